@@ -1,0 +1,189 @@
+/*
+ * pt_gravity.h -- C ABI of the MI355X (gfx950) gravity path:
+ * universe -> Morton keys -> radix sort -> octree -> CentroidData upward pass
+ * -> TransposedDownTraverser<GravityVisitor> lists -> P-C / P-P interactions
+ * -> kick/drift.
+ *
+ * The reference (paratreet/paratreet) has no FFI: its path is driven by
+ * Charm++ entry methods and C++ templates.  Each entry point below names the
+ * reference function or entry method it stands in for, so a host that used
+ * the reference's Driver loop (src/Driver.h:183-288) can call these in the
+ * same order.  All pointers are plain host or device pointers; no torch or
+ * HIP types appear in the signatures (the stream is passed as void*).
+ *
+ * Error behaviour: every call returns a pt_status (0 = PT_OK, < 0 = error)
+ * and records a message retrievable with pt_last_error(ctx).  Where the
+ * reference aborts (CkAbort / CkAssert / CmiAbort) the library returns an
+ * error instead; it never aborts the process.
+ *
+ * Threading: one context per host thread; a context owns one HIP stream and
+ * all its device buffers.  Calls are synchronous with respect to the host
+ * unless documented otherwise.
+ */
+#ifndef PT_GRAVITY_H_
+#define PT_GRAVITY_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pt_ctx pt_ctx;
+
+typedef enum {
+  PT_OK = 0,
+  PT_ERR_ARG = -1,         /* invalid argument */
+  PT_ERR_HIP = -2,         /* HIP runtime / launch failure */
+  PT_ERR_UNSUPPORTED = -3, /* configuration outside this path (e.g. kd tree) */
+  PT_ERR_CAPACITY = -4,    /* a bounded device structure overflowed */
+  PT_ERR_STATE = -5        /* stage called before its inputs exist */
+} pt_status;
+
+/* paratreet::TreeType and paratreet::DecompType (src/Configuration.h:14-29). */
+typedef enum { PT_TREE_OCT = 0, PT_TREE_BINARY_OCT = 1, PT_TREE_KD = 2, PT_TREE_LONGEST = 3 } pt_tree_type;
+typedef enum { PT_DECOMP_OCT = 0, PT_DECOMP_BINARY_OCT = 1, PT_DECOMP_SFC = 2, PT_DECOMP_KD = 3,
+               PT_DECOMP_LONGEST = 4 } pt_decomp_type;
+
+/* Arithmetic form of the interaction kernels. */
+typedef enum { PT_FP64 = 0, PT_FP32C = 1 } pt_precision;
+
+/* Where caller-supplied arrays live. */
+typedef enum { PT_HOST = 0, PT_DEVICE = 1 } pt_memory;
+
+/* The fields of paratreet::Configuration (src/Configuration.h:84-125) that
+ * the gravity path reads.  CLI letters as registered in register_fields
+ * (src/Configuration.h:132-153). */
+typedef struct pt_config {
+  int32_t min_n_subtrees;         /* '-n'; one GPU holds one subtree (tp_key 1) */
+  int32_t min_n_partitions;       /* '-p'; one partition */
+  int32_t max_particles_per_leaf; /* '-l'; bucket size B */
+  int32_t decomp_type;            /* '-d'; pt_decomp_type, only PT_DECOMP_OCT */
+  int32_t tree_type;              /* '-t'; pt_tree_type, only PT_TREE_OCT */
+  int32_t num_iterations;         /* '-i'; used by pt_run */
+  double dsoft;                   /* '-e' dSoft; applied to every particle if has_dsoft */
+  int32_t has_dsoft;              /* Configuration::origin_of("dSoft") != Unknown */
+  int32_t periodic;               /* bPeriodic; must be 0 (non-periodic path) */
+} pt_config;
+
+/* Interaction counters, src/ThreadStateHolder.h:10-13 (COUNT_INTERACTIONS). */
+typedef struct pt_counters {
+  uint64_t n_node_ints;  /* += target bucket size per node() call */
+  uint64_t n_part_ints;  /* += n_source * n_target per leaf() call */
+  uint64_t n_opens;      /* open() returned true */
+  uint64_t n_closes;     /* open() returned false */
+} pt_counters;
+
+/* Stage timings in milliseconds (HIP events on the context stream). */
+typedef enum {
+  PT_T_UNIVERSE = 0, PT_T_KEYS, PT_T_SORT, PT_T_BUILD, PT_T_MOMENTS, PT_T_TRAVERSE, PT_T_FORCE,
+  PT_T_INTEGRATE, PT_T_COUNT
+} pt_timer;
+
+/* Device pointers owned by the context (valid until the next call that
+ * resizes it).  Particle arrays in the caller's original order. */
+typedef struct pt_device_view {
+  int64_t n;
+  double *x, *y, *z, *vx, *vy, *vz, *mass, *soft;
+  double *ax, *ay, *az, *pot;
+  uint64_t* keys;        /* original order */
+  uint64_t* sorted_keys; /* sorted order */
+  uint32_t* perm;        /* sorted slot -> original index */
+  void* stream;          /* hipStream_t */
+} pt_device_view;
+
+/* Reference defaults (examples/Main.C:26-41) restricted to this path:
+ * max_particles_per_leaf = 12, octree, one subtree/partition. */
+void pt_config_default(pt_config* cfg);
+
+/* Creates a context on `device`; `stream` may be NULL (the context creates its
+ * own default-priority stream) or an existing hipStream_t. */
+pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx** out);
+void pt_ctx_destroy(pt_ctx* ctx);
+const char* pt_last_error(const pt_ctx* ctx);
+pt_status pt_get_device_view(pt_ctx* ctx, pt_device_view* out);
+
+/* Reader::load (src/Reader.C:11-108) + Reader::setSoft (src/Reader.C:110-116):
+ * takes the particle set (SoA, original order).  vx/vy/vz may be NULL (zero
+ * velocities). */
+pt_status pt_set_particles(pt_ctx* ctx, int64_t n, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy,
+                           const double* vz, const double* mass, const double* soft,
+                           pt_memory where);
+
+/* Driver::remakeUniverse (src/Driver.h:67-80) over the BoundingBox reduction
+ * (src/BoundingBox.C:29-44,74-85).  box_out (may be NULL) = {lesser, greater}. */
+pt_status pt_remake_universe(pt_ctx* ctx, double box_out[6]);
+
+/* Decomposition::assignKeys (src/Decomposition.C:58-64) /
+ * Particle::adjustNewUniverse (src/Particle.C:23-35): Morton key | 1<<63. */
+pt_status pt_assign_keys(pt_ctx* ctx);
+
+/* Subtree::buildTree's std::sort (src/Subtree.h:281), Reader::localSort
+ * (src/Reader.C:237-241): ascending key, ties by original index. */
+pt_status pt_sort(pt_ctx* ctx);
+
+/* Subtree::buildTree / recursiveBuild (src/Subtree.h:275-349). */
+pt_status pt_build_tree(pt_ctx* ctx);
+
+/* CentroidData leaf constructor + Subtree::populateTree (src/Subtree.h:351-384,
+ * examples/CentroidData.h:41-73). */
+pt_status pt_upward(pt_ctx* ctx);
+
+/* Partition::startDown<GravityVisitor> (src/Partition.h:129-134) traversal
+ * part: builds per-bucket interaction lists for GravityVisitor(offset, theta)
+ * (examples/GravityVisitor.h:16-20) and the counters of
+ * src/Traverser.h:15-37. */
+pt_status pt_traverse(pt_ctx* ctx, double theta, const double offset[3], pt_counters* counters);
+
+/* The visitor calls of the same traversal: GravityVisitor::node/leaf
+ * (examples/GravityVisitor.h:144-224) over the lists, writing acceleration and
+ * potential per particle (the reference accumulates onto zeroed particles). */
+pt_status pt_interact(pt_ctx* ctx, pt_precision precision);
+
+/* One force evaluation: universe, keys, sort, build, upward, traverse,
+ * interact -- paratreet::traversalFn with everything before it in one
+ * Driver::run iteration (src/Driver.h:190-221, examples/Gravity.C:29-33). */
+pt_status pt_compute_forces(pt_ctx* ctx, double theta, const double offset[3],
+                            pt_precision precision, pt_counters* counters);
+
+/* Partition::kick + Partition::perturb (src/Partition.h:329-379 ->
+ * src/Particle.C:7-21) with the last computed accelerations. */
+pt_status pt_kick_drift(pt_ctx* ctx, double dt);
+
+/* Driver::run (src/Driver.h:183-288) restricted to this path: `steps`
+ * iterations of compute_forces + kick_drift with a fixed dt. */
+pt_status pt_run(pt_ctx* ctx, int steps, double dt, double theta, pt_precision precision,
+                 pt_counters* counters);
+
+/* FP64 direct sum with the leaf() kernel for the given original-order target
+ * indices, over all particles (error reference).  acc_out: nt x 3 host. */
+pt_status pt_direct_sum(pt_ctx* ctx, const int64_t* targets, int64_t nt, double* acc_out);
+
+/* ---- read-back (host buffers, synchronous) ---- */
+pt_status pt_get_universe(pt_ctx* ctx, double box[6]);
+pt_status pt_get_keys(pt_ctx* ctx, uint64_t* keys);                 /* original order */
+pt_status pt_get_sorted(pt_ctx* ctx, uint64_t* sorted_keys, uint32_t* perm);
+pt_status pt_get_tree_size(pt_ctx* ctx, int64_t* n_nodes, int64_t* n_buckets, int32_t* max_depth);
+/* Node table, level order.  Any output may be NULL. */
+pt_status pt_get_nodes(pt_ctx* ctx, uint64_t* key, int32_t* type, int32_t* depth, int32_t* begin,
+                       int32_t* count, int32_t* parent, int32_t* first_child);
+/* CentroidData per node: 18 doubles per node in the order radius soft mass
+ * cm[3] q[6](xx xy xz yy yz zz) lesser[3] greater[3], node-major. */
+pt_status pt_get_node_data(pt_ctx* ctx, double* out);
+pt_status pt_get_buckets(pt_ctx* ctx, int32_t* bucket_nodes);       /* DFS order */
+pt_status pt_get_lists_size(pt_ctx* ctx, int64_t* n_pp, int64_t* n_pc);
+/* Lists per bucket in canonical DFS order (ascending left-aligned key). */
+pt_status pt_get_lists(pt_ctx* ctx, int64_t* pp_off, int32_t* pp, int64_t* pc_off, int32_t* pc);
+/* Accelerations / potentials in original particle order (n x 3 and n). */
+pt_status pt_get_accelerations(pt_ctx* ctx, double* acc, double* pot);
+pt_status pt_get_state(pt_ctx* ctx, double* pos, double* vel);       /* n x 3 each */
+pt_status pt_get_counters(pt_ctx* ctx, pt_counters* out);
+pt_status pt_set_timing(pt_ctx* ctx, int enabled);
+pt_status pt_get_timings(pt_ctx* ctx, float ms[PT_T_COUNT]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_GRAVITY_H_ */

@@ -1,0 +1,245 @@
+"""ctypes binding of the CPU oracle (oracle/pt_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package
+(paratreet_amd) never imports this module.
+
+Everything here takes numpy arrays; the C side is single-source-of-truth for the
+arithmetic (see the header comment of pt_oracle.c for how each function is
+pinned).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libpt_oracle.so")
+
+OR_EMPTY, OR_LEAF, OR_INTERNAL = 0, 1, 2
+
+NODE_DTYPE = np.dtype([
+    ("key", np.uint64), ("type", np.int32), ("depth", np.int32),
+    ("begin", np.int64), ("count", np.int64), ("parent", np.int32),
+    ("child", np.int32, (8,)),
+], align=True)
+
+CDATA_DTYPE = np.dtype([
+    ("radius", np.float64), ("soft", np.float64), ("mass", np.float64),
+    ("cm", np.float64, (3,)), ("q", np.float64, (6,)),
+    ("lo", np.float64, (3,)), ("hi", np.float64, (3,)), ("count", np.int64),
+], align=True)
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc; no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+
+
+def _declare(L):
+    L.or_universe.argtypes = [_P, _P, _P, _I64, _P]
+    L.or_keys.argtypes = [_P, _P, _P, _I64, _P, _P]
+    L.or_sort.argtypes = [_P, _I64, _P, _P]
+    L.or_build.argtypes = [_P, _I64, ctypes.c_int, _P, _I64]
+    L.or_build.restype = _I64
+    L.or_moments.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P]
+    L.or_buckets.argtypes = [_P, _I64, _P]
+    L.or_buckets.restype = _I64
+    L.or_gravity.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _D, _P, _P, _I64,
+                             _P, _P, _P, _P, _P, ctypes.c_int]
+    L.or_lists.argtypes = [_P, _I64, _P, _D, _P, _P, _P, _P, _P, _P, _P]
+    L.or_direct.argtypes = [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, ctypes.c_int]
+    L.or_spline.argtypes = [_D, _D, _P, _P]
+    L.or_splineq.argtypes = [_D, _D, _D, _P, _P, _P, _P]
+    L.or_intersect_box_point.argtypes = [_P, _P, _P, _D]
+    L.or_intersect_box_point.restype = ctypes.c_int
+    L.or_kick_drift.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _D]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def universe(x, y, z):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    box = np.zeros(6)
+    lib().or_universe(_p(x), _p(y), _p(z), len(x), _p(box))
+    return box
+
+
+def keys(x, y, z, box):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    box = _f64(box)
+    k = np.zeros(len(x), dtype=np.uint64)
+    lib().or_keys(_p(x), _p(y), _p(z), len(x), _p(box), _p(k))
+    return k
+
+
+def sort(k):
+    k = np.ascontiguousarray(k, dtype=np.uint64)
+    sk = np.zeros_like(k)
+    perm = np.zeros(len(k), dtype=np.int32)
+    lib().or_sort(_p(k), len(k), _p(sk), _p(perm))
+    return sk, perm
+
+
+def build_tree(sorted_keys, bucket):
+    sk = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    cap = max(64, 4 * len(sk) // max(bucket, 1) + 64)
+    while True:
+        nodes = np.zeros(cap, dtype=NODE_DTYPE)
+        nn = lib().or_build(_p(sk), len(sk), bucket, _p(nodes), cap)
+        if nn >= 0:
+            return nodes[:nn].copy()
+        cap = -nn
+
+
+def moments(nodes, x, y, z, m, eps):
+    nodes = np.ascontiguousarray(nodes)
+    out = np.zeros(len(nodes), dtype=CDATA_DTYPE)
+    x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
+    lib().or_moments(_p(nodes), len(nodes), _p(x), _p(y), _p(z), _p(m), _p(eps), _p(out))
+    return out
+
+
+def buckets(nodes):
+    nodes = np.ascontiguousarray(nodes)
+    nb = lib().or_buckets(_p(nodes), len(nodes), None)
+    out = np.zeros(nb, dtype=np.int32)
+    lib().or_buckets(_p(nodes), len(nodes), _p(out))
+    return out
+
+
+def gravity(nodes, data, x, y, z, m, eps, theta, offset=(0.0, 0.0, 0.0),
+            bucket_sel=None, nthreads=0):
+    """Accelerations and potentials in sorted order + counters
+    {n_node_ints, n_part_ints, n_opens, n_closes}."""
+    n = len(x)
+    x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
+    off = _f64(offset)
+    ax, ay, az, pot = (np.zeros(n) for _ in range(4))
+    cnt = np.zeros(4, dtype=np.uint64)
+    sel = None if bucket_sel is None else np.ascontiguousarray(bucket_sel, dtype=np.int32)
+    lib().or_gravity(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m), _p(eps),
+                     float(theta), _p(off), _p(sel), 0 if sel is None else len(sel),
+                     _p(ax), _p(ay), _p(az), _p(pot), _p(cnt), int(nthreads))
+    return np.stack([ax, ay, az], axis=1), pot, cnt
+
+
+def lists(nodes, data, theta, offset=(0.0, 0.0, 0.0)):
+    """Per-bucket interaction lists in DFS order: (pp_off, pp, pc_off, pc) with
+    node ids into `nodes`."""
+    nb = len(buckets(nodes))
+    off = _f64(offset)
+    ppc = np.zeros(nb, dtype=np.int64)
+    pcc = np.zeros(nb, dtype=np.int64)
+    L = lib()
+    L.or_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), _p(ppc), _p(pcc),
+               None, None, None, None)
+    pp_off = np.zeros(nb + 1, dtype=np.int64)
+    pc_off = np.zeros(nb + 1, dtype=np.int64)
+    np.cumsum(ppc, out=pp_off[1:])
+    np.cumsum(pcc, out=pc_off[1:])
+    pp = np.zeros(max(1, pp_off[-1]), dtype=np.int32)
+    pc = np.zeros(max(1, pc_off[-1]), dtype=np.int32)
+    L.or_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), _p(ppc), _p(pcc),
+               _p(pp_off), _p(pc_off), _p(pp), _p(pc))
+    return pp_off, pp[:pp_off[-1]], pc_off, pc[:pc_off[-1]]
+
+
+def direct(x, y, z, m, eps, targets, nthreads=0):
+    x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    ax, ay, az = (np.zeros(len(t)) for _ in range(3))
+    lib().or_direct(_p(x), _p(y), _p(z), _p(m), _p(eps), len(x), _p(t), len(t),
+                    _p(ax), _p(ay), _p(az), int(nthreads))
+    return np.stack([ax, ay, az], axis=1)
+
+
+def spline(r2, twoh):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().or_spline(float(r2), float(twoh), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def splineq(invr, r2, twoh):
+    v = [ctypes.c_double() for _ in range(4)]
+    lib().or_splineq(float(invr), float(r2), float(twoh), *[ctypes.byref(t) for t in v])
+    return tuple(t.value for t in v)
+
+
+def intersect_box_point(lo, hi, p, rsq):
+    lo, hi, p = _f64(lo), _f64(hi), _f64(p)
+    return bool(lib().or_intersect_box_point(_p(lo), _p(hi), _p(p), float(rsq)))
+
+
+def kick_drift(pos, vel, acc, dt):
+    """Returns new (pos, vel); inputs are (n, 3) arrays."""
+    x, y, z = (np.array(pos[:, k], dtype=np.float64) for k in range(3))
+    vx, vy, vz = (np.array(vel[:, k], dtype=np.float64) for k in range(3))
+    ax, ay, az = (_f64(acc[:, k]) for k in range(3))
+    lib().or_kick_drift(_p(x), _p(y), _p(z), _p(vx), _p(vy), _p(vz), _p(ax), _p(ay), _p(az),
+                        len(x), float(dt))
+    return np.stack([x, y, z], axis=1), np.stack([vx, vy, vz], axis=1)
+
+
+class Pipeline:
+    """Full oracle pipeline on one particle set (original order in, sorted
+    order inside).  Attributes mirror the stages of the reference path."""
+
+    def __init__(self, pos, mass, eps, bucket, theta, offset=(0.0, 0.0, 0.0)):
+        pos = np.asarray(pos, dtype=np.float64)
+        self.n = len(pos)
+        self.bucket = bucket
+        self.theta = theta
+        self.offset = offset
+        self.box = universe(pos[:, 0], pos[:, 1], pos[:, 2])
+        self.keys = keys(pos[:, 0], pos[:, 1], pos[:, 2], self.box)
+        self.sorted_keys, self.perm = sort(self.keys)
+        self.x = pos[self.perm, 0].copy()
+        self.y = pos[self.perm, 1].copy()
+        self.z = pos[self.perm, 2].copy()
+        self.m = np.asarray(mass, dtype=np.float64)[self.perm].copy()
+        self.eps = np.asarray(eps, dtype=np.float64)[self.perm].copy()
+        self.nodes = build_tree(self.sorted_keys, bucket)
+        self.data = moments(self.nodes, self.x, self.y, self.z, self.m, self.eps)
+        self.bucket_nodes = buckets(self.nodes)
+
+    def gravity(self, bucket_sel=None, nthreads=0):
+        acc, pot, cnt = gravity(self.nodes, self.data, self.x, self.y, self.z, self.m,
+                                self.eps, self.theta, self.offset, bucket_sel, nthreads)
+        return acc, pot, cnt
+
+    def gravity_unsorted(self, nthreads=0):
+        """Accelerations/potentials in the caller's original particle order."""
+        acc, pot, cnt = self.gravity(nthreads=nthreads)
+        out_a = np.empty_like(acc)
+        out_p = np.empty_like(pot)
+        out_a[self.perm] = acc
+        out_p[self.perm] = pot
+        return out_a, out_p, cnt
+
+    def lists(self):
+        return lists(self.nodes, self.data, self.theta, self.offset)

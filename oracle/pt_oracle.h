@@ -1,0 +1,134 @@
+/*
+ * pt_oracle.h -- CPU restatement of ParaTreeT's gravity hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path (paratreet_amd, libpt_gravity.so) never links,
+ * loads or falls back to it.
+ *
+ * Every function restates one piece of the reference (paratreet/paratreet) in
+ * plain C with Real = double (the reference built with -DUSE_DOUBLE_FP, no
+ * -DHEXADECAPOLE, Morton keys).  Arithmetic follows the reference's operation
+ * order so results are reproducible bit for bit when compiled with
+ * -ffp-contract=off (see oracle/Makefile).
+ *
+ * Pieces whose source is NOT in the reference tree (the unvendored
+ * N-BodyShop/utility submodule: SFC::generateKey, Space::intersect,
+ * OrientedBox) follow the explicit definitions written next to each function;
+ * how each is pinned is stated in the header comment of pt_oracle.c.
+ */
+#ifndef PT_ORACLE_H_
+#define PT_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node types, numbered as in src/Node.h:82-95 restricted to the local tree. */
+enum { OR_EMPTY = 0, OR_LEAF = 1, OR_INTERNAL = 2 };
+
+/* One octree node (DFS pre-order table).  Mirrors Node<Data> fields
+ * key/type/depth/parent and the particle range implied by recursiveBuild
+ * (src/Subtree.h:313-349). child[] = -1 for leaves. */
+typedef struct {
+  uint64_t key;
+  int32_t type;
+  int32_t depth;
+  int64_t begin;
+  int64_t count;
+  int32_t parent;
+  int32_t child[8];
+} or_node;
+
+/* CentroidData (examples/CentroidData.h:24-91) with MultipoleMoments
+ * (examples/MultipoleMoments.h:161-201, non-HEXADECAPOLE) flattened.
+ * q = {xx, xy, xz, yy, yz, zz}.  Box lo/hi = OrientedBox lesser/greater. */
+typedef struct {
+  double radius, soft, mass;
+  double cm[3];
+  double q[6];
+  double lo[3], hi[3];
+  int64_t count;
+} or_cdata;
+
+/* Driver::remakeUniverse (src/Driver.h:67-80) over the particle bbox. */
+void or_universe(const double* x, const double* y, const double* z, int64_t n,
+                 double box_out[6]);
+
+/* Decomposition::assignKeys (src/Decomposition.C:58-64): Morton key + bit 63. */
+void or_keys(const double* x, const double* y, const double* z, int64_t n,
+             const double box[6], uint64_t* keys);
+
+/* Subtree::buildTree's sort (src/Subtree.h:281): ascending key, ties by
+ * original index.  perm[i] = original index of sorted slot i. */
+void or_sort(const uint64_t* keys, int64_t n, uint64_t* sorted_keys, int32_t* perm);
+
+/* Subtree::buildTree / recursiveBuild (src/Subtree.h:275-349).  Writes nodes
+ * in DFS pre-order; returns the node count, or -(needed) if cap is too small. */
+int64_t or_build(const uint64_t* sorted_keys, int64_t n, int bucket,
+                 or_node* nodes, int64_t cap);
+
+/* Leaf CentroidData + Subtree::populateTree queue-order upward pass
+ * (examples/CentroidData.h:41-73, examples/MultipoleMoments.h:204-353,458-482,
+ * src/Subtree.h:351-384).  Inputs are in sorted order. */
+void or_moments(const or_node* nodes, int64_t nn,
+                const double* x, const double* y, const double* z,
+                const double* m, const double* eps, or_cdata* out);
+
+/* Number of buckets (non-empty leaves) and their node ids in DFS order. */
+int64_t or_buckets(const or_node* nodes, int64_t nn, int32_t* bucket_nodes);
+
+/* TransposedDownTraverser<GravityVisitor> (src/Traverser.h:154-217) per
+ * bucket, with GravityVisitor::open/node/leaf (examples/GravityVisitor.h).
+ * bucket_sel: NULL = all buckets, else the n_sel bucket ordinals to evaluate.
+ * acc/pot are overwritten (zeroed) for the selected buckets' particles and
+ * accumulate in DFS interaction order.  counters = {n_node_ints, n_part_ints,
+ * n_opens, n_closes} (src/ThreadStateHolder.h:10-13), summed over the
+ * selected buckets.  nthreads <= 0 uses the OpenMP default. */
+void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
+                const double* x, const double* y, const double* z,
+                const double* m, const double* eps,
+                double theta, const double offset[3],
+                const int32_t* bucket_sel, int64_t n_sel,
+                double* ax, double* ay, double* az, double* pot,
+                uint64_t counters[4], int nthreads);
+
+/* Interaction lists in DFS order, per bucket.  First call with pp/pc NULL
+ * fills pp_count/pc_count (per bucket); second call with offsets (exclusive
+ * prefix sums) fills node ids.  kind: leaf() sources go to pp, node() sources
+ * to pc. */
+void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
+              double theta, const double offset[3],
+              int64_t* pp_count, int64_t* pc_count,
+              const int64_t* pp_off, const int64_t* pc_off,
+              int32_t* pp, int32_t* pc);
+
+/* Direct O(N*M) sum with the leaf() kernel and SPLINE softening, 2h = eps_i +
+ * eps_j, skipping r^2 == 0 (examples/GravityVisitor.h:87-120,144-160). */
+void or_direct(const double* x, const double* y, const double* z,
+               const double* m, const double* eps, int64_t n,
+               const int64_t* targets, int64_t nt,
+               double* ax, double* ay, double* az, int nthreads);
+
+/* GravityVisitor::SPLINE / SPLINEQ exposed for closed-form tests. */
+void or_spline(double r2, double twoh, double* a, double* b);
+void or_splineq(double invr, double r2, double twoh,
+                double* a, double* b, double* c, double* d);
+
+/* Space::intersect(box, point, rsq) as defined in pt_oracle.c. */
+int or_intersect_box_point(const double lo[3], const double hi[3],
+                           const double p[3], double rsq);
+
+/* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
+void or_kick_drift(double* x, double* y, double* z,
+                   double* vx, double* vy, double* vz,
+                   const double* ax, const double* ay, const double* az,
+                   int64_t n, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_ORACLE_H_ */

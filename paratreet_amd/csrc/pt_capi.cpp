@@ -1,0 +1,839 @@
+// C ABI of the gfx950 gravity path: context, device buffers and the stage
+// sequence of one Driver::run iteration (src/Driver.h:183-288) for the
+// gravity visitor.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pt_gravity.h"
+#include "pt_common.h"
+#include "pt_kernels.h"
+
+namespace {
+
+constexpr int kMaxBoxParts = 1024;
+
+enum Stage {
+  S_NONE = 0,
+  S_PARTICLES,
+  S_UNIVERSE,
+  S_KEYS,
+  S_SORTED,
+  S_TREE,
+  S_MOMENTS,
+  S_LISTS,
+  S_FORCES
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  int64_t cap = 0;
+};
+
+}  // namespace
+
+struct pt_ctx {
+  pt_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  int stage = S_NONE;
+  bool have_forces = false;
+
+  int64_t n = 0;
+  // particle state, original order
+  DBuf<double> x, y, z, vx, vy, vz, m, e, ax, ay, az, pot;
+  // keys and sorted copies
+  DBuf<uint64_t> keys, skeys;
+  DBuf<uint32_t> perm;
+  DBuf<double> xs, ys, zs, ms, es, axs, ays, azs, pots;
+  // universe
+  DBuf<double> box, bbox, box_parts;
+  double h_box[6] = {0, 0, 0, 0, 0, 0};
+  // tree
+  int64_t nn = 0, nb = 0;
+  int32_t max_depth = 0;
+  DBuf<uint64_t> nkey;
+  DBuf<int32_t> nbegin, ncount, ndepth, ntype, nparent, nfirst;
+  DBuf<int32_t> level_nodes;
+  std::vector<int64_t> level_off;  // internal nodes of depth d: [level_off[d], level_off[d+1])
+  DBuf<uint32_t> flag, fpos, head, head_pos, u32_total;
+  DBuf<int32_t> bucket_node;
+  DBuf<double> nd;
+  DBuf<int32_t> ord_h;
+  DBuf<uint64_t> ord_b;
+  DBuf<TravNode> tn;
+  DBuf<TopoNode> topo;
+  DBuf<GravNode> gn;
+  DBuf<GravNode32> gn32;
+  DBuf<Bucket> bk;
+  // lists
+  DBuf<int32_t> pp_cnt, pc_cnt, pp, pc;
+  DBuf<int64_t> pp_off, pc_off, i64_total;
+  int64_t n_pp = 0, n_pc = 0;
+  double theta = 0.7;
+  double offset[3] = {0, 0, 0};
+  DBuf<unsigned long long> counters;
+  DBuf<int32_t> derr;
+  pt_counters h_counters{};
+  // scratch
+  DBuf<char> scratch;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[PT_T_COUNT][2] = {};
+  bool ev_used[PT_T_COUNT] = {};
+  float ms_t[PT_T_COUNT] = {};
+};
+
+namespace {
+
+pt_status fail(pt_ctx* c, pt_status st, const std::string& msg) {
+  if (c) c->err = msg;
+  return st;
+}
+
+pt_status hip_fail(pt_ctx* c, const char* what, hipError_t e) {
+  return fail(c, PT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PT_TRY_HIP(c, expr)                                   \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return hip_fail((c), #expr, _e);    \
+  } while (0)
+
+#define PT_TRY_LAUNCH(c, expr)                                              \
+  do {                                                                      \
+    if ((expr) != 0) {                                                      \
+      hipError_t _e = hipGetLastError();                                    \
+      return hip_fail((c), #expr, _e == hipSuccess ? hipErrorUnknown : _e); \
+    }                                                                       \
+  } while (0)
+
+template <typename T>
+hipError_t ensure(DBuf<T>& b, int64_t count) {
+  if (count <= b.cap) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return e;
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  int64_t c = std::max<int64_t>(count, 64);
+  hipError_t e = hipMalloc((void**)&b.p, (size_t)c * sizeof(T));
+  if (e != hipSuccess) return e;
+  b.cap = c;
+  return hipSuccess;
+}
+
+// Grows while keeping the first `keep` elements.
+template <typename T>
+hipError_t grow_keep(DBuf<T>& b, int64_t count, int64_t keep, hipStream_t s) {
+  if (count <= b.cap) return hipSuccess;
+  int64_t c = std::max<int64_t>(count, 2 * b.cap);
+  T* np = nullptr;
+  hipError_t e = hipMalloc((void**)&np, (size_t)c * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (b.p && keep > 0) {
+    e = hipMemcpyAsync(np, b.p, (size_t)keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  if (b.p) (void)hipFree(b.p);
+  b.p = np;
+  b.cap = c;
+  return hipSuccess;
+}
+
+template <typename T>
+void release(DBuf<T>& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+PtTree tree_of(pt_ctx* c) {
+  PtTree t;
+  t.key = c->nkey.p;
+  t.begin = c->nbegin.p;
+  t.count = c->ncount.p;
+  t.depth = c->ndepth.p;
+  t.type = c->ntype.p;
+  t.parent = c->nparent.p;
+  t.first_child = c->nfirst.p;
+  return t;
+}
+
+struct Timer {
+  pt_ctx* c;
+  int id;
+  Timer(pt_ctx* c_, int id_) : c(c_), id(id_) {
+    if (c->timing) {
+      (void)hipEventRecord(c->ev[id][0], c->stream);
+    }
+  }
+  ~Timer() {
+    if (c->timing) {
+      (void)hipEventRecord(c->ev[id][1], c->stream);
+      c->ev_used[id] = true;
+    }
+  }
+};
+
+pt_status check_ctx(pt_ctx* c) {
+  if (!c) return PT_ERR_ARG;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hip_fail(c, "hipSetDevice", e);
+  return PT_OK;
+}
+
+pt_status need(pt_ctx* c, int stage, const char* what) {
+  if (c->stage < stage) return fail(c, PT_ERR_STATE, std::string(what) + " called before its inputs exist");
+  return PT_OK;
+}
+
+pt_status ensure_scratch(pt_ctx* c, int64_t bytes) {
+  PT_TRY_HIP(c, ensure(c->scratch, bytes));
+  return PT_OK;
+}
+
+// Node table capacity with contents preserved up to c->nn.
+pt_status ensure_nodes(pt_ctx* c, int64_t count) {
+  const int64_t keep = c->nn;
+  PT_TRY_HIP(c, grow_keep(c->nkey, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->nbegin, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->ncount, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->ndepth, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->ntype, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->nparent, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->nfirst, count, keep, c->stream));
+  PT_TRY_HIP(c, grow_keep(c->level_nodes, count, keep, c->stream));
+  return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pt_config_default(pt_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->min_n_subtrees = 1;
+  cfg->min_n_partitions = 1;
+  cfg->max_particles_per_leaf = 12;
+  cfg->decomp_type = PT_DECOMP_OCT;
+  cfg->tree_type = PT_TREE_OCT;
+  cfg->num_iterations = 1;
+  cfg->dsoft = 0.0;
+  cfg->has_dsoft = 0;
+  cfg->periodic = 0;
+}
+
+pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx** out) {
+  if (!cfg || !out) return PT_ERR_ARG;
+  *out = nullptr;
+  if (cfg->tree_type != PT_TREE_OCT || cfg->decomp_type != PT_DECOMP_OCT) return PT_ERR_UNSUPPORTED;
+  if (cfg->max_particles_per_leaf < 1) return PT_ERR_ARG;
+  if (cfg->periodic) return PT_ERR_UNSUPPORTED;
+  pt_ctx* c = new pt_ctx();
+  c->cfg = *cfg;
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return PT_ERR_HIP;
+  }
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+      delete c;
+      return PT_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  for (int i = 0; i < PT_T_COUNT; i++) {
+    (void)hipEventCreate(&c->ev[i][0]);
+    (void)hipEventCreate(&c->ev[i][1]);
+  }
+  if (ensure(c->box, 6) != hipSuccess || ensure(c->bbox, 6) != hipSuccess ||
+      ensure(c->box_parts, 6 * kMaxBoxParts) != hipSuccess || ensure(c->counters, 4) != hipSuccess ||
+      ensure(c->derr, 1) != hipSuccess || ensure(c->u32_total, 4) != hipSuccess ||
+      ensure(c->i64_total, 4) != hipSuccess) {
+    pt_ctx_destroy(c);
+    return PT_ERR_HIP;
+  }
+  *out = c;
+  return PT_OK;
+}
+
+void pt_ctx_destroy(pt_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release(c->x); release(c->y); release(c->z); release(c->vx); release(c->vy); release(c->vz);
+  release(c->m); release(c->e); release(c->ax); release(c->ay); release(c->az); release(c->pot);
+  release(c->keys); release(c->skeys); release(c->perm);
+  release(c->xs); release(c->ys); release(c->zs); release(c->ms); release(c->es);
+  release(c->axs); release(c->ays); release(c->azs); release(c->pots);
+  release(c->box); release(c->bbox); release(c->box_parts);
+  release(c->nkey); release(c->nbegin); release(c->ncount); release(c->ndepth); release(c->ntype);
+  release(c->nparent); release(c->nfirst); release(c->level_nodes);
+  release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
+  release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
+  release(c->tn); release(c->topo); release(c->gn); release(c->gn32); release(c->bk);
+  release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
+  release(c->pp_off); release(c->pc_off); release(c->i64_total);
+  release(c->counters); release(c->derr); release(c->scratch);
+  for (int i = 0; i < PT_T_COUNT; i++) {
+    if (c->ev[i][0]) (void)hipEventDestroy(c->ev[i][0]);
+    if (c->ev[i][1]) (void)hipEventDestroy(c->ev[i][1]);
+  }
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pt_last_error(const pt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+pt_status pt_get_device_view(pt_ctx* c, pt_device_view* out) {
+  if (!c || !out) return PT_ERR_ARG;
+  out->n = c->n;
+  out->x = c->x.p; out->y = c->y.p; out->z = c->z.p;
+  out->vx = c->vx.p; out->vy = c->vy.p; out->vz = c->vz.p;
+  out->mass = c->m.p; out->soft = c->e.p;
+  out->ax = c->ax.p; out->ay = c->ay.p; out->az = c->az.p; out->pot = c->pot.p;
+  out->keys = c->keys.p; out->sorted_keys = c->skeys.p; out->perm = c->perm.p;
+  out->stream = (void*)c->stream;
+  return PT_OK;
+}
+
+pt_status pt_set_particles(pt_ctx* c, int64_t n, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy,
+                           const double* vz, const double* mass, const double* soft,
+                           pt_memory where) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if (n < 0 || n > 0x7fffffffll) return fail(c, PT_ERR_ARG, "particle count out of range");
+  if (n > 0 && (!x || !y || !z || !mass || (!soft && !c->cfg.has_dsoft)))
+    return fail(c, PT_ERR_ARG, "missing particle array");
+  DBuf<double>* arrs[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->m, &c->e,
+                          &c->ax, &c->ay, &c->az, &c->pot, &c->xs, &c->ys, &c->zs,
+                          &c->ms, &c->es, &c->axs, &c->ays, &c->azs, &c->pots};
+  for (auto* a : arrs) PT_TRY_HIP(c, ensure(*a, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->keys, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->skeys, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->perm, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->head, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->head_pos, std::max<int64_t>(n, 1)));
+  PT_TRY_HIP(c, ensure(c->bucket_node, std::max<int64_t>(n, 1)));
+  c->n = n;
+  c->have_forces = false;
+  const hipMemcpyKind kind = where == PT_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (n > 0) {
+    PT_TRY_HIP(c, hipMemcpyAsync(c->x.p, x, bytes, kind, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(c->y.p, y, bytes, kind, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(c->z.p, z, bytes, kind, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(c->m.p, mass, bytes, kind, c->stream));
+    const double* vv[3] = {vx, vy, vz};
+    double* vd[3] = {c->vx.p, c->vy.p, c->vz.p};
+    for (int k = 0; k < 3; k++) {
+      if (vv[k]) {
+        PT_TRY_HIP(c, hipMemcpyAsync(vd[k], vv[k], bytes, kind, c->stream));
+      } else {
+        PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)vd[k], n, 0ull, c->stream));
+      }
+    }
+    if (c->cfg.has_dsoft) {
+      // Reader::setSoft: every particle gets dSoft
+      unsigned long long bits;
+      std::memcpy(&bits, &c->cfg.dsoft, sizeof(bits));
+      PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->e.p, n, bits, c->stream));
+    } else {
+      PT_TRY_HIP(c, hipMemcpyAsync(c->e.p, soft, bytes, kind, c->stream));
+    }
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->ax.p, n, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->ay.p, n, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->az.p, n, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->pot.p, n, 0ull, c->stream));
+  }
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->stage = S_PARTICLES;
+  return PT_OK;
+}
+
+pt_status pt_remake_universe(pt_ctx* c, double box_out[6]) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_PARTICLES, "pt_remake_universe"))) return st;
+  if (c->n == 0) return fail(c, PT_ERR_ARG, "no particles");
+  {
+    Timer t(c, PT_T_UNIVERSE);
+    PT_TRY_LAUNCH(c, pt_launch_universe(c->x.p, c->y.p, c->z.p, c->n, c->box_parts.p, kMaxBoxParts,
+                                        c->bbox.p, c->box.p, c->stream));
+  }
+  if (box_out) {
+    PT_TRY_HIP(c, hipMemcpyAsync(c->h_box, c->box.p, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(box_out, c->h_box, sizeof(c->h_box));
+  }
+  c->stage = S_UNIVERSE;
+  return PT_OK;
+}
+
+pt_status pt_assign_keys(pt_ctx* c) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_UNIVERSE, "pt_assign_keys"))) return st;
+  Timer t(c, PT_T_KEYS);
+  PT_TRY_LAUNCH(c, pt_launch_keys(c->x.p, c->y.p, c->z.p, c->n, c->box.p, c->keys.p, c->perm.p,
+                                  c->stream));
+  c->stage = S_KEYS;
+  return PT_OK;
+}
+
+pt_status pt_sort(pt_ctx* c) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_KEYS, "pt_sort"))) return st;
+  if ((st = ensure_scratch(c, pt_sort_scratch_bytes(c->n)))) return st;
+  Timer t(c, PT_T_SORT);
+  PT_TRY_HIP(c, hipMemcpyAsync(c->skeys.p, c->keys.p, (size_t)c->n * 8, hipMemcpyDeviceToDevice,
+                               c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_iota(c->perm.p, c->n, c->stream));
+  PT_TRY_LAUNCH(c, pt_radix_sort(c->skeys.p, c->perm.p, c->n, 0, 64, c->scratch.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->x.p, c->y.p, c->z.p, c->m.p, c->e.p,
+                                    c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, c->stream));
+  c->stage = S_SORTED;
+  return PT_OK;
+}
+
+pt_status pt_build_tree(pt_ctx* c) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_SORTED, "pt_build_tree"))) return st;
+  const int B = c->cfg.max_particles_per_leaf;
+  Timer t(c, PT_T_BUILD);
+  c->nn = 0;
+  int64_t est = std::max<int64_t>(1024, 8 * (2 * c->n / (B + 1)) + 64);
+  if ((st = ensure_nodes(c, est))) return st;
+  PtTree tr = tree_of(c);
+  PT_TRY_LAUNCH(c, pt_launch_root(c->n, B, tr, c->stream));
+  c->nn = 1;
+  c->level_off.assign(1, 0);
+  c->max_depth = 0;
+  int64_t nf = (c->n > B) ? 1 : 0;
+  if (nf) {
+    // level 0 frontier = {root}
+    PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->level_nodes.p, 1, 0u, c->stream));
+  }
+  c->level_off.push_back(nf);
+  int depth = 0;
+  while (nf > 0) {
+    const int64_t base = c->nn;
+    const int64_t m = 8 * nf;
+    if ((st = ensure_nodes(c, base + m))) return st;
+    tr = tree_of(c);
+    PT_TRY_HIP(c, ensure(c->flag, m));
+    PT_TRY_HIP(c, ensure(c->fpos, m));
+    if ((st = ensure_scratch(c, pt_scan_scratch_bytes(m)))) return st;
+    const int64_t f0 = c->level_off[depth];
+    PT_TRY_LAUNCH(c, pt_launch_split(c->skeys.p, c->level_nodes.p + f0, (int32_t)nf, (int32_t)base, B,
+                                     depth + 1, tr, c->flag.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_exclusive_scan_u32(c->flag.p, c->fpos.p, m, c->scratch.p, c->u32_total.p,
+                                           c->stream));
+    uint32_t nnext = 0;
+    PT_TRY_HIP(c, hipMemcpyAsync(&nnext, c->u32_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    c->nn = base + m;
+    depth++;
+    c->max_depth = depth;
+    const int64_t f1 = c->level_off[depth];
+    if (nnext > 0) {
+      PT_TRY_LAUNCH(c, pt_launch_compact(c->flag.p, c->fpos.p, m, (int32_t)base,
+                                         c->level_nodes.p + f1, c->stream));
+    }
+    c->level_off.push_back(f1 + nnext);
+    nf = nnext;
+    if (depth > PT_MAX_DEPTH) return fail(c, PT_ERR_CAPACITY, "tree deeper than the key allows");
+  }
+  // buckets: non-empty leaves in particle (= DFS) order
+  if ((st = ensure_scratch(c, pt_scan_scratch_bytes(c->n)))) return st;
+  PT_TRY_LAUNCH(c, pt_launch_buckets(tr, c->nn, c->n, c->head.p, c->head_pos.p, c->scratch.p,
+                                     c->u32_total.p, c->bucket_node.p, c->stream));
+  uint32_t nb = 0;
+  PT_TRY_HIP(c, hipMemcpyAsync(&nb, c->u32_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->nb = nb;
+  c->stage = S_TREE;
+  return PT_OK;
+}
+
+pt_status pt_upward(pt_ctx* c) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_TREE, "pt_upward"))) return st;
+  Timer t(c, PT_T_MOMENTS);
+  const int64_t nn = c->nn;
+  PT_TRY_HIP(c, ensure(c->nd, PT_CD_NFIELDS * nn));
+  PT_TRY_HIP(c, ensure(c->ord_h, nn));
+  PT_TRY_HIP(c, ensure(c->ord_b, nn));
+  PT_TRY_HIP(c, ensure(c->tn, nn));
+  PT_TRY_HIP(c, ensure(c->topo, nn));
+  PT_TRY_HIP(c, ensure(c->gn, nn));
+  PT_TRY_HIP(c, ensure(c->gn32, nn));
+  PT_TRY_HIP(c, ensure(c->bk, std::max<int64_t>(c->nb, 1)));
+  PtTree tr = tree_of(c);
+  PT_TRY_LAUNCH(c, pt_launch_moments_init(c->nd.p, nn, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_leaf_moments(tr, c->bucket_node.p, c->nb, c->xs.p, c->ys.p, c->zs.p,
+                                          c->ms.p, c->es.p, c->nd.p, nn, c->stream));
+  // internal nodes, deepest level first
+  const int nlev = (int)c->level_off.size() - 1;
+  for (int d = nlev - 1; d >= 0; d--) {
+    const int64_t a = c->level_off[d], b = c->level_off[d + 1];
+    if (b > a) {
+      PT_TRY_LAUNCH(c, pt_launch_upward_level(tr, c->level_nodes.p + a, (int32_t)(b - a), c->nd.p, nn,
+                                              c->ord_h.p, c->ord_b.p, c->stream));
+    }
+  }
+  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tn.p, c->topo.p,
+                                  c->gn.p, c->gn32.p, c->bk.p, c->stream));
+  c->stage = S_MOMENTS;
+  return PT_OK;
+}
+
+pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counters* counters) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_MOMENTS, "pt_traverse"))) return st;
+  if (!(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
+  c->theta = theta;
+  for (int k = 0; k < 3; k++) c->offset[k] = offset ? offset[k] : 0.0;
+  // GravityVisitor(offset, theta) (examples/GravityVisitor.h:16-19)
+  const double opening_geometry_factor_squared = 4.0 / 3.0;
+  const double gf = opening_geometry_factor_squared / (theta * theta);
+  const int64_t nb = c->nb;
+  {
+    Timer t(c, PT_T_TRAVERSE);
+    PT_TRY_HIP(c, ensure(c->pp_cnt, std::max<int64_t>(nb, 1)));
+    PT_TRY_HIP(c, ensure(c->pc_cnt, std::max<int64_t>(nb, 1)));
+    PT_TRY_HIP(c, ensure(c->pp_off, nb + 1));
+    PT_TRY_HIP(c, ensure(c->pc_off, nb + 1));
+    if ((st = ensure_scratch(c, pt_scan_scratch_bytes(nb + 1)))) return st;
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->counters.p, 4, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_traverse(false, c->tn.p, c->topo.p, c->bk.p, nb, gf, c->offset,
+                                        c->pp_cnt.p, c->pc_cnt.p, nullptr, nullptr, nullptr,
+                                        nullptr, c->counters.p, c->derr.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pp_cnt.p, c->pp_off.p, nb, c->scratch.p,
+                                                  c->pp_off.p + nb, c->stream));
+    PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pc_cnt.p, c->pc_off.p, nb, c->scratch.p,
+                                                  c->pc_off.p + nb, c->stream));
+    int64_t tot[2] = {0, 0};
+    int32_t herr = 0;
+    PT_TRY_HIP(c, hipMemcpyAsync(&tot[0], c->pp_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(&tot[1], c->pc_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(&c->h_counters, c->counters.p, sizeof(pt_counters),
+                                 hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    if (herr) {
+      c->stage = S_MOMENTS;
+      return fail(c, PT_ERR_CAPACITY, "traversal stack overflow (PT_TRAV_STACK)");
+    }
+    c->n_pp = tot[0];
+    c->n_pc = tot[1];
+    PT_TRY_HIP(c, ensure(c->pp, std::max<int64_t>(c->n_pp, 1)));
+    PT_TRY_HIP(c, ensure(c->pc, std::max<int64_t>(c->n_pc, 1)));
+    PT_TRY_LAUNCH(c, pt_launch_traverse(true, c->tn.p, c->topo.p, c->bk.p, nb, gf, c->offset,
+                                        nullptr, nullptr, c->pp_off.p, c->pc_off.p, c->pp.p,
+                                        c->pc.p, nullptr, c->derr.p, c->stream));
+  }
+  if (counters) *counters = c->h_counters;
+  c->stage = S_LISTS;
+  return PT_OK;
+}
+
+pt_status pt_interact(pt_ctx* c, pt_precision precision) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_LISTS, "pt_interact"))) return st;
+  if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  {
+    Timer t(c, PT_T_FORCE);
+    PT_TRY_LAUNCH(c, pt_launch_force((int)precision, c->bk.p, c->nb, c->gn.p, c->gn32.p, c->topo.p,
+                                     c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p,
+                                     c->pp.p, c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
+                                     c->azs.p, c->pots.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
+                                      c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+  }
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->stage = S_FORCES;
+  c->have_forces = true;
+  return PT_OK;
+}
+
+pt_status pt_compute_forces(pt_ctx* c, double theta, const double offset[3],
+                            pt_precision precision, pt_counters* counters) {
+  pt_status st;
+  if ((st = pt_remake_universe(c, nullptr))) return st;
+  if ((st = pt_assign_keys(c))) return st;
+  if ((st = pt_sort(c))) return st;
+  if ((st = pt_build_tree(c))) return st;
+  if ((st = pt_upward(c))) return st;
+  if ((st = pt_traverse(c, theta, offset, counters))) return st;
+  return pt_interact(c, precision);
+}
+
+pt_status pt_kick_drift(pt_ctx* c, double dt) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_FORCES, "pt_kick_drift"))) return st;
+  {
+    Timer t(c, PT_T_INTEGRATE);
+    PT_TRY_LAUNCH(c, pt_launch_kick_drift(c->n, dt, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
+                                          c->vz.p, c->ax.p, c->ay.p, c->az.p, c->stream));
+  }
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  // positions moved: everything downstream must be rebuilt (Partition::rebuild)
+  c->stage = S_PARTICLES;
+  return PT_OK;
+}
+
+pt_status pt_run(pt_ctx* c, int steps, double dt, double theta, pt_precision precision,
+                 pt_counters* counters) {
+  pt_status st;
+  pt_counters total{};
+  for (int i = 0; i < steps; i++) {
+    pt_counters k{};
+    if ((st = pt_compute_forces(c, theta, nullptr, precision, &k))) return st;
+    total.n_node_ints += k.n_node_ints;
+    total.n_part_ints += k.n_part_ints;
+    total.n_opens += k.n_opens;
+    total.n_closes += k.n_closes;
+    if ((st = pt_kick_drift(c, dt))) return st;
+  }
+  if (counters) *counters = total;
+  return PT_OK;
+}
+
+pt_status pt_direct_sum(pt_ctx* c, const int64_t* targets, int64_t nt, double* acc_out) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_PARTICLES, "pt_direct_sum"))) return st;
+  if (nt < 0 || (nt > 0 && (!targets || !acc_out))) return fail(c, PT_ERR_ARG, "bad targets");
+  for (int64_t i = 0; i < nt; i++)
+    if (targets[i] < 0 || targets[i] >= c->n) return fail(c, PT_ERR_ARG, "target index out of range");
+  if (nt == 0) return PT_OK;
+  int64_t* dt = nullptr;
+  double* dout = nullptr;
+  PT_TRY_HIP(c, hipMalloc((void**)&dt, nt * sizeof(int64_t)));
+  hipError_t e = hipMalloc((void**)&dout, nt * 3 * sizeof(double));
+  if (e != hipSuccess) {
+    (void)hipFree(dt);
+    return hip_fail(c, "hipMalloc", e);
+  }
+  e = hipMemcpyAsync(dt, targets, nt * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    if (pt_launch_direct(dt, nt, c->x.p, c->y.p, c->z.p, c->m.p, c->e.p, c->n, dout, c->stream) != 0)
+      e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(acc_out, dout, nt * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dt);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return hip_fail(c, "pt_direct_sum", e);
+  return PT_OK;
+}
+
+// ---------------------------------------------------------------- read-back
+
+static pt_status d2h(pt_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!dst || bytes == 0) return PT_OK;
+  PT_TRY_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+pt_status pt_get_universe(pt_ctx* c, double box[6]) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_UNIVERSE, "pt_get_universe"))) return st;
+  return d2h(c, box, c->box.p, 6 * sizeof(double));
+}
+
+pt_status pt_get_keys(pt_ctx* c, uint64_t* keys) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_KEYS, "pt_get_keys"))) return st;
+  return d2h(c, keys, c->keys.p, (size_t)c->n * 8);
+}
+
+pt_status pt_get_sorted(pt_ctx* c, uint64_t* sorted_keys, uint32_t* perm) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_SORTED, "pt_get_sorted"))) return st;
+  if ((st = d2h(c, sorted_keys, c->skeys.p, (size_t)c->n * 8))) return st;
+  return d2h(c, perm, c->perm.p, (size_t)c->n * 4);
+}
+
+pt_status pt_get_tree_size(pt_ctx* c, int64_t* n_nodes, int64_t* n_buckets, int32_t* max_depth) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_TREE, "pt_get_tree_size"))) return st;
+  if (n_nodes) *n_nodes = c->nn;
+  if (n_buckets) *n_buckets = c->nb;
+  if (max_depth) *max_depth = c->max_depth;
+  return PT_OK;
+}
+
+pt_status pt_get_nodes(pt_ctx* c, uint64_t* key, int32_t* type, int32_t* depth, int32_t* begin,
+                       int32_t* count, int32_t* parent, int32_t* first_child) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_TREE, "pt_get_nodes"))) return st;
+  const size_t nn = (size_t)c->nn;
+  if ((st = d2h(c, key, c->nkey.p, nn * 8))) return st;
+  if ((st = d2h(c, type, c->ntype.p, nn * 4))) return st;
+  if ((st = d2h(c, depth, c->ndepth.p, nn * 4))) return st;
+  if ((st = d2h(c, begin, c->nbegin.p, nn * 4))) return st;
+  if ((st = d2h(c, count, c->ncount.p, nn * 4))) return st;
+  if ((st = d2h(c, parent, c->nparent.p, nn * 4))) return st;
+  return d2h(c, first_child, c->nfirst.p, nn * 4);
+}
+
+pt_status pt_get_node_data(pt_ctx* c, double* out) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_MOMENTS, "pt_get_node_data"))) return st;
+  const int64_t nn = c->nn;
+  std::vector<double> soa((size_t)nn * PT_CD_NFIELDS);
+  if ((st = d2h(c, soa.data(), c->nd.p, soa.size() * sizeof(double)))) return st;
+  for (int64_t i = 0; i < nn; i++)
+    for (int f = 0; f < PT_CD_NFIELDS; f++) out[i * PT_CD_NFIELDS + f] = soa[(size_t)f * nn + i];
+  return PT_OK;
+}
+
+pt_status pt_get_buckets(pt_ctx* c, int32_t* bucket_nodes) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_TREE, "pt_get_buckets"))) return st;
+  return d2h(c, bucket_nodes, c->bucket_node.p, (size_t)c->nb * 4);
+}
+
+pt_status pt_get_lists_size(pt_ctx* c, int64_t* n_pp, int64_t* n_pc) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_LISTS, "pt_get_lists_size"))) return st;
+  if (n_pp) *n_pp = c->n_pp;
+  if (n_pc) *n_pc = c->n_pc;
+  return PT_OK;
+}
+
+pt_status pt_get_lists(pt_ctx* c, int64_t* pp_off, int32_t* pp, int64_t* pc_off, int32_t* pc) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_LISTS, "pt_get_lists"))) return st;
+  const int64_t nb = c->nb;
+  std::vector<uint64_t> keys((size_t)c->nn);
+  if ((st = d2h(c, keys.data(), c->nkey.p, keys.size() * 8))) return st;
+  auto left_aligned = [&](int32_t id) {
+    const uint64_t k = keys[(size_t)id];
+    const int depth = (63 - __builtin_clzll(k)) / 3;
+    return k << (64 - (3 * depth + 1));
+  };
+  struct L {
+    int64_t* off;
+    int32_t* ids;
+    const DBuf<int64_t>* doff;
+    const DBuf<int32_t>* dids;
+    int64_t n;
+  } lists[2] = {{pp_off, pp, &c->pp_off, &c->pp, c->n_pp}, {pc_off, pc, &c->pc_off, &c->pc, c->n_pc}};
+  for (auto& l : lists) {
+    if (!l.off || !l.ids) continue;
+    if ((st = d2h(c, l.off, l.doff->p, (size_t)(nb + 1) * 8))) return st;
+    if ((st = d2h(c, l.ids, l.dids->p, (size_t)l.n * 4))) return st;
+    // canonical order inside each bucket: DFS order = ascending left-aligned key
+    for (int64_t b = 0; b < nb; b++) {
+      std::sort(l.ids + l.off[b], l.ids + l.off[b + 1],
+                [&](int32_t a, int32_t q) { return left_aligned(a) < left_aligned(q); });
+    }
+  }
+  return PT_OK;
+}
+
+pt_status pt_get_accelerations(pt_ctx* c, double* acc, double* pot) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if (!c->have_forces) return fail(c, PT_ERR_STATE, "pt_get_accelerations before any force evaluation");
+  const size_t n = (size_t)c->n;
+  std::vector<double> a(n), b(n), d(n);
+  if ((st = d2h(c, a.data(), c->ax.p, n * 8))) return st;
+  if ((st = d2h(c, b.data(), c->ay.p, n * 8))) return st;
+  if ((st = d2h(c, d.data(), c->az.p, n * 8))) return st;
+  if (acc) {
+    for (size_t i = 0; i < n; i++) {
+      acc[3 * i] = a[i];
+      acc[3 * i + 1] = b[i];
+      acc[3 * i + 2] = d[i];
+    }
+  }
+  return d2h(c, pot, c->pot.p, n * 8);
+}
+
+pt_status pt_get_state(pt_ctx* c, double* pos, double* vel) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_PARTICLES, "pt_get_state"))) return st;
+  const size_t n = (size_t)c->n;
+  std::vector<double> t(n);
+  DBuf<double>* src[2][3] = {{&c->x, &c->y, &c->z}, {&c->vx, &c->vy, &c->vz}};
+  double* dst[2] = {pos, vel};
+  for (int a = 0; a < 2; a++) {
+    if (!dst[a]) continue;
+    for (int k = 0; k < 3; k++) {
+      if ((st = d2h(c, t.data(), src[a][k]->p, n * 8))) return st;
+      for (size_t i = 0; i < n; i++) dst[a][3 * i + k] = t[i];
+    }
+  }
+  return PT_OK;
+}
+
+pt_status pt_get_counters(pt_ctx* c, pt_counters* out) {
+  if (!c || !out) return PT_ERR_ARG;
+  *out = c->h_counters;
+  return PT_OK;
+}
+
+pt_status pt_set_timing(pt_ctx* c, int enabled) {
+  if (!c) return PT_ERR_ARG;
+  c->timing = enabled != 0;
+  for (int i = 0; i < PT_T_COUNT; i++) c->ev_used[i] = false;
+  return PT_OK;
+}
+
+pt_status pt_get_timings(pt_ctx* c, float ms[PT_T_COUNT]) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < PT_T_COUNT; i++) {
+    ms[i] = 0.f;
+    if (c->ev_used[i]) (void)hipEventElapsedTime(&ms[i], c->ev[i][0], c->ev[i][1]);
+  }
+  return PT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Shared device definitions for the gfx950 gravity path (keys, sort, octree,
+// moments, traversal, interaction kernels).  Internal header: the public C ABI
+// is include/pt_gravity.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PT_WAVE 64
+#define PT_KEY_BITS 64
+#define PT_MAX_DEPTH 21       // 63 Morton bits / 3 bits per octree level
+#define PT_MIN_PARTICLE_NODE 6  // GravityVisitor::nMinParticleNode
+
+// Node types, same numbering as the oracle (EmptyLeaf, Leaf, Internal).
+enum { PT_NODE_EMPTY = 0, PT_NODE_LEAF = 1, PT_NODE_INTERNAL = 2 };
+
+// Per-node moment record fields (CentroidData flattened), SoA index order used
+// by pt_get_node_data: radius soft mass cmx cmy cmz xx xy xz yy yz zz lox loy
+// loz hix hiy hiz.
+#define PT_CD_RADIUS 0
+#define PT_CD_SOFT 1
+#define PT_CD_MASS 2
+#define PT_CD_CM 3
+#define PT_CD_Q 6
+#define PT_CD_LO 12
+#define PT_CD_HI 15
+#define PT_CD_NFIELDS 18
+
+// Packed record read by the traversal: centre of mass and radius.
+struct __align__(32) TravNode {
+  double cx, cy, cz, radius;
+};
+// Packed topology record read by the traversal.
+struct __align__(16) TopoNode {
+  int32_t count;
+  int32_t first_child;  // -1 for leaves
+  int32_t type;
+  int32_t begin;
+};
+// Packed source record for particle-cell interactions (FP64 path).
+struct __align__(32) GravNode {
+  double cx, cy, cz, mass;
+  double soft, qxx, qxy, qxz;
+  double qyy, qyz, qzz, pad;
+};
+// Packed source record for particle-cell interactions (FP32 path): the centre
+// stays FP64 (it is shifted per bucket), the rest is FP32.
+struct __align__(16) GravNode32 {
+  double cx, cy, cz;
+  float mass, soft;
+  float qxx, qxy, qxz, qyy;
+  float qyz, qzz, pad0, pad1;
+};
+// Bucket (target leaf) record: particle range, box, centre of mass, softening.
+struct __align__(32) Bucket {
+  double lox, loy, loz, hix;
+  double hiy, hiz, cx, cy;
+  double cz, soft, pad0, pad1;
+  int32_t node, begin, count, pad2;
+};
+
+// Interaction counters in the order {n_node_ints, n_part_ints, n_opens,
+// n_closes} (src/ThreadStateHolder.h:10-13).
+struct PtCounters {
+  unsigned long long v[4];
+};
+
+__device__ __forceinline__ int pt_lane() { return __lane_id(); }
+
+// Wave-wide inclusive prefix sum of a 32-bit value.
+__device__ __forceinline__ int pt_wave_incl_scan(int v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < PT_WAVE; d <<= 1) {
+    const int o = __shfl_up(v, d, PT_WAVE);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double pt_wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, PT_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long pt_ballot(bool p) {
+  return __ballot(p);
+}
+
+// Number of set lanes below the calling lane in mask.
+__device__ __forceinline__ int pt_rank_in(unsigned long long mask) {
+  return __popcll(mask & ((1ull << __lane_id()) - 1ull));
+}
+
+// Host-side error check helper used inside the C ABI implementation.
+#define PT_HIP_OK(expr) ((expr) == hipSuccess)

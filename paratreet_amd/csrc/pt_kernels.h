@@ -1,0 +1,87 @@
+// Host-side launch wrappers of the gfx950 kernels (internal to
+// libpt_gravity.so).  Every wrapper enqueues on the given stream and returns 0
+// or -1 (launch error).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_common.h"
+
+// Stack capacity (opened nodes pending expansion) per wavefront in the
+// traversal; exceeding it is reported as an error, never silently truncated.
+#define PT_TRAV_STACK 2048
+
+// Device node table (SoA, level order).
+struct PtTree {
+  uint64_t* key;
+  int32_t* begin;
+  int32_t* count;
+  int32_t* depth;
+  int32_t* type;
+  int32_t* parent;
+  int32_t* first_child;
+};
+
+int64_t pt_scan_scratch_bytes(int64_t n);
+int pt_exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* scratch,
+                          uint32_t* total, hipStream_t s);
+int pt_exclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, void* scratch,
+                                 int64_t* total, hipStream_t s);
+
+int pt_launch_universe(const double* x, const double* y, const double* z, int64_t n,
+                       double* scratch_parts, int max_parts, double* bbox_out, double* box,
+                       hipStream_t s);
+int pt_launch_keys(const double* x, const double* y, const double* z, int64_t n,
+                   const double* box, uint64_t* keys, uint32_t* idx, hipStream_t s);
+
+int64_t pt_sort_scratch_bytes(int64_t n);
+int pt_radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int end_bit,
+                  void* scratch, hipStream_t s);
+int pt_launch_iota(uint32_t* v, int64_t n, hipStream_t s);
+
+int pt_launch_root(int64_t n, int bucket, const PtTree& t, hipStream_t s);
+int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, int32_t base,
+                    int bucket, int child_depth, const PtTree& t, uint32_t* flag, hipStream_t s);
+int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
+                      int32_t* next, hipStream_t s);
+int pt_launch_buckets(const PtTree& t, int64_t nn, int64_t n, uint32_t* head, uint32_t* head_pos,
+                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node, hipStream_t s);
+
+int pt_launch_moments_init(double* nd, int64_t nn, hipStream_t s);
+int pt_launch_leaf_moments(const PtTree& t, const int32_t* bucket_node, int64_t nb,
+                           const double* x, const double* y, const double* z, const double* m,
+                           const double* e, double* nd, int64_t nn, hipStream_t s);
+int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
+                           int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
+int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
+                   int64_t nb, TravNode* tn, TopoNode* topo, GravNode* gn, GravNode32* gn32,
+                   Bucket* bk, hipStream_t s);
+
+int pt_launch_traverse(bool fill, const TravNode* tn, const TopoNode* topo, const Bucket* bk,
+                       int64_t nb, double gf, const double off[3], int32_t* pp_cnt,
+                       int32_t* pc_cnt, const int64_t* pp_off, const int64_t* pc_off,
+                       int32_t* pp, int32_t* pc, unsigned long long* counters, int32_t* err,
+                       hipStream_t s);
+
+int pt_launch_force(int precision, const Bucket* bk, int64_t nb, const GravNode* gn,
+                    const GravNode32* gn32, const TopoNode* topo, const double* x,
+                    const double* y, const double* z, const double* m, const double* e,
+                    const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
+                    const int32_t* pc, const double off[3], double* ax, double* ay, double* az,
+                    double* pot, hipStream_t s);
+
+int pt_launch_gather(const uint32_t* perm, int64_t n, const double* x, const double* y,
+                     const double* z, const double* m, const double* e, double* xs, double* ys,
+                     double* zs, double* ms, double* es, hipStream_t s);
+int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,
+                     const double* a2, const double* a3, double* o0, double* o1, double* o2,
+                     double* o3, hipStream_t s);
+int pt_launch_kick_drift(int64_t n, double dt, double* x, double* y, double* z, double* vx,
+                         double* vy, double* vz, const double* ax, const double* ay,
+                         const double* az, hipStream_t s);
+int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const double* y,
+                     const double* z, const double* m, const double* e, int64_t n, double* out,
+                     hipStream_t s);
+int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);

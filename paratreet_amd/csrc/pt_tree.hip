@@ -1,0 +1,139 @@
+// Level-parallel octree build over sorted Morton keys.
+//
+// Reference: Subtree::buildTree / recursiveBuild / getType (src/Subtree.h:
+// 262-349), OctTree::findChildsLastParticle (src/Modularization.C:60-65) and
+// the key helpers of src/Utility.h:9-130.  The reference recurses depth-first;
+// here every internal node of one level is split by one thread per
+// (node, child): child i of a node covers [first key >= left-aligned(child
+// key), first key >= left-aligned(child key + 1)), the last child taking the
+// remainder, which is exactly the range recursiveBuild assigns.  The 8
+// children of a node get consecutive ids, so a node stores only its first
+// child.  Node ids are in level order; node identity across implementations
+// is the key.
+#include "pt_common.h"
+#include "pt_kernels.h"
+
+namespace {
+
+// Utility::mssb64_pos (src/Utility.h:9-21) as a count-leading-zeros.
+__device__ __forceinline__ int mssb64_pos(uint64_t x) { return x ? 63 - __clzll(x) : -1; }
+
+// Utility::removeLeadingZeros (src/Utility.h:127-130) for branch factor 8.
+__device__ __forceinline__ uint64_t left_align(uint64_t k) {
+  const int depth = mssb64_pos(k) / 3;
+  return k << (PT_KEY_BITS - (3 * depth + 1));
+}
+
+// Utility::binarySearchComp with a >= comparison (src/Utility.h:34-50).
+__device__ __forceinline__ int64_t first_ge(const uint64_t* __restrict__ k, int64_t lo,
+                                            int64_t hi, uint64_t s) {
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (k[mid] >= s) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Subtree::getType (src/Subtree.h:262-267).  At the last key level no split
+// is possible, so an over-full node there stays a leaf.
+__device__ __forceinline__ int node_type(int64_t n, int bucket, int depth) {
+  if (n == 0) return PT_NODE_EMPTY;
+  if (n <= bucket) return PT_NODE_LEAF;
+  if (depth >= PT_MAX_DEPTH) return PT_NODE_LEAF;
+  return PT_NODE_INTERNAL;
+}
+
+__global__ void root_kernel(int64_t n, int bucket, PtTree t) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  t.key[0] = 1;
+  t.begin[0] = 0;
+  t.count[0] = (int32_t)n;
+  t.depth[0] = 0;
+  t.type[0] = node_type(n, bucket, 0);
+  t.parent[0] = -1;
+  t.first_child[0] = -1;
+}
+
+__global__ void __launch_bounds__(256)
+split_kernel(const uint64_t* __restrict__ keys, const int32_t* __restrict__ frontier,
+             int32_t nf, int32_t base, int bucket, int child_depth, PtTree t,
+             uint32_t* __restrict__ internal_flag) {
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= (int64_t)nf * 8) return;
+  const int f = (int)(gid >> 3);
+  const int c = (int)(gid & 7);
+  const int32_t p = frontier[f];
+  const int64_t b = t.begin[p];
+  const int64_t e = b + t.count[p];
+  const uint64_t child_key = (t.key[p] << 3) | (uint64_t)c;
+  const int64_t lo = (c == 0) ? b : first_ge(keys, b, e, left_align(child_key));
+  const int64_t hi = (c == 7) ? e : first_ge(keys, b, e, left_align(child_key + 1));
+  const int32_t id = base + (int32_t)gid;
+  const int ty = node_type(hi - lo, bucket, child_depth);
+  t.key[id] = child_key;
+  t.begin[id] = (int32_t)lo;
+  t.count[id] = (int32_t)(hi - lo);
+  t.depth[id] = child_depth;
+  t.type[id] = ty;
+  t.parent[id] = p;
+  t.first_child[id] = -1;
+  if (c == 0) t.first_child[p] = base + 8 * f;
+  internal_flag[gid] = (ty == PT_NODE_INTERNAL) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t m,
+               int32_t base, int32_t* __restrict__ next) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < m && flag[i]) next[pos[i]] = base + (int32_t)i;
+}
+
+__global__ void __launch_bounds__(256)
+leaf_heads_kernel(const PtTree t, int64_t nn, uint32_t* __restrict__ head) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < nn && t.type[i] == PT_NODE_LEAF) head[t.begin[i]] = 1u;
+}
+
+__global__ void __launch_bounds__(256)
+bucket_ids_kernel(const PtTree t, int64_t nn, const uint32_t* __restrict__ head_pos,
+                  int32_t* __restrict__ bucket_node) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < nn && t.type[i] == PT_NODE_LEAF) bucket_node[head_pos[t.begin[i]]] = (int32_t)i;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int pt_launch_root(int64_t n, int bucket, const PtTree& t, hipStream_t s) {
+  root_kernel<<<1, 64, 0, s>>>(n, bucket, t);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, int32_t base,
+                    int bucket, int child_depth, const PtTree& t, uint32_t* flag,
+                    hipStream_t s) {
+  if (nf <= 0) return 0;
+  split_kernel<<<grid_for((int64_t)nf * 8), 256, 0, s>>>(keys, frontier, nf, base, bucket,
+                                                          child_depth, t, flag);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
+                      int32_t* next, hipStream_t s) {
+  if (m <= 0) return 0;
+  compact_kernel<<<grid_for(m), 256, 0, s>>>(flag, pos, m, base, next);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_buckets(const PtTree& t, int64_t nn, int64_t n, uint32_t* head, uint32_t* head_pos,
+                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node,
+                      hipStream_t s) {
+  if (n <= 0) return 0;
+  if (pt_launch_fill_u32(head, n, 0u, s) != 0) return -1;
+  leaf_heads_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, head);
+  if (pt_exclusive_scan_u32(head, head_pos, n, scan_scratch, nb_out, s) != 0) return -1;
+  bucket_ids_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, head_pos, bucket_node);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
