@@ -1,0 +1,176 @@
+"""Benchmark: interactions/sec of one gravity force evaluation on one MI355X.
+
+Workload (default): 1M equal-mass Plummer sphere (G=M=a=1, r <= 10a, seeded),
+bucket 16, theta 0.7, softening 0.005, monopole+quadrupole.  One step = one full
+force evaluation from positions already resident in HBM: universe box, Morton
+keys, radix sort, octree build, upward pass, traversal lists, P-C + P-P
+interactions (FP32 bucket-shifted Kahan form = the headline; the plain FP64
+form is timed the same way for the ratio).
+
+value = (n_part_ints + n_node_ints) / seconds per step (counter definitions of
+src/Traverser.h:15-37).  The roofline object covers the interaction kernel:
+algorithmic FLOPs per launch (20 per P-P pair, 67 per P-C particle-node
+interaction) over its average duration, from HIP events recorded on the
+stream the kernel runs on.  Errors are measured against an FP64 direct sum on
+4096 seeded random targets.  cpu_baseline times the oracle (CPU restatement,
+OpenMP over buckets) on the host cores over the same 1M-particle tree.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # name: (n, bucket, theta, eps, generator)
+    "plummer1m": (1_000_000, 16, 0.7, 0.005, "plummer"),
+    "plummer100k": (100_000, 12, 0.7, 0.01, "plummer"),
+    "halo16m": (16_000_000, 32, 0.7, 0.01, "halo"),
+}
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+FLOPS_PP = 20.0
+FLOPS_PC = 67.0
+
+
+def make_input(name):
+    from paratreet_amd import ics
+    n, bucket, theta, eps, gen = CONFIGS[name]
+    if gen == "plummer":
+        pos, vel, m = ics.plummer(n, seed=2024)
+    else:
+        pos, vel, m = ics.halo_mixture(n, n_halos=64, seed=4242)
+    return pos, vel, m, np.full(n, eps), bucket, theta
+
+
+def time_path(d, theta, precision, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        d.compute_forces(theta=theta, precision=precision)
+    d.set_timing(True)
+    force_ms = []
+    stage = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cnt = None
+    for _ in range(steps):
+        cnt = d.compute_forces(theta=theta, precision=precision)
+        tm = d.timings()
+        force_ms.append(tm["force"])
+        stage = tm
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    d.set_timing(False)
+    ms_step = (t1 - t0) * 1000.0 / steps
+    return ms_step, cnt, float(np.mean(force_ms)), stage
+
+
+def cpu_baseline(pos, m, eps, bucket, theta):
+    """Oracle (port) on the host cores: full force evaluation of the same tree."""
+    from oracle import oracle as O
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    P = O.Pipeline(pos, m, eps, bucket=bucket, theta=theta)
+    t0 = time.perf_counter()
+    _, _, cnt = P.gravity(nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    ints = float(cnt[0] + cnt[1])
+    return {"value": ints / dt, "unit": "interactions/s", "cores": nthreads, "kind": "port",
+            "sample": "full force evaluation (all %d buckets, %.3g interactions) of the same "
+                      "tree, OpenMP over buckets, %.1f s" % (len(P.bucket_nodes), ints, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="plummer1m", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-error", action="store_true", help="skip the direct-sum error leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design, see the documentation)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from paratreet_amd.gravity import Configuration, Driver, PT_FP32C, PT_FP64
+
+    pos, vel, m, eps, bucket, theta = make_input(args.config)
+    n = len(pos)
+    d = Driver(Configuration(max_particles_per_leaf=bucket))
+    d.load(pos, m, eps, vel)
+
+    ms32, cnt32, f32_ms, st32 = time_path(d, theta, PT_FP32C, args.steps, args.warmup)
+    acc32, _ = d.accelerations()
+    ms64, cnt64, f64_ms, st64 = time_path(d, theta, PT_FP64, args.steps, args.warmup)
+    acc64, _ = d.accelerations()
+    ints = float(cnt32[0] + cnt32[1])
+    value = ints / (ms32 / 1000.0)
+    value64 = ints / (ms64 / 1000.0)
+    flops = FLOPS_PP * cnt32[1] + FLOPS_PC * cnt32[0]
+    achieved32 = flops / (f32_ms / 1000.0) / 1e12
+    achieved64 = flops / (f64_ms / 1000.0) / 1e12
+
+    err = {}
+    if not args.no_error:
+        rng = np.random.default_rng(99)
+        tgt = np.sort(rng.choice(n, 4096, replace=False))
+        ref = d.direct_sum(tgt)
+        nr = np.linalg.norm(ref, axis=1)
+        for name, a in (("fp32c", acc32), ("fp64", acc64)):
+            e = np.linalg.norm(a[tgt] - ref, axis=1) / nr
+            err[name + "_vs_direct_rms"] = float(np.sqrt(np.mean(e * e)))
+            err[name + "_vs_direct_p99"] = float(np.percentile(e, 99))
+        e = np.linalg.norm(acc32 - acc64, axis=1) / np.linalg.norm(acc64, axis=1)
+        err["fp32c_vs_fp64_rms"] = float(np.sqrt(np.mean(e * e)))
+        err["fp32c_vs_fp64_p99"] = float(np.percentile(e, 99))
+    d.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pos, m, eps, bucket, theta)
+
+    out = {
+        "metric": "interactions/sec, 1M Plummer θ=0.7, FP32-comp vs FP64 GPU; RMS force err",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 interactions (Kahan-compensated), f64 keys/tree/moments",
+        "data": "synthetic (seeded Plummer generator)",
+        "config": {"workload": "%s: n=%d, bucket=%d, theta=%.2f, eps=%g, monopole+quadrupole, "
+                               "full force evaluation per step" % (args.config, n, bucket, theta,
+                                                                   eps[0]),
+                   "n": n, "bucket": bucket, "theta": theta, "parallelism": "single GPU"},
+        "interactions_per_step": ints,
+        "n_part_ints": int(cnt32[1]), "n_node_ints": int(cnt32[0]),
+        "fp64_value": value64, "fp64_ms_per_step": ms64,
+        "fp32c_over_fp64": value / value64,
+        "force_kernel_ms": {"fp32c": f32_ms, "fp64": f64_ms},
+        "force_only_ips": {"fp32c": ints / (f32_ms / 1000.0), "fp64": ints / (f64_ms / 1000.0)},
+        "stage_ms_fp32c": st32, "stage_ms_fp64": st64,
+        "roofline": {"bound": "valu", "achieved": achieved32, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "force32c_kernel",
+                     "fp64_kernel": {"achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
+                                     "frac": achieved64 / PEAK_FP64_TFLOPS}},
+        "errors": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

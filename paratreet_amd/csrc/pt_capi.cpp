@@ -68,11 +68,11 @@ struct pt_ctx {
   DBuf<double> nd;
   DBuf<int32_t> ord_h;
   DBuf<uint64_t> ord_b;
-  DBuf<TravNode> tn;
-  DBuf<TopoNode> topo;
+  DBuf<TreeNode> tnode;
   DBuf<GravNode> gn;
-  DBuf<GravNode32> gn32;
   DBuf<Bucket> bk;
+  DBuf<unsigned long long> cpart;  // per-block counter partials
+  bool lists_valid = false;
   // lists
   DBuf<int32_t> pp_cnt, pc_cnt, pp, pc;
   DBuf<int64_t> pp_off, pc_off, i64_total;
@@ -289,7 +289,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->nparent); release(c->nfirst); release(c->level_nodes);
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
-  release(c->tn); release(c->topo); release(c->gn); release(c->gn32); release(c->bk);
+  release(c->tnode); release(c->gn); release(c->bk); release(c->cpart);
   release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
@@ -489,10 +489,8 @@ pt_status pt_upward(pt_ctx* c) {
   PT_TRY_HIP(c, ensure(c->nd, PT_CD_NFIELDS * nn));
   PT_TRY_HIP(c, ensure(c->ord_h, nn));
   PT_TRY_HIP(c, ensure(c->ord_b, nn));
-  PT_TRY_HIP(c, ensure(c->tn, nn));
-  PT_TRY_HIP(c, ensure(c->topo, nn));
+  PT_TRY_HIP(c, ensure(c->tnode, nn));
   PT_TRY_HIP(c, ensure(c->gn, nn));
-  PT_TRY_HIP(c, ensure(c->gn32, nn));
   PT_TRY_HIP(c, ensure(c->bk, std::max<int64_t>(c->nb, 1)));
   PtTree tr = tree_of(c);
   PT_TRY_LAUNCH(c, pt_launch_moments_init(c->nd.p, nn, c->stream));
@@ -507,9 +505,30 @@ pt_status pt_upward(pt_ctx* c) {
                                               c->ord_h.p, c->ord_b.p, c->stream));
     }
   }
-  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tn.p, c->topo.p,
-                                  c->gn.p, c->gn32.p, c->bk.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tnode.p, c->gn.p,
+                                  c->bk.p, c->stream));
   c->stage = S_MOMENTS;
+  c->lists_valid = false;
+  return PT_OK;
+}
+
+static pt_status set_visitor(pt_ctx* c, double theta, const double offset[3], double* gf) {
+  if (!(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
+  c->theta = theta;
+  for (int k = 0; k < 3; k++) c->offset[k] = offset ? offset[k] : 0.0;
+  // GravityVisitor(offset, theta) (examples/GravityVisitor.h:16-19)
+  const double opening_geometry_factor_squared = 4.0 / 3.0;
+  *gf = opening_geometry_factor_squared / (theta * theta);
+  return PT_OK;
+}
+
+// Reads back the reduced counters and the overflow flag (synchronises).
+static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr) {
+  PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblocks, c->counters.p, c->stream));
+  PT_TRY_HIP(c, hipMemcpyAsync(herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipMemcpyAsync(&c->h_counters, c->counters.p, sizeof(pt_counters),
+                               hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   return PT_OK;
 }
 
@@ -517,69 +536,99 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
   pt_status st = check_ctx(c);
   if (st) return st;
   if ((st = need(c, S_MOMENTS, "pt_traverse"))) return st;
-  if (!(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
-  c->theta = theta;
-  for (int k = 0; k < 3; k++) c->offset[k] = offset ? offset[k] : 0.0;
-  // GravityVisitor(offset, theta) (examples/GravityVisitor.h:16-19)
-  const double opening_geometry_factor_squared = 4.0 / 3.0;
-  const double gf = opening_geometry_factor_squared / (theta * theta);
+  double gf;
+  if ((st = set_visitor(c, theta, offset, &gf))) return st;
   const int64_t nb = c->nb;
   {
     Timer t(c, PT_T_TRAVERSE);
+    const int64_t nblk = pt_traverse_blocks(nb);
     PT_TRY_HIP(c, ensure(c->pp_cnt, std::max<int64_t>(nb, 1)));
     PT_TRY_HIP(c, ensure(c->pc_cnt, std::max<int64_t>(nb, 1)));
     PT_TRY_HIP(c, ensure(c->pp_off, nb + 1));
     PT_TRY_HIP(c, ensure(c->pc_off, nb + 1));
+    PT_TRY_HIP(c, ensure(c->cpart, std::max<int64_t>(nblk, 1) * 4));
     if ((st = ensure_scratch(c, pt_scan_scratch_bytes(nb + 1)))) return st;
-    PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->counters.p, 4, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, std::max<int64_t>(nblk, 1) * 4, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
-    PT_TRY_LAUNCH(c, pt_launch_traverse(false, c->tn.p, c->topo.p, c->bk.p, nb, gf, c->offset,
-                                        c->pp_cnt.p, c->pc_cnt.p, nullptr, nullptr, nullptr,
-                                        nullptr, c->counters.p, c->derr.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_traverse(false, c->tnode.p, c->bk.p, nb, gf, c->offset, c->pp_cnt.p,
+                                        c->pc_cnt.p, nullptr, nullptr, nullptr, nullptr,
+                                        c->cpart.p, c->derr.p, c->stream));
     PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pp_cnt.p, c->pp_off.p, nb, c->scratch.p,
                                                   c->pp_off.p + nb, c->stream));
     PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pc_cnt.p, c->pc_off.p, nb, c->scratch.p,
                                                   c->pc_off.p + nb, c->stream));
     int64_t tot[2] = {0, 0};
-    int32_t herr = 0;
     PT_TRY_HIP(c, hipMemcpyAsync(&tot[0], c->pp_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
     PT_TRY_HIP(c, hipMemcpyAsync(&tot[1], c->pc_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-    PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
-    PT_TRY_HIP(c, hipMemcpyAsync(&c->h_counters, c->counters.p, sizeof(pt_counters),
-                                 hipMemcpyDeviceToHost, c->stream));
-    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    if (herr) {
-      c->stage = S_MOMENTS;
-      return fail(c, PT_ERR_CAPACITY, "traversal stack overflow (PT_TRAV_STACK)");
-    }
+    int32_t herr = 0;
+    if ((st = finish_counters(c, std::max<int64_t>(nblk, 1), &herr))) return st;
+    if (herr) return fail(c, PT_ERR_CAPACITY, "traversal stack overflow (PT_WALK_STACK)");
     c->n_pp = tot[0];
     c->n_pc = tot[1];
     PT_TRY_HIP(c, ensure(c->pp, std::max<int64_t>(c->n_pp, 1)));
     PT_TRY_HIP(c, ensure(c->pc, std::max<int64_t>(c->n_pc, 1)));
-    PT_TRY_LAUNCH(c, pt_launch_traverse(true, c->tn.p, c->topo.p, c->bk.p, nb, gf, c->offset,
-                                        nullptr, nullptr, c->pp_off.p, c->pc_off.p, c->pp.p,
-                                        c->pc.p, nullptr, c->derr.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_traverse(true, c->tnode.p, c->bk.p, nb, gf, c->offset, nullptr,
+                                        nullptr, c->pp_off.p, c->pc_off.p, c->pp.p, c->pc.p,
+                                        c->cpart.p, c->derr.p, c->stream));
   }
   if (counters) *counters = c->h_counters;
-  c->stage = S_LISTS;
+  c->lists_valid = true;
   return PT_OK;
 }
 
 pt_status pt_interact(pt_ctx* c, pt_precision precision) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  if ((st = need(c, S_LISTS, "pt_interact"))) return st;
+  if (!c->lists_valid) return fail(c, PT_ERR_STATE, "pt_interact called before pt_traverse");
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
   {
     Timer t(c, PT_T_FORCE);
-    PT_TRY_LAUNCH(c, pt_launch_force((int)precision, c->bk.p, c->nb, c->gn.p, c->gn32.p, c->topo.p,
-                                     c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p,
-                                     c->pp.p, c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
-                                     c->azs.p, c->pots.p, c->stream));
-    PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                      c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p, c->xs.p,
+                                        c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
+                                        c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
+                                        c->azs.p, c->pots.p, c->stream));
   }
+  PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
+                                    c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->stage = S_FORCES;
+  c->have_forces = true;
+  return PT_OK;
+}
+
+// Fused walk + interactions (no lists), with a large-stack rerun if the
+// 512-entry walk stack overflowed for some bucket.
+static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
+                               pt_precision precision, pt_counters* counters) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_MOMENTS, "pt_compute_forces"))) return st;
+  if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  double gf;
+  if ((st = set_visitor(c, theta, offset, &gf))) return st;
+  const int64_t nb = c->nb;
+  const int64_t nblk = std::max<int64_t>(pt_gravity_blocks(nb), 1);
+  PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
+  for (int attempt = 0; attempt < 2; attempt++) {
+    PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
+    {
+      Timer t(c, PT_T_FORCE);
+      PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
+                                         c->gn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, gf,
+                                         c->offset, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
+                                         c->cpart.p, c->derr.p, c->stream));
+    }
+    int32_t herr = 0;
+    if ((st = finish_counters(c, nblk, &herr))) return st;
+    if (!herr) break;
+    if (attempt == 1) return fail(c, PT_ERR_CAPACITY, "walk stack overflow at 1536 entries");
+  }
+  PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
+                                    c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  if (counters) *counters = c->h_counters;
+  c->lists_valid = false;
   c->stage = S_FORCES;
   c->have_forces = true;
   return PT_OK;
@@ -593,14 +642,13 @@ pt_status pt_compute_forces(pt_ctx* c, double theta, const double offset[3],
   if ((st = pt_sort(c))) return st;
   if ((st = pt_build_tree(c))) return st;
   if ((st = pt_upward(c))) return st;
-  if ((st = pt_traverse(c, theta, offset, counters))) return st;
-  return pt_interact(c, precision);
+  return gravity_fused(c, theta, offset, precision, counters);
 }
 
 pt_status pt_kick_drift(pt_ctx* c, double dt) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  if ((st = need(c, S_FORCES, "pt_kick_drift"))) return st;
+  if (!c->have_forces) return fail(c, PT_ERR_STATE, "pt_kick_drift before any force evaluation");
   {
     Timer t(c, PT_T_INTEGRATE);
     PT_TRY_LAUNCH(c, pt_launch_kick_drift(c->n, dt, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
@@ -609,6 +657,7 @@ pt_status pt_kick_drift(pt_ctx* c, double dt) {
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   // positions moved: everything downstream must be rebuilt (Partition::rebuild)
   c->stage = S_PARTICLES;
+  c->lists_valid = false;
   return PT_OK;
 }
 
@@ -737,7 +786,7 @@ pt_status pt_get_buckets(pt_ctx* c, int32_t* bucket_nodes) {
 pt_status pt_get_lists_size(pt_ctx* c, int64_t* n_pp, int64_t* n_pc) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  if ((st = need(c, S_LISTS, "pt_get_lists_size"))) return st;
+  if (!c->lists_valid) return fail(c, PT_ERR_STATE, "no interaction lists (call pt_traverse)");
   if (n_pp) *n_pp = c->n_pp;
   if (n_pc) *n_pc = c->n_pc;
   return PT_OK;
@@ -746,7 +795,7 @@ pt_status pt_get_lists_size(pt_ctx* c, int64_t* n_pp, int64_t* n_pc) {
 pt_status pt_get_lists(pt_ctx* c, int64_t* pp_off, int32_t* pp, int64_t* pc_off, int32_t* pc) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  if ((st = need(c, S_LISTS, "pt_get_lists"))) return st;
+  if (!c->lists_valid) return fail(c, PT_ERR_STATE, "no interaction lists (call pt_traverse)");
   const int64_t nb = c->nb;
   std::vector<uint64_t> keys((size_t)c->nn);
   if ((st = d2h(c, keys.data(), c->nkey.p, keys.size() * 8))) return st;

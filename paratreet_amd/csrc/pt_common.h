@@ -26,30 +26,22 @@ enum { PT_NODE_EMPTY = 0, PT_NODE_LEAF = 1, PT_NODE_INTERNAL = 2 };
 #define PT_CD_HI 15
 #define PT_CD_NFIELDS 18
 
-// Packed record read by the traversal: centre of mass and radius.
-struct __align__(32) TravNode {
+// Packed node record read by the tree walk: centre of mass, radius and
+// topology in one 64-byte line (the 8 children of a node are consecutive
+// records, so a wavefront reads them as 512 contiguous bytes).
+struct __align__(64) TreeNode {
   double cx, cy, cz, radius;
-};
-// Packed topology record read by the traversal.
-struct __align__(16) TopoNode {
   int32_t count;
   int32_t first_child;  // -1 for leaves
   int32_t type;
   int32_t begin;
+  int32_t pad[4];
 };
 // Packed source record for particle-cell interactions (FP64 path).
 struct __align__(32) GravNode {
   double cx, cy, cz, mass;
   double soft, qxx, qxy, qxz;
   double qyy, qyz, qzz, pad;
-};
-// Packed source record for particle-cell interactions (FP32 path): the centre
-// stays FP64 (it is shifted per bucket), the rest is FP32.
-struct __align__(16) GravNode32 {
-  double cx, cy, cz;
-  float mass, soft;
-  float qxx, qxy, qxz, qyy;
-  float qyz, qzz, pad0, pad1;
 };
 // Bucket (target leaf) record: particle range, box, centre of mass, softening.
 struct __align__(32) Bucket {

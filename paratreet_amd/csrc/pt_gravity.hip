@@ -1,0 +1,719 @@
+// GravityVisitor interactions on gfx950: list-driven and fused walk+interact.
+//
+// Reference: GravityVisitor::node with openSoftening / addGravity / SPLINEQ and
+// GravityVisitor::leaf with SPLINE (examples/GravityVisitor.h:37-160,184-224),
+// applied through SpatialNode::applyAcceleration / applyPotential
+// (src/Node.h:30-38).  Potential comes from node()'s quadrupole branch only,
+// as in the reference (addGravity and leaf() add none).
+//
+// Work mapping: one wavefront per target bucket (buckets hold ~5 particles).
+// With nt targets the lanes are laid out as lane = s * nt + t: lane holds
+// target t in registers and is source slot s of S = 64 / nt slots.  Sources
+// (closed nodes for node(), particles of leaf sources for leaf()) are staged
+// 64 at a time into LDS by one lane each -- the staging lane also converts
+// them (FP32 shift, openSoftening decision) -- and every lane then walks the
+// chunk with stride S.  All 64 lanes do useful pair work except the
+// 64 - S * nt remainder.  At the end the S partial sums of each target are
+// reduced through LDS in a fixed order.
+//
+// Two arithmetic forms:
+//   FP64  -- the reference's expressions in double (sqrt and 1/x included);
+//   FP32C -- positions shifted to the bucket's box centre in FP64 and rounded
+//            to FP32, interaction arithmetic in FP32 with the hardware
+//            reciprocal square root, each chunk's partial sum (<= 64/S terms)
+//            added to the per-lane accumulators with Kahan compensation;
+//            partials are combined in FP64.  The openSoftening choice
+//            (monopole vs quadrupole) is made in FP64 in both forms.
+//
+// Kernels:
+//   interact_kernel  -- reads the per-bucket lists of pt_traverse;
+//   gravity_kernel   -- walks the tree itself (pt_walk.h, same decisions) and
+//                       evaluates each 64-entry chunk of closed nodes / leaf
+//                       sources as soon as the walk has produced it, so no
+//                       list ever reaches HBM.
+//
+// This file is compiled with -ffp-contract=off because it contains the walk's
+// opening test; the interaction arithmetic uses explicit fma() where the
+// reference's separately rounded order does not matter for the 1e-12 bound.
+#include "pt_common.h"
+#include "pt_kernels.h"
+#include "pt_walk.h"
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * PT_WAVE;
+constexpr int kGroupMax = PT_WAVE;  // targets per pass (one per lane at most)
+
+// ---------------------------------------------------------------------------
+// arithmetic
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double spline_b64(double r2, double twoh) {
+  const double r = sqrt(r2);
+  if (r < twoh) {
+    const double dih = 2.0 / twoh;
+    const double u = r * dih;
+    if (u < 1.0) {
+      return dih * dih * dih * (4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u);
+    }
+    const double dir = 1.0 / r;
+    return -1.0 / 15.0 * dir * dir * dir +
+           dih * dih * dih * (8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u - 1.0 / 6.0 * u * u * u);
+  }
+  const double a = 1.0 / r;
+  return a * a * a;
+}
+
+template <typename R>
+__device__ __forceinline__ void splineq(R dir, R r2, R twoh, R& a, R& b, R& c, R& d) {
+  if (r2 < twoh * twoh) {
+    const R dih = R(2.0) / twoh;
+    const R u = dih / dir;
+    if (u < R(1.0)) {
+      a = dih * R(7.0) / R(5.0) - R(2.0 / 3.0) * u * u + R(3.0 / 10.0) * u * u * u * u -
+          R(1.0 / 10.0) * u * u * u * u * u;
+      b = dih * dih * dih * R(4.0) / R(3.0) - R(6.0 / 5.0) * u * u + R(1.0 / 2.0) * u * u * u;
+      c = dih * dih * dih * dih * dih * R(12.0) / R(5.0) - R(3.0 / 2.0) * u;
+      d = R(3.0 / 2.0) * dih * dih * dih * dih * dih * dih * dir;
+    } else {
+      a = R(-1.0 / 15.0) * dir + dih * R(8.0 / 5.0) - R(4.0 / 3.0) * u * u + u * u * u -
+          R(3.0 / 10.0) * u * u * u * u + R(1.0 / 30.0) * u * u * u * u * u;
+      b = R(-1.0 / 15.0) * dir * dir * dir + dih * dih * dih * R(8.0) / R(3.0) - R(3.0) * u +
+          R(6.0 / 5.0) * u * u - R(1.0 / 6.0) * u * u * u;
+      c = R(-1.0 / 5.0) * dir * dir * dir * dir * dir + R(3.0) * dih * dih * dih * dih * dir +
+          dih * dih * dih * dih * dih * R(-12.0) / R(5.0) + R(1.0 / 2.0) * u;
+      d = -dir * dir * dir * dir * dir * dir * dir +
+          R(3.0) * dih * dih * dih * dih * dir * dir * dir -
+          R(1.0 / 2.0) * dih * dih * dih * dih * dih * dih * dir;
+    }
+  } else {
+    a = dir;
+    b = a * a * a;
+    c = R(3.0) * b * a * a;
+    d = R(5.0) * c * a * a;
+  }
+}
+
+__device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
+
+__device__ __forceinline__ float spline_b32(float r2, float twoh) {
+  const float rinv = rsqrt32(r2);
+  const float r = r2 * rinv;
+  if (r < twoh) {
+    const float dih = 2.0f / twoh;
+    const float u = r * dih;
+    if (u < 1.0f) {
+      return dih * dih * dih * (4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u);
+    }
+    return -1.0f / 15.0f * rinv * rinv * rinv +
+           dih * dih * dih * (8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u - 1.0f / 6.0f * u * u * u);
+  }
+  return rinv * rinv * rinv;
+}
+
+// GravityVisitor::openSoftening (examples/GravityVisitor.h:122-128) for source
+// centre (already offset) and softening against the target bucket.
+__device__ __forceinline__ bool open_softening(double sx, double sy, double sz, double ssoft,
+                                               const Bucket& b) {
+  const double sr = 2.0 * ssoft;
+  {
+    const double dx = sx - b.cx, dy = sy - b.cy, dz = sz - b.cz;
+    const double dsq = dx * dx + dy * dy + dz * dz;
+    const double rs = sr + 2.0 * b.soft;
+    if (dsq <= rs * rs) return true;
+  }
+  return pt_intersect_box_point(b, sx, sy, sz, sr * sr);
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// ---------------------------------------------------------------------------
+// LDS source records
+// ---------------------------------------------------------------------------
+
+struct SrcC64 {  // closed node, FP64
+  double cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, mono;
+};
+struct SrcP64 {  // source particle, FP64
+  double x, y, z, m, e, pad;
+};
+struct SrcC32 {  // closed node, FP32, centre in the bucket frame
+  float cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, mono;
+};
+struct SrcP32 {  // source particle, FP32, bucket frame
+  float x, y, z, m, e, pad0, pad1, pad2;
+};
+
+// Per-wave LDS working set: the staged chunk (also the final reduction area),
+// the walk stack and the pending closed-node / leaf-source buffers (a step
+// adds at most 64 entries to a buffer holding fewer than 64).
+template <bool F32, int STACK>
+struct __align__(16) WaveLds {
+  static constexpr int kStage = 64 * (F32 ? sizeof(SrcC32) : sizeof(SrcC64));
+  static_assert(64 * 4 * sizeof(double) <= kStage, "reduction area");
+  unsigned char stage[kStage];
+  int32_t stack[STACK];
+  int32_t pcbuf[2 * PT_WAVE];
+  int32_t ppbeg[2 * PT_WAVE];
+  int32_t ppcnt[2 * PT_WAVE];
+  int32_t fl_beg[PT_WAVE];
+  int32_t fl_exc[PT_WAVE];
+};
+
+// Per-lane target and accumulator state for one target group.
+template <bool F32>
+struct Lanes;
+
+template <>
+struct Lanes<false> {
+  double tx, ty, tz, te;
+  double ax, ay, az, ap;
+};
+template <>
+struct Lanes<true> {
+  float tx, ty, tz, te;
+  float ax, ay, az, ap;
+  float cx, cy, cz, cp;  // Kahan compensations
+};
+
+struct Group {
+  int ng;      // targets in this pass
+  int S;       // source slots = 64 / ng
+  int s, t;    // this lane's slot and target
+  bool act;    // lane participates
+  int iters;   // ceil(64 / S)
+  int64_t first;  // sorted index of the group's first target
+};
+
+__device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
+  Group g;
+  const int rem = b.count - g0;
+  g.ng = rem < kGroupMax ? rem : kGroupMax;
+  g.S = PT_WAVE / g.ng;
+  const int lane = __lane_id();
+  g.s = lane / g.ng;
+  g.t = lane - g.s * g.ng;
+  g.act = g.s < g.S;
+  g.iters = (PT_WAVE + g.S - 1) / g.S;
+  g.first = (int64_t)b.begin + g0;
+  return g;
+}
+
+template <bool F32>
+__device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
+                                             const double* __restrict__ px,
+                                             const double* __restrict__ py,
+                                             const double* __restrict__ pz,
+                                             const double* __restrict__ pe, double ccx,
+                                             double ccy, double ccz) {
+  const int64_t i = g.first + (g.act ? g.t : 0);
+  if (F32) {
+    L.tx = (float)(px[i] - ccx);
+    L.ty = (float)(py[i] - ccy);
+    L.tz = (float)(pz[i] - ccz);
+    L.te = (float)pe[i];
+  } else {
+    L.tx = px[i];
+    L.ty = py[i];
+    L.tz = pz[i];
+    L.te = pe[i];
+  }
+  L.ax = L.ay = L.az = L.ap = 0;
+  if constexpr (F32) L.cx = L.cy = L.cz = L.cp = 0.f;
+}
+
+// ---- staging (one lane per source) ----------------------------------------
+
+template <bool F32, typename W_t>
+__device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
+                                           const GravNode* __restrict__ gn, const Bucket& b,
+                                           double ox, double oy, double oz, double ccx,
+                                           double ccy, double ccz) {
+  const GravNode s = gn[id];
+  const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
+  const bool mono = open_softening(cx, cy, cz, s.soft, b);
+  if constexpr (F32) {
+    SrcC32 r;
+    r.cx = (float)(cx - ccx);
+    r.cy = (float)(cy - ccy);
+    r.cz = (float)(cz - ccz);
+    r.mass = (float)s.mass;
+    r.soft = (float)s.soft;
+    r.qxx = (float)s.qxx;
+    r.qxy = (float)s.qxy;
+    r.qxz = (float)s.qxz;
+    r.qyy = (float)s.qyy;
+    r.qyz = (float)s.qyz;
+    r.qzz = (float)s.qzz;
+    r.mono = mono ? 1.f : 0.f;
+    reinterpret_cast<SrcC32*>(W.stage)[i] = r;
+  } else {
+    SrcC64 r;
+    r.cx = s.cx;
+    r.cy = s.cy;
+    r.cz = s.cz;
+    r.mass = s.mass;
+    r.soft = s.soft;
+    r.qxx = s.qxx;
+    r.qxy = s.qxy;
+    r.qxz = s.qxz;
+    r.qyy = s.qyy;
+    r.qyz = s.qyz;
+    r.qzz = s.qzz;
+    r.mono = mono ? 1.0 : 0.0;
+    reinterpret_cast<SrcC64*>(W.stage)[i] = r;
+  }
+}
+
+template <bool F32, typename W_t>
+__device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
+                                               const double* __restrict__ px,
+                                               const double* __restrict__ py,
+                                               const double* __restrict__ pz,
+                                               const double* __restrict__ pm,
+                                               const double* __restrict__ pe, double ox,
+                                               double oy, double oz, double ccx, double ccy,
+                                               double ccz) {
+  if constexpr (F32) {
+    SrcP32 r;
+    r.x = (float)((px[j] + ox) - ccx);
+    r.y = (float)((py[j] + oy) - ccy);
+    r.z = (float)((pz[j] + oz) - ccz);
+    r.m = (float)pm[j];
+    r.e = (float)pe[j];
+    r.pad0 = r.pad1 = r.pad2 = 0.f;
+    reinterpret_cast<SrcP32*>(W.stage)[i] = r;
+  } else {
+    SrcP64 r;
+    r.x = px[j];
+    r.y = py[j];
+    r.z = pz[j];
+    r.m = pm[j];
+    r.e = pe[j];
+    r.pad = 0.0;
+    reinterpret_cast<SrcP64*>(W.stage)[i] = r;
+  }
+}
+
+// ---- chunk evaluation (all lanes) -------------------------------------------
+
+template <bool F32, typename W_t>
+__device__ __forceinline__ void eval_nodes(const W_t& W, int nsrc, const Group& g,
+                                           Lanes<F32>& L, double ox, double oy, double oz) {
+  if constexpr (F32) {
+    float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
+    const SrcC32* src = reinterpret_cast<const SrcC32*>(W.stage);
+    for (int k = 0; k < g.iters; k++) {
+      const int j = g.s + k * g.S;
+      if (g.act && j < nsrc) {
+        const SrcC32 s = src[j];
+        if (s.mono != 0.f) {
+          const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
+          const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (rsq != 0.f) {
+            const float ri = rsqrt32(rsq);
+            const float f = s.mass * ri * ri * ri;
+            px = fmaf(dx, f, px);
+            py = fmaf(dy, f, py);
+            pz = fmaf(dz, f, pz);
+          }
+        } else {
+          const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
+          const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+          const float dir = rsqrt32(rsq);
+          float a, bb, c, d;
+          splineq<float>(dir, rsq, s.soft + L.te, a, bb, c, d);
+          const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
+          const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
+          const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
+          const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+          const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
+          const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * tr));
+          pp += fmaf(-s.mass, a, fmaf(-c, qir, bb * tr));
+          px += fmaf(-qir3, rx, c * qx);
+          py += fmaf(-qir3, ry, c * qy);
+          pz += fmaf(-qir3, rz, c * qz);
+        }
+      }
+    }
+    kahan_add(L.ax, L.cx, px);
+    kahan_add(L.ay, L.cy, py);
+    kahan_add(L.az, L.cz, pz);
+    kahan_add(L.ap, L.cp, pp);
+  } else {
+    const SrcC64* src = reinterpret_cast<const SrcC64*>(W.stage);
+    for (int k = 0; k < g.iters; k++) {
+      const int j = g.s + k * g.S;
+      if (g.act && j < nsrc) {
+        const SrcC64 s = src[j];
+        if (s.mono != 0.0) {
+          // addGravity: unsoftened monopole, no potential
+          const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
+          const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+          if (rsq != 0) {
+            const double f = s.mass / (rsq * sqrt(rsq));
+            L.ax = fma(dx, f, L.ax);
+            L.ay = fma(dy, f, L.ay);
+            L.az = fma(dz, f, L.az);
+          }
+        } else {
+          const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
+          const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
+          const double dir = 1.0 / sqrt(rsq);
+          double a, bb, c, d;
+          splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
+          const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
+          const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
+          const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
+          const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
+          const double tr = 0.5 * (s.qxx + s.qyy + s.qzz);
+          const double qir3 = fma(bb, s.mass, fma(d, qir, -c * tr));
+          L.ap += fma(-s.mass, a, fma(-c, qir, bb * tr));
+          L.ax += fma(-qir3, rx, c * qx);
+          L.ay += fma(-qir3, ry, c * qy);
+          L.az += fma(-qir3, rz, c * qz);
+        }
+      }
+    }
+  }
+}
+
+template <bool F32, typename W_t>
+__device__ __forceinline__ void eval_particles(const W_t& W, int nsrc, const Group& g,
+                                               Lanes<F32>& L, double ox, double oy, double oz) {
+  if constexpr (F32) {
+    float px = 0.f, py = 0.f, pz = 0.f;
+    const SrcP32* src = reinterpret_cast<const SrcP32*>(W.stage);
+    for (int k = 0; k < g.iters; k++) {
+      const int j = g.s + k * g.S;
+      if (g.act && j < nsrc) {
+        const SrcP32 s = src[j];
+        const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
+        const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        if (rsq != 0.f) {
+          const float f = spline_b32(rsq, s.e + L.te) * s.m;
+          px = fmaf(dx, f, px);
+          py = fmaf(dy, f, py);
+          pz = fmaf(dz, f, pz);
+        }
+      }
+    }
+    kahan_add(L.ax, L.cx, px);
+    kahan_add(L.ay, L.cy, py);
+    kahan_add(L.az, L.cz, pz);
+  } else {
+    const SrcP64* src = reinterpret_cast<const SrcP64*>(W.stage);
+    for (int k = 0; k < g.iters; k++) {
+      const int j = g.s + k * g.S;
+      if (g.act && j < nsrc) {
+        const SrcP64 s = src[j];
+        const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
+        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (rsq != 0) {
+          const double f = spline_b64(rsq, s.e + L.te) * s.m;
+          L.ax = fma(dx, f, L.ax);
+          L.ay = fma(dy, f, L.ay);
+          L.az = fma(dz, f, L.az);
+        }
+      }
+    }
+  }
+}
+
+// Flattens up to 64 leaf sources (begin/count in the calling lane, count 0 =
+// none) into particles and evaluates them 64 at a time.
+template <bool F32, typename W_t>
+__device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, const Group& g,
+                                          Lanes<F32>& L, const double* __restrict__ px,
+                                          const double* __restrict__ py,
+                                          const double* __restrict__ pz,
+                                          const double* __restrict__ pm,
+                                          const double* __restrict__ pe, double ox, double oy,
+                                          double oz, double ccx, double ccy, double ccz) {
+  const int lane = __lane_id();
+  const int incl = pt_wave_incl_scan(cnt);
+  const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
+  W.fl_beg[lane] = beg;
+  W.fl_exc[lane] = incl - cnt;
+  __builtin_amdgcn_wave_barrier();
+  for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+    const int slot = s0 + lane;
+    if (slot < total) {
+      int lo = 0, hi = PT_WAVE - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (W.fl_exc[mid] <= slot) lo = mid;
+        else hi = mid - 1;
+      }
+      const int64_t j = (int64_t)W.fl_beg[lo] + (slot - W.fl_exc[lo]);
+      stage_particle<F32, W_t>(W, lane, j, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
+    eval_particles<F32, W_t>(W, nsrc, g, L, ox, oy, oz);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <bool F32, typename W_t>
+__device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Group& g,
+                                         Lanes<F32>& L, const GravNode* __restrict__ gn,
+                                         const Bucket& b, double ox, double oy, double oz,
+                                         double ccx, double ccy, double ccz) {
+  if (__lane_id() < nsrc) stage_node<F32, W_t>(W, __lane_id(), id, gn, b, ox, oy, oz, ccx, ccy, ccz);
+  __builtin_amdgcn_wave_barrier();
+  eval_nodes<F32, W_t>(W, nsrc, g, L, ox, oy, oz);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Sums the S partials of each target (fixed order) and stores the results.
+template <bool F32, typename W_t>
+__device__ __forceinline__ void finish_group(W_t& W, const Group& g, const Lanes<F32>& L,
+                                             double* __restrict__ ax, double* __restrict__ ay,
+                                             double* __restrict__ az, double* __restrict__ pot) {
+  double* red = reinterpret_cast<double*>(W.stage);
+  const int lane = __lane_id();
+  double v0, v1, v2, v3;
+  if constexpr (F32) {
+    v0 = (double)L.ax - (double)L.cx;
+    v1 = (double)L.ay - (double)L.cy;
+    v2 = (double)L.az - (double)L.cz;
+    v3 = (double)L.ap - (double)L.cp;
+  } else {
+    v0 = L.ax;
+    v1 = L.ay;
+    v2 = L.az;
+    v3 = L.ap;
+  }
+  red[lane * 4 + 0] = g.act ? v0 : 0.0;
+  red[lane * 4 + 1] = g.act ? v1 : 0.0;
+  red[lane * 4 + 2] = g.act ? v2 : 0.0;
+  red[lane * 4 + 3] = g.act ? v3 : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  if (lane < g.ng) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int s = 0; s < g.S; s++) {
+      const int l = s * g.ng + lane;
+      s0 += red[l * 4 + 0];
+      s1 += red[l * 4 + 1];
+      s2 += red[l * 4 + 2];
+      s3 += red[l * 4 + 3];
+    }
+    const int64_t i = g.first + lane;
+    ax[i] = s0;
+    ay[i] = s1;
+    az[i] = s2;
+    pot[i] = s3;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+template <bool F32>
+__global__ void __launch_bounds__(kThreads)
+interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+                const GravNode* __restrict__ gn, const double* __restrict__ px,
+                const double* __restrict__ py, const double* __restrict__ pz,
+                const double* __restrict__ pm, const double* __restrict__ pe,
+                const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
+                const int64_t* __restrict__ pc_off, const int32_t* __restrict__ pc, double ox,
+                double oy, double oz, double* __restrict__ ax, double* __restrict__ ay,
+                double* __restrict__ az, double* __restrict__ pot) {
+  using W_t = WaveLds<F32, 8>;
+  __shared__ W_t lds[kWaves];
+  const int wave = threadIdx.x / PT_WAVE;
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  if (bi >= nb) return;
+  W_t& W = lds[wave];
+  const Bucket b = bk[bi];
+  const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+               ccz = (b.loz + b.hiz) * 0.5;
+  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
+  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
+  for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
+    const Group g = make_group(b, g0);
+    Lanes<F32> L;
+    load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+    for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
+      const int nsrc = (pc1 - e0) < PT_WAVE ? (int)(pc1 - e0) : PT_WAVE;
+      const int32_t id = lane < nsrc ? pc[e0 + lane] : 0;
+      do_nodes<F32, W_t>(W, id, nsrc, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+    }
+    for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
+      int32_t beg = 0, cnt = 0;
+      if (l0 + lane < pp1) {
+        const TreeNode nd = nodes[pp[l0 + lane]];
+        beg = nd.begin;
+        cnt = nd.count;
+      }
+      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+    }
+    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
+  }
+}
+
+template <bool F32, int STACK>
+__global__ void __launch_bounds__(kThreads)
+gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+               const GravNode* __restrict__ gn, const double* __restrict__ px,
+               const double* __restrict__ py, const double* __restrict__ pz,
+               const double* __restrict__ pm, const double* __restrict__ pe, double gf,
+               double ox, double oy, double oz, double* __restrict__ ax,
+               double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
+               unsigned long long* __restrict__ partials, int32_t* __restrict__ err) {
+  using W_t = WaveLds<F32, STACK>;
+  __shared__ W_t lds[kWaves];
+  __shared__ unsigned long long s_cnt[kWaves][4];
+  const int wave = threadIdx.x / PT_WAVE;
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t blk = pt_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t bi_raw = blk * kWaves + wave;
+  const bool live = bi_raw < nb;
+  const int64_t bi = live ? bi_raw : nb - 1;
+  W_t& W = lds[wave];
+  const Bucket b = bk[bi];
+  const int64_t nt = b.count;
+  const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+               ccz = (b.loz + b.hiz) * 0.5;
+  WalkState total{0, 0, 0, 0, 0, false};
+  for (int g0 = 0; live && g0 < b.count; g0 += kGroupMax) {
+    const Group g = make_group(b, g0);
+    Lanes<F32> L;
+    load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+    WalkState ws{0, 0, 0, 0, 0, false};
+    int npc = 0, npp = 0;
+    {
+      const WalkLane r = pt_walk_root(nodes, b, nt, gf, ox, oy, oz, W.stack, ws);
+      if (r.is_closed) {
+        if (lane == 0) W.pcbuf[0] = 0;
+        npc = 1;
+      } else if (r.is_leaf) {
+        if (lane == 0) {
+          W.ppbeg[0] = r.begin;
+          W.ppcnt[0] = r.count;
+        }
+        npp = 1;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (ws.sp > 0) {
+      unsigned long long leafm, closem;
+      const WalkLane r =
+          pt_walk_step<STACK>(nodes, b, nt, gf, ox, oy, oz, W.stack, ws, leafm, closem);
+      if (r.is_closed) W.pcbuf[npc + pt_rank_in(closem)] = r.node;
+      if (r.is_leaf) {
+        const int k = npp + pt_rank_in(leafm);
+        W.ppbeg[k] = r.begin;
+        W.ppcnt[k] = r.count;
+      }
+      npc += __popcll(closem);
+      npp += __popcll(leafm);
+      __builtin_amdgcn_wave_barrier();
+      if (npc >= PT_WAVE) {
+        do_nodes<F32, W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+        const int rest = npc - PT_WAVE;
+        const int32_t v = lane < rest ? W.pcbuf[PT_WAVE + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rest) W.pcbuf[lane] = v;
+        npc = rest;
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (npp >= PT_WAVE) {
+        do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, ox, oy, oz,
+                       ccx, ccy, ccz);
+        const int rest = npp - PT_WAVE;
+        const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
+        const int32_t vc = lane < rest ? W.ppcnt[PT_WAVE + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rest) {
+          W.ppbeg[lane] = vb;
+          W.ppcnt[lane] = vc;
+        }
+        npp = rest;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    if (npc > 0) {
+      const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
+      do_nodes<F32, W_t>(W, id, npc, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+    }
+    if (npp > 0) {
+      const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
+      const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
+      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+    }
+    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
+    if (g0 == 0) total = ws;
+    else total.overflow |= ws.overflow;
+  }
+  if (lane == 0) {
+    if (live && total.overflow) atomicExch(err, 1);
+    s_cnt[wave][0] = live ? total.c_node : 0;
+    s_cnt[wave][1] = live ? total.c_part : 0;
+    s_cnt[wave][2] = live ? total.c_open : 0;
+    s_cnt[wave][3] = live ? total.c_close : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kWaves; w++) v += s_cnt[w][threadIdx.x];
+    partials[blk * 4 + threadIdx.x] = v;
+  }
+}
+
+inline unsigned grid_for(int64_t nb) { return (unsigned)((nb + kWaves - 1) / kWaves); }
+
+}  // namespace
+
+int64_t pt_gravity_blocks(int64_t nb) { return (nb + kWaves - 1) / kWaves; }
+
+int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                       const GravNode* gn, const double* x, const double* y, const double* z,
+                       const double* m, const double* e, const int64_t* pp_off, const int32_t* pp,
+                       const int64_t* pc_off, const int32_t* pc, const double off[3], double* ax,
+                       double* ay, double* az, double* pot, hipStream_t s) {
+  if (nb <= 0) return 0;
+  if (precision == 0) {
+    interact_kernel<false><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, x, y, z, m, e,
+                                                             pp_off, pp, pc_off, pc, off[0],
+                                                             off[1], off[2], ax, ay, az, pot);
+  } else {
+    interact_kernel<true><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, x, y, z, m, e,
+                                                            pp_off, pp, pc_off, pc, off[0], off[1],
+                                                            off[2], ax, ay, az, pot);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// big_stack selects the 1536-entry walk stack (used after a 512-entry walk
+// reported an overflow; 1536 keeps the FP64 work-group under 64 KiB of LDS).
+int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
+                      const TreeNode* nodes, const GravNode* gn, const double* x,
+                      const double* y, const double* z, const double* m, const double* e,
+                      double gf, const double off[3], double* ax, double* ay, double* az,
+                      double* pot, unsigned long long* partials, int32_t* err, hipStream_t s) {
+  if (nb <= 0) return 0;
+#define PT_GRAVITY_LAUNCH(F32, STACK)                                                          \
+  gravity_kernel<F32, STACK><<<grid_for(nb), kThreads, 0, s>>>(                                \
+      bk, nb, nodes, gn, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot, partials, \
+      err)
+  if (precision == 0) {
+    if (big_stack) PT_GRAVITY_LAUNCH(false, 1536);
+    else PT_GRAVITY_LAUNCH(false, 512);
+  } else {
+    if (big_stack) PT_GRAVITY_LAUNCH(true, 1536);
+    else PT_GRAVITY_LAUNCH(true, 512);
+  }
+#undef PT_GRAVITY_LAUNCH
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -285,25 +285,23 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
   cd_store(d, nd, nn, p);
 }
 
-// Packs the traversal and interaction records from the SoA node table.
+// Packs the walk and interaction records from the SoA node table.
 __global__ void __launch_bounds__(256)
 pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
-                  TravNode* __restrict__ tn, TopoNode* __restrict__ topo,
-                  GravNode* __restrict__ gn, GravNode32* __restrict__ gn32) {
+                  TreeNode* __restrict__ tnode, GravNode* __restrict__ gn) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= nn) return;
-  TravNode a;
+  TreeNode a;
   a.cx = nd[(PT_CD_CM + 0) * nn + i];
   a.cy = nd[(PT_CD_CM + 1) * nn + i];
   a.cz = nd[(PT_CD_CM + 2) * nn + i];
   a.radius = nd[PT_CD_RADIUS * nn + i];
-  tn[i] = a;
-  TopoNode b;
-  b.count = t.count[i];
-  b.first_child = t.first_child[i];
-  b.type = t.type[i];
-  b.begin = t.begin[i];
-  topo[i] = b;
+  a.count = t.count[i];
+  a.first_child = t.first_child[i];
+  a.type = t.type[i];
+  a.begin = t.begin[i];
+  a.pad[0] = a.pad[1] = a.pad[2] = a.pad[3] = 0;
+  tnode[i] = a;
   GravNode g;
   g.cx = a.cx;
   g.cy = a.cy;
@@ -318,21 +316,6 @@ pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
   g.qzz = nd[(PT_CD_Q + 5) * nn + i];
   g.pad = 0.0;
   gn[i] = g;
-  GravNode32 h;
-  h.cx = a.cx;
-  h.cy = a.cy;
-  h.cz = a.cz;
-  h.mass = (float)g.mass;
-  h.soft = (float)g.soft;
-  h.qxx = (float)g.qxx;
-  h.qxy = (float)g.qxy;
-  h.qxz = (float)g.qxz;
-  h.qyy = (float)g.qyy;
-  h.qyz = (float)g.qyz;
-  h.qzz = (float)g.qzz;
-  h.pad0 = 0.f;
-  h.pad1 = 0.f;
-  gn32[i] = h;
 }
 
 __global__ void __launch_bounds__(256)
@@ -388,9 +371,8 @@ int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t 
 }
 
 int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
-                   int64_t nb, TravNode* tn, TopoNode* topo, GravNode* gn, GravNode32* gn32,
-                   Bucket* bk, hipStream_t s) {
-  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tn, topo, gn, gn32);
+                   int64_t nb, TreeNode* tnode, GravNode* gn, Bucket* bk, hipStream_t s) {
+  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tnode, gn);
   if (nb > 0) pack_buckets_kernel<<<grid_for(nb), 256, 0, s>>>(t, nd, nn, bucket_node, nb, bk);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -1,0 +1,168 @@
+// Wavefront tree walk shared by the list-producing traversal and the fused
+// walk-and-interact kernel, so both evaluate exactly the same open() decisions.
+//
+// Reference: TransposedDownTraverser::recurse (src/Traverser.h:154-217) seen
+// from one target bucket, with GravityVisitor::open (examples/GravityVisitor.h:
+// 162-164,180).  A Leaf source yields leaf() (self included: CallSelfLeaf), an
+// EmptyLeaf nothing, an Internal source an open() test: opened -> its 8
+// children are visited, closed -> node().  The per-bucket set of calls does not
+// depend on the visit order, so the wavefront visits 8 opened nodes' children
+// per step (64 lanes): the LDS stack holds the first-child ids of opened
+// nodes, whose 8 children are consecutive records.
+//
+// Included only by translation units compiled with -ffp-contract=off so the
+// opening test rounds exactly like the reference's separately rounded
+// expression.
+#pragma once
+
+#include "pt_common.h"
+
+// Walk stack entries per wavefront in the list-producing traversal.
+#define PT_WALK_STACK 1024
+
+// Space::intersect(box, point, rsq): per-axis clamp distance, dsq <= rsq.
+__device__ __forceinline__ bool pt_intersect_box_point(const Bucket& b, double px, double py,
+                                                       double pz, double rsq) {
+  double dsq = 0.0;
+  if (px < b.lox) {
+    const double t = b.lox - px;
+    dsq += t * t;
+  } else if (px > b.hix) {
+    const double t = px - b.hix;
+    dsq += t * t;
+  }
+  if (py < b.loy) {
+    const double t = b.loy - py;
+    dsq += t * t;
+  } else if (py > b.hiy) {
+    const double t = py - b.hiy;
+    dsq += t * t;
+  }
+  if (pz < b.loz) {
+    const double t = b.loz - pz;
+    dsq += t * t;
+  } else if (pz > b.hiz) {
+    const double t = pz - b.hiz;
+    dsq += t * t;
+  }
+  return dsq <= rsq;
+}
+
+// GravityVisitor::open (examples/GravityVisitor.h:162-164,180).
+__device__ __forceinline__ bool pt_gv_open(const TreeNode& s, const Bucket& b, double gf,
+                                           double ox, double oy, double oz) {
+  if (s.count <= PT_MIN_PARTICLE_NODE) return true;
+  const double dataRsq = s.radius * s.radius * gf;
+  return pt_intersect_box_point(b, s.cx + ox, s.cy + oy, s.cz + oz, dataRsq);
+}
+
+// Result of one walk step for the calling lane.
+struct WalkLane {
+  int32_t node;      // visited node id (valid if active)
+  int32_t count;     // its particle count
+  int32_t begin;     // its first particle
+  bool is_leaf;      // -> leaf() (P-P)
+  bool is_closed;    // -> node() (P-C)
+};
+
+// Per-wave walk state (wave-uniform scalars).
+struct WalkState {
+  int sp;                      // stack size
+  unsigned long long c_open, c_close, c_node, c_part;
+  bool overflow;
+};
+
+// Visits the root (node 0).  Returns the lane-0 style result for the root:
+// is_leaf / is_closed are wave-uniform here; lane 0 owns the append.
+__device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ nodes,
+                                                 const Bucket& b, int64_t nt, double gf,
+                                                 double ox, double oy, double oz,
+                                                 int32_t* __restrict__ stack, WalkState& ws) {
+  WalkLane r{0, 0, 0, false, false};
+  const TreeNode root = nodes[0];
+  r.count = root.count;
+  r.begin = root.begin;
+  if (root.type == PT_NODE_LEAF) {
+    r.is_leaf = true;
+    ws.c_part += (unsigned long long)root.count * (unsigned long long)nt;
+  } else if (root.type == PT_NODE_INTERNAL) {
+    if (pt_gv_open(root, b, gf, ox, oy, oz)) {
+      if (__lane_id() == 0) stack[0] = root.first_child;
+      ws.sp = 1;
+      ws.c_open++;
+    } else {
+      r.is_closed = true;
+      ws.c_close++;
+      ws.c_node += (unsigned long long)nt;
+    }
+  }
+  return r;
+}
+
+// One step: pops up to 8 opened nodes and tests their 64 children.  Returns
+// this lane's child and its classification; pushes opened children.  A push
+// beyond STACK entries sets ws.overflow and ends the walk (the caller reports
+// it; nothing is silently dropped).
+template <int STACK>
+__device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ nodes,
+                                                 const Bucket& b, int64_t nt, double gf,
+                                                 double ox, double oy, double oz,
+                                                 int32_t* __restrict__ stack, WalkState& ws,
+                                                 unsigned long long& leafm,
+                                                 unsigned long long& closem) {
+  const int lane = __lane_id();
+  const int k = ws.sp < 8 ? ws.sp : 8;
+  ws.sp -= k;
+  const int slot = lane >> 3;
+  const bool active = slot < k;
+  WalkLane r{-1, 0, 0, false, false};
+  bool op = false;
+  int32_t fc = -1;
+  if (active) {
+    r.node = stack[ws.sp + slot] + (lane & 7);
+    const TreeNode nd = nodes[r.node];
+    r.count = nd.count;
+    r.begin = nd.begin;
+    fc = nd.first_child;
+    if (nd.type == PT_NODE_LEAF) {
+      r.is_leaf = true;
+    } else if (nd.type == PT_NODE_INTERNAL) {
+      op = pt_gv_open(nd, b, gf, ox, oy, oz);
+      r.is_closed = !op;
+    }
+  }
+  leafm = __ballot(r.is_leaf);
+  closem = __ballot(r.is_closed);
+  const unsigned long long openm = __ballot(op);
+  const int n_open = __popcll(openm);
+  if (ws.sp + n_open > STACK) {
+    ws.overflow = true;
+    ws.sp = 0;
+    leafm = closem = 0;
+    r.is_leaf = r.is_closed = false;
+    return r;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (op) stack[ws.sp + pt_rank_in(openm)] = fc;
+  __builtin_amdgcn_wave_barrier();
+  ws.sp += n_open;
+  ws.c_open += n_open;
+  const int nc = __popcll(closem);
+  ws.c_close += nc;
+  ws.c_node += (unsigned long long)nc * (unsigned long long)nt;
+  int lc = r.is_leaf ? r.count : 0;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) lc += __shfl_xor(lc, d, PT_WAVE);
+  ws.c_part += (unsigned long long)lc * (unsigned long long)nt;
+  return r;
+}
+
+// XCD-aware bucket order: work-groups are dispatched round-robin over the 8
+// XCDs, so group i is given the (i % 8)-th contiguous eighth of the buckets;
+// neighbouring buckets (which share most of their source nodes) then run on
+// the same XCD and share its L2.
+__device__ __forceinline__ int64_t pt_xcd_block(int64_t i, int64_t g) {
+  const int64_t g8 = (g / 8) * 8;
+  if (i >= g8) return i;
+  return (i % 8) * (g8 / 8) + i / 8;
+}

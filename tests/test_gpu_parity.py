@@ -165,6 +165,27 @@ def test_forces_fp32c_precision(pair):
     assert rms <= FP32C_RMS_TOL, name
 
 
+def test_fused_path_matches_oracle(pair):
+    """pt_compute_forces (fused walk + interactions, no lists) against the
+    oracle: counters exact, FP64 within 1e-12, FP32C within 1e-5 RMS."""
+    name, P, d, _, _ = pair
+    cnt = d.compute_forces(theta=P.theta, precision=0)
+    acc, pot = d.accelerations()
+    oacc, opot, ocnt = P.gravity_unsorted()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt), name
+    e = rel_err(acc, oacc)
+    ep = np.abs(pot - opot) / np.abs(opot)
+    print("%s fused fp64 vs oracle: max rel acc %.3g pot %.3g" % (name, e.max(), ep.max()))
+    assert e.max() <= FP64_TOL and ep.max() <= FP64_TOL, name
+    cnt32 = d.compute_forces(theta=P.theta, precision=1)
+    a32, _ = d.accelerations()
+    assert tuple(cnt32) == tuple(cnt), name
+    e32 = rel_err(a32, acc)
+    rms = float(np.sqrt(np.mean(e32 * e32)))
+    print("%s fused fp32c vs fp64: rms %.3g max %.3g" % (name, rms, e32.max()))
+    assert rms <= FP32C_RMS_TOL, name
+
+
 def test_lambs_tree_error_vs_reference_direct():
     """Tree force error on the reference's accuracy fixture against its
     direct.acc: same order as the oracle's (quadrupole, theta 0.7)."""
