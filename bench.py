@@ -70,6 +70,21 @@ def time_path(d, theta, precision, steps, warmup):
     return ms_step, cnt, float(np.mean(force_ms)), stage
 
 
+def time_list_path(d, theta, precision):
+    """One evaluation through the list path (traverse -> lists -> interact),
+    timed per stage, to separate the walk from the interactions."""
+    from paratreet_amd.gravity import GravityVisitor
+    d.remake_universe()
+    d.assign_keys()
+    d.build_tree()
+    d.set_timing(True)
+    d.traverse(GravityVisitor((0.0, 0.0, 0.0), theta))
+    d.interact(precision)
+    tm = d.timings()
+    d.set_timing(False)
+    return {"traverse_2pass": tm["traverse"], "interact": tm["force"]}
+
+
 def cpu_baseline(pos, m, eps, bucket, theta):
     """Oracle (port) on the host cores: full force evaluation of the same tree."""
     from oracle import oracle as O
@@ -118,6 +133,9 @@ def main():
     achieved32 = flops / (f32_ms / 1000.0) / 1e12
     achieved64 = flops / (f64_ms / 1000.0) / 1e12
 
+    list_path = {"fp32c": time_list_path(d, theta, PT_FP32C),
+                 "fp64": time_list_path(d, theta, PT_FP64)}
+
     err = {}
     if not args.no_error:
         rng = np.random.default_rng(99)
@@ -161,6 +179,7 @@ def main():
         "force_kernel_ms": {"fp32c": f32_ms, "fp64": f64_ms},
         "force_only_ips": {"fp32c": ints / (f32_ms / 1000.0), "fp64": ints / (f64_ms / 1000.0)},
         "stage_ms_fp32c": st32, "stage_ms_fp64": st64,
+        "list_path_ms": list_path,
         "roofline": {"bound": "valu", "achieved": achieved32, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "force32c_kernel",

@@ -97,21 +97,6 @@ __device__ __forceinline__ void splineq(R dir, R r2, R twoh, R& a, R& b, R& c, R
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
 
-__device__ __forceinline__ float spline_b32(float r2, float twoh) {
-  const float rinv = rsqrt32(r2);
-  const float r = r2 * rinv;
-  if (r < twoh) {
-    const float dih = 2.0f / twoh;
-    const float u = r * dih;
-    if (u < 1.0f) {
-      return dih * dih * dih * (4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u);
-    }
-    return -1.0f / 15.0f * rinv * rinv * rinv +
-           dih * dih * dih * (8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u - 1.0f / 6.0f * u * u * u);
-  }
-  return rinv * rinv * rinv;
-}
-
 // GravityVisitor::openSoftening (examples/GravityVisitor.h:122-128) for source
 // centre (already offset) and softening against the target bucket.
 __device__ __forceinline__ bool open_softening(double sx, double sy, double sz, double ssoft,
@@ -150,12 +135,19 @@ struct SrcP32 {  // source particle, FP32, bucket frame
   float x, y, z, m, e, pad0, pad1, pad2;
 };
 
+// A staged chunk holds up to 64 real sources; the slots the stride-S walk
+// reads beyond them hold null sources (zero mass and moments, far away), so
+// the evaluation loops need no bounds tests.  With S = 64 / nt the walk reads
+// S * ceil(64 / S) <= 84 slots (nt = 3).
+constexpr int kSlots = 96;
+constexpr double kNullPos = 1.0e6;
+
 // Per-wave LDS working set: the staged chunk (also the final reduction area),
 // the walk stack and the pending closed-node / leaf-source buffers (a step
 // adds at most 64 entries to a buffer holding fewer than 64).
 template <bool F32, int STACK>
 struct __align__(16) WaveLds {
-  static constexpr int kStage = 64 * (F32 ? sizeof(SrcC32) : sizeof(SrcC64));
+  static constexpr int kStage = kSlots * (F32 ? sizeof(SrcC32) : sizeof(SrcC64));
   static_assert(64 * 4 * sizeof(double) <= kStage, "reduction area");
   unsigned char stage[kStage];
   int32_t stack[STACK];
@@ -185,7 +177,7 @@ struct Lanes<true> {
 struct Group {
   int ng;      // targets in this pass
   int S;       // source slots = 64 / ng
-  int s, t;    // this lane's slot and target
+  int s, t;    // this lane's slot and target (inactive lanes read slot 0)
   bool act;    // lane participates
   int iters;   // ceil(64 / S)
   int64_t first;  // sorted index of the group's first target
@@ -197,9 +189,10 @@ __device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
   g.ng = rem < kGroupMax ? rem : kGroupMax;
   g.S = PT_WAVE / g.ng;
   const int lane = __lane_id();
-  g.s = lane / g.ng;
-  g.t = lane - g.s * g.ng;
-  g.act = g.s < g.S;
+  const int s = lane / g.ng;
+  g.act = s < g.S;
+  g.s = g.act ? s : 0;
+  g.t = g.act ? lane - s * g.ng : 0;
   g.iters = (PT_WAVE + g.S - 1) / g.S;
   g.first = (int64_t)b.begin + g0;
   return g;
@@ -212,12 +205,13 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
                                              const double* __restrict__ pz,
                                              const double* __restrict__ pe, double ccx,
                                              double ccy, double ccz) {
-  const int64_t i = g.first + (g.act ? g.t : 0);
-  if (F32) {
+  const int64_t i = g.first + g.t;
+  if constexpr (F32) {
     L.tx = (float)(px[i] - ccx);
     L.ty = (float)(py[i] - ccy);
     L.tz = (float)(pz[i] - ccz);
     L.te = (float)pe[i];
+    L.cx = L.cy = L.cz = L.cp = 0.f;
   } else {
     L.tx = px[i];
     L.ty = py[i];
@@ -225,7 +219,6 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
     L.te = pe[i];
   }
   L.ax = L.ay = L.az = L.ap = 0;
-  if constexpr (F32) L.cx = L.cy = L.cz = L.cp = 0.f;
 }
 
 // ---- staging (one lane per source) ----------------------------------------
@@ -272,6 +265,19 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
 }
 
 template <bool F32, typename W_t>
+__device__ __forceinline__ void stage_null_node(W_t& W, int i) {
+  if constexpr (F32) {
+    SrcC32 r{};
+    r.cx = r.cy = r.cz = (float)kNullPos;
+    reinterpret_cast<SrcC32*>(W.stage)[i] = r;
+  } else {
+    SrcC64 r{};
+    r.cx = r.cy = r.cz = kNullPos;
+    reinterpret_cast<SrcC64*>(W.stage)[i] = r;
+  }
+}
+
+template <bool F32, typename W_t>
 __device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
                                                const double* __restrict__ px,
                                                const double* __restrict__ py,
@@ -301,45 +307,66 @@ __device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
   }
 }
 
-// ---- chunk evaluation (all lanes) -------------------------------------------
+template <bool F32, typename W_t>
+__device__ __forceinline__ void stage_null_particle(W_t& W, int i) {
+  if constexpr (F32) {
+    SrcP32 r{};
+    r.x = r.y = r.z = (float)kNullPos;
+    reinterpret_cast<SrcP32*>(W.stage)[i] = r;
+  } else {
+    SrcP64 r{};
+    r.x = r.y = r.z = kNullPos;
+    reinterpret_cast<SrcP64*>(W.stage)[i] = r;
+  }
+}
+
+// Null sources for the slots [nsrc, S * iters) the evaluation will read.
+template <bool F32, bool NODE, typename W_t>
+__device__ __forceinline__ void pad_chunk(W_t& W, int nsrc, const Group& g) {
+  const int end = g.S * g.iters;
+  const int lane = __lane_id();
+  for (int k = nsrc + lane; k < end; k += PT_WAVE) {
+    if (NODE) stage_null_node<F32, W_t>(W, k);
+    else stage_null_particle<F32, W_t>(W, k);
+  }
+}
+
+// ---- chunk evaluation (all lanes, no bounds tests) -------------------------
 
 template <bool F32, typename W_t>
-__device__ __forceinline__ void eval_nodes(const W_t& W, int nsrc, const Group& g,
-                                           Lanes<F32>& L, double ox, double oy, double oz) {
+__device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F32>& L,
+                                           double ox, double oy, double oz) {
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
     const SrcC32* src = reinterpret_cast<const SrcC32*>(W.stage);
+#pragma unroll 2
     for (int k = 0; k < g.iters; k++) {
-      const int j = g.s + k * g.S;
-      if (g.act && j < nsrc) {
-        const SrcC32 s = src[j];
-        if (s.mono != 0.f) {
-          const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
-          const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (rsq != 0.f) {
-            const float ri = rsqrt32(rsq);
-            const float f = s.mass * ri * ri * ri;
-            px = fmaf(dx, f, px);
-            py = fmaf(dy, f, py);
-            pz = fmaf(dz, f, pz);
-          }
-        } else {
-          const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
-          const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-          const float dir = rsqrt32(rsq);
-          float a, bb, c, d;
-          splineq<float>(dir, rsq, s.soft + L.te, a, bb, c, d);
-          const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
-          const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
-          const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
-          const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
-          const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
-          const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * tr));
-          pp += fmaf(-s.mass, a, fmaf(-c, qir, bb * tr));
-          px += fmaf(-qir3, rx, c * qx);
-          py += fmaf(-qir3, ry, c * qy);
-          pz += fmaf(-qir3, rz, c * qz);
-        }
+      const SrcC32 s = src[g.s + k * g.S];
+      if (s.mono != 0.f) {
+        // addGravity: unsoftened monopole, no potential
+        const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
+        const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float ri = rsq != 0.f ? rsqrt32(rsq) : 0.f;
+        const float f = s.mass * (ri * ri * ri);
+        px = fmaf(dx, f, px);
+        py = fmaf(dy, f, py);
+        pz = fmaf(dz, f, pz);
+      } else {
+        const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
+        const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+        const float dir = rsqrt32(rsq);
+        float a, bb, c, d;
+        splineq<float>(dir, rsq, s.soft + L.te, a, bb, c, d);
+        const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
+        const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
+        const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
+        const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+        const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
+        const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * tr));
+        pp += fmaf(-s.mass, a, fmaf(-c, qir, bb * tr));
+        px += fmaf(-qir3, rx, c * qx);
+        py += fmaf(-qir3, ry, c * qy);
+        pz += fmaf(-qir3, rz, c * qz);
       }
     }
     kahan_add(L.ax, L.cx, px);
@@ -349,60 +376,67 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, int nsrc, const Group& 
   } else {
     const SrcC64* src = reinterpret_cast<const SrcC64*>(W.stage);
     for (int k = 0; k < g.iters; k++) {
-      const int j = g.s + k * g.S;
-      if (g.act && j < nsrc) {
-        const SrcC64 s = src[j];
-        if (s.mono != 0.0) {
-          // addGravity: unsoftened monopole, no potential
-          const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
-          const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-          if (rsq != 0) {
-            const double f = s.mass / (rsq * sqrt(rsq));
-            L.ax = fma(dx, f, L.ax);
-            L.ay = fma(dy, f, L.ay);
-            L.az = fma(dz, f, L.az);
-          }
-        } else {
-          const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
-          const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-          const double dir = 1.0 / sqrt(rsq);
-          double a, bb, c, d;
-          splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
-          const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
-          const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
-          const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
-          const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
-          const double tr = 0.5 * (s.qxx + s.qyy + s.qzz);
-          const double qir3 = fma(bb, s.mass, fma(d, qir, -c * tr));
-          L.ap += fma(-s.mass, a, fma(-c, qir, bb * tr));
-          L.ax += fma(-qir3, rx, c * qx);
-          L.ay += fma(-qir3, ry, c * qy);
-          L.az += fma(-qir3, rz, c * qz);
-        }
+      const SrcC64 s = src[g.s + k * g.S];
+      if (s.mono != 0.0) {
+        // addGravity: unsoftened monopole, no potential
+        const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
+        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+        double f = s.mass / (rsq * sqrt(rsq));
+        f = rsq != 0 ? f : 0.0;
+        L.ax = fma(dx, f, L.ax);
+        L.ay = fma(dy, f, L.ay);
+        L.az = fma(dz, f, L.az);
+      } else {
+        const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
+        const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
+        const double dir = 1.0 / sqrt(rsq);
+        double a, bb, c, d;
+        splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
+        const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
+        const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
+        const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
+        const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
+        const double tr = 0.5 * (s.qxx + s.qyy + s.qzz);
+        const double qir3 = fma(bb, s.mass, fma(d, qir, -c * tr));
+        L.ap += fma(-s.mass, a, fma(-c, qir, bb * tr));
+        L.ax += fma(-qir3, rx, c * qx);
+        L.ay += fma(-qir3, ry, c * qy);
+        L.az += fma(-qir3, rz, c * qz);
       }
     }
   }
 }
 
 template <bool F32, typename W_t>
-__device__ __forceinline__ void eval_particles(const W_t& W, int nsrc, const Group& g,
-                                               Lanes<F32>& L, double ox, double oy, double oz) {
+__device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lanes<F32>& L,
+                                               double ox, double oy, double oz) {
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f;
     const SrcP32* src = reinterpret_cast<const SrcP32*>(W.stage);
+#pragma unroll 2
     for (int k = 0; k < g.iters; k++) {
-      const int j = g.s + k * g.S;
-      if (g.act && j < nsrc) {
-        const SrcP32 s = src[j];
-        const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
-        const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (rsq != 0.f) {
-          const float f = spline_b32(rsq, s.e + L.te) * s.m;
-          px = fmaf(dx, f, px);
-          py = fmaf(dy, f, py);
-          pz = fmaf(dz, f, pz);
-        }
+      const SrcP32 s = src[g.s + k * g.S];
+      const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
+      const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float twoh = s.e + L.te;
+      const float rinv = rsq != 0.f ? rsqrt32(rsq) : 0.f;
+      const float r = rsq * rinv;
+      float b = rinv * rinv * rinv;
+      if (r < twoh) {
+        // GravityVisitor::SPLINE softened branches (rare)
+        const float dih = 2.0f * __builtin_amdgcn_rcpf(twoh);
+        const float u = r * dih;
+        const float dih3 = dih * dih * dih;
+        b = u < 1.0f
+                ? dih3 * (4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u)
+                : -1.0f / 15.0f * rinv * rinv * rinv +
+                      dih3 * (8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u - 1.0f / 6.0f * u * u * u);
+        b = rsq != 0.f ? b : 0.f;
       }
+      const float f = b * s.m;
+      px = fmaf(dx, f, px);
+      py = fmaf(dy, f, py);
+      pz = fmaf(dz, f, pz);
     }
     kahan_add(L.ax, L.cx, px);
     kahan_add(L.ay, L.cy, py);
@@ -410,18 +444,14 @@ __device__ __forceinline__ void eval_particles(const W_t& W, int nsrc, const Gro
   } else {
     const SrcP64* src = reinterpret_cast<const SrcP64*>(W.stage);
     for (int k = 0; k < g.iters; k++) {
-      const int j = g.s + k * g.S;
-      if (g.act && j < nsrc) {
-        const SrcP64 s = src[j];
-        const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
-        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (rsq != 0) {
-          const double f = spline_b64(rsq, s.e + L.te) * s.m;
-          L.ax = fma(dx, f, L.ax);
-          L.ay = fma(dy, f, L.ay);
-          L.az = fma(dz, f, L.az);
-        }
-      }
+      const SrcP64 s = src[g.s + k * g.S];
+      const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
+      const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+      double f = spline_b64(rsq, s.e + L.te) * s.m;
+      f = rsq != 0 ? f : 0.0;
+      L.ax = fma(dx, f, L.ax);
+      L.ay = fma(dy, f, L.ay);
+      L.az = fma(dz, f, L.az);
     }
   }
 }
@@ -454,9 +484,10 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
       const int64_t j = (int64_t)W.fl_beg[lo] + (slot - W.fl_exc[lo]);
       stage_particle<F32, W_t>(W, lane, j, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
     }
-    __builtin_amdgcn_wave_barrier();
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
-    eval_particles<F32, W_t>(W, nsrc, g, L, ox, oy, oz);
+    pad_chunk<F32, false, W_t>(W, nsrc, g);
+    __builtin_amdgcn_wave_barrier();
+    eval_particles<F32, W_t>(W, g, L, ox, oy, oz);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -467,8 +498,9 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
                                          const Bucket& b, double ox, double oy, double oz,
                                          double ccx, double ccy, double ccz) {
   if (__lane_id() < nsrc) stage_node<F32, W_t>(W, __lane_id(), id, gn, b, ox, oy, oz, ccx, ccy, ccz);
+  pad_chunk<F32, true, W_t>(W, nsrc, g);
   __builtin_amdgcn_wave_barrier();
-  eval_nodes<F32, W_t>(W, nsrc, g, L, ox, oy, oz);
+  eval_nodes<F32, W_t>(W, g, L, ox, oy, oz);
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -530,7 +562,7 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
                 double* __restrict__ az, double* __restrict__ pot) {
   using W_t = WaveLds<F32, 8>;
   __shared__ W_t lds[kWaves];
-  const int wave = threadIdx.x / PT_WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
   if (bi >= nb) return;
@@ -574,7 +606,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
   using W_t = WaveLds<F32, STACK>;
   __shared__ W_t lds[kWaves];
   __shared__ unsigned long long s_cnt[kWaves][4];
-  const int wave = threadIdx.x / PT_WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t blk = pt_xcd_block(blockIdx.x, gridDim.x);
   const int64_t bi_raw = blk * kWaves + wave;

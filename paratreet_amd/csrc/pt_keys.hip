@@ -53,25 +53,44 @@ bbox_partial_kernel(const double* __restrict__ x, const double* __restrict__ y,
 
 // Reduces the per-block boxes and applies remakeUniverse: a cube centred on
 // the bbox centre with half-side fEps * 0.5 * (largest extent).
-__global__ void universe_kernel(const double* __restrict__ part, int nparts,
-                                double* __restrict__ bbox_out, double* __restrict__ box) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kBoxThreads)
+universe_kernel(const double* __restrict__ part, int nparts, double* __restrict__ bbox_out,
+                double* __restrict__ box) {
+  __shared__ double s[6][kBoxThreads];
   double lo[3] = {INFINITY, INFINITY, INFINITY};
   double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nparts; b++) {
+  for (int b = threadIdx.x; b < nparts; b += kBoxThreads) {
     for (int k = 0; k < 3; k++) {
       const double a = part[b * 6 + k];
       const double c = part[b * 6 + 3 + k];
-      if (a < lo[k]) lo[k] = a;
-      if (c > hi[k]) hi[k] = c;
+      lo[k] = a < lo[k] ? a : lo[k];
+      hi[k] = c > hi[k] ? c : hi[k];
     }
   }
+  for (int k = 0; k < 3; k++) {
+    s[k][threadIdx.x] = lo[k];
+    s[3 + k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      for (int k = 0; k < 3; k++) {
+        const double a = s[k][threadIdx.x + w];
+        if (a < s[k][threadIdx.x]) s[k][threadIdx.x] = a;
+        const double c = s[3 + k][threadIdx.x + w];
+        if (c > s[3 + k][threadIdx.x]) s[3 + k][threadIdx.x] = c;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   double bs[3], bc[3];
   for (int k = 0; k < 3; k++) {
-    bbox_out[k] = lo[k];
-    bbox_out[3 + k] = hi[k];
-    bs[k] = hi[k] - lo[k];
-    bc[k] = (lo[k] + hi[k]) / 2.0;
+    const double l = s[k][0], h = s[3 + k][0];
+    bbox_out[k] = l;
+    bbox_out[3 + k] = h;
+    bs[k] = h - l;
+    bc[k] = (l + h) / 2.0;
   }
   double mx = (bs[0] > bs[1]) ? bs[0] : bs[1];
   mx = (mx > bs[2]) ? mx : bs[2];
@@ -125,7 +144,7 @@ int pt_launch_universe(const double* x, const double* y, const double* z, int64_
   if (nb > max_parts) nb = max_parts;
   if (nb < 1) nb = 1;
   bbox_partial_kernel<<<nb, kBoxThreads, 0, s>>>(x, y, z, n, scratch_parts);
-  universe_kernel<<<1, 64, 0, s>>>(scratch_parts, nb, bbox_out, box);
+  universe_kernel<<<1, kBoxThreads, 0, s>>>(scratch_parts, nb, bbox_out, box);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

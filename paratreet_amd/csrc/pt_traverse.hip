@@ -28,7 +28,7 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
                 int32_t* __restrict__ err) {
   __shared__ int32_t stack[kTravWaves][PT_WALK_STACK];
   __shared__ unsigned long long s_cnt[kTravWaves][4];
-  const int wave = threadIdx.x / PT_WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t blk = pt_xcd_block(blockIdx.x, gridDim.x);
   const int64_t bi_raw = blk * kTravWaves + wave;
@@ -87,12 +87,14 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
   }
 }
 
+// Sums the per-block counter partials: each block reduces a strided share and
+// adds it to `out` (zeroed beforehand) with four atomics.
 __global__ void __launch_bounds__(256)
 reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblocks,
                        unsigned long long* __restrict__ out) {
   __shared__ unsigned long long s[4][256];
   unsigned long long v[4] = {0, 0, 0, 0};
-  for (int64_t b = threadIdx.x; b < nblocks; b += 256)
+  for (int64_t b = blockIdx.x * 256ll + threadIdx.x; b < nblocks; b += (int64_t)gridDim.x * 256)
     for (int k = 0; k < 4; k++) v[k] += part[b * 4 + k];
   for (int k = 0; k < 4; k++) s[k][threadIdx.x] = v[k];
   __syncthreads();
@@ -101,7 +103,7 @@ reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblo
       for (int k = 0; k < 4; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < 4) atomicAdd(&out[threadIdx.x], s[threadIdx.x][0]);
 }
 
 }  // namespace
@@ -110,7 +112,11 @@ int64_t pt_traverse_blocks(int64_t nb) { return (nb + kTravWaves - 1) / kTravWav
 
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
                               unsigned long long* out, hipStream_t s) {
-  reduce_counters_kernel<<<1, 256, 0, s>>>(part, nblocks, out);
+  if (pt_launch_fill_u64(out, 4, 0ull, s) != 0) return -1;
+  int64_t grid = (nblocks + 255) / 256;
+  if (grid > 256) grid = 256;
+  if (grid < 1) grid = 1;
+  reduce_counters_kernel<<<(unsigned)grid, 256, 0, s>>>(part, nblocks, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
