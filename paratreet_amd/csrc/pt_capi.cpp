@@ -70,6 +70,7 @@ struct pt_ctx {
   DBuf<uint64_t> ord_b;
   DBuf<TreeNode> tnode;
   DBuf<GravNode> gn;
+  DBuf<GravNode32> gn32;
   DBuf<Bucket> bk;
   DBuf<unsigned long long> cpart;  // per-block counter partials
   bool lists_valid = false;
@@ -289,7 +290,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->nparent); release(c->nfirst); release(c->level_nodes);
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
-  release(c->tnode); release(c->gn); release(c->bk); release(c->cpart);
+  release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
   release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
@@ -491,6 +492,7 @@ pt_status pt_upward(pt_ctx* c) {
   PT_TRY_HIP(c, ensure(c->ord_b, nn));
   PT_TRY_HIP(c, ensure(c->tnode, nn));
   PT_TRY_HIP(c, ensure(c->gn, nn));
+  PT_TRY_HIP(c, ensure(c->gn32, nn));
   PT_TRY_HIP(c, ensure(c->bk, std::max<int64_t>(c->nb, 1)));
   PtTree tr = tree_of(c);
   PT_TRY_LAUNCH(c, pt_launch_moments_init(c->nd.p, nn, c->stream));
@@ -506,7 +508,7 @@ pt_status pt_upward(pt_ctx* c) {
     }
   }
   PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tnode.p, c->gn.p,
-                                  c->bk.p, c->stream));
+                                  c->gn32.p, c->bk.p, c->stream));
   c->stage = S_MOMENTS;
   c->lists_valid = false;
   return PT_OK;
@@ -583,7 +585,7 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
   {
     Timer t(c, PT_T_FORCE);
-    PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p, c->xs.p,
+    PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
                                         c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
                                         c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
                                         c->azs.p, c->pots.p, c->stream));
@@ -615,7 +617,7 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
     {
       Timer t(c, PT_T_FORCE);
       PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
-                                         c->gn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, gf,
+                                         c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, gf,
                                          c->offset, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
                                          c->cpart.p, c->derr.p, c->stream));
     }

@@ -38,10 +38,16 @@ struct __align__(64) TreeNode {
   int32_t pad[4];
 };
 // Packed source record for particle-cell interactions (FP64 path).
+// tr = 0.5 * (qxx + qyy + qzz) as GravityVisitor::node computes it.
 struct __align__(32) GravNode {
-  double cx, cy, cz, mass;
-  double soft, qxx, qxy, qxz;
-  double qyy, qyz, qzz, pad;
+  double cx, cy, cz, soft;
+  double mass, qxx, qxy, qxz;
+  double qyy, qyz, qzz, tr;
+};
+// FP32 copy of a node's moments for the compensated-FP32 interactions
+// (the centre stays FP64: it is shifted per target bucket).
+struct __align__(32) GravNode32 {
+  float mass, soft, qxx, qxy, qxz, qyy, qyz, qzz;
 };
 // Bucket (target leaf) record: particle range, box, centre of mass, softening.
 struct __align__(32) Bucket {

@@ -122,14 +122,16 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 // LDS source records
 // ---------------------------------------------------------------------------
 
+// Closed-node records: soft < 0 marks a node that openSoftening sends to
+// addGravity (unsoftened monopole); tr = 0.5 * trace(Q).
 struct SrcC64 {  // closed node, FP64
-  double cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, mono;
+  double cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, tr;
 };
 struct SrcP64 {  // source particle, FP64
   double x, y, z, m, e, pad;
 };
 struct SrcC32 {  // closed node, FP32, centre in the bucket frame
-  float cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, mono;
+  float cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, tr;
 };
 struct SrcP32 {  // source particle, FP32, bucket frame
   float x, y, z, m, e, pad0, pad1, pad2;
@@ -154,8 +156,7 @@ struct __align__(16) WaveLds {
   int32_t pcbuf[2 * PT_WAVE];
   int32_t ppbeg[2 * PT_WAVE];
   int32_t ppcnt[2 * PT_WAVE];
-  int32_t fl_beg[PT_WAVE];
-  int32_t fl_exc[PT_WAVE];
+  int32_t fl_mark[PT_WAVE];
 };
 
 // Per-lane target and accumulator state for one target group.
@@ -173,6 +174,47 @@ struct Lanes<true> {
   float ax, ay, az, ap;
   float cx, cy, cz, cp;  // Kahan compensations
 };
+
+// The target bucket in its own (box-centre) frame, FP32, for the quick
+// openSoftening rejection of the FP32 path.
+struct BucketF {
+  float lox, loy, loz, hix, hiy, hiz, cx, cy, cz, soft, ext;
+};
+
+__device__ __forceinline__ BucketF bucket_frame(const Bucket& b, double ccx, double ccy,
+                                                double ccz) {
+  BucketF f;
+  f.lox = (float)(b.lox - ccx);
+  f.loy = (float)(b.loy - ccy);
+  f.loz = (float)(b.loz - ccz);
+  f.hix = (float)(b.hix - ccx);
+  f.hiy = (float)(b.hiy - ccy);
+  f.hiz = (float)(b.hiz - ccz);
+  f.cx = (float)(b.cx - ccx);
+  f.cy = (float)(b.cy - ccy);
+  f.cz = (float)(b.cz - ccz);
+  f.soft = (float)b.soft;
+  f.ext = (f.hix - f.lox) + (f.hiy - f.loy) + (f.hiz - f.loz);
+  return f;
+}
+
+// True when the FP32 estimate proves openSoftening false: both distances
+// exceed their thresholds by a margin (1e-5 relative on the radius plus
+// 1e-6 of the coordinate magnitudes) far above the FP32 rounding of the
+// shifted coordinates.  Otherwise the exact FP64 test decides.
+__device__ __forceinline__ bool softening_far32(float sx, float sy, float sz, float ssoft,
+                                                const BucketF& f) {
+  const float eps = 1e-6f * (fabsf(sx) + fabsf(sy) + fabsf(sz) + f.ext);
+  const float rb = 2.0f * ssoft * 1.00001f + eps;
+  const float ra = (2.0f * ssoft + 2.0f * f.soft) * 1.00001f + eps;
+  const float dx = sx - f.cx, dy = sy - f.cy, dz = sz - f.cz;
+  const float da = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float ex = fmaxf(fmaxf(f.lox - sx, sx - f.hix), 0.f);
+  const float ey = fmaxf(fmaxf(f.loy - sy, sy - f.hiy), 0.f);
+  const float ez = fmaxf(fmaxf(f.loz - sz, sz - f.hiz), 0.f);
+  const float db = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+  return da > ra * ra && db > rb * rb;
+}
 
 struct Group {
   int ng;      // targets in this pass
@@ -225,41 +267,46 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
 
 template <bool F32, typename W_t>
 __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
-                                           const GravNode* __restrict__ gn, const Bucket& b,
-                                           double ox, double oy, double oz, double ccx,
-                                           double ccy, double ccz) {
-  const GravNode s = gn[id];
-  const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
-  const bool mono = open_softening(cx, cy, cz, s.soft, b);
+                                           const GravNode* __restrict__ gn,
+                                           const GravNode32* __restrict__ gn32, const Bucket& b,
+                                           const BucketF& bf, double ox, double oy, double oz,
+                                           double ccx, double ccy, double ccz) {
+  const double cx = gn[id].cx + ox, cy = gn[id].cy + oy, cz = gn[id].cz + oz;
+  const double soft = gn[id].soft;
   if constexpr (F32) {
+    const GravNode32 h = gn32[id];
     SrcC32 r;
     r.cx = (float)(cx - ccx);
     r.cy = (float)(cy - ccy);
     r.cz = (float)(cz - ccz);
-    r.mass = (float)s.mass;
-    r.soft = (float)s.soft;
-    r.qxx = (float)s.qxx;
-    r.qxy = (float)s.qxy;
-    r.qxz = (float)s.qxz;
-    r.qyy = (float)s.qyy;
-    r.qyz = (float)s.qyz;
-    r.qzz = (float)s.qzz;
-    r.mono = mono ? 1.f : 0.f;
+    const bool mono = softening_far32(r.cx, r.cy, r.cz, h.soft, bf)
+                          ? false
+                          : open_softening(cx, cy, cz, soft, b);
+    r.mass = h.mass;
+    r.soft = mono ? -1.f : h.soft;
+    r.qxx = h.qxx;
+    r.qxy = h.qxy;
+    r.qxz = h.qxz;
+    r.qyy = h.qyy;
+    r.qyz = h.qyz;
+    r.qzz = h.qzz;
+    r.tr = 0.5f * (h.qxx + h.qyy + h.qzz);
     reinterpret_cast<SrcC32*>(W.stage)[i] = r;
   } else {
+    const GravNode s = gn[id];
     SrcC64 r;
     r.cx = s.cx;
     r.cy = s.cy;
     r.cz = s.cz;
     r.mass = s.mass;
-    r.soft = s.soft;
+    r.soft = open_softening(cx, cy, cz, s.soft, b) ? -1.0 : s.soft;
     r.qxx = s.qxx;
     r.qxy = s.qxy;
     r.qxz = s.qxz;
     r.qyy = s.qyy;
     r.qyz = s.qyz;
     r.qzz = s.qzz;
-    r.mono = mono ? 1.0 : 0.0;
+    r.tr = s.tr;
     reinterpret_cast<SrcC64*>(W.stage)[i] = r;
   }
 }
@@ -342,11 +389,11 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
 #pragma unroll 2
     for (int k = 0; k < g.iters; k++) {
       const SrcC32 s = src[g.s + k * g.S];
-      if (s.mono != 0.f) {
+      if (s.soft < 0.f) {
         // addGravity: unsoftened monopole, no potential
         const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
         const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ri = rsq != 0.f ? rsqrt32(rsq) : 0.f;
+        const float ri = fminf(rsqrt32(rsq), 1e10f);
         const float f = s.mass * (ri * ri * ri);
         px = fmaf(dx, f, px);
         py = fmaf(dy, f, py);
@@ -355,18 +402,31 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
         const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
         const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
         const float dir = rsqrt32(rsq);
+        const float twoh = s.soft + L.te;
         float a, bb, c, d;
-        splineq<float>(dir, rsq, s.soft + L.te, a, bb, c, d);
+        if (rsq < twoh * twoh) {
+          splineq<float>(dir, rsq, twoh, a, bb, c, d);
+        } else {
+          const float dir2 = dir * dir;
+          a = dir;
+          bb = dir * dir2;
+          c = 3.0f * bb * dir2;
+          d = 5.0f * c * dir2;
+        }
         const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
         const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
         const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
         const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
-        const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
-        const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * tr));
-        pp += fmaf(-s.mass, a, fmaf(-c, qir, bb * tr));
-        px += fmaf(-qir3, rx, c * qx);
-        py += fmaf(-qir3, ry, c * qy);
-        pz += fmaf(-qir3, rz, c * qz);
+        const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
+        pp = fmaf(-s.mass, a, pp);
+        pp = fmaf(-c, qir, pp);
+        pp = fmaf(bb, s.tr, pp);
+        px = fmaf(-qir3, rx, px);
+        px = fmaf(c, qx, px);
+        py = fmaf(-qir3, ry, py);
+        py = fmaf(c, qy, py);
+        pz = fmaf(-qir3, rz, pz);
+        pz = fmaf(c, qz, pz);
       }
     }
     kahan_add(L.ax, L.cx, px);
@@ -377,7 +437,7 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
     const SrcC64* src = reinterpret_cast<const SrcC64*>(W.stage);
     for (int k = 0; k < g.iters; k++) {
       const SrcC64 s = src[g.s + k * g.S];
-      if (s.mono != 0.0) {
+      if (s.soft < 0.0) {
         // addGravity: unsoftened monopole, no potential
         const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
         const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -396,9 +456,8 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
         const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
         const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
         const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
-        const double tr = 0.5 * (s.qxx + s.qyy + s.qzz);
-        const double qir3 = fma(bb, s.mass, fma(d, qir, -c * tr));
-        L.ap += fma(-s.mass, a, fma(-c, qir, bb * tr));
+        const double qir3 = fma(bb, s.mass, fma(d, qir, -c * s.tr));
+        L.ap += fma(-s.mass, a, fma(-c, qir, bb * s.tr));
         L.ax += fma(-qir3, rx, c * qx);
         L.ay += fma(-qir3, ry, c * qy);
         L.az += fma(-qir3, rz, c * qz);
@@ -419,11 +478,13 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
       const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       const float twoh = s.e + L.te;
-      const float rinv = rsq != 0.f ? rsqrt32(rsq) : 0.f;
-      const float r = rsq * rinv;
+      // rsq == 0 (the target itself): the clamp keeps b finite, and dx = dy =
+      // dz = 0 then contributes exactly zero
+      const float rinv = fminf(rsqrt32(rsq), 1e10f);
       float b = rinv * rinv * rinv;
-      if (r < twoh) {
-        // GravityVisitor::SPLINE softened branches (rare)
+      if (rsq < twoh * twoh) {
+        // GravityVisitor::SPLINE softened branches
+        const float r = rsq * rinv;
         const float dih = 2.0f * __builtin_amdgcn_rcpf(twoh);
         const float u = r * dih;
         const float dih3 = dih * dih * dih;
@@ -431,7 +492,6 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
                 ? dih3 * (4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u)
                 : -1.0f / 15.0f * rinv * rinv * rinv +
                       dih3 * (8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u - 1.0f / 6.0f * u * u * u);
-        b = rsq != 0.f ? b : 0.f;
       }
       const float f = b * s.m;
       px = fmaf(dx, f, px);
@@ -457,7 +517,9 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
 }
 
 // Flattens up to 64 leaf sources (begin/count in the calling lane, count 0 =
-// none) into particles and evaluates them 64 at a time.
+// none) into particles and evaluates them 64 at a time.  Each window marks
+// the slot where every leaf starts and an inclusive max-scan over the 64
+// slots gives each slot its leaf.
 template <bool F32, typename W_t>
 __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, const Group& g,
                                           Lanes<F32>& L, const double* __restrict__ px,
@@ -469,19 +531,26 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
   const int lane = __lane_id();
   const int incl = pt_wave_incl_scan(cnt);
   const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
-  W.fl_beg[lane] = beg;
-  W.fl_exc[lane] = incl - cnt;
-  __builtin_amdgcn_wave_barrier();
+  const int exc = incl - cnt;
+  int carry = 0;
   for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+    W.fl_mark[lane] = -1;
+    __builtin_amdgcn_wave_barrier();
+    if (cnt > 0 && exc >= s0 && exc < s0 + PT_WAVE) W.fl_mark[exc - s0] = lane;
+    __builtin_amdgcn_wave_barrier();
+    int leaf = W.fl_mark[lane];
+#pragma unroll
+    for (int d = 1; d < PT_WAVE; d <<= 1) {
+      const int o = __shfl_up(leaf, d, PT_WAVE);
+      if (lane >= d) leaf = leaf > o ? leaf : o;
+    }
+    leaf = leaf > carry ? leaf : carry;
+    carry = __shfl(leaf, PT_WAVE - 1, PT_WAVE);
+    const int lbeg = __shfl(beg, leaf, PT_WAVE);
+    const int lexc = __shfl(exc, leaf, PT_WAVE);
     const int slot = s0 + lane;
     if (slot < total) {
-      int lo = 0, hi = PT_WAVE - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (W.fl_exc[mid] <= slot) lo = mid;
-        else hi = mid - 1;
-      }
-      const int64_t j = (int64_t)W.fl_beg[lo] + (slot - W.fl_exc[lo]);
+      const int64_t j = (int64_t)lbeg + (slot - lexc);
       stage_particle<F32, W_t>(W, lane, j, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
     }
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
@@ -495,9 +564,11 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
 template <bool F32, typename W_t>
 __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Group& g,
                                          Lanes<F32>& L, const GravNode* __restrict__ gn,
-                                         const Bucket& b, double ox, double oy, double oz,
+                                         const GravNode32* __restrict__ gn32, const Bucket& b,
+                                         const BucketF& bf, double ox, double oy, double oz,
                                          double ccx, double ccy, double ccz) {
-  if (__lane_id() < nsrc) stage_node<F32, W_t>(W, __lane_id(), id, gn, b, ox, oy, oz, ccx, ccy, ccz);
+  if (__lane_id() < nsrc)
+    stage_node<F32, W_t>(W, __lane_id(), id, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
   pad_chunk<F32, true, W_t>(W, nsrc, g);
   __builtin_amdgcn_wave_barrier();
   eval_nodes<F32, W_t>(W, g, L, ox, oy, oz);
@@ -553,7 +624,8 @@ __device__ __forceinline__ void finish_group(W_t& W, const Group& g, const Lanes
 template <bool F32>
 __global__ void __launch_bounds__(kThreads)
 interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
-                const GravNode* __restrict__ gn, const double* __restrict__ px,
+                const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+                const double* __restrict__ px,
                 const double* __restrict__ py, const double* __restrict__ pz,
                 const double* __restrict__ pm, const double* __restrict__ pe,
                 const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
@@ -570,6 +642,7 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
   const Bucket b = bk[bi];
   const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
                ccz = (b.loz + b.hiz) * 0.5;
+  const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
   const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
   const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
   for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
@@ -579,7 +652,7 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
       const int nsrc = (pc1 - e0) < PT_WAVE ? (int)(pc1 - e0) : PT_WAVE;
       const int32_t id = lane < nsrc ? pc[e0 + lane] : 0;
-      do_nodes<F32, W_t>(W, id, nsrc, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+      do_nodes<F32, W_t>(W, id, nsrc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
     }
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       int32_t beg = 0, cnt = 0;
@@ -597,7 +670,8 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
 template <bool F32, int STACK>
 __global__ void __launch_bounds__(kThreads)
 gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
-               const GravNode* __restrict__ gn, const double* __restrict__ px,
+               const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+               const double* __restrict__ px,
                const double* __restrict__ py, const double* __restrict__ pz,
                const double* __restrict__ pm, const double* __restrict__ pe, double gf,
                double ox, double oy, double oz, double* __restrict__ ax,
@@ -617,15 +691,19 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
   const int64_t nt = b.count;
   const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
                ccz = (b.loz + b.hiz) * 0.5;
-  WalkState total{0, 0, 0, 0, 0, false};
+  const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
+  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
+  bool overflow = false;
   for (int g0 = 0; live && g0 < b.count; g0 += kGroupMax) {
     const Group g = make_group(b, g0);
+    const bool count = g0 == 0;  // the walk repeats per target group; count once
     Lanes<F32> L;
     load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
-    WalkState ws{0, 0, 0, 0, 0, false};
+    int sp = 0;
     int npc = 0, npp = 0;
+    WalkCounts wc;
     {
-      const WalkLane r = pt_walk_root(nodes, b, nt, gf, ox, oy, oz, W.stack, ws);
+      const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, W.stack, sp, wc);
       if (r.is_closed) {
         if (lane == 0) W.pcbuf[0] = 0;
         npc = 1;
@@ -636,12 +714,24 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
         }
         npp = 1;
       }
+      if (count) {
+        c_open += wc.n_open;
+        c_close += wc.n_close;
+        c_node += (unsigned long long)wc.n_close * nt;
+        c_part += (unsigned long long)wc.leaf_parts * nt;
+      }
     }
     __builtin_amdgcn_wave_barrier();
-    while (ws.sp > 0) {
+    while (sp > 0) {
       unsigned long long leafm, closem;
-      const WalkLane r =
-          pt_walk_step<STACK>(nodes, b, nt, gf, ox, oy, oz, W.stack, ws, leafm, closem);
+      const WalkLane r = pt_walk_step<STACK>(nodes, b, gf, ox, oy, oz, W.stack, sp, overflow,
+                                             wc, leafm, closem);
+      if (count) {
+        c_open += wc.n_open;
+        c_close += wc.n_close;
+        c_node += (unsigned long long)wc.n_close * nt;
+        c_part += (unsigned long long)wc.leaf_parts * nt;
+      }
       if (r.is_closed) W.pcbuf[npc + pt_rank_in(closem)] = r.node;
       if (r.is_leaf) {
         const int k = npp + pt_rank_in(leafm);
@@ -652,7 +742,8 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
       npp += __popcll(leafm);
       __builtin_amdgcn_wave_barrier();
       if (npc >= PT_WAVE) {
-        do_nodes<F32, W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+        do_nodes<F32, W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy,
+                           ccz);
         const int rest = npc - PT_WAVE;
         const int32_t v = lane < rest ? W.pcbuf[PT_WAVE + lane] : 0;
         __builtin_amdgcn_wave_barrier();
@@ -662,7 +753,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
       }
       if (npp >= PT_WAVE) {
         do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, ox, oy, oz,
-                       ccx, ccy, ccz);
+                            ccx, ccy, ccz);
         const int rest = npp - PT_WAVE;
         const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
         const int32_t vc = lane < rest ? W.ppcnt[PT_WAVE + lane] : 0;
@@ -677,7 +768,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
     }
     if (npc > 0) {
       const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
-      do_nodes<F32, W_t>(W, id, npc, g, L, gn, b, ox, oy, oz, ccx, ccy, ccz);
+      do_nodes<F32, W_t>(W, id, npc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
     }
     if (npp > 0) {
       const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
@@ -685,15 +776,13 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
       do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
     }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
-    if (g0 == 0) total = ws;
-    else total.overflow |= ws.overflow;
   }
   if (lane == 0) {
-    if (live && total.overflow) atomicExch(err, 1);
-    s_cnt[wave][0] = live ? total.c_node : 0;
-    s_cnt[wave][1] = live ? total.c_part : 0;
-    s_cnt[wave][2] = live ? total.c_open : 0;
-    s_cnt[wave][3] = live ? total.c_close : 0;
+    if (live && overflow) atomicExch(err, 1);
+    s_cnt[wave][0] = c_node;
+    s_cnt[wave][1] = c_part;
+    s_cnt[wave][2] = c_open;
+    s_cnt[wave][3] = c_close;
   }
   __syncthreads();
   if (threadIdx.x < 4) {
@@ -710,18 +799,18 @@ inline unsigned grid_for(int64_t nb) { return (unsigned)((nb + kWaves - 1) / kWa
 int64_t pt_gravity_blocks(int64_t nb) { return (nb + kWaves - 1) / kWaves; }
 
 int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                       const GravNode* gn, const double* x, const double* y, const double* z,
+                       const GravNode* gn, const GravNode32* gn32, const double* x, const double* y, const double* z,
                        const double* m, const double* e, const int64_t* pp_off, const int32_t* pp,
                        const int64_t* pc_off, const int32_t* pc, const double off[3], double* ax,
                        double* ay, double* az, double* pot, hipStream_t s) {
   if (nb <= 0) return 0;
   if (precision == 0) {
-    interact_kernel<false><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, x, y, z, m, e,
-                                                             pp_off, pp, pc_off, pc, off[0],
+    interact_kernel<false><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, gn32, x, y, z, m,
+                                                             e, pp_off, pp, pc_off, pc, off[0],
                                                              off[1], off[2], ax, ay, az, pot);
   } else {
-    interact_kernel<true><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, x, y, z, m, e,
-                                                            pp_off, pp, pc_off, pc, off[0], off[1],
+    interact_kernel<true><<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, gn, gn32, x, y, z, m,
+                                                            e, pp_off, pp, pc_off, pc, off[0], off[1],
                                                             off[2], ax, ay, az, pot);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -730,14 +819,15 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
 // big_stack selects the 1536-entry walk stack (used after a 512-entry walk
 // reported an overflow; 1536 keeps the FP64 work-group under 64 KiB of LDS).
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
-                      const TreeNode* nodes, const GravNode* gn, const double* x,
-                      const double* y, const double* z, const double* m, const double* e,
-                      double gf, const double off[3], double* ax, double* ay, double* az,
+                      const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
+                      const double* x, const double* y, const double* z, const double* m,
+                      const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
                       double* pot, unsigned long long* partials, int32_t* err, hipStream_t s) {
   if (nb <= 0) return 0;
 #define PT_GRAVITY_LAUNCH(F32, STACK)                                                          \
   gravity_kernel<F32, STACK><<<grid_for(nb), kThreads, 0, s>>>(                                \
-      bk, nb, nodes, gn, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot, partials, \
+      bk, nb, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot,      \
+      partials,                                                                                  \
       err)
   if (precision == 0) {
     if (big_stack) PT_GRAVITY_LAUNCH(false, 1536);

@@ -51,7 +51,8 @@ int pt_launch_leaf_moments(const PtTree& t, const int32_t* bucket_node, int64_t 
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
                            int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
 int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
-                   int64_t nb, TreeNode* tnode, GravNode* gn, Bucket* bk, hipStream_t s);
+                   int64_t nb, TreeNode* tnode, GravNode* gn, GravNode32* gn32, Bucket* bk,
+                   hipStream_t s);
 
 // List-producing traversal (count pass when fill is false).  partials holds
 // pt_traverse_blocks(nb) x 4 counters, reduced by pt_launch_reduce_counters.
@@ -65,7 +66,7 @@ int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
 
 // Interactions over the lists (precision 0 = FP64, 1 = FP32C).
 int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                       const GravNode* gn, const double* x, const double* y, const double* z,
+                       const GravNode* gn, const GravNode32* gn32, const double* x, const double* y, const double* z,
                        const double* m, const double* e, const int64_t* pp_off, const int32_t* pp,
                        const int64_t* pc_off, const int32_t* pc, const double off[3], double* ax,
                        double* ay, double* az, double* pot, hipStream_t s);
@@ -73,9 +74,9 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
 // Fused walk + interactions; partials holds pt_gravity_blocks(nb) x 4.
 int64_t pt_gravity_blocks(int64_t nb);
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
-                      const TreeNode* nodes, const GravNode* gn, const double* x,
-                      const double* y, const double* z, const double* m, const double* e,
-                      double gf, const double off[3], double* ax, double* ay, double* az,
+                      const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
+                      const double* x, const double* y, const double* z, const double* m,
+                      const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
                       double* pot, unsigned long long* partials, int32_t* err, hipStream_t s);
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const double* x, const double* y,

@@ -288,7 +288,8 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
 // Packs the walk and interaction records from the SoA node table.
 __global__ void __launch_bounds__(256)
 pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
-                  TreeNode* __restrict__ tnode, GravNode* __restrict__ gn) {
+                  TreeNode* __restrict__ tnode, GravNode* __restrict__ gn,
+                  GravNode32* __restrict__ gn32) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= nn) return;
   TreeNode a;
@@ -314,8 +315,18 @@ pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
   g.qyy = nd[(PT_CD_Q + 3) * nn + i];
   g.qyz = nd[(PT_CD_Q + 4) * nn + i];
   g.qzz = nd[(PT_CD_Q + 5) * nn + i];
-  g.pad = 0.0;
+  g.tr = 0.5 * (g.qxx + g.qyy + g.qzz);
   gn[i] = g;
+  GravNode32 h;
+  h.mass = (float)g.mass;
+  h.soft = (float)g.soft;
+  h.qxx = (float)g.qxx;
+  h.qxy = (float)g.qxy;
+  h.qxz = (float)g.qxz;
+  h.qyy = (float)g.qyy;
+  h.qyz = (float)g.qyz;
+  h.qzz = (float)g.qzz;
+  gn32[i] = h;
 }
 
 __global__ void __launch_bounds__(256)
@@ -371,8 +382,9 @@ int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t 
 }
 
 int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
-                   int64_t nb, TreeNode* tnode, GravNode* gn, Bucket* bk, hipStream_t s) {
-  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tnode, gn);
+                   int64_t nb, TreeNode* tnode, GravNode* gn, GravNode32* gn32, Bucket* bk,
+                   hipStream_t s) {
+  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tnode, gn, gn32);
   if (nb > 0) pack_buckets_kernel<<<grid_for(nb), 256, 0, s>>>(t, nd, nn, bucket_node, nb, bk);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

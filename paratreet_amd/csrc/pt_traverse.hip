@@ -38,12 +38,15 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
   const int64_t nt = b.count;
   int32_t* st = stack[wave];
 
-  WalkState ws{0, 0, 0, 0, 0, false};
+  int sp = 0;
+  bool overflow = false;
+  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
   int64_t npp = 0, npc = 0;
   int32_t* ppw = FILL ? pp + pp_off[bi] : nullptr;
   int32_t* pcw = FILL ? pc + pc_off[bi] : nullptr;
   if (live) {
-    const WalkLane r = pt_walk_root(nodes, b, nt, gf, ox, oy, oz, st, ws);
+    WalkCounts wc;
+    const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, st, sp, wc);
     if (r.is_leaf) {
       if (FILL && lane == 0) ppw[0] = 0;
       npp = 1;
@@ -51,29 +54,38 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
       if (FILL && lane == 0) pcw[0] = 0;
       npc = 1;
     }
-    while (ws.sp > 0) {
+    c_open += wc.n_open;
+    c_close += wc.n_close;
+    c_node += (unsigned long long)wc.n_close * nt;
+    c_part += (unsigned long long)wc.leaf_parts * nt;
+    while (sp > 0) {
       unsigned long long leafm, closem;
-      const WalkLane r2 = pt_walk_step<PT_WALK_STACK>(nodes, b, nt, gf, ox, oy, oz, st, ws, leafm, closem);
+      const WalkLane r2 = pt_walk_step<PT_WALK_STACK>(nodes, b, gf, ox, oy, oz, st, sp, overflow,
+                                                      wc, leafm, closem);
       if (FILL) {
         if (r2.is_leaf) ppw[npp + pt_rank_in(leafm)] = r2.node;
         if (r2.is_closed) pcw[npc + pt_rank_in(closem)] = r2.node;
       }
       npp += __popcll(leafm);
       npc += __popcll(closem);
+      c_open += wc.n_open;
+      c_close += wc.n_close;
+      c_node += (unsigned long long)wc.n_close * nt;
+      c_part += (unsigned long long)wc.leaf_parts * nt;
     }
   }
   if (lane == 0) {
     if (live) {
-      if (ws.overflow) atomicExch(err, 1);
+      if (overflow) atomicExch(err, 1);
       if (!FILL) {
         pp_cnt[bi] = (int32_t)npp;
         pc_cnt[bi] = (int32_t)npc;
       }
     }
-    s_cnt[wave][0] = live ? ws.c_node : 0;
-    s_cnt[wave][1] = live ? ws.c_part : 0;
-    s_cnt[wave][2] = live ? ws.c_open : 0;
-    s_cnt[wave][3] = live ? ws.c_close : 0;
+    s_cnt[wave][0] = c_node;
+    s_cnt[wave][1] = c_part;
+    s_cnt[wave][2] = c_open;
+    s_cnt[wave][3] = c_close;
   }
   if (!FILL) {
     // one partial per block (a global atomic per wavefront would serialise

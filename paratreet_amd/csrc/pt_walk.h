@@ -65,35 +65,36 @@ struct WalkLane {
   bool is_closed;    // -> node() (P-C)
 };
 
-// Per-wave walk state (wave-uniform scalars).
-struct WalkState {
-  int sp;                      // stack size
-  unsigned long long c_open, c_close, c_node, c_part;
-  bool overflow;
+// Wave-uniform per-step counts for the interaction counters.
+struct WalkCounts {
+  int n_open;     // open() returned true
+  int n_close;    // open() returned false (node() calls)
+  int leaf_parts; // sum of source-leaf particle counts (leaf() calls)
 };
 
-// Visits the root (node 0).  Returns the lane-0 style result for the root:
-// is_leaf / is_closed are wave-uniform here; lane 0 owns the append.
+// Visits the root (node 0): pushes its first child when it is opened.  sp and
+// the counts are wave-uniform; lane 0 owns the stack write.
 __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ nodes,
-                                                 const Bucket& b, int64_t nt, double gf,
-                                                 double ox, double oy, double oz,
-                                                 int32_t* __restrict__ stack, WalkState& ws) {
+                                                 const Bucket& b, double gf, double ox,
+                                                 double oy, double oz,
+                                                 int32_t* __restrict__ stack, int& sp,
+                                                 WalkCounts& wc) {
   WalkLane r{0, 0, 0, false, false};
+  wc = WalkCounts{0, 0, 0};
   const TreeNode root = nodes[0];
   r.count = root.count;
   r.begin = root.begin;
   if (root.type == PT_NODE_LEAF) {
     r.is_leaf = true;
-    ws.c_part += (unsigned long long)root.count * (unsigned long long)nt;
+    wc.leaf_parts = root.count;
   } else if (root.type == PT_NODE_INTERNAL) {
     if (pt_gv_open(root, b, gf, ox, oy, oz)) {
       if (__lane_id() == 0) stack[0] = root.first_child;
-      ws.sp = 1;
-      ws.c_open++;
+      sp = 1;
+      wc.n_open = 1;
     } else {
       r.is_closed = true;
-      ws.c_close++;
-      ws.c_node += (unsigned long long)nt;
+      wc.n_close = 1;
     }
   }
   return r;
@@ -101,25 +102,26 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
 
 // One step: pops up to 8 opened nodes and tests their 64 children.  Returns
 // this lane's child and its classification; pushes opened children.  A push
-// beyond STACK entries sets ws.overflow and ends the walk (the caller reports
+// beyond STACK entries sets overflow and ends the walk (the caller reports
 // it; nothing is silently dropped).
 template <int STACK>
 __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ nodes,
-                                                 const Bucket& b, int64_t nt, double gf,
-                                                 double ox, double oy, double oz,
-                                                 int32_t* __restrict__ stack, WalkState& ws,
+                                                 const Bucket& b, double gf, double ox,
+                                                 double oy, double oz,
+                                                 int32_t* __restrict__ stack, int& sp,
+                                                 bool& overflow, WalkCounts& wc,
                                                  unsigned long long& leafm,
                                                  unsigned long long& closem) {
   const int lane = __lane_id();
-  const int k = ws.sp < 8 ? ws.sp : 8;
-  ws.sp -= k;
+  const int k = sp < 8 ? sp : 8;
+  sp -= k;
   const int slot = lane >> 3;
   const bool active = slot < k;
   WalkLane r{-1, 0, 0, false, false};
   bool op = false;
   int32_t fc = -1;
   if (active) {
-    r.node = stack[ws.sp + slot] + (lane & 7);
+    r.node = stack[sp + slot] + (lane & 7);
     const TreeNode nd = nodes[r.node];
     r.count = nd.count;
     r.begin = nd.begin;
@@ -135,25 +137,24 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   closem = __ballot(r.is_closed);
   const unsigned long long openm = __ballot(op);
   const int n_open = __popcll(openm);
-  if (ws.sp + n_open > STACK) {
-    ws.overflow = true;
-    ws.sp = 0;
+  if (sp + n_open > STACK) {
+    overflow = true;
+    sp = 0;
     leafm = closem = 0;
     r.is_leaf = r.is_closed = false;
+    wc = WalkCounts{0, 0, 0};
     return r;
   }
   __builtin_amdgcn_wave_barrier();
-  if (op) stack[ws.sp + pt_rank_in(openm)] = fc;
+  if (op) stack[sp + pt_rank_in(openm)] = fc;
   __builtin_amdgcn_wave_barrier();
-  ws.sp += n_open;
-  ws.c_open += n_open;
-  const int nc = __popcll(closem);
-  ws.c_close += nc;
-  ws.c_node += (unsigned long long)nc * (unsigned long long)nt;
+  sp += n_open;
   int lc = r.is_leaf ? r.count : 0;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) lc += __shfl_xor(lc, d, PT_WAVE);
-  ws.c_part += (unsigned long long)lc * (unsigned long long)nt;
+  wc.n_open = n_open;
+  wc.n_close = __popcll(closem);
+  wc.leaf_parts = __builtin_amdgcn_readfirstlane(lc);
   return r;
 }
 
