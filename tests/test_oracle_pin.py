@@ -1,0 +1,314 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Each test fixes a part of the oracle against something independent of it:
+the reference's own fixture (tests/gravity/direct.acc), closed forms, brute
+force on small inputs, or invariants a wrong index/sign/term would break.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paratreet_amd import ics
+from tests.helpers import left_aligned
+
+
+def _lambs():
+    d = np.load("tests/golden/lambs_30k.npz")
+    return (d["pos"].astype(np.float64), d["mass"].astype(np.float64),
+            d["soft"].astype(np.float64), d["direct_acc"])
+
+
+# ---------------------------------------------------------------- leaf/SPLINE
+
+def test_direct_sum_reproduces_reference_fixture():
+    """or_direct (leaf() kernel + SPLINE, 2h = eps_i + eps_j) against the
+    reference's direct.acc for the lambs 30K fixture (tests/gravity/)."""
+    pos, m, e, ref = _lambs()
+    rng = np.random.default_rng(3)
+    t = rng.choice(len(pos), 600, replace=False)
+    acc = O.direct(pos[:, 0], pos[:, 1], pos[:, 2], m, e, t)
+    rel = np.linalg.norm(acc - ref[t], axis=1) / np.linalg.norm(ref[t], axis=1)
+    # the fixture was written from float32 positions with '%.14g'
+    assert np.sqrt(np.mean(rel ** 2)) < 3e-8
+    assert rel.max() < 3e-7
+
+
+def test_plummer_softening_does_not_reproduce_fixture():
+    """Negative control: a Plummer-softened kernel misses direct.acc by ~10%,
+    so the fixture really distinguishes the spline kernel."""
+    pos, m, e, ref = _lambs()
+    t = np.arange(0, 30000, 500)
+    acc = np.zeros((len(t), 3))
+    for k, i in enumerate(t):
+        d = pos - pos[i]
+        r2 = (d * d).sum(1) + (e + e[i]) ** 2 / 4
+        w = m / r2 ** 1.5
+        w[i] = 0
+        acc[k] = (d * w[:, None]).sum(0)
+    rel = np.linalg.norm(acc - ref[t], axis=1) / np.linalg.norm(ref[t], axis=1)
+    assert np.sqrt(np.mean(rel ** 2)) > 1e-3
+
+
+@pytest.mark.parametrize("twoh", [0.02, 0.5, 3.0])
+def test_spline_force_is_minus_gradient_of_potential(twoh):
+    """SPLINE returns potential term a(r) and force term b(r) of the cubic
+    spline; b = -(1/r) da/dr at every radius (central differences)."""
+    for r in np.concatenate([np.linspace(0.05, 2.5, 60) * twoh]):
+        h = 1e-6 * twoh
+        ap, _ = O.spline((r + h) ** 2, twoh)
+        am, _ = O.spline((r - h) ** 2, twoh)
+        _, b = O.spline(r * r, twoh)
+        dadr = (ap - am) / (2 * h)
+        assert abs(-dadr / r - b) <= 1e-6 * max(abs(b), 1e-12), (r, twoh)
+
+
+@pytest.mark.parametrize("twoh", [0.01, 1.0])
+def test_spline_continuity_and_far_field(twoh):
+    for u in (1.0, 2.0):  # branch boundaries r = twoh/2, r = twoh
+        r = u * twoh / 2
+        a0, b0 = O.spline((r * (1 - 1e-12)) ** 2, twoh)
+        a1, b1 = O.spline((r * (1 + 1e-12)) ** 2, twoh)
+        assert abs(a0 - a1) <= 1e-9 * abs(a0) and abs(b0 - b1) <= 1e-9 * abs(b0)
+    r = 1.7 * twoh
+    a, b = O.spline(r * r, twoh)
+    assert a == 1.0 / r and b == (1.0 / r) ** 3
+
+
+def test_splineq_far_field_closed_form():
+    """Outside the softening SPLINEQ returns 1/r, 1/r^3, 3/r^5, 15/r^7."""
+    for r in (0.3, 1.0, 7.5):
+        a, b, c, d = O.splineq(1.0 / r, r * r, 0.01)
+        assert np.allclose([a, b, c, d], [1 / r, r ** -3, 3 * r ** -5, 15 * r ** -7], rtol=1e-14)
+
+
+# ---------------------------------------------------------------- keys
+
+def _keys_numpy(pos, box):
+    """Independent Morton keys: per-axis floor quantisation, bitwise loop."""
+    q = np.empty((len(pos), 3), dtype=np.uint64)
+    for k in range(3):
+        u = (pos[:, k] - box[k]) / (box[3 + k] - box[k])
+        q[:, k] = np.clip(np.floor(u * 2.0 ** 21), 0, 2 ** 21 - 1).astype(np.uint64)
+    key = np.zeros(len(pos), dtype=np.uint64)
+    for bit in range(20, -1, -1):
+        for k in range(3):
+            key = (key << np.uint64(1)) | ((q[:, k] >> np.uint64(bit)) & np.uint64(1))
+    return key | (np.uint64(1) << np.uint64(63))
+
+
+def test_keys_match_independent_interleave():
+    rng = np.random.default_rng(5)
+    pos = rng.standard_normal((5000, 3)) * [1.0, 3.0, 0.2]
+    box = O.universe(pos[:, 0], pos[:, 1], pos[:, 2])
+    assert np.array_equal(O.keys(pos[:, 0], pos[:, 1], pos[:, 2], box), _keys_numpy(pos, box))
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_keys_brute_force_cells(level):
+    """Every cell centre of a 2^level grid gets the key whose top 3*level bits
+    interleave the cell indices (x most significant)."""
+    n = 2 ** level
+    box = np.array([0.0, 0.0, 0.0, 1.0, 1.0, 1.0])
+    idx = np.array([(i, j, k) for i in range(n) for j in range(n) for k in range(n)])
+    pos = (idx + 0.5) / n
+    keys = O.keys(pos[:, 0], pos[:, 1], pos[:, 2], box)
+    for (i, j, k), key in zip(idx, keys):
+        expect = 1
+        for b in range(level - 1, -1, -1):
+            expect = (expect << 3) | (((i >> b) & 1) << 2) | (((j >> b) & 1) << 1) | ((k >> b) & 1)
+        assert int(key) >> (63 - 3 * level) == expect
+
+
+def test_universe_is_slopped_cube():
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(-2, 5, (1000, 3)) * [1, 0.5, 0.25]
+    box = O.universe(pos[:, 0], pos[:, 1], pos[:, 2])
+    ext = box[3:] - box[:3]
+    assert np.allclose(ext, ext[0], rtol=1e-15)
+    assert np.isclose(ext[0], (pos.max(0) - pos.min(0)).max() * (1 + 1.91e-6), rtol=1e-15)
+    assert np.all(box[:3] < pos.min(0)) and np.all(box[3:] > pos.max(0))
+
+
+def test_sort_is_stable_lexsort():
+    rng = np.random.default_rng(2)
+    k = rng.integers(0, 50, 4000).astype(np.uint64) | (np.uint64(1) << np.uint64(63))
+    sk, perm = O.sort(k)
+    ref = np.lexsort((np.arange(len(k)), k))
+    assert np.array_equal(perm, ref) and np.array_equal(sk, k[ref])
+
+
+# ---------------------------------------------------------------- tree
+
+def _small_pipeline(n=3000, bucket=8, seed=4, theta=0.7):
+    pos, _, m = ics.plummer(n, seed=seed)
+    return O.Pipeline(pos, m, np.full(n, 0.02), bucket=bucket, theta=theta)
+
+
+def test_tree_brute_force_prefix_counts():
+    """Every node's range is exactly the particles carrying its key prefix."""
+    P = _small_pipeline(2000, 6)
+    sk = P.sorted_keys
+    for nd in P.nodes:
+        shift = np.uint64(63 - 3 * nd["depth"])
+        prefix = sk >> shift
+        members = np.nonzero(prefix == np.uint64(nd["key"]))[0]
+        if nd["count"] == 0:
+            assert members.size == 0
+        else:
+            assert members[0] == nd["begin"] and members.size == nd["count"]
+            assert members[-1] == nd["begin"] + nd["count"] - 1
+
+
+def test_tree_invariants():
+    P = _small_pipeline(5000, 12)
+    nodes = P.nodes
+    for i, nd in enumerate(nodes):
+        if nd["type"] == O.OR_INTERNAL:
+            assert nd["count"] > 12
+            ch = nodes[nd["child"]]
+            assert len(ch) == 8 and ch["count"].sum() == nd["count"]
+            assert np.all(ch["key"] == (nd["key"] << np.uint64(3)) + np.arange(8, dtype=np.uint64))
+            starts = np.concatenate([[nd["begin"]], nd["begin"] + np.cumsum(ch["count"])[:-1]])
+            assert np.array_equal(ch["begin"][ch["count"] > 0], starts[ch["count"] > 0])
+            assert np.all(nd["child"] > i)  # DFS pre-order
+        elif nd["type"] == O.OR_LEAF:
+            assert 1 <= nd["count"] <= 12
+        else:
+            assert nd["count"] == 0
+    # buckets tile the particles in order
+    b = nodes[P.bucket_nodes]
+    assert b["begin"][0] == 0 and np.all(b["begin"][1:] == b["begin"][:-1] + b["count"][:-1])
+    assert b["count"].sum() == len(P.x)
+    # DFS order == ascending left-aligned key among disjoint buckets
+    la = left_aligned(b["key"])
+    assert np.all(la[1:] > la[:-1])
+
+
+# ---------------------------------------------------------------- moments
+
+def test_moments_closed_forms():
+    P = _small_pipeline(4000, 10)
+    x = np.stack([P.x, P.y, P.z], 1)
+    for nd, d in zip(P.nodes, P.data):
+        if nd["count"] == 0:
+            assert d["mass"] == 0 and d["radius"] == 0 and np.all(np.isinf(d["lo"]))
+            continue
+        sl = slice(nd["begin"], nd["begin"] + nd["count"])
+        xm, mm, em = x[sl], P.m[sl], P.eps[sl]
+        M = mm.sum()
+        cm = (mm[:, None] * xm).sum(0) / M
+        r = xm - cm
+        Q = np.einsum("i,ij,ik->jk", mm, r, r)
+        q = np.array([Q[0, 0], Q[0, 1], Q[0, 2], Q[1, 1], Q[1, 2], Q[2, 2]])
+        assert np.isclose(d["mass"], M, rtol=1e-13)
+        assert np.allclose(d["cm"], cm, rtol=0, atol=1e-13 * (1 + np.abs(cm).max()))
+        scale = M * (np.abs(r).max() ** 2 + 1e-300)
+        assert np.allclose(d["q"], q, rtol=0, atol=1e-11 * scale)
+        assert np.isclose(d["soft"], (mm * em).sum() / M, rtol=1e-13)
+        assert np.array_equal(d["lo"], xm.min(0)) and np.array_equal(d["hi"], xm.max(0))
+        assert d["count"] == nd["count"]
+        if nd["count"] == 1:
+            assert d["radius"] == 1.0
+        elif nd["type"] == O.OR_LEAF:   # calculateRadiusBox
+            assert np.isclose(d["radius"], 0.5 * np.linalg.norm(d["hi"] - d["lo"]), rtol=1e-15)
+        else:                            # calculateRadiusFarthestCorner
+            far = np.maximum(d["cm"] - d["lo"], d["hi"] - d["cm"])
+            assert np.isclose(d["radius"], np.linalg.norm(far), rtol=1e-15)
+
+
+# ---------------------------------------------------------------- open / lists
+
+def test_intersect_box_point_ties_and_cases():
+    lo, hi = np.array([0.0, 0.0, 0.0]), np.array([1.0, 2.0, 3.0])
+    assert O.intersect_box_point(lo, hi, [0.5, 1.0, 1.0], 0.0)        # inside
+    assert O.intersect_box_point(lo, hi, [2.0, 1.0, 1.0], 1.0)        # tie dsq == rsq
+    assert not O.intersect_box_point(lo, hi, [2.0, 1.0, 1.0], 0.999999)
+    assert O.intersect_box_point(lo, hi, [-1.0, -1.0, 1.0], 2.0)      # corner region
+    assert not O.intersect_box_point(lo, hi, [-1.0, -1.0, -1.0], 2.999)
+    assert O.intersect_box_point(lo, hi, [-1.0, -1.0, -1.0], 3.0)
+
+
+def test_lists_partition_sources_and_respect_opening():
+    """Per bucket, the P-P leaves plus the P-C nodes cover every particle
+    exactly once; each P-C node fails open() and every proper ancestor of a
+    listed node passes it (brute force with the predicate)."""
+    P = _small_pipeline(3000, 8, theta=0.6)
+    pp_off, pp, pc_off, pc = P.lists()
+    nodes, data = P.nodes, P.data
+    gf = (4.0 / 3.0) / 0.6 ** 2
+
+    def opens(src, tgt):
+        if data[src]["count"] <= 6:
+            return True
+        return O.intersect_box_point(data[tgt]["lo"], data[tgt]["hi"], data[src]["cm"],
+                                     data[src]["radius"] * data[src]["radius"] * gf)
+
+    n = len(P.x)
+    for bi, bn in enumerate(P.bucket_nodes[::7]):
+        b = bi * 7
+        cover = np.zeros(n, dtype=np.int64)
+        for s in pp[pp_off[b]:pp_off[b + 1]]:
+            assert nodes[s]["type"] == O.OR_LEAF
+            cover[nodes[s]["begin"]:nodes[s]["begin"] + nodes[s]["count"]] += 1
+        for s in pc[pc_off[b]:pc_off[b + 1]]:
+            assert nodes[s]["type"] == O.OR_INTERNAL and not opens(s, bn)
+            cover[nodes[s]["begin"]:nodes[s]["begin"] + nodes[s]["count"]] += 1
+        assert np.all(cover == 1)
+        for s in np.concatenate([pp[pp_off[b]:pp_off[b + 1]], pc[pc_off[b]:pc_off[b + 1]]]):
+            a = nodes[s]["parent"]
+            while a >= 0:
+                assert opens(a, bn)
+                a = nodes[a]["parent"]
+
+
+def test_counters_follow_lists():
+    P = _small_pipeline(3000, 8)
+    pp_off, pp, pc_off, pc = P.lists()
+    _, _, cnt = P.gravity()
+    nt = P.nodes["count"][P.bucket_nodes]
+    n_node = sum(int(nt[b]) * int(pc_off[b + 1] - pc_off[b]) for b in range(len(nt)))
+    n_part = sum(int(nt[b]) * int(P.nodes["count"][pp[pp_off[b]:pp_off[b + 1]]].sum())
+                 for b in range(len(nt)))
+    assert int(cnt[0]) == n_node and int(cnt[1]) == n_part
+    assert int(cnt[3]) == len(pc)
+
+
+# ---------------------------------------------------------------- forces
+
+def test_quadrupole_of_far_cluster():
+    """node(): a compact cluster seen from far away: the quadrupole error falls
+    as (size/distance)^3 (monopole would fall as ^2)."""
+    rng = np.random.default_rng(9)
+    clus = rng.standard_normal((64, 3)) * [0.3, 0.1, 0.05]
+    errs = []
+    for dist in (8.0, 16.0, 32.0):
+        tgt = np.array([[dist, 0.0, 0.0], [dist, 0.001, 0.0]])
+        pos = np.concatenate([clus, tgt])
+        m = np.concatenate([np.full(64, 1.0 / 64), [1e-12, 1e-12]])
+        P = O.Pipeline(pos, m, np.full(66, 1e-6), bucket=2, theta=1.0)
+        acc, _, cnt = P.gravity_unsorted()
+        ref = O.direct(pos[:, 0], pos[:, 1], pos[:, 2], m, np.full(66, 1e-6), [64])
+        errs.append(np.linalg.norm(acc[64] - ref[0]) / np.linalg.norm(ref[0]))
+        assert cnt[0] > 0  # the target used the node expansion
+    assert errs[1] < errs[0] / 5 and errs[2] < errs[1] / 5
+
+
+def test_tree_force_error_on_reference_fixture():
+    """Full oracle force on lambs 30K (B=12, theta=0.7, quadrupole) against
+    direct.acc: the reference states RMS .0008 / max .03 for its (more
+    accurate) hexadecapole build; the quadrupole error is a few 1e-3."""
+    pos, m, e, ref = _lambs()
+    P = O.Pipeline(pos, m, e, bucket=12, theta=0.7)
+    acc, _, _ = P.gravity_unsorted()
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert 5e-4 < np.sqrt(np.mean(rel ** 2)) < 5e-3
+    assert rel.max() < 0.1
+
+
+def test_kick_drift_closed_form():
+    rng = np.random.default_rng(0)
+    x, v, a = (rng.standard_normal((50, 3)) for _ in range(3))
+    dt = 1e-3
+    xn, vn = O.kick_drift(x, v, a, dt)
+    assert np.allclose(vn, v + a * dt, rtol=0, atol=1e-15)
+    assert np.allclose(xn, x + (v + a * dt) * dt, rtol=0, atol=1e-15)
