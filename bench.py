@@ -32,6 +32,10 @@ CONFIGS = {
     "plummer100k": (100_000, 12, 0.7, 0.01, "plummer"),
     "halo16m": (16_000_000, 32, 0.7, 0.01, "halo"),
 }
+# HBM bytes per dispatch of gravity_kernel<FP32C> on this workload: FETCH_SIZE
+# + WRITE_SIZE from two rocprofv3 counter passes (write counter calibrated
+# against an 8 MB device copy: factor 1.0).
+TRAFFIC_BYTES_FP32C = (94534.8 + 33620.9) * 1024
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
@@ -181,7 +185,8 @@ def main():
         "stage_ms_fp32c": st32, "stage_ms_fp64": st64,
         "list_path_ms": list_path,
         "roofline": {"bound": "valu", "achieved": achieved32, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_BYTES_FP32C if args.config == "plummer1m" else None,
                      "kernel": "force32c_kernel",
                      "fp64_kernel": {"achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
                                      "frac": achieved64 / PEAK_FP64_TFLOPS}},

@@ -1,0 +1,374 @@
+// paratreet/gravity.hpp -- C++ host interface of the MI355X gravity path.
+//
+// Mirrors the reference's user-facing types for this path so code written
+// against them reads the same:
+//   paratreet::Configuration  -- src/Configuration.h:84-187 (fields, CLI
+//                                letters, '-x' key = value files as
+//                                Loadable::parse, src/Loadable.C:93-184)
+//   paratreet::TreeSpec       -- src/TreeSpec.h:8-35 (octree only here)
+//   paratreet::GravityVisitor -- examples/GravityVisitor.h:9-20 (flags and
+//                                opening factors)
+//   paratreet::CentroidData   -- examples/CentroidData.h:24-91 (node data as
+//                                computed on the GPU)
+//   paratreet::Driver         -- the Driver::run stages (src/Driver.h:183-288)
+//                                with Partition::startDown<GravityVisitor>
+//                                (src/Partition.h:129-134)
+// Everything computes on the GPU through the C ABI of pt_gravity.h; errors
+// throw paratreet::Error carrying the library's status and message.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../pt_gravity.h"
+
+namespace paratreet {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int st, const std::string& msg) : std::runtime_error(msg), status(st) {}
+};
+
+enum class TreeType { eOct = 0, eBinaryOct, eKd, eLongest, eInvalid = 100 };
+enum class DecompType { eOct = 0, eBinaryOct, eSfc, eKd, eLongest, eInvalid = 100 };
+
+inline TreeType treeTypeFromString(const std::string& s) {
+  if (s == "oct") return TreeType::eOct;
+  if (s == "binoct") return TreeType::eBinaryOct;
+  if (s == "kd") return TreeType::eKd;
+  if (s == "longest") return TreeType::eLongest;
+  throw Error(PT_ERR_ARG, "-- invalid tree type value, " + s + " --");
+}
+
+inline DecompType decompTypeFromString(const std::string& s) {
+  if (s == "oct") return DecompType::eOct;
+  if (s == "binoct") return DecompType::eBinaryOct;
+  if (s == "sfc") return DecompType::eSfc;
+  if (s == "kd") return DecompType::eKd;
+  if (s == "longest") return DecompType::eLongest;
+  throw Error(PT_ERR_ARG, "-- invalid decomp type value, " + s + " --");
+}
+
+// paratreet::Configuration, gravity subset.  Defaults follow
+// ExMain::setDefaults (examples/Main.C:26-41) except tree/decomposition,
+// which default to the octree this path implements.
+struct Configuration {
+  int min_n_subtrees = 1;          // -n nSubtreesMin
+  int min_n_partitions = 1;        // -p nPartitionsMin
+  int max_particles_per_leaf = 12; // -l nParticlesPerLeafMax
+  DecompType decomp_type = DecompType::eOct;  // -d achDecompType
+  TreeType tree_type = TreeType::eOct;        // -t achTreeType
+  int num_iterations = 3;          // -i nIterations
+  double dSoft = 0.0;              // -e dSoft
+  bool dSoft_set = false;          // Configuration::origin_of("dSoft") != Unknown
+  std::string input_file;          // -f achInputFile
+  std::string output_file;         // -v achOutputFile
+  int periodic = 0;                // bPeriodic
+
+  int branchFactor() const {
+    switch (tree_type) {
+      case TreeType::eOct: return 8;
+      case TreeType::eBinaryOct:
+      case TreeType::eKd:
+      case TreeType::eLongest: return 2;
+      default: throw Error(PT_ERR_ARG, "Tree type is invalid");
+    }
+  }
+
+  // Loadable::parse: "-<letter> value" pairs and "-x file"; returns the
+  // arguments it did not consume (application flags such as -m).
+  std::vector<std::string> parse(int argc, char** argv) {
+    std::vector<std::string> rest;
+    for (int i = 1; i < argc; i++) {
+      const std::string a = argv[i];
+      if (a.size() == 2 && a[0] == '-' && i + 1 < argc && setLetter(a[1], argv[i + 1])) {
+        i++;
+      } else if (a == "-x" && i + 1 < argc) {
+        loadParamFile(argv[++i]);
+      } else {
+        rest.push_back(a);
+      }
+    }
+    return rest;
+  }
+
+  void loadParamFile(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw Error(PT_ERR_ARG, "cannot open parameter file " + path);
+    std::string line;
+    while (std::getline(f, line)) {
+      const auto hash = line.find('#');
+      if (hash != std::string::npos) line.resize(hash);
+      const auto eq = line.find('=');
+      if (eq == std::string::npos) continue;
+      const std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
+      static const char* names[][2] = {{"nSubtreesMin", "n"}, {"nPartitionsMin", "p"},
+                                       {"nParticlesPerLeafMax", "l"}, {"achDecompType", "d"},
+                                       {"achTreeType", "t"}, {"nIterations", "i"},
+                                       {"dSoft", "e"}, {"achInputFile", "f"},
+                                       {"achOutputFile", "v"}};
+      for (auto& nm : names)
+        if (k == nm[0]) setLetter(nm[1][0], v);
+    }
+  }
+
+  pt_config toC() const {
+    pt_config c;
+    pt_config_default(&c);
+    c.min_n_subtrees = min_n_subtrees;
+    c.min_n_partitions = min_n_partitions;
+    c.max_particles_per_leaf = max_particles_per_leaf;
+    c.decomp_type = (int32_t)decomp_type;
+    c.tree_type = (int32_t)tree_type;
+    c.num_iterations = num_iterations;
+    c.dsoft = dSoft;
+    c.has_dsoft = dSoft_set ? 1 : 0;
+    c.periodic = periodic;
+    return c;
+  }
+
+ private:
+  static std::string trim(const std::string& s) {
+    const auto b = s.find_first_not_of(" \t\r\n");
+    if (b == std::string::npos) return "";
+    const auto e = s.find_last_not_of(" \t\r\n");
+    return s.substr(b, e - b + 1);
+  }
+  bool setLetter(char c, const std::string& v) {
+    switch (c) {
+      case 'n': min_n_subtrees = std::atoi(v.c_str()); return true;
+      case 'p': min_n_partitions = std::atoi(v.c_str()); return true;
+      case 'l': max_particles_per_leaf = std::atoi(v.c_str()); return true;
+      case 'd': decomp_type = decompTypeFromString(v); return true;
+      case 't': tree_type = treeTypeFromString(v); return true;
+      case 'i': num_iterations = std::atoi(v.c_str()); return true;
+      case 'e': dSoft = std::atof(v.c_str()); dSoft_set = true; return true;
+      case 'f': input_file = v; return true;
+      case 'v': output_file = v; return true;
+      default: return false;
+    }
+  }
+};
+
+// TreeSpec restricted to this path: TreeType::eOct with DecompType::eOct and
+// one subtree per GPU.  Other combinations are rejected with an error.
+class TreeSpec {
+ public:
+  explicit TreeSpec(const Configuration& c) : config_(c) {
+    if (c.tree_type != TreeType::eOct || c.decomp_type != DecompType::eOct)
+      throw Error(PT_ERR_UNSUPPORTED, "only the octree ('-t oct -d oct') runs on the GPU path");
+  }
+  int getBranchFactor() const { return 8; }
+
+ private:
+  Configuration config_;
+};
+
+// GravityVisitor (examples/GravityVisitor.h:9-20): constructed exactly as the
+// reference does; the open/node/leaf bodies run on the GPU.
+class GravityVisitor {
+ public:
+  static constexpr bool CallSelfLeaf = true;
+  static constexpr bool ForceEvenDepth = true;
+  static constexpr bool TargetMustBeLeaf = true;
+  static constexpr double opening_geometry_factor_squared = 4.0 / 3.0;
+  static constexpr int nMinParticleNode = 6;
+
+  GravityVisitor() : offset{0, 0, 0}, theta(0.7) { setFactors(); }
+  GravityVisitor(const double off[3], double th) : offset{off[0], off[1], off[2]}, theta(th) {
+    setFactors();
+  }
+
+  double offset[3];
+  double theta;
+  double gravity_factor = 0;
+  double monopole_gravity_factor = 0;
+
+ private:
+  void setFactors() {
+    gravity_factor = opening_geometry_factor_squared / (theta * theta);
+    monopole_gravity_factor = std::sqrt(opening_geometry_factor_squared) / (theta * theta * theta * theta);
+  }
+};
+
+// CentroidData as computed on the GPU (MultipoleMoments + OrientedBox + count).
+struct CentroidData {
+  double radius, soft, totalMass;
+  double cm[3];
+  double xx, xy, xz, yy, yz, zz;
+  double lesser[3], greater[3];
+  int32_t count;
+};
+
+// The stages of one Driver::run iteration for the gravity visitor on one GPU.
+class Driver {
+ public:
+  explicit Driver(const Configuration& cfg, int device = 0) : config_(cfg), spec_(cfg) {
+    const pt_config c = cfg.toC();
+    check(pt_ctx_create(&c, device, nullptr, &ctx_), "pt_ctx_create");
+  }
+  ~Driver() { pt_ctx_destroy(ctx_); }
+  Driver(const Driver&) = delete;
+  Driver& operator=(const Driver&) = delete;
+
+  // Reader::load + setSoft: original-order host arrays.
+  void load(const std::vector<double>& pos, const std::vector<double>& vel,
+            const std::vector<double>& mass, const std::vector<double>& soft) {
+    n_ = (int64_t)mass.size();
+    std::vector<double> x(n_), y(n_), z(n_), vx(n_), vy(n_), vz(n_);
+    for (int64_t i = 0; i < n_; i++) {
+      x[i] = pos[3 * i]; y[i] = pos[3 * i + 1]; z[i] = pos[3 * i + 2];
+      vx[i] = vel.empty() ? 0.0 : vel[3 * i];
+      vy[i] = vel.empty() ? 0.0 : vel[3 * i + 1];
+      vz[i] = vel.empty() ? 0.0 : vel[3 * i + 2];
+    }
+    check(pt_set_particles(ctx_, n_, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
+                           mass.data(), soft.empty() ? nullptr : soft.data(), PT_HOST),
+          "pt_set_particles");
+  }
+
+  // remakeUniverse + assignKeys + Subtree::buildTree (sort, recursiveBuild,
+  // populateTree).
+  void buildTree() {
+    double box[6];
+    check(pt_remake_universe(ctx_, box), "pt_remake_universe");
+    check(pt_assign_keys(ctx_), "pt_assign_keys");
+    check(pt_sort(ctx_), "pt_sort");
+    check(pt_build_tree(ctx_), "pt_build_tree");
+    check(pt_upward(ctx_), "pt_upward");
+  }
+
+  // Partition::startDown<GravityVisitor>: interaction lists + visitor calls.
+  pt_counters startDown(const GravityVisitor& v, pt_precision prec = PT_FP64) {
+    pt_counters c{};
+    check(pt_traverse(ctx_, v.theta, v.offset, &c), "pt_traverse");
+    check(pt_interact(ctx_, prec), "pt_interact");
+    return c;
+  }
+
+  // Everything above in one fused evaluation (no lists kept).
+  pt_counters computeForces(const GravityVisitor& v, pt_precision prec = PT_FP64) {
+    pt_counters c{};
+    check(pt_compute_forces(ctx_, v.theta, v.offset, prec, &c), "pt_compute_forces");
+    return c;
+  }
+
+  // Partition::kick + Partition::perturb.
+  void kickDrift(double dt) { check(pt_kick_drift(ctx_, dt), "pt_kick_drift"); }
+
+  std::vector<double> accelerations() {
+    std::vector<double> acc(3 * n_), pot(n_);
+    check(pt_get_accelerations(ctx_, acc.data(), pot.data()), "pt_get_accelerations");
+    return acc;
+  }
+
+  std::vector<CentroidData> nodeData() {
+    int64_t nn = 0, nb = 0;
+    int32_t md = 0;
+    check(pt_get_tree_size(ctx_, &nn, &nb, &md), "pt_get_tree_size");
+    std::vector<double> raw(18 * nn);
+    std::vector<int32_t> count(nn);
+    check(pt_get_node_data(ctx_, raw.data()), "pt_get_node_data");
+    check(pt_get_nodes(ctx_, nullptr, nullptr, nullptr, nullptr, count.data(), nullptr, nullptr),
+          "pt_get_nodes");
+    std::vector<CentroidData> out(nn);
+    for (int64_t i = 0; i < nn; i++) {
+      const double* r = &raw[18 * i];
+      CentroidData& d = out[i];
+      d.radius = r[0]; d.soft = r[1]; d.totalMass = r[2];
+      for (int k = 0; k < 3; k++) { d.cm[k] = r[3 + k]; d.lesser[k] = r[12 + k]; d.greater[k] = r[15 + k]; }
+      d.xx = r[6]; d.xy = r[7]; d.xz = r[8]; d.yy = r[9]; d.yz = r[10]; d.zz = r[11];
+      d.count = count[i];
+    }
+    return out;
+  }
+
+  // Writer::do_write format (src/Writer.C:31-72): N, then the x, y and z
+  // blocks, one "%.14g" value per line, original particle order.
+  void outputParticleAccelerations(const std::string& prefix) {
+    const std::vector<double> acc = accelerations();
+    FILE* f = std::fopen((prefix + ".acc").c_str(), "w");
+    if (!f) throw Error(PT_ERR_ARG, "cannot write " + prefix + ".acc");
+    std::fprintf(f, "%lld\n", (long long)n_);
+    for (int k = 0; k < 3; k++)
+      for (int64_t i = 0; i < n_; i++) std::fprintf(f, "%.14g\n", acc[3 * i + k]);
+    std::fclose(f);
+  }
+
+  int64_t size() const { return n_; }
+  pt_ctx* context() { return ctx_; }
+  const Configuration& config() const { return config_; }
+
+ private:
+  void check(int st, const char* what) {
+    if (st != PT_OK) throw Error(st, std::string(what) + ": " + (ctx_ ? pt_last_error(ctx_) : "no context"));
+  }
+  Configuration config_;
+  TreeSpec spec_;
+  pt_ctx* ctx_ = nullptr;
+  int64_t n_ = 0;
+};
+
+// Tipsy standard binary reader (big-endian), as Reader::load consumes it
+// (src/Reader.C:11-108): gas, dark and star records; soft = hsmooth / eps /
+// star eps, then Reader::setSoft when dSoft is configured.
+struct TipsyData {
+  double time = 0;
+  std::vector<double> pos, vel, mass, soft;
+};
+
+inline TipsyData readTipsy(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(PT_ERR_ARG, "cannot open tipsy file " + path);
+  auto rd = [&](void* p, size_t n) {
+    f.read((char*)p, (std::streamsize)n);
+    if (!f) throw Error(PT_ERR_ARG, "short tipsy file " + path);
+  };
+  auto be32 = [](const unsigned char* b) {
+    return (uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | (uint32_t)b[3];
+  };
+  unsigned char h[32];
+  rd(h, 32);
+  uint64_t tb = 0;
+  for (int i = 0; i < 8; i++) tb = (tb << 8) | h[i];
+  TipsyData t;
+  std::memcpy(&t.time, &tb, 8);
+  const int nbodies = (int)be32(h + 8), nsph = (int)be32(h + 16), ndark = (int)be32(h + 20),
+            nstar = (int)be32(h + 24);
+  if (nsph + ndark + nstar != nbodies) throw Error(PT_ERR_ARG, "inconsistent tipsy header");
+  auto readf = [&](int cnt, std::vector<float>& out) {
+    std::vector<unsigned char> raw(4 * (size_t)cnt);
+    rd(raw.data(), raw.size());
+    out.resize(cnt);
+    for (int i = 0; i < cnt; i++) {
+      const uint32_t u = be32(&raw[4 * (size_t)i]);
+      std::memcpy(&out[i], &u, 4);
+    }
+  };
+  std::vector<float> g, d, s;
+  readf(12 * nsph, g);
+  readf(9 * ndark, d);
+  readf(11 * nstar, s);
+  auto add = [&](const std::vector<float>& r, int w, int cnt, int soft_idx) {
+    for (int i = 0; i < cnt; i++) {
+      const float* q = &r[(size_t)w * i];
+      t.mass.push_back(q[0]);
+      for (int k = 0; k < 3; k++) { t.pos.push_back(q[1 + k]); t.vel.push_back(q[4 + k]); }
+      t.soft.push_back(soft_idx >= 0 ? q[soft_idx] : 0.0);
+    }
+  };
+  add(g, 12, nsph, 9);
+  add(d, 9, ndark, 7);
+  add(s, 11, nstar, -1);
+  return t;
+}
+
+}  // namespace paratreet

@@ -50,6 +50,22 @@ def _compile(name, extra):
     return obj
 
 
+HOST_MAIN = os.path.join(CSRC, "host", "pt_gravity_main.cpp")
+BIN = os.path.join(HERE, "bin", "pt_gravity")
+
+
+def build_host_app():
+    """The C++ example driver (g++, links libpt_gravity.so; no HIP types)."""
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           HOST_MAIN, "-o", BIN, "-L" + HERE, "-lpt_gravity", "-Wl,-rpath,$ORIGIN/..",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    return BIN
+
+
 def build(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -60,6 +76,7 @@ def build(jobs=8):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    build_host_app()
     return LIB
 
 

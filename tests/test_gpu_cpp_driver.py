@@ -1,0 +1,61 @@
+"""The C++ host driver (paratreet/gravity.hpp + pt_gravity_main.cpp) end to
+end on the GPU: Tipsy in, '.acc' out (the reference's acc_test.sh flow), checked
+against the oracle's tree accelerations (1e-12) and the reference's direct.acc."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paratreet_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "paratreet_amd", "bin", "pt_gravity")
+
+
+def _write_tipsy(path, pos, vel, mass, soft):
+    n = len(mass)
+    hdr = np.array([(1.0, n, 3, 0, n, 0, 0)], dtype=np.dtype(">f8,>i4,>i4,>i4,>i4,>i4,>i4"))
+    rec = np.zeros((n, 9), dtype=">f4")
+    rec[:, 0] = mass
+    rec[:, 1:4] = pos
+    rec[:, 4:7] = vel
+    rec[:, 7] = soft
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(rec.tobytes())
+
+
+def test_cpp_driver_lambs_acc_file(tmp_path):
+    d = np.load(os.path.join(ROOT, "tests", "golden", "lambs_30k.npz"))
+    pos, vel, m, e = d["pos"], d["vel"], d["mass"], d["soft"]
+    f = tmp_path / "lambs.tipsy"
+    _write_tipsy(f, pos, vel, m, e)
+    out = tmp_path / "run"
+    r = subprocess.run([BIN, "-f", str(f), "-v", str(out), "-l", "12", "-i", "1", "-m"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    acc = ics.read_acc(str(out) + ".acc")
+    P = O.Pipeline(pos.astype(np.float64), m.astype(np.float64), e.astype(np.float64),
+                   bucket=12, theta=0.7)
+    oacc, _, cnt = P.gravity_unsorted()
+    rel = np.linalg.norm(acc - oacc, axis=1) / np.linalg.norm(oacc, axis=1)
+    # '%.14g' text output limits agreement to ~1e-14 relative
+    assert rel.max() < 1e-12
+    line = [l for l in r.stdout.splitlines() if "Iteration 0" in l][0]
+    assert "%d node-particle" % cnt[0] in line and "%d bucket-particle" % cnt[1] in line
+    ref = d["direct_acc"]
+    err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.sqrt(np.mean(err ** 2)) < 5e-3
+
+
+def test_cpp_driver_rejects_unsupported_tree(tmp_path):
+    pos, vel, m = ics.plummer(100, seed=1)
+    f = tmp_path / "p.tipsy"
+    _write_tipsy(f, pos, vel, m, np.full(100, 0.01))
+    r = subprocess.run([BIN, "-f", str(f), "-t", "kd"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "octree" in r.stderr
