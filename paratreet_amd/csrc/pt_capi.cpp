@@ -61,7 +61,7 @@ struct pt_ctx {
   int32_t max_depth = 0;
   DBuf<uint64_t> nkey;
   DBuf<int32_t> nbegin, ncount, ndepth, ntype, nparent, nfirst;
-  DBuf<int32_t> level_nodes;
+  DBuf<int32_t> level_nodes, level_cnt;
   std::vector<int64_t> level_off;  // internal nodes of depth d: [level_off[d], level_off[d+1])
   DBuf<uint32_t> flag, fpos, head, head_pos, u32_total;
   DBuf<int32_t> bucket_node;
@@ -288,6 +288,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->box); release(c->bbox); release(c->box_parts);
   release(c->nkey); release(c->nbegin); release(c->ncount); release(c->ndepth); release(c->ntype);
   release(c->nparent); release(c->nfirst); release(c->level_nodes);
+  release(c->level_cnt);
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
@@ -418,58 +419,68 @@ pt_status pt_sort(pt_ctx* c) {
   return PT_OK;
 }
 
+// Level-parallel build driven from the device: one split launch per level
+// (internal-node counts stay on the device), a host check every 4 levels, and
+// a rebuild with larger tables if a capacity was exceeded.
 pt_status pt_build_tree(pt_ctx* c) {
   pt_status st = check_ctx(c);
   if (st) return st;
   if ((st = need(c, S_SORTED, "pt_build_tree"))) return st;
   const int B = c->cfg.max_particles_per_leaf;
   Timer t(c, PT_T_BUILD);
-  c->nn = 0;
-  int64_t est = std::max<int64_t>(1024, 8 * (2 * c->n / (B + 1)) + 64);
-  if ((st = ensure_nodes(c, est))) return st;
-  PtTree tr = tree_of(c);
-  PT_TRY_LAUNCH(c, pt_launch_root(c->n, B, tr, c->stream));
-  c->nn = 1;
-  c->level_off.assign(1, 0);
-  c->max_depth = 0;
-  int64_t nf = (c->n > B) ? 1 : 0;
-  if (nf) {
-    // level 0 frontier = {root}
+  // internal nodes of one level are disjoint and hold more than B particles
+  const int64_t per_level = c->n / (B + 1) + 1;
+  int64_t level_cap = std::max<int64_t>(64, 2 * per_level);
+  PT_TRY_HIP(c, ensure(c->level_cnt, PT_MAX_DEPTH + 2));
+  for (int attempt = 0;; attempt++) {
+    const int64_t node_cap = 1 + 8 * level_cap;
+    c->nn = 0;
+    if ((st = ensure_nodes(c, node_cap))) return st;
+    PT_TRY_HIP(c, ensure(c->level_nodes, level_cap));
+    PtTree tr = tree_of(c);
+    PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_root(c->n, B, tr, c->level_cnt.p, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->level_nodes.p, 1, 0u, c->stream));
-  }
-  c->level_off.push_back(nf);
-  int depth = 0;
-  while (nf > 0) {
-    const int64_t base = c->nn;
-    const int64_t m = 8 * nf;
-    if ((st = ensure_nodes(c, base + m))) return st;
-    tr = tree_of(c);
-    PT_TRY_HIP(c, ensure(c->flag, m));
-    PT_TRY_HIP(c, ensure(c->fpos, m));
-    if ((st = ensure_scratch(c, pt_scan_scratch_bytes(m)))) return st;
-    const int64_t f0 = c->level_off[depth];
-    PT_TRY_LAUNCH(c, pt_launch_split(c->skeys.p, c->level_nodes.p + f0, (int32_t)nf, (int32_t)base, B,
-                                     depth + 1, tr, c->flag.p, c->stream));
-    PT_TRY_LAUNCH(c, pt_exclusive_scan_u32(c->flag.p, c->fpos.p, m, c->scratch.p, c->u32_total.p,
-                                           c->stream));
-    uint32_t nnext = 0;
-    PT_TRY_HIP(c, hipMemcpyAsync(&nnext, c->u32_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    c->nn = base + m;
-    depth++;
-    c->max_depth = depth;
-    const int64_t f1 = c->level_off[depth];
-    if (nnext > 0) {
-      PT_TRY_LAUNCH(c, pt_launch_compact(c->flag.p, c->fpos.p, m, (int32_t)base,
-                                         c->level_nodes.p + f1, c->stream));
+    int32_t cnt[PT_MAX_DEPTH + 2] = {0};
+    int32_t herr = 0;
+    int64_t bound = 1;  // upper bound of the level's internal-node count
+    int depth = 0;
+    for (; depth <= PT_MAX_DEPTH - 1; depth++) {
+      PT_TRY_LAUNCH(c, pt_launch_split_level(c->skeys.p, c->level_nodes.p, c->level_cnt.p, depth,
+                                             bound, B, tr, node_cap, level_cap, c->derr.p,
+                                             c->stream));
+      bound = std::min<int64_t>(bound * 8, per_level);
+      if (depth % 4 == 3) {
+        PT_TRY_HIP(c, hipMemcpyAsync(cnt, c->level_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost,
+                                     c->stream));
+        PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+        if (cnt[depth + 1] == 0) break;
+      }
     }
-    c->level_off.push_back(f1 + nnext);
-    nf = nnext;
-    if (depth > PT_MAX_DEPTH) return fail(c, PT_ERR_CAPACITY, "tree deeper than the key allows");
+    PT_TRY_HIP(c, hipMemcpyAsync(cnt, c->level_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost,
+                                 c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    if (herr) {
+      if (attempt >= 3) return fail(c, PT_ERR_CAPACITY, "octree node table overflow");
+      level_cap *= 4;
+      continue;
+    }
+    // level table: internal nodes of depth d are [level_off[d], level_off[d+1])
+    c->level_off.assign(1, 0);
+    c->max_depth = 0;
+    int64_t internal = 0;
+    for (int d = 0; d <= PT_MAX_DEPTH; d++) {
+      internal += cnt[d];
+      c->level_off.push_back(internal);
+      if (cnt[d] > 0) c->max_depth = d + 1;
+    }
+    c->nn = 1 + 8 * internal;
+    break;
   }
   // buckets: non-empty leaves in particle (= DFS) order
   if ((st = ensure_scratch(c, pt_scan_scratch_bytes(c->n)))) return st;
+  PtTree tr = tree_of(c);
   PT_TRY_LAUNCH(c, pt_launch_buckets(tr, c->nn, c->n, c->head.p, c->head_pos.p, c->scratch.p,
                                      c->u32_total.p, c->bucket_node.p, c->stream));
   uint32_t nb = 0;

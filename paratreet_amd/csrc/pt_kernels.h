@@ -36,7 +36,10 @@ int pt_radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int 
                   void* scratch, hipStream_t s);
 int pt_launch_iota(uint32_t* v, int64_t n, hipStream_t s);
 
-int pt_launch_root(int64_t n, int bucket, const PtTree& t, hipStream_t s);
+int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s);
+int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
+                          int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
+                          int64_t level_cap, int32_t* overflow, hipStream_t s);
 int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, int32_t base,
                     int bucket, int child_depth, const PtTree& t, uint32_t* flag, hipStream_t s);
 int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,

@@ -216,7 +216,9 @@ __device__ __forceinline__ uint64_t left_align(uint64_t k) {
   return k << (PT_KEY_BITS - (3 * depth + 1));
 }
 
-// One thread per internal node of one level (deepest level first).
+// One thread per internal node of one level (deepest level first).  The
+// children's pop order is built in registers (fully unrolled loops over the
+// 8 children, the order packed as 8 nibbles) so nothing spills to scratch.
 __global__ void __launch_bounds__(256)
 upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
               double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
@@ -225,46 +227,59 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
   if (li >= nl) return;
   const int32_t p = level_nodes[li];
   const int32_t fc = t.first_child[p];
-  // pop order of the 8 children
-  int order[8];
-  int no = 0;
   int ctype[8];
-  for (int c = 0; c < 8; c++) ctype[c] = t.type[fc + c];
-  for (int c = 0; c < 8; c++)
-    if (ctype[c] == PT_NODE_LEAF) order[no++] = c;
-  for (int c = 0; c < 8; c++)
-    if (ctype[c] == PT_NODE_EMPTY) order[no++] = c;
-  const int n_leafish = no;
+  int32_t h[8];
+  uint64_t b[8];
+  int n_leaf = 0, n_empty = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    ctype[c] = t.type[fc + c];
+    h[c] = -1;
+    b[c] = 0;
+    if (ctype[c] == PT_NODE_INTERNAL) {
+      h[c] = ord_h[fc + c];
+      b[c] = ord_b[fc + c];
+    }
+    n_leaf += ctype[c] == PT_NODE_LEAF;
+    n_empty += ctype[c] == PT_NODE_EMPTY;
+  }
+  // position of every child in the pop order: non-empty leaves, empty leaves
+  // (each in child order), then internal children ascending in (h, b)
+  uint32_t order = 0;
   int32_t hmax = -1;
   uint64_t bmax = 0;
+  int k_leaf = 0, k_empty = 0;
+#pragma unroll
   for (int c = 0; c < 8; c++) {
-    if (ctype[c] != PT_NODE_INTERNAL) continue;
-    const int32_t h = ord_h[fc + c];
-    const uint64_t b = ord_b[fc + c];
-    // insertion into order[n_leafish..no) ascending by (h, b)
-    int j = no;
-    while (j > n_leafish) {
-      const int oc = order[j - 1];
-      const int32_t oh = ord_h[fc + oc];
-      const uint64_t ob = ord_b[fc + oc];
-      if (oh < h || (oh == h && ob < b)) break;
-      order[j] = oc;
-      j--;
+    int pos;
+    if (ctype[c] == PT_NODE_LEAF) {
+      pos = k_leaf++;
+    } else if (ctype[c] == PT_NODE_EMPTY) {
+      pos = n_leaf + k_empty++;
+    } else {
+      int rank = 0;
+#pragma unroll
+      for (int o = 0; o < 8; o++) {
+        const bool before = ctype[o] == PT_NODE_INTERNAL &&
+                            (h[o] < h[c] || (h[o] == h[c] && b[o] < b[c]));
+        rank += before ? 1 : 0;
+      }
+      pos = n_leaf + n_empty + rank;
+      if (h[c] > hmax || (h[c] == hmax && b[c] > bmax)) {
+        hmax = h[c];
+        bmax = b[c];
+      }
     }
-    order[j] = c;
-    no++;
-    if (h > hmax || (h == hmax && b > bmax)) {
-      hmax = h;
-      bmax = b;
-    }
+    order |= (uint32_t)c << (4 * pos);
   }
   if (hmax < 0) {
     // frontier node: h = 0, b = rank of the last-popped leaf child
     uint64_t bl = 0;
+#pragma unroll
     for (int c = 0; c < 8; c++) {
       uint64_t r = left_align(t.key[fc + c]) >> 1;
       if (ctype[c] == PT_NODE_EMPTY) r |= (1ull << 63);
-      if (r > bl) bl = r;
+      bl = r > bl ? r : bl;
     }
     ord_h[p] = 0;
     ord_b[p] = bl;
@@ -276,10 +291,9 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
   cd_load(d, nd, nn, p, 0);
   d.count = 0;
   for (int j = 0; j < 8; j++) {
-    const int c = order[j];
+    const int c = (order >> (4 * j)) & 15;
     CData cd;
     cd_load(cd, nd, nn, fc + c, t.count[fc + c]);
-    if (ctype[c] == PT_NODE_EMPTY) cd.count = 0;
     cd_merge(d, cd);
   }
   cd_store(d, nd, nn, p);

@@ -44,8 +44,10 @@ __device__ __forceinline__ int node_type(int64_t n, int bucket, int depth) {
   return PT_NODE_INTERNAL;
 }
 
-__global__ void root_kernel(int64_t n, int bucket, PtTree t) {
+__global__ void root_kernel(int64_t n, int bucket, PtTree t, int32_t* __restrict__ level_cnt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int l = 0; l <= PT_MAX_DEPTH + 1; l++) level_cnt[l] = 0;
+  level_cnt[0] = node_type(n, bucket, 0) == PT_NODE_INTERNAL ? 1 : 0;
   t.key[0] = 1;
   t.begin[0] = 0;
   t.count[0] = (int32_t)n;
@@ -82,6 +84,55 @@ split_kernel(const uint64_t* __restrict__ keys, const int32_t* __restrict__ fron
   internal_flag[gid] = (ty == PT_NODE_INTERNAL) ? 1u : 0u;
 }
 
+// Device-driven level split: level d's internal nodes are
+// level_nodes[off_d, off_d + level_cnt[d]) with off_d = sum of the counts of
+// the levels above; their children get ids 1 + 8 * off_d + 8 f + c, and the
+// internal children are appended to level d + 1 with an atomic counter (their
+// order there is irrelevant: children stay contiguous per parent, merge order
+// and walk order depend only on the tree).  Writes beyond the capacities set
+// *overflow instead.
+__global__ void __launch_bounds__(256)
+split_level_kernel(const uint64_t* __restrict__ keys, int32_t* __restrict__ level_nodes,
+                   int32_t* __restrict__ level_cnt, int d, int bucket, PtTree t,
+                   int64_t node_cap, int64_t level_cap, int32_t* __restrict__ overflow) {
+  int64_t f0 = 0;
+  for (int l = 0; l < d; l++) f0 += level_cnt[l];
+  const int64_t nf = level_cnt[d];
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  if (gid >= nf * 8) return;
+  const int64_t f = gid >> 3;
+  const int c = (int)(gid & 7);
+  const int32_t p = level_nodes[f0 + f];
+  const int64_t base = 1 + 8 * f0;
+  const int64_t id = base + gid;
+  if (id >= node_cap) {
+    atomicExch(overflow, 1);
+    return;
+  }
+  const int64_t b = t.begin[p];
+  const int64_t e = b + t.count[p];
+  const uint64_t child_key = (t.key[p] << 3) | (uint64_t)c;
+  const int64_t lo = (c == 0) ? b : first_ge(keys, b, e, left_align(child_key));
+  const int64_t hi = (c == 7) ? e : first_ge(keys, b, e, left_align(child_key + 1));
+  const int ty = node_type(hi - lo, bucket, d + 1);
+  t.key[id] = child_key;
+  t.begin[id] = (int32_t)lo;
+  t.count[id] = (int32_t)(hi - lo);
+  t.depth[id] = d + 1;
+  t.type[id] = ty;
+  t.parent[id] = p;
+  t.first_child[id] = -1;
+  if (c == 0) t.first_child[p] = (int32_t)(base + 8 * f);
+  if (ty == PT_NODE_INTERNAL) {
+    const int64_t slot = f0 + nf + atomicAdd(&level_cnt[d + 1], 1);
+    if (slot >= level_cap) {
+      atomicExch(overflow, 1);
+      return;
+    }
+    level_nodes[slot] = (int32_t)id;
+  }
+}
+
 __global__ void __launch_bounds__(256)
 compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t m,
                int32_t base, int32_t* __restrict__ next) {
@@ -106,8 +157,8 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-int pt_launch_root(int64_t n, int bucket, const PtTree& t, hipStream_t s) {
-  root_kernel<<<1, 64, 0, s>>>(n, bucket, t);
+int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s) {
+  root_kernel<<<1, 64, 0, s>>>(n, bucket, t, level_cnt);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -117,6 +168,16 @@ int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, i
   if (nf <= 0) return 0;
   split_kernel<<<grid_for((int64_t)nf * 8), 256, 0, s>>>(keys, frontier, nf, base, bucket,
                                                           child_depth, t, flag);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
+                          int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
+                          int64_t level_cap, int32_t* overflow, hipStream_t s) {
+  if (max_frontier <= 0) return 0;
+  split_level_kernel<<<grid_for(max_frontier * 8), 256, 0, s>>>(keys, level_nodes, level_cnt, d,
+                                                                 bucket, t, node_cap, level_cap,
+                                                                 overflow);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
