@@ -48,6 +48,10 @@ typedef enum { PT_DECOMP_OCT = 0, PT_DECOMP_BINARY_OCT = 1, PT_DECOMP_SFC = 2, P
 /* Arithmetic form of the interaction kernels. */
 typedef enum { PT_FP64 = 0, PT_FP32C = 1 } pt_precision;
 
+/* Force model of a trajectory: the tree (GravityVisitor over the octree) or
+ * the FP64 direct sum of the leaf() kernel over all pairs (validation). */
+typedef enum { PT_FORCE_TREE = 0, PT_FORCE_DIRECT = 1 } pt_force_mode;
+
 /* Where caller-supplied arrays live. */
 typedef enum { PT_HOST = 0, PT_DEVICE = 1 } pt_memory;
 
@@ -77,7 +81,7 @@ typedef struct pt_counters {
 /* Stage timings in milliseconds (HIP events on the context stream). */
 typedef enum {
   PT_T_UNIVERSE = 0, PT_T_KEYS, PT_T_SORT, PT_T_BUILD, PT_T_MOMENTS, PT_T_TRAVERSE, PT_T_FORCE,
-  PT_T_INTEGRATE, PT_T_COUNT
+  PT_T_INTEGRATE, PT_T_ADJOINT, PT_T_COUNT
 } pt_timer;
 
 /* Device pointers owned by the context (valid until the next call that
@@ -159,6 +163,38 @@ pt_status pt_run(pt_ctx* ctx, int steps, double dt, double theta, pt_precision p
 /* FP64 direct sum with the leaf() kernel for the given original-order target
  * indices, over all particles (error reference).  acc_out: nt x 3 host. */
 pt_status pt_direct_sum(pt_ctx* ctx, const int64_t* targets, int64_t nt, double* acc_out);
+
+/* ---- differentiation (no counterpart in the reference) ----
+ * The reference has no adjoint; these entry points add reverse-mode
+ * gradients of the Driver::run loop above.  Arrays are n x 3 row-major
+ * (xyz interleaved) in original particle order, host or device per `where`.
+ *
+ * Vector-Jacobian product of one force evaluation at the current positions:
+ * for w (n x 3) it returns grad_x = (da/dx)^T w (n x 3) and
+ * grad_m = (da/dm)^T w (n), on the same traversal lists as pt_traverse
+ * (tree) or over all pairs (direct); accelerations are those of
+ * GravityVisitor::node/leaf (examples/GravityVisitor.h:144-224), opening
+ * decisions held fixed. */
+pt_status pt_force_vjp(pt_ctx* ctx, double theta, pt_force_mode mode, const double* w,
+                       double* grad_x, double* grad_m, pt_memory where);
+
+/* pt_run (Driver::run, src/Driver.h:183-288, with Partition::kick/perturb,
+ * src/Particle.C:7-21) from the current state, keeping a checkpoint of
+ * positions and velocities every `checkpoint_every` steps (<= 0: one
+ * checkpoint at step 0) for pt_leapfrog_adjoint.  The final state is left in
+ * the context (pt_get_state). */
+pt_status pt_leapfrog_forward(pt_ctx* ctx, int32_t steps, double dt, double theta,
+                              pt_precision precision, pt_force_mode mode,
+                              int32_t checkpoint_every, pt_counters* counters);
+
+/* Reverse sweep of the last pt_leapfrog_forward: given the cotangents of the
+ * final positions and velocities (lam_x, lam_v: n x 3, either may be NULL for
+ * zero) returns the gradients with respect to the initial positions,
+ * velocities (n x 3) and masses (n).  Steps between checkpoints are
+ * recomputed with the forward's force model and precision; the adjoint
+ * arithmetic is FP64.  The final state is restored afterwards. */
+pt_status pt_leapfrog_adjoint(pt_ctx* ctx, const double* lam_x, const double* lam_v,
+                              double* grad_x, double* grad_v, double* grad_m, pt_memory where);
 
 /* ---- read-back (host buffers, synchronous) ---- */
 pt_status pt_get_universe(pt_ctx* ctx, double box[6]);

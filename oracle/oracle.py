@@ -72,6 +72,7 @@ def _declare(L):
     L.or_intersect_box_point.argtypes = [_P, _P, _P, _D]
     L.or_intersect_box_point.restype = ctypes.c_int
     L.or_kick_drift.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _D]
+    L.or_direct_vjp.argtypes = [_P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P]
 
 
 def _p(a):
@@ -176,6 +177,19 @@ def direct(x, y, z, m, eps, targets, nthreads=0):
     lib().or_direct(_p(x), _p(y), _p(z), _p(m), _p(eps), len(x), _p(t), len(t),
                     _p(ax), _p(ay), _p(az), int(nthreads))
     return np.stack([ax, ay, az], axis=1)
+
+
+def direct_vjp(pos, m, eps, w):
+    """(da/dx)^T w (n x 3) and (da/dm)^T w (n) of the direct sum."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (_f64(pos[:, k]) for k in range(3))
+    w = np.asarray(w, dtype=np.float64)
+    wx, wy, wz = (_f64(w[:, k]) for k in range(3))
+    n = len(x)
+    gx, gy, gz, gm = (np.zeros(n) for _ in range(4))
+    lib().or_direct_vjp(_p(x), _p(y), _p(z), _p(_f64(m)), _p(_f64(eps)), n, _p(wx), _p(wy),
+                        _p(wz), _p(gx), _p(gy), _p(gz), _p(gm))
+    return np.stack([gx, gy, gz], axis=1), gm
 
 
 def spline(r2, twoh):

@@ -23,6 +23,10 @@
  *                              Q = sum m (x - cm)(x - cm)^T, exact bbox, radius
  *                              formulae, for every node.
  *   or_intersect_box_point  -- brute-force predicate cases with exact ties.
+ *   or_direct_vjp           -- central finite differences of or_direct on
+ *                              small sets (positions and masses, every
+ *                              softening branch), and the symmetry identity
+ *                              sum_i g_i = 0 for translation invariance.
  *   or_gravity / or_lists   -- interaction sets against a brute-force
  *                              enumeration of the opening predicate; forces
  *                              against or_direct (tree error ~1e-3 at
@@ -741,6 +745,56 @@ void or_direct(const double* x, const double* y, const double* z,
     az[s] = sz;
   }
   (void)nthreads;
+}
+
+/* d b / d r of GravityVisitor::SPLINE's force factor b
+ * (examples/GravityVisitor.h:82-120), differentiated term by term with
+ * u = 2 r / twoh. */
+static double spline_db_dr(double r, double twoh) {
+  if (r < twoh) {
+    const double dih = 2.0 / twoh;
+    const double u = r * dih;
+    if (u < 1.0) return pow(dih, 4) * (-2.4 * u + 1.5 * u * u);
+    return 0.2 / pow(r, 4) + pow(dih, 4) * (-3.0 + 2.4 * u - 0.5 * u * u);
+  }
+  return -3.0 / pow(r, 4);
+}
+
+/* Vector-Jacobian product of the direct sum (no counterpart in the
+ * reference): for cotangents w on the accelerations of or_direct (all
+ * targets), g = (da/dx)^T w and gm = (da/dm)^T w.  Serial loop over ordered
+ * pairs with the explicit 3x3 block da_i/dd_ij = m_j (b I + b'/r d d^T),
+ * d = x_j - x_i. */
+void or_direct_vjp(const double* x, const double* y, const double* z,
+                   const double* m, const double* eps, int64_t n,
+                   const double* wx, const double* wy, const double* wz,
+                   double* gx, double* gy, double* gz, double* gm) {
+  for (int64_t i = 0; i < n; i++) gx[i] = gy[i] = gz[i] = gm[i] = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    const double w[3] = {wx[i], wy[i], wz[i]};
+    for (int64_t j = 0; j < n; j++) {
+      const double d[3] = {x[j] - x[i], y[j] - y[i], z[j] - z[i]};
+      const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (rsq == 0) continue;
+      const double r = sqrt(rsq);
+      double a, b;
+      or_spline(rsq, eps[j] + eps[i], &a, &b);
+      const double bp = spline_db_dr(r, eps[j] + eps[i]);
+      double J[3][3];
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) J[p][q] = m[j] * ((p == q ? b : 0.0) + bp / r * d[p] * d[q]);
+      double jt[3] = {0, 0, 0};
+      for (int q = 0; q < 3; q++)
+        for (int p = 0; p < 3; p++) jt[q] += J[p][q] * w[p];
+      gx[i] -= jt[0];
+      gy[i] -= jt[1];
+      gz[i] -= jt[2];
+      gx[j] += jt[0];
+      gy[j] += jt[1];
+      gz[j] += jt[2];
+      gm[j] += b * (d[0] * w[0] + d[1] * w[1] + d[2] * w[2]);
+    }
+  }
 }
 
 void or_kick_drift(double* x, double* y, double* z,

@@ -122,6 +122,13 @@ int or_intersect_box_point(const double lo[3], const double hi[3],
                            const double p[3], double rsq);
 
 /* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
+/* (da/dx)^T w and (da/dm)^T w of or_direct over all targets (test-only
+ * reference for the adjoint). */
+void or_direct_vjp(const double* x, const double* y, const double* z,
+                   const double* m, const double* eps, int64_t n,
+                   const double* wx, const double* wy, const double* wz,
+                   double* gx, double* gy, double* gz, double* gm);
+
 void or_kick_drift(double* x, double* y, double* z,
                    double* vx, double* vy, double* vz,
                    const double* ax, const double* ay, const double* az,

@@ -14,7 +14,9 @@ PT_TREE_OCT, PT_TREE_BINARY_OCT, PT_TREE_KD, PT_TREE_LONGEST = 0, 1, 2, 3
 PT_DECOMP_OCT, PT_DECOMP_BINARY_OCT, PT_DECOMP_SFC, PT_DECOMP_KD, PT_DECOMP_LONGEST = 0, 1, 2, 3, 4
 PT_FP64, PT_FP32C = 0, 1
 PT_HOST, PT_DEVICE = 0, 1
-TIMERS = ("universe", "keys", "sort", "build", "moments", "traverse", "force", "integrate")
+PT_FORCE_TREE, PT_FORCE_DIRECT = 0, 1
+TIMERS = ("universe", "keys", "sort", "build", "moments", "traverse", "force", "integrate",
+          "adjoint")
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = (
@@ -25,6 +27,7 @@ EXPORTS = (
     "pt_get_keys", "pt_get_sorted", "pt_get_tree_size", "pt_get_nodes", "pt_get_node_data",
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
+    "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint",
 )
 
 
@@ -71,6 +74,19 @@ class PtError(RuntimeError):
 _lib = None
 
 
+def _init_torch_first():
+    """When torch is in the process, let it bring up its HIP state before the
+    library does: torch reports no GPU if the runtime was initialised by
+    another user first.  Both resolve to the same libamdhip64.so.7."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        try:
+            torch.cuda.is_available()
+        except Exception:
+            pass
+
+
 def load():
     """Loads libpt_gravity.so; raises if it has not been built."""
     global _lib
@@ -80,6 +96,7 @@ def load():
         raise RuntimeError("libpt_gravity.so is not built (%s); run "
                            "`python -c 'import __graft_entry__ as g; g.build()'` or "
                            "`python paratreet_amd/build.py`" % LIB_PATH)
+    _init_torch_first()
     L = ctypes.CDLL(LIB_PATH)
     P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
     sig = {
@@ -114,6 +131,9 @@ def load():
         "pt_get_counters": (I32, [P, P]),
         "pt_set_timing": (I32, [P, ctypes.c_int]),
         "pt_get_timings": (I32, [P, P]),
+        "pt_force_vjp": (I32, [P, D, ctypes.c_int, P, P, P, ctypes.c_int]),
+        "pt_leapfrog_forward": (I32, [P, I32, D, D, ctypes.c_int, ctypes.c_int, I32, P]),
+        "pt_leapfrog_adjoint": (I32, [P, P, P, P, P, P, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,618 @@
+// Adjoint (vector-Jacobian products) of the gravity interactions, FP64.
+//
+// For a cotangent w_i on every particle's acceleration these kernels compute
+//   g_j  = sum_i (da_i/dx_j)^T w_i      (positions)
+//   gm_j = sum_i (da_i/dm_j)^T w_i      (masses)
+// for the tree force of GravityVisitor::node / leaf
+// (examples/GravityVisitor.h:37-160,184-224) on the same per-bucket
+// interaction lists the forward pass used, and for the FP64 direct sum.
+//
+// leaf() pair (target t, source j, d = x_j + off - x_t):  a_t += m_j b(|d|) d:
+//   da_t/dd = m_j K,  K = b I + (b'/|d|) d d^T  (symmetric), so the target
+//   gets -m_j K w_t, the source +m_j K w_t and the source mass b d.w_t.
+// node() pair (target t, node s, r = x_t - cm_s - off), quadrupole form
+//   a = -qir3 r + c q,  q = Q r,  qir = r.q / 2,  qir3 = b M + d qir - c tr:
+//   (da/dr)^T w = -qir3 w - k1 r (r.w) - d q (r.w) + c Q w + (c'/|r|) r (q.w),
+//   k1 = (M b' + qir d' - tr c') / |r|;
+//   the target gets it, the node centre gets minus it, the node mass
+//   -b (r.w), each stored quadrupole component d(a.w)/dQ, and the node
+//   softening d(a.w)/dtwoh (softened branch only).
+// addGravity (the openSoftening branch): a = M dd/|dd|^3, dd = cm + off - x_t,
+//   T = M (I/|dd|^3 - 3 dd dd^T/|dd|^5); the target gets -T w, the centre
+//   +T w, the mass dd.w/|dd|^3.
+// Node cotangents then flow to member particles through the closed forms of
+// the moments (MultipoleMoments::operator+=, examples/MultipoleMoments.h:
+// 204-270, is the parallel-axis merge of M = sum m, cm = sum m x / M,
+// Q = sum m u u^T with u = x - cm, soft = sum m e / M):
+//   dcm/dx_j = m_j/M I,  dQ_kl/dx_j = m_j (e_k u_l + u_k e_l),
+//   dM/dm_j = 1, dcm/dm_j = u_j/M, dQ/dm_j = u_j u_j^T, dsoft/dm_j = (e_j - soft)/M.
+// Opening decisions (open, openSoftening) are piecewise constant and held
+// fixed.
+#include "pt_common.h"
+#include "pt_kernels.h"
+#include "pt_walk.h"
+
+namespace {
+
+constexpr int kAdjWaves = 4;
+constexpr int kAdjThreads = kAdjWaves * PT_WAVE;
+constexpr int kANT = 8;       // targets per pass over a bucket's lists
+constexpr int kNodeAdj = 11;  // node cotangent: cm(3), M, Q(6: xx xy xz yy yz zz), soft
+
+// GravityVisitor::SPLINE force factor b and db/dr (examples/GravityVisitor.h:
+// 82-120); r is |d|, twoh the summed softening.
+__device__ __forceinline__ void spline_b_db(double r, double twoh, double& b, double& db) {
+  if (r < twoh) {
+    const double dih = 2.0 / twoh;
+    const double u = r * dih;
+    if (u < 1.0) {
+      b = dih * dih * dih * (4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u);
+      db = dih * dih * dih * dih * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
+    } else {
+      const double dir = 1.0 / r;
+      b = -1.0 / 15.0 * dir * dir * dir +
+          dih * dih * dih * (8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u - 1.0 / 6.0 * u * u * u);
+      db = 1.0 / 5.0 * dir * dir * dir * dir +
+           dih * dih * dih * dih * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
+    }
+  } else {
+    const double a = 1.0 / r;
+    b = a * a * a;
+    db = -3.0 * b * a;
+  }
+}
+
+// GravityVisitor::SPLINEQ b, c, d (examples/GravityVisitor.h:37-78, the
+// expressions as written there), their r-derivatives (br, cr, dr_) and their
+// twoh-derivatives at fixed r (bt, ct, dt_).
+struct SplineQ {
+  double b, c, d, br, cr, dr_, bt, ct, dt_;
+};
+
+__device__ __forceinline__ SplineQ splineq_grad(double r, double r2, double twoh) {
+  SplineQ s;
+  const double dir = 1.0 / r;
+  if (r2 < twoh * twoh) {
+    const double dih = 2.0 / twoh;
+    const double u = dih * r;
+    const double dih2 = dih * dih, dih3 = dih2 * dih, dih4 = dih3 * dih, dih5 = dih4 * dih,
+                 dih6 = dih5 * dih;
+    double bD, cD, dD;  // derivatives with respect to dih at fixed r
+    if (u < 1.0) {
+      s.b = dih3 * 4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u;
+      s.c = dih5 * 12.0 / 5.0 - 3.0 / 2.0 * u;
+      s.d = 3.0 / 2.0 * dih6 * dir;
+      s.br = dih * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
+      s.cr = -3.0 / 2.0 * dih;
+      s.dr_ = -3.0 / 2.0 * dih6 * dir * dir;
+      bD = 4.0 * dih2 + r * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
+      cD = 12.0 * dih4 - 3.0 / 2.0 * r;
+      dD = 9.0 * dih5 * dir;
+    } else {
+      const double dir2 = dir * dir, dir3 = dir2 * dir;
+      s.b = -1.0 / 15.0 * dir3 + dih3 * 8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u -
+            1.0 / 6.0 * u * u * u;
+      s.c = -1.0 / 5.0 * dir3 * dir2 + 3.0 * dih4 * dir - dih5 * 12.0 / 5.0 + 1.0 / 2.0 * u;
+      s.d = -dir3 * dir3 * dir + 3.0 * dih4 * dir3 - 1.0 / 2.0 * dih6 * dir;
+      s.br = 1.0 / 5.0 * dir2 * dir2 + dih * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
+      s.cr = dir3 * dir3 - 3.0 * dih4 * dir2 + 1.0 / 2.0 * dih;
+      s.dr_ = 7.0 * dir3 * dir3 * dir2 - 9.0 * dih4 * dir2 * dir2 + 1.0 / 2.0 * dih6 * dir2;
+      bD = 8.0 * dih2 + r * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
+      cD = 12.0 * dih3 * dir - 12.0 * dih4 + 1.0 / 2.0 * r;
+      dD = 12.0 * dih3 * dir3 - 3.0 * dih5 * dir;
+    }
+    const double dd_dt = -0.5 * dih2;  // d(dih)/d(twoh)
+    s.bt = bD * dd_dt;
+    s.ct = cD * dd_dt;
+    s.dt_ = dD * dd_dt;
+  } else {
+    const double a = dir;
+    s.b = a * a * a;
+    s.c = 3.0 * s.b * a * a;
+    s.d = 5.0 * s.c * a * a;
+    s.br = -3.0 * s.b * a;
+    s.cr = -5.0 * s.c * a;
+    s.dr_ = -7.0 * s.d * a;
+    s.bt = s.ct = s.dt_ = 0.0;
+  }
+  return s;
+}
+
+struct TgtA {
+  double x, y, z, e, wx, wy, wz, pad;
+};
+
+// One wave per bucket.  Node sources: one list entry per lane, looping over
+// the bucket's targets (8 per pass) from LDS; the node cotangent is summed
+// over the targets of a pass before one atomic add per field.  Leaf sources:
+// flattened to one source particle per lane; the source's gradient is summed
+// over the targets of a pass before its atomic add.  Target gradients are
+// wave-reduced per pass.
+__global__ void __launch_bounds__(kAdjThreads)
+adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+                    const GravNode* __restrict__ gn, const double* __restrict__ px,
+                    const double* __restrict__ py, const double* __restrict__ pz,
+                    const double* __restrict__ pm, const double* __restrict__ pe,
+                    const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
+                    const int64_t* __restrict__ pc_off, const int32_t* __restrict__ pc,
+                    double ox, double oy, double oz, const double* __restrict__ wx,
+                    const double* __restrict__ wy, const double* __restrict__ wz,
+                    double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
+                    double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn) {
+  __shared__ TgtA s_t[kAdjWaves][kANT];
+  __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
+  __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t bi = (int64_t)blockIdx.x * kAdjWaves + wave;
+  if (bi >= nb) return;
+  const Bucket b = bk[bi];
+  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
+  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
+  for (int g0 = 0; g0 < b.count; g0 += kANT) {
+    const int ng = (b.count - g0) < kANT ? (b.count - g0) : kANT;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < ng) {
+      const int64_t i = (int64_t)b.begin + g0 + lane;
+      s_t[wave][lane] = TgtA{px[i], py[i], pz[i], pe[i], wx[i], wy[i], wz[i], 0.0};
+    }
+    __builtin_amdgcn_wave_barrier();
+    double tg[kANT][3];
+#pragma unroll
+    for (int t = 0; t < kANT; t++) tg[t][0] = tg[t][1] = tg[t][2] = 0.0;
+
+    // ---- node() sources
+    for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
+      const int64_t e = e0 + lane;
+      if (e < pc1) {
+        const int32_t sid = pc[e];
+        const GravNode s = gn[sid];
+        const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
+        const bool mono = pt_open_softening(cx, cy, cz, s.soft, b);
+        double lam[kNodeAdj];
+#pragma unroll
+        for (int k = 0; k < kNodeAdj; k++) lam[k] = 0.0;
+#pragma unroll
+        for (int t = 0; t < kANT; t++) {
+          if (t < ng) {
+            const TgtA tt = s_t[wave][t];
+            if (mono) {
+              const double dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
+              const double r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 != 0) {
+                const double ir = 1.0 / sqrt(r2);
+                const double ir3 = ir * ir * ir;
+                const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+                const double f = -3.0 * s.mass * ir3 * ir * ir * dw;
+                const double Tx = s.mass * ir3 * tt.wx + f * dx;
+                const double Ty = s.mass * ir3 * tt.wy + f * dy;
+                const double Tz = s.mass * ir3 * tt.wz + f * dz;
+                tg[t][0] -= Tx;
+                tg[t][1] -= Ty;
+                tg[t][2] -= Tz;
+                lam[0] += Tx;
+                lam[1] += Ty;
+                lam[2] += Tz;
+                lam[3] += dw * ir3;
+              }
+            } else {
+              const double rx = (tt.x - s.cx) - ox, ry = (tt.y - s.cy) - oy,
+                           rz = (tt.z - s.cz) - oz;
+              const double r2 = rx * rx + ry * ry + rz * rz;
+              const double r = sqrt(r2);
+              const SplineQ q = splineq_grad(r, r2, s.soft + tt.e);
+              const double qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
+              const double qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
+              const double qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
+              const double qir = 0.5 * (qx * rx + qy * ry + qz * rz);
+              const double qir3 = q.b * s.mass + q.d * qir - q.c * s.tr;
+              const double rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
+              const double qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
+              const double Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
+              const double Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
+              const double Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
+              const double ir = 1.0 / r;
+              const double k1 = (s.mass * q.br + qir * q.dr_ - s.tr * q.cr) * ir;
+              const double k2 = q.cr * ir;
+              const double Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
+                                (k2 * qw) * rx;
+              const double Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
+                                (k2 * qw) * ry;
+              const double Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
+                                (k2 * qw) * rz;
+              tg[t][0] += Jx;
+              tg[t][1] += Jy;
+              tg[t][2] += Jz;
+              lam[0] -= Jx;
+              lam[1] -= Jy;
+              lam[2] -= Jz;
+              lam[3] -= q.b * rw;
+              const double hd = -0.5 * q.d * rw, hc = 0.5 * q.c * rw;
+              lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
+              lam[5] += 2.0 * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
+              lam[6] += 2.0 * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
+              lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
+              lam[8] += 2.0 * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
+              lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
+              lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * s.tr) * rw + q.ct * qw;
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kNodeAdj; k++)
+          if (lam[k] != 0.0) atomicAdd(&lam_node[(int64_t)k * nn + sid], lam[k]);
+      }
+    }
+
+    // ---- leaf() sources, flattened to particles
+    for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
+      const int64_t l = l0 + lane;
+      int32_t cnt = 0, beg = 0;
+      if (l < pp1) {
+        const TreeNode nd = nodes[pp[l]];
+        cnt = nd.count;
+        beg = nd.begin;
+      }
+      const int incl = pt_wave_incl_scan(cnt);
+      const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
+      __builtin_amdgcn_wave_barrier();
+      s_beg[wave][lane] = beg;
+      s_exc[wave][lane] = incl - cnt;
+      __builtin_amdgcn_wave_barrier();
+      for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+        const int slot = s0 + lane;
+        if (slot < total) {
+          // the last entry whose exclusive start is <= slot holds the slot
+          // (entries with zero count share the start of the next entry and
+          // come before it, so the last such entry is the non-empty one)
+          int lo = 0, hi = PT_WAVE - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_exc[wave][mid] <= slot) lo = mid;
+            else hi = mid - 1;
+          }
+          const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
+          const double jx = px[j] + ox, jy = py[j] + oy, jz = pz[j] + oz;
+          const double jm = pm[j], je = pe[j];
+          double sgx = 0.0, sgy = 0.0, sgz = 0.0, sgm = 0.0;
+#pragma unroll
+          for (int t = 0; t < kANT; t++) {
+            if (t < ng) {
+              const TgtA tt = s_t[wave][t];
+              const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
+              const double r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 != 0) {
+                const double r = sqrt(r2);
+                double bb, db;
+                spline_b_db(r, je + tt.e, bb, db);
+                const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+                const double f = db / r * dw;
+                const double Kx = jm * (bb * tt.wx + f * dx);
+                const double Ky = jm * (bb * tt.wy + f * dy);
+                const double Kz = jm * (bb * tt.wz + f * dz);
+                tg[t][0] -= Kx;
+                tg[t][1] -= Ky;
+                tg[t][2] -= Kz;
+                sgx += Kx;
+                sgy += Ky;
+                sgz += Kz;
+                sgm += bb * dw;
+              }
+            }
+          }
+          atomicAdd(&gx[j], sgx);
+          atomicAdd(&gy[j], sgy);
+          atomicAdd(&gz[j], sgz);
+          atomicAdd(&gmass[j], sgm);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+
+#pragma unroll
+    for (int t = 0; t < kANT; t++) {
+      if (t < ng) {
+        const double rx = pt_wave_sum(tg[t][0]);
+        const double ry = pt_wave_sum(tg[t][1]);
+        const double rz = pt_wave_sum(tg[t][2]);
+        if (lane == t) {
+          const int64_t i = (int64_t)b.begin + g0 + t;
+          atomicAdd(&gx[i], rx);
+          atomicAdd(&gy[i], ry);
+          atomicAdd(&gz[i], rz);
+        }
+      }
+    }
+  }
+}
+
+// Pushes node cotangents to every particle through the closed-form moments of
+// its leaf and each ancestor (parent links up to the root).  head/head_pos
+// give the particle's bucket (head_pos[j] + head[j] - 1).
+__global__ void __launch_bounds__(256)
+adjoint_down_kernel(int64_t n, const uint32_t* __restrict__ head,
+                    const uint32_t* __restrict__ head_pos, const int32_t* __restrict__ bucket_node,
+                    const int32_t* __restrict__ parent, const GravNode* __restrict__ gn,
+                    const double* __restrict__ lam_node, int64_t nn,
+                    const double* __restrict__ px, const double* __restrict__ py,
+                    const double* __restrict__ pz, const double* __restrict__ pm,
+                    const double* __restrict__ pe, double* __restrict__ gx,
+                    double* __restrict__ gy, double* __restrict__ gz,
+                    double* __restrict__ gmass) {
+  const int64_t j = blockIdx.x * 256ll + threadIdx.x;
+  if (j >= n) return;
+  const int64_t bucket = (int64_t)head_pos[j] + head[j] - 1;
+  int32_t s = bucket_node[bucket];
+  const double x = px[j], y = py[j], z = pz[j], m = pm[j], e = pe[j];
+  double ax = 0.0, ay = 0.0, az = 0.0, am = 0.0;
+  while (s >= 0) {
+    const GravNode g = gn[s];
+    if (g.mass != 0.0) {
+      double l[kNodeAdj];
+#pragma unroll
+      for (int k = 0; k < kNodeAdj; k++) l[k] = lam_node[(int64_t)k * nn + s];
+      const double ux = x - g.cx, uy = y - g.cy, uz = z - g.cz;
+      const double im = 1.0 / g.mass;
+      ax += m * (im * l[0] + 2.0 * l[4] * ux + l[5] * uy + l[6] * uz);
+      ay += m * (im * l[1] + l[5] * ux + 2.0 * l[7] * uy + l[8] * uz);
+      az += m * (im * l[2] + l[6] * ux + l[8] * uy + 2.0 * l[9] * uz);
+      am += l[3] + (l[0] * ux + l[1] * uy + l[2] * uz) * im + l[4] * ux * ux + l[5] * ux * uy +
+            l[6] * ux * uz + l[7] * uy * uy + l[8] * uy * uz + l[9] * uz * uz +
+            l[10] * (e - g.soft) * im;
+    }
+    s = parent[s];
+  }
+  gx[j] += ax;
+  gy[j] += ay;
+  gz[j] += az;
+  gmass[j] += am;
+}
+
+// FP64 direct sum of the leaf() kernel over every pair, one block per target.
+__global__ void __launch_bounds__(256)
+direct_all_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                  const double* __restrict__ z, const double* __restrict__ m,
+                  const double* __restrict__ e, double* __restrict__ ax, double* __restrict__ ay,
+                  double* __restrict__ az, double* __restrict__ pot) {
+  __shared__ double red[3][256 / PT_WAVE];
+  const int64_t i = blockIdx.x;
+  const double tx = x[i], ty = y[i], tz = z[i], te = e[i];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) {
+    const double dx = x[j] - tx, dy = y[j] - ty, dz = z[j] - tz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 != 0) {
+      double b, db;
+      spline_b_db(sqrt(r2), e[j] + te, b, db);
+      const double f = b * m[j];
+      sx += dx * f;
+      sy += dy * f;
+      sz += dz * f;
+    }
+  }
+  sx = pt_wave_sum(sx);
+  sy = pt_wave_sum(sy);
+  sz = pt_wave_sum(sz);
+  const int wave = threadIdx.x / PT_WAVE;
+  if ((threadIdx.x & (PT_WAVE - 1)) == 0) {
+    red[0][wave] = sx;
+    red[1][wave] = sy;
+    red[2][wave] = sz;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[3] = {0, 0, 0};
+    for (int w = 0; w < 256 / PT_WAVE; w++)
+      for (int k = 0; k < 3; k++) r[k] += red[k][w];
+    ax[i] = r[0];
+    ay[i] = r[1];
+    az[i] = r[2];
+    pot[i] = 0.0;
+  }
+}
+
+// Exact VJP of the direct sum, one block per particle i:
+//   g_i += sum_j K_ij (m_i w_j - m_j w_i),  gm_i += sum_j b_ij (x_i - x_j).w_j.
+__global__ void __launch_bounds__(256)
+direct_adjoint_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                      const double* __restrict__ z, const double* __restrict__ m,
+                      const double* __restrict__ e, const double* __restrict__ wx,
+                      const double* __restrict__ wy, const double* __restrict__ wz,
+                      double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
+                      double* __restrict__ gmass) {
+  __shared__ double red[4][256 / PT_WAVE];
+  const int64_t i = blockIdx.x;
+  const double tx = x[i], ty = y[i], tz = z[i], te = e[i], tm = m[i];
+  const double twx = wx[i], twy = wy[i], twz = wz[i];
+  double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) {
+    const double dx = x[j] - tx, dy = y[j] - ty, dz = z[j] - tz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 != 0) {
+      const double r = sqrt(r2);
+      double b, db;
+      spline_b_db(r, e[j] + te, b, db);
+      const double vx = tm * wx[j] - m[j] * twx;
+      const double vy = tm * wy[j] - m[j] * twy;
+      const double vz = tm * wz[j] - m[j] * twz;
+      const double dv = dx * vx + dy * vy + dz * vz;
+      const double f = db / r * dv;
+      sx += b * vx + f * dx;
+      sy += b * vy + f * dy;
+      sz += b * vz + f * dz;
+      sm -= b * (dx * wx[j] + dy * wy[j] + dz * wz[j]);
+    }
+  }
+  sx = pt_wave_sum(sx);
+  sy = pt_wave_sum(sy);
+  sz = pt_wave_sum(sz);
+  sm = pt_wave_sum(sm);
+  const int wave = threadIdx.x / PT_WAVE;
+  if ((threadIdx.x & (PT_WAVE - 1)) == 0) {
+    red[0][wave] = sx;
+    red[1][wave] = sy;
+    red[2][wave] = sz;
+    red[3][wave] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 256 / PT_WAVE; w++)
+      for (int k = 0; k < 4; k++) r[k] += red[k][w];
+    gx[i] += r[0];
+    gy[i] += r[1];
+    gz[i] += r[2];
+    gmass[i] += r[3];
+  }
+}
+
+// Reverse of one kick-drift step (v' = v + dt a(x), x' = x + dt v'):
+//   w = dt^2 lx' + dt lv'  (cotangent of a(x)),  lv = lv' + dt lx'.
+// lx itself becomes lx' + (da/dx)^T w once the VJP has been added.
+__global__ void __launch_bounds__(256)
+adjoint_w_kernel(int64_t n, double dt, const double* __restrict__ lx,
+                 const double* __restrict__ ly, const double* __restrict__ lz,
+                 double* __restrict__ lvx, double* __restrict__ lvy, double* __restrict__ lvz,
+                 double* __restrict__ wx, double* __restrict__ wy, double* __restrict__ wz) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const double dt2 = dt * dt;
+  wx[i] = dt2 * lx[i] + dt * lvx[i];
+  wy[i] = dt2 * ly[i] + dt * lvy[i];
+  wz[i] = dt2 * lz[i] + dt * lvz[i];
+  lvx[i] += dt * lx[i];
+  lvy[i] += dt * ly[i];
+  lvz[i] += dt * lz[i];
+}
+
+__global__ void __launch_bounds__(256)
+gather3_kernel(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ a,
+               const double* __restrict__ b, const double* __restrict__ c,
+               double* __restrict__ as, double* __restrict__ bs, double* __restrict__ cs) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  as[i] = a[j];
+  bs[i] = b[j];
+  cs[i] = c[j];
+}
+
+// out[perm[i]] += in[i] for four arrays (sorted -> caller order).
+__global__ void __launch_bounds__(256)
+scatter_add4_kernel(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ a,
+                    const double* __restrict__ b, const double* __restrict__ c,
+                    const double* __restrict__ d, double* __restrict__ oa,
+                    double* __restrict__ ob, double* __restrict__ oc, double* __restrict__ od) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  oa[j] += a[i];
+  ob[j] += b[i];
+  oc[j] += c[i];
+  od[j] += d[i];
+}
+
+// n x 3 interleaved <-> three arrays.
+__global__ void __launch_bounds__(256)
+split3_kernel(int64_t n, const double* __restrict__ v, double* __restrict__ a,
+              double* __restrict__ b, double* __restrict__ c) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  a[i] = v[3 * i];
+  b[i] = v[3 * i + 1];
+  c[i] = v[3 * i + 2];
+}
+
+__global__ void __launch_bounds__(256)
+join3_kernel(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+             const double* __restrict__ c, double* __restrict__ v) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  v[3 * i] = a[i];
+  v[3 * i + 1] = b[i];
+  v[3 * i + 2] = c[i];
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int pt_node_adjoint_fields() { return kNodeAdj; }
+
+int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const GravNode* gn, const double* x, const double* y,
+                            const double* z, const double* m, const double* e,
+                            const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
+                            const int32_t* pc, const double off[3], const double* wx,
+                            const double* wy, const double* wz, double* gx, double* gy,
+                            double* gz, double* gm, double* lam_node, int64_t nn, hipStream_t s) {
+  if (nb <= 0) return 0;
+  adjoint_list_kernel<<<(unsigned)((nb + kAdjWaves - 1) / kAdjWaves), kAdjThreads, 0, s>>>(
+      bk, nb, nodes, gn, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2], wx, wy,
+      wz, gx, gy, gz, gm, lam_node, nn);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_adjoint_down(int64_t n, const uint32_t* head, const uint32_t* head_pos,
+                           const int32_t* bucket_node, const int32_t* parent, const GravNode* gn,
+                           const double* lam_node, int64_t nn, const double* x, const double* y,
+                           const double* z, const double* m, const double* e, double* gx,
+                           double* gy, double* gz, double* gm, hipStream_t s) {
+  if (n <= 0) return 0;
+  adjoint_down_kernel<<<grid_for(n), 256, 0, s>>>(n, head, head_pos, bucket_node, parent, gn,
+                                                  lam_node, nn, x, y, z, m, e, gx, gy, gz, gm);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_direct_all(int64_t n, const double* x, const double* y, const double* z,
+                         const double* m, const double* e, double* ax, double* ay, double* az,
+                         double* pot, hipStream_t s) {
+  if (n <= 0) return 0;
+  direct_all_kernel<<<(unsigned)n, 256, 0, s>>>(n, x, y, z, m, e, ax, ay, az, pot);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_direct_adjoint(int64_t n, const double* x, const double* y, const double* z,
+                             const double* m, const double* e, const double* wx,
+                             const double* wy, const double* wz, double* gx, double* gy,
+                             double* gz, double* gm, hipStream_t s) {
+  if (n <= 0) return 0;
+  direct_adjoint_kernel<<<(unsigned)n, 256, 0, s>>>(n, x, y, z, m, e, wx, wy, wz, gx, gy, gz, gm);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_adjoint_w(int64_t n, double dt, const double* lx, const double* ly,
+                        const double* lz, double* lvx, double* lvy, double* lvz, double* wx,
+                        double* wy, double* wz, hipStream_t s) {
+  if (n <= 0) return 0;
+  adjoint_w_kernel<<<grid_for(n), 256, 0, s>>>(n, dt, lx, ly, lz, lvx, lvy, lvz, wx, wy, wz);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_gather3(const uint32_t* perm, int64_t n, const double* a, const double* b,
+                      const double* c, double* as, double* bs, double* cs, hipStream_t s) {
+  if (n <= 0) return 0;
+  gather3_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, a, b, c, as, bs, cs);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_scatter_add4(const uint32_t* perm, int64_t n, const double* a, const double* b,
+                           const double* c, const double* d, double* oa, double* ob, double* oc,
+                           double* od, hipStream_t s) {
+  if (n <= 0) return 0;
+  scatter_add4_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, a, b, c, d, oa, ob, oc, od);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_split3(int64_t n, const double* v, double* a, double* b, double* c, hipStream_t s) {
+  if (n <= 0) return 0;
+  split3_kernel<<<grid_for(n), 256, 0, s>>>(n, v, a, b, c);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_join3(int64_t n, const double* a, const double* b, const double* c, double* v,
+                    hipStream_t s) {
+  if (n <= 0) return 0;
+  join3_kernel<<<grid_for(n), 256, 0, s>>>(n, a, b, c, v);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

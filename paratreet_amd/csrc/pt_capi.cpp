@@ -83,6 +83,16 @@ struct pt_ctx {
   DBuf<unsigned long long> counters;
   DBuf<int32_t> derr;
   pt_counters h_counters{};
+  // leapfrog trajectory and adjoint
+  DBuf<double> ckpt;      // checkpoints, 6n each (x y z vx vy vz), + final state
+  DBuf<double> seg;       // positions of the steps of one segment, 3n each
+  DBuf<double> adj;       // adjoint work arrays (17n)
+  DBuf<double> lam_node;  // node cotangents, field-major
+  bool traj_valid = false;
+  int32_t traj_steps = 0, traj_every = 1;
+  int traj_mode = 0, traj_prec = 0;
+  double traj_dt = 0.0, traj_theta = 0.7;
+  int64_t traj_n = 0;
   // scratch
   DBuf<char> scratch;
   // timing
@@ -295,6 +305,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
+  release(c->ckpt); release(c->seg); release(c->adj); release(c->lam_node);
   for (int i = 0; i < PT_T_COUNT; i++) {
     if (c->ev[i][0]) (void)hipEventDestroy(c->ev[i][0]);
     if (c->ev[i][1]) (void)hipEventDestroy(c->ev[i][1]);
@@ -338,6 +349,7 @@ pt_status pt_set_particles(pt_ctx* c, int64_t n, const double* x, const double* 
   PT_TRY_HIP(c, ensure(c->bucket_node, std::max<int64_t>(n, 1)));
   c->n = n;
   c->have_forces = false;
+  c->traj_valid = false;
   const hipMemcpyKind kind = where == PT_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   const size_t bytes = (size_t)n * sizeof(double);
   if (n > 0) {
@@ -720,6 +732,280 @@ pt_status pt_direct_sum(pt_ctx* c, const int64_t* targets, int64_t nt, double* a
   if (e != hipSuccess) return hip_fail(c, "pt_direct_sum", e);
   return PT_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- adjoint
+
+namespace {
+
+// Adjoint work arrays inside c->adj (17n doubles).
+struct AdjViews {
+  double *lx, *ly, *lz, *lvx, *lvy, *lvz, *lm;   // caller order
+  double *wx, *wy, *wz;                          // caller order
+  double *wsx, *wsy, *wsz, *gsx, *gsy, *gsz, *gsm;  // sorted order
+};
+
+pt_status adj_views(pt_ctx* c, AdjViews* v) {
+  const int64_t n = std::max<int64_t>(c->n, 1);
+  PT_TRY_HIP(c, ensure(c->adj, 17 * n));
+  double* p = c->adj.p;
+  double** f[17] = {&v->lx, &v->ly, &v->lz, &v->lvx, &v->lvy, &v->lvz, &v->lm, &v->wx, &v->wy,
+                    &v->wz, &v->wsx, &v->wsy, &v->wsz, &v->gsx, &v->gsy, &v->gsz, &v->gsm};
+  for (int k = 0; k < 17; k++) *f[k] = p + k * n;
+  return PT_OK;
+}
+
+pt_status zero(pt_ctx* c, double* p, int64_t count) {
+  PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)p, count, 0ull, c->stream));
+  return PT_OK;
+}
+
+// n x 3 caller array (host or device) -> three device arrays; `stage` is a
+// 3n device buffer used for host input.  NULL input gives zeros.
+pt_status load3(pt_ctx* c, const double* v, pt_memory where, double* a, double* b, double* d,
+                double* stage) {
+  const int64_t n = c->n;
+  if (!v) {
+    pt_status st;
+    if ((st = zero(c, a, n)) || (st = zero(c, b, n)) || (st = zero(c, d, n))) return st;
+    return PT_OK;
+  }
+  const double* src = v;
+  if (where == PT_HOST) {
+    PT_TRY_HIP(c, hipMemcpyAsync(stage, v, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    src = stage;
+  }
+  PT_TRY_LAUNCH(c, pt_launch_split3(n, src, a, b, d, c->stream));
+  return PT_OK;
+}
+
+pt_status store3(pt_ctx* c, const double* a, const double* b, const double* d, double* out,
+                 pt_memory where, double* stage) {
+  const int64_t n = c->n;
+  if (!out) return PT_OK;
+  double* dst = where == PT_HOST ? stage : out;
+  PT_TRY_LAUNCH(c, pt_launch_join3(n, a, b, d, dst, c->stream));
+  if (where == PT_HOST) {
+    PT_TRY_HIP(c, hipMemcpyAsync(out, stage, (size_t)n * 24, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return PT_OK;
+}
+
+pt_status store1(pt_ctx* c, const double* a, double* out, pt_memory where) {
+  if (!out) return PT_OK;
+  const hipMemcpyKind kind = where == PT_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  PT_TRY_HIP(c, hipMemcpyAsync(out, a, (size_t)c->n * 8, kind, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+pt_status copy_d2d(pt_ctx* c, double* dst, const double* src, int64_t count) {
+  PT_TRY_HIP(c, hipMemcpyAsync(dst, src, (size_t)count * 8, hipMemcpyDeviceToDevice, c->stream));
+  return PT_OK;
+}
+
+pt_status save_state(pt_ctx* c, double* dst) {
+  const int64_t n = c->n;
+  const double* src[6] = {c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p, c->vz.p};
+  pt_status st;
+  for (int k = 0; k < 6; k++)
+    if ((st = copy_d2d(c, dst + k * n, src[k], n))) return st;
+  return PT_OK;
+}
+
+pt_status load_state(pt_ctx* c, const double* src, bool with_velocities) {
+  const int64_t n = c->n;
+  double* dst[6] = {c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p, c->vz.p};
+  pt_status st;
+  for (int k = 0; k < (with_velocities ? 6 : 3); k++)
+    if ((st = copy_d2d(c, dst[k], src + k * n, n))) return st;
+  c->stage = S_PARTICLES;
+  c->have_forces = false;
+  c->lists_valid = false;
+  return PT_OK;
+}
+
+// Accelerations at the current positions with the given force model.
+pt_status traj_force(pt_ctx* c, int mode, double theta, pt_precision prec, pt_counters* k) {
+  if (mode == PT_FORCE_TREE) return pt_compute_forces(c, theta, nullptr, prec, k);
+  pt_status st;
+  if ((st = need(c, S_PARTICLES, "direct forces"))) return st;
+  {
+    Timer t(c, PT_T_FORCE);
+    PT_TRY_LAUNCH(c, pt_launch_direct_all(c->n, c->x.p, c->y.p, c->z.p, c->m.p, c->e.p, c->ax.p,
+                                          c->ay.p, c->az.p, c->pot.p, c->stream));
+  }
+  if (k) {
+    *k = pt_counters{};
+    k->n_part_ints = (uint64_t)c->n * (uint64_t)c->n;
+  }
+  c->have_forces = true;
+  return PT_OK;
+}
+
+// g (caller order) += (da/dx)^T w, gm += (da/dm)^T w at the current
+// positions; w in caller order.
+pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, const double* wx,
+                         const double* wy, const double* wz, double* gx, double* gy, double* gz,
+                         double* gm) {
+  pt_status st;
+  const int64_t n = c->n;
+  if (mode == PT_FORCE_DIRECT) {
+    Timer t(c, PT_T_ADJOINT);
+    PT_TRY_LAUNCH(c, pt_launch_direct_adjoint(n, c->x.p, c->y.p, c->z.p, c->m.p, c->e.p, wx, wy,
+                                              wz, gx, gy, gz, gm, c->stream));
+    return PT_OK;
+  }
+  if ((st = pt_remake_universe(c, nullptr))) return st;
+  if ((st = pt_assign_keys(c))) return st;
+  if ((st = pt_sort(c))) return st;
+  if ((st = pt_build_tree(c))) return st;
+  if ((st = pt_upward(c))) return st;
+  if ((st = pt_traverse(c, theta, nullptr, nullptr))) return st;
+  const int64_t nn = c->nn;
+  const int nf = pt_node_adjoint_fields();
+  PT_TRY_HIP(c, ensure(c->lam_node, nf * std::max<int64_t>(nn, 1)));
+  Timer t(c, PT_T_ADJOINT);
+  if ((st = zero(c, c->lam_node.p, nf * nn))) return st;
+  if ((st = zero(c, v.gsx, 4 * std::max<int64_t>(n, 1)))) return st;  // gsx gsy gsz gsm
+  PT_TRY_LAUNCH(c, pt_launch_gather3(c->perm.p, n, wx, wy, wz, v.wsx, v.wsy, v.wsz, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_adjoint_lists(c->bk.p, c->nb, c->tnode.p, c->gn.p, c->xs.p, c->ys.p,
+                                           c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
+                                           c->pc_off.p, c->pc.p, c->offset, v.wsx, v.wsy, v.wsz,
+                                           v.gsx, v.gsy, v.gsz, v.gsm, c->lam_node.p, nn,
+                                           c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_adjoint_down(n, c->head.p, c->head_pos.p, c->bucket_node.p,
+                                          c->nparent.p, c->gn.p, c->lam_node.p, nn, c->xs.p,
+                                          c->ys.p, c->zs.p, c->ms.p, c->es.p, v.gsx, v.gsy,
+                                          v.gsz, v.gsm, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_scatter_add4(c->perm.p, n, v.gsx, v.gsy, v.gsz, v.gsm, gx, gy, gz,
+                                          gm, c->stream));
+  return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_force_vjp(pt_ctx* c, double theta, pt_force_mode mode, const double* w,
+                       double* grad_x, double* grad_m, pt_memory where) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_PARTICLES, "pt_force_vjp"))) return st;
+  if (mode != PT_FORCE_TREE && mode != PT_FORCE_DIRECT) return fail(c, PT_ERR_ARG, "bad force mode");
+  if (!w) return fail(c, PT_ERR_ARG, "missing cotangent");
+  if (c->n == 0) return PT_OK;
+  if (mode == PT_FORCE_TREE && !(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
+  AdjViews v;
+  if ((st = adj_views(c, &v))) return st;
+  // the sorted-order block (gsx..) doubles as the host staging area before
+  // and after the tree stages use it
+  if ((st = load3(c, w, where, v.wx, v.wy, v.wz, v.gsx))) return st;
+  if ((st = zero(c, v.lx, 3 * c->n)) || (st = zero(c, v.lm, c->n))) return st;
+  if ((st = vjp_accumulate(c, mode, theta, v, v.wx, v.wy, v.wz, v.lx, v.ly, v.lz, v.lm))) return st;
+  if ((st = store3(c, v.lx, v.ly, v.lz, grad_x, where, v.gsx))) return st;
+  if ((st = store1(c, v.lm, grad_m, where))) return st;
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+pt_status pt_leapfrog_forward(pt_ctx* c, int32_t steps, double dt, double theta,
+                              pt_precision precision, pt_force_mode mode,
+                              int32_t checkpoint_every, pt_counters* counters) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_PARTICLES, "pt_leapfrog_forward"))) return st;
+  if (steps < 0) return fail(c, PT_ERR_ARG, "negative step count");
+  if (mode != PT_FORCE_TREE && mode != PT_FORCE_DIRECT) return fail(c, PT_ERR_ARG, "bad force mode");
+  if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  if (c->n == 0) return fail(c, PT_ERR_ARG, "no particles");
+  const int32_t every = checkpoint_every > 0 ? checkpoint_every : std::max<int32_t>(steps, 1);
+  const int64_t nseg = std::max<int64_t>(1, (steps + every - 1) / every);
+  const int64_t n = c->n;
+  c->traj_valid = false;
+  PT_TRY_HIP(c, ensure(c->ckpt, (nseg + 1) * 6 * n));
+  pt_counters total{};
+  if (steps == 0 && (st = save_state(c, c->ckpt.p))) return st;
+  for (int32_t k = 0; k < steps; k++) {
+    if (k % every == 0 && (st = save_state(c, c->ckpt.p + (int64_t)(k / every) * 6 * n))) return st;
+    pt_counters kc{};
+    if ((st = traj_force(c, mode, theta, precision, &kc))) return st;
+    total.n_node_ints += kc.n_node_ints;
+    total.n_part_ints += kc.n_part_ints;
+    total.n_opens += kc.n_opens;
+    total.n_closes += kc.n_closes;
+    if ((st = pt_kick_drift(c, dt))) return st;
+  }
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->traj_valid = true;
+  c->traj_steps = steps;
+  c->traj_every = every;
+  c->traj_mode = mode;
+  c->traj_prec = precision;
+  c->traj_dt = dt;
+  c->traj_theta = theta;
+  c->traj_n = n;
+  if (counters) *counters = total;
+  return PT_OK;
+}
+
+pt_status pt_leapfrog_adjoint(pt_ctx* c, const double* lam_x, const double* lam_v,
+                              double* grad_x, double* grad_v, double* grad_m, pt_memory where) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if (!c->traj_valid || c->traj_n != c->n)
+    return fail(c, PT_ERR_STATE, "pt_leapfrog_adjoint without a matching pt_leapfrog_forward");
+  const int64_t n = c->n;
+  const int32_t steps = c->traj_steps, every = c->traj_every;
+  const int64_t nseg = std::max<int64_t>(1, (steps + every - 1) / every);
+  const int mode = c->traj_mode;
+  const pt_precision prec = (pt_precision)c->traj_prec;
+  AdjViews v;
+  if ((st = adj_views(c, &v))) return st;
+  PT_TRY_HIP(c, ensure(c->seg, (int64_t)std::min(every, std::max(steps, 1)) * 3 * n));
+  double* final_state = c->ckpt.p + nseg * 6 * n;
+  if ((st = save_state(c, final_state))) return st;
+  if ((st = load3(c, lam_x, where, v.lx, v.ly, v.lz, v.gsx))) return st;
+  if ((st = load3(c, lam_v, where, v.lvx, v.lvy, v.lvz, v.gsx))) return st;
+  if ((st = zero(c, v.lm, n))) return st;
+  for (int64_t s = nseg - 1; s >= 0 && steps > 0; s--) {
+    const int32_t k0 = (int32_t)(s * every);
+    const int32_t k1 = std::min(steps, k0 + every);
+    // recompute the segment's positions from its checkpoint
+    if ((st = load_state(c, c->ckpt.p + s * 6 * n, true))) return st;
+    for (int32_t k = k0; k < k1; k++) {
+      double* xk = c->seg.p + (int64_t)(k - k0) * 3 * n;
+      if ((st = copy_d2d(c, xk, c->x.p, n)) || (st = copy_d2d(c, xk + n, c->y.p, n)) ||
+          (st = copy_d2d(c, xk + 2 * n, c->z.p, n)))
+        return st;
+      if (k + 1 < k1) {
+        if ((st = traj_force(c, mode, c->traj_theta, prec, nullptr))) return st;
+        if ((st = pt_kick_drift(c, c->traj_dt))) return st;
+      }
+    }
+    // reverse through the segment
+    for (int32_t k = k1 - 1; k >= k0; k--) {
+      if ((st = load_state(c, c->seg.p + (int64_t)(k - k0) * 3 * n, false))) return st;
+      PT_TRY_LAUNCH(c, pt_launch_adjoint_w(n, c->traj_dt, v.lx, v.ly, v.lz, v.lvx, v.lvy, v.lvz,
+                                           v.wx, v.wy, v.wz, c->stream));
+      if ((st = vjp_accumulate(c, mode, c->traj_theta, v, v.wx, v.wy, v.wz, v.lx, v.ly, v.lz,
+                               v.lm)))
+        return st;
+    }
+  }
+  if ((st = store3(c, v.lx, v.ly, v.lz, grad_x, where, v.gsx))) return st;
+  if ((st = store3(c, v.lvx, v.lvy, v.lvz, grad_v, where, v.gsx))) return st;
+  if ((st = store1(c, v.lm, grad_m, where))) return st;
+  if ((st = load_state(c, final_state, true))) return st;
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ---------------------------------------------------------------- read-back
 

@@ -97,20 +97,7 @@ __device__ __forceinline__ void splineq(R dir, R r2, R twoh, R& a, R& b, R& c, R
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// GravityVisitor::openSoftening (examples/GravityVisitor.h:122-128) for source
-// centre (already offset) and softening against the target bucket.
-__device__ __forceinline__ bool open_softening(double sx, double sy, double sz, double ssoft,
-                                               const Bucket& b) {
-  const double sr = 2.0 * ssoft;
-  {
-    const double dx = sx - b.cx, dy = sy - b.cy, dz = sz - b.cz;
-    const double dsq = dx * dx + dy * dy + dz * dz;
-    const double rs = sr + 2.0 * b.soft;
-    if (dsq <= rs * rs) return true;
-  }
-  return pt_intersect_box_point(b, sx, sy, sz, sr * sr);
-}
-
+// Compensated (Kahan) FP32 accumulation.
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   const float y = v - c;
   const float t = s + y;
@@ -281,7 +268,7 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
     r.cz = (float)(cz - ccz);
     const bool mono = softening_far32(r.cx, r.cy, r.cz, h.soft, bf)
                           ? false
-                          : open_softening(cx, cy, cz, soft, b);
+                          : pt_open_softening(cx, cy, cz, soft, b);
     r.mass = h.mass;
     r.soft = mono ? -1.f : h.soft;
     r.qxx = h.qxx;
@@ -299,7 +286,7 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
     r.cy = s.cy;
     r.cz = s.cz;
     r.mass = s.mass;
-    r.soft = open_softening(cx, cy, cz, s.soft, b) ? -1.0 : s.soft;
+    r.soft = pt_open_softening(cx, cy, cz, s.soft, b) ? -1.0 : s.soft;
     r.qxx = s.qxx;
     r.qxy = s.qxy;
     r.qxz = s.qxz;

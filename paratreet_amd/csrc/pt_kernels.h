@@ -96,3 +96,37 @@ int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const 
                      hipStream_t s);
 int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+
+// Adjoint of the interactions (FP64).  lam_node holds pt_node_adjoint_fields()
+// x nn doubles (field-major) and must be zero on entry; g* accumulate.
+int pt_node_adjoint_fields();
+int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const GravNode* gn, const double* x, const double* y,
+                            const double* z, const double* m, const double* e,
+                            const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
+                            const int32_t* pc, const double off[3], const double* wx,
+                            const double* wy, const double* wz, double* gx, double* gy,
+                            double* gz, double* gm, double* lam_node, int64_t nn, hipStream_t s);
+int pt_launch_adjoint_down(int64_t n, const uint32_t* head, const uint32_t* head_pos,
+                           const int32_t* bucket_node, const int32_t* parent, const GravNode* gn,
+                           const double* lam_node, int64_t nn, const double* x, const double* y,
+                           const double* z, const double* m, const double* e, double* gx,
+                           double* gy, double* gz, double* gm, hipStream_t s);
+int pt_launch_direct_all(int64_t n, const double* x, const double* y, const double* z,
+                         const double* m, const double* e, double* ax, double* ay, double* az,
+                         double* pot, hipStream_t s);
+int pt_launch_direct_adjoint(int64_t n, const double* x, const double* y, const double* z,
+                             const double* m, const double* e, const double* wx,
+                             const double* wy, const double* wz, double* gx, double* gy,
+                             double* gz, double* gm, hipStream_t s);
+int pt_launch_adjoint_w(int64_t n, double dt, const double* lx, const double* ly,
+                        const double* lz, double* lvx, double* lvy, double* lvz, double* wx,
+                        double* wy, double* wz, hipStream_t s);
+int pt_launch_gather3(const uint32_t* perm, int64_t n, const double* a, const double* b,
+                      const double* c, double* as, double* bs, double* cs, hipStream_t s);
+int pt_launch_scatter_add4(const uint32_t* perm, int64_t n, const double* a, const double* b,
+                           const double* c, const double* d, double* oa, double* ob, double* oc,
+                           double* od, hipStream_t s);
+int pt_launch_split3(int64_t n, const double* v, double* a, double* b, double* c, hipStream_t s);
+int pt_launch_join3(int64_t n, const double* a, const double* b, const double* c, double* v,
+                    hipStream_t s);

@@ -56,6 +56,21 @@ __device__ __forceinline__ bool pt_gv_open(const TreeNode& s, const Bucket& b, d
   return pt_intersect_box_point(b, s.cx + ox, s.cy + oy, s.cz + oz, dataRsq);
 }
 
+// GravityVisitor::openSoftening (examples/GravityVisitor.h:122-128) with the
+// bucket as target: the source sphere (cm + offset, 2 soft) against the
+// bucket's sphere (cm, 2 soft), then against the bucket's box.
+__device__ __forceinline__ bool pt_open_softening(double sx, double sy, double sz, double ssoft,
+                                                  const Bucket& b) {
+  const double sr = 2.0 * ssoft;
+  {
+    const double dx = sx - b.cx, dy = sy - b.cy, dz = sz - b.cz;
+    const double dsq = dx * dx + dy * dy + dz * dz;
+    const double rs = sr + 2.0 * b.soft;
+    if (dsq <= rs * rs) return true;
+  }
+  return pt_intersect_box_point(b, sx, sy, sz, sr * sr);
+}
+
 // Result of one walk step for the calling lane.
 struct WalkLane {
   int32_t node;      // visited node id (valid if active)

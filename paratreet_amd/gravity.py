@@ -26,6 +26,8 @@ from . import _lib
 
 PT_FP64 = _lib.PT_FP64
 PT_FP32C = _lib.PT_FP32C
+FORCE_TREE = _lib.PT_FORCE_TREE
+FORCE_DIRECT = _lib.PT_FORCE_DIRECT
 
 _TREE_NAMES = {"oct": _lib.PT_TREE_OCT, "binoct": _lib.PT_TREE_BINARY_OCT,
                "kd": _lib.PT_TREE_KD, "longest": _lib.PT_TREE_LONGEST}
@@ -261,6 +263,36 @@ class Driver:
         out = np.zeros((len(t), 3))
         self._check(self.L.pt_direct_sum(self.ctx, _ptr(t), len(t), _ptr(out)))
         return out
+
+    # ---- differentiation (no counterpart in the reference) -----------
+    def force_vjp(self, w, theta=0.7, mode=FORCE_TREE):
+        """(da/dx)^T w and (da/dm)^T w at the current positions (host arrays)."""
+        w = _f64(w).reshape(self.n, 3)
+        gx = np.zeros((self.n, 3))
+        gm = np.zeros(self.n)
+        self._check(self.L.pt_force_vjp(self.ctx, float(theta), int(mode), _ptr(w), _ptr(gx),
+                                        _ptr(gm), _lib.PT_HOST))
+        return gx, gm
+
+    def leapfrog_forward(self, steps, dt, theta=0.7, precision=PT_FP64, mode=FORCE_TREE,
+                         checkpoint_every=0):
+        """pt_run with checkpoints kept for leapfrog_adjoint; returns counters."""
+        cnt = _lib.PtCounters()
+        self._check(self.L.pt_leapfrog_forward(self.ctx, int(steps), float(dt), float(theta),
+                                               int(precision), int(mode), int(checkpoint_every),
+                                               ctypes.byref(cnt)))
+        return cnt.as_tuple()
+
+    def leapfrog_adjoint(self, lam_x=None, lam_v=None):
+        """Gradients (x0, v0, m) of <lam_x, x_T> + <lam_v, v_T> (host arrays)."""
+        lx = None if lam_x is None else _f64(lam_x).reshape(self.n, 3)
+        lv = None if lam_v is None else _f64(lam_v).reshape(self.n, 3)
+        gx = np.zeros((self.n, 3))
+        gv = np.zeros((self.n, 3))
+        gm = np.zeros(self.n)
+        self._check(self.L.pt_leapfrog_adjoint(self.ctx, _ptr(lx), _ptr(lv), _ptr(gx), _ptr(gv),
+                                               _ptr(gm), _lib.PT_HOST))
+        return gx, gv, gm
 
     # ---- read-back ----------------------------------------------------
     def universe(self):

@@ -8,6 +8,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def pytest_sessionstart(session):
+    # torch reports no GPU if the HIP runtime was first initialised by the
+    # library, so torch comes up first (a no-op without a GPU)
+    import torch
+    torch.cuda.is_available()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libpt_gravity.so")
     config.addinivalue_line("markers", "slow: long-running CPU test")

@@ -312,3 +312,58 @@ def test_kick_drift_closed_form():
     xn, vn = O.kick_drift(x, v, a, dt)
     assert np.allclose(vn, v + a * dt, rtol=0, atol=1e-15)
     assert np.allclose(xn, x + (v + a * dt) * dt, rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------- adjoint
+
+def _vjp_set(n=36, seed=5):
+    """Small cluster whose pair distances cover the three SPLINE branches
+    (u < 1, 1 <= u < 2, beyond 2h) for softening 0.15."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.5, 0.5, (n, 3))
+    m = rng.uniform(0.5, 1.5, n) / n
+    e = np.full(n, 0.15)
+    r = np.linalg.norm(pos[:, None] - pos[None], axis=2)[np.triu_indices(n, 1)]
+    twoh = 0.3
+    assert (r < twoh / 2).any() and ((r >= twoh / 2) & (r < twoh)).any() and (r >= twoh).any()
+    return pos, m, e, rng.standard_normal((n, 3))
+
+
+def _dot_acc(pos, m, e, w):
+    acc = O.direct(pos[:, 0], pos[:, 1], pos[:, 2], m, e, np.arange(len(pos)))
+    return float(np.sum(acc * w))
+
+
+def test_direct_vjp_matches_finite_differences():
+    """or_direct_vjp against central differences of or_direct (h = 1e-6) for
+    every position coordinate and every mass."""
+    pos, m, e, w = _vjp_set()
+    g, gm = O.direct_vjp(pos, m, e, w)
+    h = 1e-6
+    fd = np.zeros_like(pos)
+    for i in range(len(pos)):
+        for k in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, k] += h
+            p2[i, k] -= h
+            fd[i, k] = (_dot_acc(p1, m, e, w) - _dot_acc(p2, m, e, w)) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 1e-6 * np.max(np.abs(g))
+    fdm = np.zeros(len(m))
+    for j in range(len(m)):
+        m1, m2 = m.copy(), m.copy()
+        hm = 1e-6 * m[j]
+        m1[j] += hm
+        m2[j] -= hm
+        fdm[j] = (_dot_acc(pos, m1, e, w) - _dot_acc(pos, m2, e, w)) / (2 * hm)
+    assert np.max(np.abs(fdm - gm)) < 1e-7 * np.max(np.abs(gm))
+
+
+def test_direct_vjp_invariants():
+    """Translation invariance (sum_j g_j = 0) and degree-1 homogeneity in the
+    masses (sum_j m_j gm_j = w . a)."""
+    pos, m, e, w = _vjp_set(seed=9)
+    g, gm = O.direct_vjp(pos, m, e, w)
+    scale = np.abs(g).max()
+    assert np.abs(g.sum(0)).max() < 1e-12 * scale * len(pos)
+    wa = _dot_acc(pos, m, e, w)
+    assert abs(np.dot(m, gm) - wa) < 1e-12 * abs(wa) * len(pos)
