@@ -103,6 +103,99 @@ def cpu_baseline(pos, m, eps, bucket, theta):
                       "tree, OpenMP over buckets, %.1f s" % (len(P.bucket_nodes), ints, dt)}
 
 
+ADJOINT_CONFIGS = {
+    # name: (n, bucket, theta, eps, generator, steps, dt, checkpoint_every, precision)
+    "plummer256k_adjoint": (262_144, 16, 0.7, 0.005, "plummer", 10, 1e-3, 0, "fp64"),
+    "halo4m_adjoint": (4_194_304, 32, 0.7, 0.01, "halo16", 50, 5e-4, 10, "fp32c"),
+}
+
+
+def _loss_grad(d, pos, vel, m, eps, steps, dt, theta, precision, mode, every):
+    """Forward + reverse sweep of L = sum m |x_T - x_0|^2; returns the
+    gradients (x0, v0, m) and the (forward, adjoint) wall times in ms."""
+    import torch
+    d.load(pos, m, eps, vel)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.leapfrog_forward(steps, dt, theta=theta, precision=precision, mode=mode,
+                       checkpoint_every=every)
+    t1 = time.perf_counter()
+    xT = d.state()[0]
+    lam = 2.0 * m[:, None] * (xT - pos)
+    t2 = time.perf_counter()
+    gx, gv, gm = d.leapfrog_adjoint(lam_x=lam)
+    t3 = time.perf_counter()
+    return (gx - lam, gv, gm + np.sum((xT - pos) ** 2, axis=1)), (t1 - t0) * 1e3, (t3 - t2) * 1e3
+
+
+def adjoint_bench(name, fd_coords, direct):
+    """Differentiable-trajectory workload: timing of the forward (with
+    checkpoints) and the reverse sweep, gradient checks against central
+    finite differences (h = 1e-6) and against the FP64 direct-sum gradient,
+    and the device memory the context holds."""
+    import torch
+    from paratreet_amd import ics
+    from paratreet_amd.gravity import (FORCE_DIRECT, FORCE_TREE, PT_FP32C, PT_FP64,
+                                       Configuration, Driver)
+    n, bucket, theta, eps0, gen, steps, dt, every, prec = ADJOINT_CONFIGS[name]
+    if gen == "plummer":
+        pos, vel, m = ics.plummer(n, seed=2025)
+    else:
+        pos, vel, m = ics.halo_mixture(n, n_halos=16, seed=4243)
+    eps = np.full(n, eps0)
+    precision = PT_FP32C if prec == "fp32c" else PT_FP64
+    free0 = torch.cuda.mem_get_info()[0]
+    d = Driver(Configuration(max_particles_per_leaf=bucket))
+    _loss_grad(d, pos, vel, m, eps, steps, dt, theta, precision, FORCE_TREE, every)  # warm-up
+    g, fwd_ms, adj_ms = _loss_grad(d, pos, vel, m, eps, steps, dt, theta, precision, FORCE_TREE,
+                                   every)
+    hbm = free0 - torch.cuda.mem_get_info()[0]
+    out = {"workload": name, "n": n, "bucket": bucket, "theta": theta, "eps": eps0,
+           "steps": steps, "dt": dt, "checkpoint_every": every or steps,
+           "forward_precision": prec, "adjoint_arithmetic": "f64",
+           "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "device_bytes_in_use": int(hbm)}
+    if precision == PT_FP32C:
+        g64, _, _ = _loss_grad(d, pos, vel, m, eps, steps, dt, theta, PT_FP64, FORCE_TREE, every)
+        out["fp32c_vs_fp64_gradient_rel_l2"] = {
+            k: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            for k, a, b in zip(("x0", "v0", "m"), g, g64)}
+    if precision == PT_FP32C:
+        # FP32 force rounding (~1e-7 of |a| per step) swamps an h = 1e-6
+        # difference of the loss; the FD check belongs to the FP64 workload
+        fd_coords = 0
+    if fd_coords:
+        rng = np.random.default_rng(7)
+        h = 1e-6
+        errs = []
+        scale = np.abs(g[0]).max()
+
+        def final(p):
+            d.load(p, m, eps, vel)
+            d.leapfrog_forward(steps, dt, theta=theta, precision=precision, mode=FORCE_TREE,
+                               checkpoint_every=every)
+            xT = d.state()[0]
+            return m * np.sum((xT - p) ** 2, axis=1)
+
+        for c in rng.choice(pos.size, fd_coords, replace=False):
+            i, k = divmod(int(c), 3)
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, k] += h
+            p2[i, k] -= h
+            fd = np.sum(final(p1) - final(p2)) / (2 * h)
+            errs.append(abs(fd - g[0][i, k]) / scale)
+        out["fd_x0_rel_err"] = {"coords": fd_coords, "h": h, "median": float(np.median(errs)),
+                                "max": float(np.max(errs))}
+    if direct:
+        gd, dfwd, dadj = _loss_grad(d, pos, vel, m, eps, steps, dt, theta, PT_FP64, FORCE_DIRECT,
+                                    every)
+        out["direct_sum_gradient_rel_l2"] = {
+            k: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            for k, a, b in zip(("x0", "v0", "m"), g, gd)}
+        out["direct_sum_ms"] = {"forward": dfwd, "adjoint": dadj}
+    d.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +204,11 @@ def main():
     ap.add_argument("--config", default="plummer1m", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the direct-sum error leg")
+    ap.add_argument("--adjoint", choices=sorted(ADJOINT_CONFIGS),
+                    help="run the differentiable-trajectory workload instead")
+    ap.add_argument("--fd-coords", type=int, default=64)
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the direct-sum gradient leg of --adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -120,6 +218,11 @@ def main():
 
     import torch
     from paratreet_amd.gravity import Configuration, Driver, PT_FP32C, PT_FP64
+    torch.cuda.is_available()
+
+    if args.adjoint:
+        adjoint_bench(args.adjoint, args.fd_coords, not args.no_direct)
+        return
 
     pos, vel, m, eps, bucket, theta = make_input(args.config)
     n = len(pos)
@@ -187,7 +290,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved32, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS,
                      "traffic": TRAFFIC_BYTES_FP32C if args.config == "plummer1m" else None,
-                     "kernel": "force32c_kernel",
+                     "kernel": "gravity_kernel<FP32C>",
                      "fp64_kernel": {"achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
                                      "frac": achieved64 / PEAK_FP64_TFLOPS}},
         "errors": err,
