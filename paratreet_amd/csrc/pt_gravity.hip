@@ -35,6 +35,8 @@
 // This file is compiled with -ffp-contract=off because it contains the walk's
 // opening test; the interaction arithmetic uses explicit fma() where the
 // reference's separately rounded order does not matter for the 1e-12 bound.
+#include <type_traits>
+
 #include "pt_common.h"
 #include "pt_kernels.h"
 #include "pt_walk.h"
@@ -154,13 +156,20 @@ template <>
 struct Lanes<false> {
   double tx, ty, tz, te;
   double ax, ay, az, ap;
+  float emax;  // largest target softening of the group
 };
 template <>
 struct Lanes<true> {
   float tx, ty, tz, te;
   float ax, ay, az, ap;
   float cx, cy, cz, cp;  // Kahan compensations
+  float emax;            // largest target softening of the group
 };
+
+template <bool F32>
+using SrcC = typename std::conditional<F32, SrcC32, SrcC64>::type;
+template <bool F32>
+using SrcP = typename std::conditional<F32, SrcP32, SrcP64>::type;
 
 // The target bucket in its own (box-centre) frame, FP32, for the quick
 // openSoftening rejection of the FP32 path.
@@ -201,6 +210,33 @@ __device__ __forceinline__ bool softening_far32(float sx, float sy, float sz, fl
   const float ez = fmaxf(fmaxf(f.loz - sz, sz - f.hiz), 0.f);
   const float db = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
   return da > ra * ra && db > rb * rb;
+}
+
+// False when no target of the bucket can come within softening reach of the
+// source: the box distance exceeds (e_s + e_max) by a margin (1e-5 relative
+// plus 1e-6 of the coordinate magnitudes) far above the rounding of the
+// FP32 bucket-frame coordinates, so every pair with this source takes the
+// unsoftened branch of SPLINE / SPLINEQ in FP32 and in FP64 alike.  Such
+// "far" sources are evaluated by the branch-free loops.
+__device__ __forceinline__ bool near32(float sx, float sy, float sz, float se, const BucketF& f,
+                                       float emax) {
+  const float eps = 1e-6f * (fabsf(sx) + fabsf(sy) + fabsf(sz) + f.ext);
+  const float reach = (se + emax) * 1.00001f + eps;
+  const float ex = fmaxf(fmaxf(f.lox - sx, sx - f.hix), 0.f);
+  const float ey = fmaxf(fmaxf(f.loy - sy, sy - f.hiy), 0.f);
+  const float ez = fmaxf(fmaxf(f.loz - sz, sz - f.hiz), 0.f);
+  const float db = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+  return !(db > reach * reach);
+}
+
+__device__ __forceinline__ float frame32(double v, double o, double cc) {
+  return (float)((v + o) - cc);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, PT_WAVE));
+  return v;
 }
 
 struct Group {
@@ -248,16 +284,19 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
     L.te = pe[i];
   }
   L.ax = L.ay = L.az = L.ap = 0;
+  L.emax = wave_max((float)L.te);
 }
 
 // ---- staging (one lane per source) ----------------------------------------
 
-template <bool F32, typename W_t>
-__device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
-                                           const GravNode* __restrict__ gn,
-                                           const GravNode32* __restrict__ gn32, const Bucket& b,
-                                           const BucketF& bf, double ox, double oy, double oz,
-                                           double ccx, double ccy, double ccz) {
+// Closed-node source record; `near` marks a node that takes the monopole
+// (openSoftening) branch or may be softened for some target of the group.
+template <bool F32>
+__device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __restrict__ gn,
+                                                 const GravNode32* __restrict__ gn32,
+                                                 const Bucket& b, const BucketF& bf, float emax,
+                                                 double ox, double oy, double oz, double ccx,
+                                                 double ccy, double ccz, bool& near) {
   const double cx = gn[id].cx + ox, cy = gn[id].cy + oy, cz = gn[id].cz + oz;
   const double soft = gn[id].soft;
   if constexpr (F32) {
@@ -278,7 +317,8 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
     r.qyz = h.qyz;
     r.qzz = h.qzz;
     r.tr = 0.5f * (h.qxx + h.qyy + h.qzz);
-    reinterpret_cast<SrcC32*>(W.stage)[i] = r;
+    near = mono || near32(r.cx, r.cy, r.cz, h.soft, bf, emax);
+    return r;
   } else {
     const GravNode s = gn[id];
     SrcC64 r;
@@ -286,7 +326,8 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
     r.cy = s.cy;
     r.cz = s.cz;
     r.mass = s.mass;
-    r.soft = pt_open_softening(cx, cy, cz, s.soft, b) ? -1.0 : s.soft;
+    const bool mono = pt_open_softening(cx, cy, cz, s.soft, b);
+    r.soft = mono ? -1.0 : s.soft;
     r.qxx = s.qxx;
     r.qxy = s.qxy;
     r.qxz = s.qxz;
@@ -294,7 +335,9 @@ __device__ __forceinline__ void stage_node(W_t& W, int i, int32_t id,
     r.qyz = s.qyz;
     r.qzz = s.qzz;
     r.tr = s.tr;
-    reinterpret_cast<SrcC64*>(W.stage)[i] = r;
+    near = mono || near32((float)(cx - ccx), (float)(cy - ccy), (float)(cz - ccz), (float)soft,
+                          bf, emax);
+    return r;
   }
 }
 
@@ -311,15 +354,17 @@ __device__ __forceinline__ void stage_null_node(W_t& W, int i) {
   }
 }
 
-template <bool F32, typename W_t>
-__device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
-                                               const double* __restrict__ px,
-                                               const double* __restrict__ py,
-                                               const double* __restrict__ pz,
-                                               const double* __restrict__ pm,
-                                               const double* __restrict__ pe, double ox,
-                                               double oy, double oz, double ccx, double ccy,
-                                               double ccz) {
+// Leaf-source particle record; `near` marks a particle that may be within
+// softening reach of some target of the group.
+template <bool F32>
+__device__ __forceinline__ SrcP<F32> particle_record(int64_t j, const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz,
+                                                     const double* __restrict__ pm,
+                                                     const double* __restrict__ pe,
+                                                     const BucketF& bf, float emax, double ox,
+                                                     double oy, double oz, double ccx, double ccy,
+                                                     double ccz, bool& near) {
   if constexpr (F32) {
     SrcP32 r;
     r.x = (float)((px[j] + ox) - ccx);
@@ -328,7 +373,8 @@ __device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
     r.m = (float)pm[j];
     r.e = (float)pe[j];
     r.pad0 = r.pad1 = r.pad2 = 0.f;
-    reinterpret_cast<SrcP32*>(W.stage)[i] = r;
+    near = near32(r.x, r.y, r.z, r.e, bf, emax);
+    return r;
   } else {
     SrcP64 r;
     r.x = px[j];
@@ -337,7 +383,9 @@ __device__ __forceinline__ void stage_particle(W_t& W, int i, int64_t j,
     r.m = pm[j];
     r.e = pe[j];
     r.pad = 0.0;
-    reinterpret_cast<SrcP64*>(W.stage)[i] = r;
+    near = near32(frame32(r.x, ox, ccx), frame32(r.y, oy, ccy), frame32(r.z, oz, ccz), (float)r.e,
+                  bf, emax);
+    return r;
   }
 }
 
@@ -366,101 +414,169 @@ __device__ __forceinline__ void pad_chunk(W_t& W, int nsrc, const Group& g) {
 }
 
 // ---- chunk evaluation (all lanes, no bounds tests) -------------------------
+//
+// A chunk holds its "near" sources in slots [0, nnear) and the rest after
+// them, so iterations k >= kfar = ceil(nnear / S) read far sources (or null
+// padding) only and run the unsoftened expressions without branches; the
+// first kfar iterations run the general code (monopole branch, SPLINE /
+// SPLINEQ softened branches).  The arithmetic of a pair is the same in both
+// loops.
+
+// One closed-node pair, general form (addGravity when s.soft < 0).
+template <bool F32>
+__device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& L,
+                                          typename std::conditional<F32, float, double>::type& px,
+                                          typename std::conditional<F32, float, double>::type& py,
+                                          typename std::conditional<F32, float, double>::type& pz,
+                                          typename std::conditional<F32, float, double>::type& pp,
+                                          double ox, double oy, double oz) {
+  if constexpr (F32) {
+    if (s.soft < 0.f) {
+      // addGravity: unsoftened monopole, no potential
+      const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
+      const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float ri = fminf(rsqrt32(rsq), 1e10f);
+      const float f = s.mass * (ri * ri * ri);
+      px = fmaf(dx, f, px);
+      py = fmaf(dy, f, py);
+      pz = fmaf(dz, f, pz);
+    } else {
+      const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
+      const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+      const float dir = rsqrt32(rsq);
+      const float twoh = s.soft + L.te;
+      float a, bb, c, d;
+      if (rsq < twoh * twoh) {
+        splineq<float>(dir, rsq, twoh, a, bb, c, d);
+      } else {
+        const float dir2 = dir * dir;
+        a = dir;
+        bb = dir * dir2;
+        c = 3.0f * bb * dir2;
+        d = 5.0f * c * dir2;
+      }
+      const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
+      const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
+      const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
+      const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+      const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
+      pp = fmaf(-s.mass, a, pp);
+      pp = fmaf(-c, qir, pp);
+      pp = fmaf(bb, s.tr, pp);
+      px = fmaf(-qir3, rx, px);
+      px = fmaf(c, qx, px);
+      py = fmaf(-qir3, ry, py);
+      py = fmaf(c, qy, py);
+      pz = fmaf(-qir3, rz, pz);
+      pz = fmaf(c, qz, pz);
+    }
+  } else {
+    if (s.soft < 0.0) {
+      const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
+      const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+      double f = s.mass / (rsq * sqrt(rsq));
+      f = rsq != 0 ? f : 0.0;
+      px = fma(dx, f, px);
+      py = fma(dy, f, py);
+      pz = fma(dz, f, pz);
+    } else {
+      const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
+      const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
+      const double dir = 1.0 / sqrt(rsq);
+      double a, bb, c, d;
+      splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
+      const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
+      const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
+      const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
+      const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
+      const double qir3 = fma(bb, s.mass, fma(d, qir, -c * s.tr));
+      pp += fma(-s.mass, a, fma(-c, qir, bb * s.tr));
+      px += fma(-qir3, rx, c * qx);
+      py += fma(-qir3, ry, c * qy);
+      pz += fma(-qir3, rz, c * qz);
+    }
+  }
+}
+
+// One closed-node pair known to take the unsoftened quadrupole branch.
+template <bool F32>
+__device__ __forceinline__ void node_pair_far(
+    const SrcC<F32>& s, const Lanes<F32>& L,
+    typename std::conditional<F32, float, double>::type& px,
+    typename std::conditional<F32, float, double>::type& py,
+    typename std::conditional<F32, float, double>::type& pz,
+    typename std::conditional<F32, float, double>::type& pp, double ox, double oy, double oz) {
+  if constexpr (F32) {
+    const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
+    const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    const float dir = rsqrt32(rsq);
+    const float dir2 = dir * dir;
+    const float a = dir;
+    const float bb = dir * dir2;
+    const float c = 3.0f * bb * dir2;
+    const float d = 5.0f * c * dir2;
+    const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
+    const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
+    const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
+    const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+    const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
+    pp = fmaf(-s.mass, a, pp);
+    pp = fmaf(-c, qir, pp);
+    pp = fmaf(bb, s.tr, pp);
+    px = fmaf(-qir3, rx, px);
+    px = fmaf(c, qx, px);
+    py = fmaf(-qir3, ry, py);
+    py = fmaf(c, qy, py);
+    pz = fmaf(-qir3, rz, pz);
+    pz = fmaf(c, qz, pz);
+  } else {
+    const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
+    const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
+    const double a = 1.0 / sqrt(rsq);
+    const double bb = a * a * a;
+    const double c = 3.0 * bb * a * a;
+    const double d = 5.0 * c * a * a;
+    const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
+    const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
+    const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
+    const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
+    const double qir3 = fma(bb, s.mass, fma(d, qir, -c * s.tr));
+    pp += fma(-s.mass, a, fma(-c, qir, bb * s.tr));
+    px += fma(-qir3, rx, c * qx);
+    py += fma(-qir3, ry, c * qy);
+    pz += fma(-qir3, rz, c * qz);
+  }
+}
 
 template <bool F32, typename W_t>
-__device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F32>& L,
+__device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F32>& L, int kfar,
                                            double ox, double oy, double oz) {
+  const SrcC<F32>* src = reinterpret_cast<const SrcC<F32>*>(W.stage);
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
-    const SrcC32* src = reinterpret_cast<const SrcC32*>(W.stage);
+    for (int k = 0; k < kfar; k++) node_pair<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
 #pragma unroll 2
-    for (int k = 0; k < g.iters; k++) {
-      const SrcC32 s = src[g.s + k * g.S];
-      if (s.soft < 0.f) {
-        // addGravity: unsoftened monopole, no potential
-        const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
-        const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ri = fminf(rsqrt32(rsq), 1e10f);
-        const float f = s.mass * (ri * ri * ri);
-        px = fmaf(dx, f, px);
-        py = fmaf(dy, f, py);
-        pz = fmaf(dz, f, pz);
-      } else {
-        const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
-        const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-        const float dir = rsqrt32(rsq);
-        const float twoh = s.soft + L.te;
-        float a, bb, c, d;
-        if (rsq < twoh * twoh) {
-          splineq<float>(dir, rsq, twoh, a, bb, c, d);
-        } else {
-          const float dir2 = dir * dir;
-          a = dir;
-          bb = dir * dir2;
-          c = 3.0f * bb * dir2;
-          d = 5.0f * c * dir2;
-        }
-        const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
-        const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
-        const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
-        const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
-        const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
-        pp = fmaf(-s.mass, a, pp);
-        pp = fmaf(-c, qir, pp);
-        pp = fmaf(bb, s.tr, pp);
-        px = fmaf(-qir3, rx, px);
-        px = fmaf(c, qx, px);
-        py = fmaf(-qir3, ry, py);
-        py = fmaf(c, qy, py);
-        pz = fmaf(-qir3, rz, pz);
-        pz = fmaf(c, qz, pz);
-      }
-    }
+    for (int k = kfar; k < g.iters; k++)
+      node_pair_far<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
     kahan_add(L.ax, L.cx, px);
     kahan_add(L.ay, L.cy, py);
     kahan_add(L.az, L.cz, pz);
     kahan_add(L.ap, L.cp, pp);
   } else {
-    const SrcC64* src = reinterpret_cast<const SrcC64*>(W.stage);
-    for (int k = 0; k < g.iters; k++) {
-      const SrcC64 s = src[g.s + k * g.S];
-      if (s.soft < 0.0) {
-        // addGravity: unsoftened monopole, no potential
-        const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
-        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-        double f = s.mass / (rsq * sqrt(rsq));
-        f = rsq != 0 ? f : 0.0;
-        L.ax = fma(dx, f, L.ax);
-        L.ay = fma(dy, f, L.ay);
-        L.az = fma(dz, f, L.az);
-      } else {
-        const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
-        const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-        const double dir = 1.0 / sqrt(rsq);
-        double a, bb, c, d;
-        splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
-        const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
-        const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
-        const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
-        const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
-        const double qir3 = fma(bb, s.mass, fma(d, qir, -c * s.tr));
-        L.ap += fma(-s.mass, a, fma(-c, qir, bb * s.tr));
-        L.ax += fma(-qir3, rx, c * qx);
-        L.ay += fma(-qir3, ry, c * qy);
-        L.az += fma(-qir3, rz, c * qz);
-      }
-    }
+    for (int k = 0; k < kfar; k++)
+      node_pair<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
+    for (int k = kfar; k < g.iters; k++)
+      node_pair_far<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
   }
 }
 
 template <bool F32, typename W_t>
 __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lanes<F32>& L,
-                                               double ox, double oy, double oz) {
+                                               int kfar, double ox, double oy, double oz) {
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f;
     const SrcP32* src = reinterpret_cast<const SrcP32*>(W.stage);
-#pragma unroll 2
-    for (int k = 0; k < g.iters; k++) {
+    for (int k = 0; k < kfar; k++) {
       const SrcP32 s = src[g.s + k * g.S];
       const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
       const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -485,17 +601,39 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       py = fmaf(dy, f, py);
       pz = fmaf(dz, f, pz);
     }
+#pragma unroll 4
+    for (int k = kfar; k < g.iters; k++) {
+      // far sources: rsq > 0 and beyond every softening reach
+      const SrcP32 s = src[g.s + k * g.S];
+      const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
+      const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float rinv = rsqrt32(rsq);
+      const float f = (rinv * rinv * rinv) * s.m;
+      px = fmaf(dx, f, px);
+      py = fmaf(dy, f, py);
+      pz = fmaf(dz, f, pz);
+    }
     kahan_add(L.ax, L.cx, px);
     kahan_add(L.ay, L.cy, py);
     kahan_add(L.az, L.cz, pz);
   } else {
     const SrcP64* src = reinterpret_cast<const SrcP64*>(W.stage);
-    for (int k = 0; k < g.iters; k++) {
+    for (int k = 0; k < kfar; k++) {
       const SrcP64 s = src[g.s + k * g.S];
       const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
       double f = spline_b64(rsq, s.e + L.te) * s.m;
       f = rsq != 0 ? f : 0.0;
+      L.ax = fma(dx, f, L.ax);
+      L.ay = fma(dy, f, L.ay);
+      L.az = fma(dz, f, L.az);
+    }
+    for (int k = kfar; k < g.iters; k++) {
+      const SrcP64 s = src[g.s + k * g.S];
+      const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
+      const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+      const double a = 1.0 / sqrt(rsq);
+      const double f = (a * a * a) * s.m;
       L.ax = fma(dx, f, L.ax);
       L.ay = fma(dy, f, L.ay);
       L.az = fma(dz, f, L.az);
@@ -513,8 +651,9 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
                                           const double* __restrict__ py,
                                           const double* __restrict__ pz,
                                           const double* __restrict__ pm,
-                                          const double* __restrict__ pe, double ox, double oy,
-                                          double oz, double ccx, double ccy, double ccz) {
+                                          const double* __restrict__ pe, const BucketF& bf,
+                                          double ox, double oy, double oz, double ccx, double ccy,
+                                          double ccz) {
   const int lane = __lane_id();
   const int incl = pt_wave_incl_scan(cnt);
   const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
@@ -536,14 +675,21 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
     const int lbeg = __shfl(beg, leaf, PT_WAVE);
     const int lexc = __shfl(exc, leaf, PT_WAVE);
     const int slot = s0 + lane;
-    if (slot < total) {
+    const bool act = slot < total;
+    bool nr = false;
+    SrcP<F32> r{};
+    if (act) {
       const int64_t j = (int64_t)lbeg + (slot - lexc);
-      stage_particle<F32, W_t>(W, lane, j, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+      r = particle_record<F32>(j, px, py, pz, pm, pe, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
     }
+    // near sources first, then far ones
+    const unsigned long long mn = pt_ballot(act && nr), mf = pt_ballot(act && !nr);
+    const int nnear = __popcll(mn);
+    if (act) reinterpret_cast<SrcP<F32>*>(W.stage)[nr ? pt_rank_in(mn) : nnear + pt_rank_in(mf)] = r;
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
     pad_chunk<F32, false, W_t>(W, nsrc, g);
     __builtin_amdgcn_wave_barrier();
-    eval_particles<F32, W_t>(W, g, L, ox, oy, oz);
+    eval_particles<F32, W_t>(W, g, L, (nnear + g.S - 1) / g.S, ox, oy, oz);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -554,11 +700,17 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
                                          const GravNode32* __restrict__ gn32, const Bucket& b,
                                          const BucketF& bf, double ox, double oy, double oz,
                                          double ccx, double ccy, double ccz) {
-  if (__lane_id() < nsrc)
-    stage_node<F32, W_t>(W, __lane_id(), id, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
+  const bool act = __lane_id() < nsrc;
+  bool nr = false;
+  SrcC<F32> r{};
+  if (act) r = node_record<F32>(id, gn, gn32, b, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
+  // near (monopole or possibly softened) nodes first, then far ones
+  const unsigned long long mn = pt_ballot(act && nr), mf = pt_ballot(act && !nr);
+  const int nnear = __popcll(mn);
+  if (act) reinterpret_cast<SrcC<F32>*>(W.stage)[nr ? pt_rank_in(mn) : nnear + pt_rank_in(mf)] = r;
   pad_chunk<F32, true, W_t>(W, nsrc, g);
   __builtin_amdgcn_wave_barrier();
-  eval_nodes<F32, W_t>(W, g, L, ox, oy, oz);
+  eval_nodes<F32, W_t>(W, g, L, (nnear + g.S - 1) / g.S, ox, oy, oz);
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -648,7 +800,7 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
         beg = nd.begin;
         cnt = nd.count;
       }
-      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
     }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
   }
@@ -739,7 +891,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
         __builtin_amdgcn_wave_barrier();
       }
       if (npp >= PT_WAVE) {
-        do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, ox, oy, oz,
+        do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox, oy, oz,
                             ccx, ccy, ccz);
         const int rest = npp - PT_WAVE;
         const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
@@ -760,7 +912,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
     if (npp > 0) {
       const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
       const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
-      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, ox, oy, oz, ccx, ccy, ccz);
+      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
     }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
   }
