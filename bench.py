@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="plummer1m", choices=sorted(CONFIGS))
+    ap.add_argument("--theta", type=float, default=None,
+                    help="opening angle override (the halo mixture is quoted at 0.5, 0.7, 0.9)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the direct-sum error leg")
     ap.add_argument("--adjoint", choices=sorted(ADJOINT_CONFIGS),
@@ -225,6 +227,8 @@ def main():
         return
 
     pos, vel, m, eps, bucket, theta = make_input(args.config)
+    if args.theta is not None:
+        theta = args.theta
     n = len(pos)
     d = Driver(Configuration(max_particles_per_leaf=bucket))
     d.load(pos, m, eps, vel)

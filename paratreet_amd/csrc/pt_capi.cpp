@@ -622,7 +622,7 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
 }
 
 // Fused walk + interactions (no lists), with a large-stack rerun if the
-// 512-entry walk stack overflowed for some bucket.
+// small walk stack (192 entries) overflowed for some bucket.
 static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
                                pt_precision precision, pt_counters* counters) {
   pt_status st = check_ctx(c);

@@ -46,6 +46,14 @@ namespace {
 constexpr int kWaves = 4;
 constexpr int kThreads = kWaves * PT_WAVE;
 constexpr int kGroupMax = PT_WAVE;  // targets per pass (one per lane at most)
+// Walk stack entries per wave.  The deepest walks measured on the workloads
+// here (a CPU replay of this walk order over 4000 sampled buckets: Plummer
+// 1M, theta 0.7: 57 entries; 64-halo mixture 4M, theta 0.5, bucket 32: 114)
+// fit the small stack with room to spare; a bucket that overflows it makes
+// the whole evaluation rerun with the big one.  The small stack keeps the
+// FP32 work-group at 5 per CU in LDS.
+constexpr int kSmallStack = 192;
+constexpr int kBigStack = 1536;
 
 // ---------------------------------------------------------------------------
 // arithmetic
@@ -242,9 +250,9 @@ __device__ __forceinline__ float wave_max(float v) {
 struct Group {
   int ng;      // targets in this pass
   int S;       // source slots = 64 / ng
+  float rS;    // 1 / S
   int s, t;    // this lane's slot and target (inactive lanes read slot 0)
   bool act;    // lane participates
-  int iters;   // ceil(64 / S)
   int64_t first;  // sorted index of the group's first target
 };
 
@@ -253,14 +261,22 @@ __device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
   const int rem = b.count - g0;
   g.ng = rem < kGroupMax ? rem : kGroupMax;
   g.S = PT_WAVE / g.ng;
+  g.rS = 1.0f / (float)g.S;
   const int lane = __lane_id();
   const int s = lane / g.ng;
   g.act = s < g.S;
   g.s = g.act ? s : 0;
   g.t = g.act ? lane - s * g.ng : 0;
-  g.iters = (PT_WAVE + g.S - 1) / g.S;
   g.first = (int64_t)b.begin + g0;
   return g;
+}
+
+// ceil(n / S) for 0 <= n <= 128 without an integer division: with
+// y = n + S - 1, floor((y + 0.5) / S) = floor(y / S) because y / S is either
+// an integer or at least 1/S above one, and the FP32 error of (y + 0.5) * rS
+// is far below 0.5 / S.
+__device__ __forceinline__ int ceil_div_s(int n, const Group& g) {
+  return (int)(((float)(n + g.S - 1) + 0.5f) * g.rS);
 }
 
 template <bool F32>
@@ -402,10 +418,11 @@ __device__ __forceinline__ void stage_null_particle(W_t& W, int i) {
   }
 }
 
-// Null sources for the slots [nsrc, S * iters) the evaluation will read.
+// Null sources for the slots [nsrc, S * ceil(nsrc / S)) the evaluation
+// will read.
 template <bool F32, bool NODE, typename W_t>
 __device__ __forceinline__ void pad_chunk(W_t& W, int nsrc, const Group& g) {
-  const int end = g.S * g.iters;
+  const int end = g.S * ceil_div_s(nsrc, g);
   const int lane = __lane_id();
   for (int k = nsrc + lane; k < end; k += PT_WAVE) {
     if (NODE) stage_null_node<F32, W_t>(W, k);
@@ -550,13 +567,13 @@ __device__ __forceinline__ void node_pair_far(
 
 template <bool F32, typename W_t>
 __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F32>& L, int kfar,
-                                           double ox, double oy, double oz) {
+                                           int kend, double ox, double oy, double oz) {
   const SrcC<F32>* src = reinterpret_cast<const SrcC<F32>*>(W.stage);
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
     for (int k = 0; k < kfar; k++) node_pair<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
 #pragma unroll 2
-    for (int k = kfar; k < g.iters; k++)
+    for (int k = kfar; k < kend; k++)
       node_pair_far<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
     kahan_add(L.ax, L.cx, px);
     kahan_add(L.ay, L.cy, py);
@@ -565,14 +582,15 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
   } else {
     for (int k = 0; k < kfar; k++)
       node_pair<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
-    for (int k = kfar; k < g.iters; k++)
+    for (int k = kfar; k < kend; k++)
       node_pair_far<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
   }
 }
 
 template <bool F32, typename W_t>
 __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lanes<F32>& L,
-                                               int kfar, double ox, double oy, double oz) {
+                                               int kfar, int kend, double ox, double oy,
+                                               double oz) {
   if constexpr (F32) {
     float px = 0.f, py = 0.f, pz = 0.f;
     const SrcP32* src = reinterpret_cast<const SrcP32*>(W.stage);
@@ -602,7 +620,7 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       pz = fmaf(dz, f, pz);
     }
 #pragma unroll 4
-    for (int k = kfar; k < g.iters; k++) {
+    for (int k = kfar; k < kend; k++) {
       // far sources: rsq > 0 and beyond every softening reach
       const SrcP32 s = src[g.s + k * g.S];
       const float dx = s.x - L.tx, dy = s.y - L.ty, dz = s.z - L.tz;
@@ -628,7 +646,7 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       L.ay = fma(dy, f, L.ay);
       L.az = fma(dz, f, L.az);
     }
-    for (int k = kfar; k < g.iters; k++) {
+    for (int k = kfar; k < kend; k++) {
       const SrcP64 s = src[g.s + k * g.S];
       const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -689,7 +707,7 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
     pad_chunk<F32, false, W_t>(W, nsrc, g);
     __builtin_amdgcn_wave_barrier();
-    eval_particles<F32, W_t>(W, g, L, (nnear + g.S - 1) / g.S, ox, oy, oz);
+    eval_particles<F32, W_t>(W, g, L, ceil_div_s(nnear, g), ceil_div_s(nsrc, g), ox, oy, oz);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -710,7 +728,7 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
   if (act) reinterpret_cast<SrcC<F32>*>(W.stage)[nr ? pt_rank_in(mn) : nnear + pt_rank_in(mf)] = r;
   pad_chunk<F32, true, W_t>(W, nsrc, g);
   __builtin_amdgcn_wave_barrier();
-  eval_nodes<F32, W_t>(W, g, L, (nnear + g.S - 1) / g.S, ox, oy, oz);
+  eval_nodes<F32, W_t>(W, g, L, ceil_div_s(nnear, g), ceil_div_s(nsrc, g), ox, oy, oz);
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -807,7 +825,7 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
 }
 
 template <bool F32, int STACK>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 5 : 1, 8)))
 gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
                const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
                const double* __restrict__ px,
@@ -955,8 +973,9 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// big_stack selects the 1536-entry walk stack (used after a 512-entry walk
-// reported an overflow; 1536 keeps the FP64 work-group under 64 KiB of LDS).
+// big_stack selects the kBigStack-entry walk stack (used after a walk with
+// the small stack reported an overflow; 1536 entries keep the FP64
+// work-group under 64 KiB of LDS).
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
                       const double* x, const double* y, const double* z, const double* m,
@@ -969,11 +988,11 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
       partials,                                                                                  \
       err)
   if (precision == 0) {
-    if (big_stack) PT_GRAVITY_LAUNCH(false, 1536);
-    else PT_GRAVITY_LAUNCH(false, 512);
+    if (big_stack) PT_GRAVITY_LAUNCH(false, kBigStack);
+    else PT_GRAVITY_LAUNCH(false, kSmallStack);
   } else {
-    if (big_stack) PT_GRAVITY_LAUNCH(true, 1536);
-    else PT_GRAVITY_LAUNCH(true, 512);
+    if (big_stack) PT_GRAVITY_LAUNCH(true, kBigStack);
+    else PT_GRAVITY_LAUNCH(true, kSmallStack);
   }
 #undef PT_GRAVITY_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
