@@ -287,6 +287,10 @@ def main():
         "n_part_ints": int(cnt32[1]), "n_node_ints": int(cnt32[0]),
         "fp64_value": value64, "fp64_ms_per_step": ms64,
         "fp32c_over_fp64": value / value64,
+        # the definition of t_force in the scope table: upward pass + traversal +
+        # interactions device time (keys, sort and build reported separately)
+        "t_force_ms": {"fp32c": st32["moments"] + f32_ms, "fp64": st64["moments"] + f64_ms},
+        "fp32c_over_fp64_t_force": (st64["moments"] + f64_ms) / (st32["moments"] + f32_ms),
         "force_kernel_ms": {"fp32c": f32_ms, "fp64": f64_ms},
         "force_only_ips": {"fp32c": ints / (f32_ms / 1000.0), "fp64": ints / (f64_ms / 1000.0)},
         "stage_ms_fp32c": st32, "stage_ms_fp64": st64,

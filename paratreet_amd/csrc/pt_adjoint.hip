@@ -32,12 +32,20 @@
 #include "pt_kernels.h"
 #include "pt_walk.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int kAdjWaves = 4;
 constexpr int kAdjThreads = kAdjWaves * PT_WAVE;
 constexpr int kANT = 8;       // targets per pass over a bucket's lists
 constexpr int kNodeAdj = 11;  // node cotangent: cm(3), M, Q(6: xx xy xz yy yz zz), soft
+// Nodes of the top levels (ids < kHotNodes in level order) are in the lists
+// of most buckets; their cotangents are accumulated into kHotParts private
+// copies (chosen by work-group) and summed afterwards, so concurrent atomic
+// adds to one address come from 1/kHotParts of the work-groups.
+constexpr int64_t kHotNodes = 4096;
+constexpr int kHotParts = 32;
 
 // GravityVisitor::SPLINE force factor b and db/dr (examples/GravityVisitor.h:
 // 82-120); r is |d|, twoh the summed softening.
@@ -138,7 +146,8 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
                     double ox, double oy, double oz, const double* __restrict__ wx,
                     const double* __restrict__ wy, const double* __restrict__ wz,
                     double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
-                    double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn) {
+                    double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn,
+                    double* __restrict__ hot, int64_t nhot) {
   __shared__ TgtA s_t[kAdjWaves][kANT];
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
@@ -238,9 +247,13 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
             }
           }
         }
+        double* dst = sid < nhot
+                          ? hot + (int64_t)(blockIdx.x % kHotParts) * kNodeAdj * nhot + sid
+                          : lam_node + sid;
+        const int64_t stride = sid < nhot ? nhot : nn;
 #pragma unroll
         for (int k = 0; k < kNodeAdj; k++)
-          if (lam[k] != 0.0) atomicAdd(&lam_node[(int64_t)k * nn + sid], lam[k]);
+          if (lam[k] != 0.0) atomicAdd(dst + (int64_t)k * stride, lam[k]);
       }
     }
 
@@ -533,9 +546,25 @@ join3_kernel(int64_t n, const double* __restrict__ a, const double* __restrict__
   v[3 * i + 2] = c[i];
 }
 
+// lam_node[k][i] = sum over the private copies, for the hot nodes i < nhot.
+__global__ void __launch_bounds__(256)
+hot_reduce_kernel(const double* __restrict__ hot, int64_t nhot, double* __restrict__ lam_node,
+                  int64_t nn) {
+  const int64_t q = blockIdx.x * 256ll + threadIdx.x;
+  if (q >= nhot * kNodeAdj) return;
+  const int64_t k = q / nhot, i = q - k * nhot;
+  double v = 0.0;
+  for (int p = 0; p < kHotParts; p++) v += hot[((int64_t)p * kNodeAdj + k) * nhot + i];
+  lam_node[k * nn + i] = v;
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+int64_t pt_adjoint_node_scratch(int64_t nn) {
+  return (int64_t)kNodeAdj * nn + (int64_t)kHotParts * kNodeAdj * std::min<int64_t>(nn, kHotNodes);
+}
 
 int pt_node_adjoint_fields() { return kNodeAdj; }
 
@@ -547,9 +576,14 @@ int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
                             const double* wy, const double* wz, double* gx, double* gy,
                             double* gz, double* gm, double* lam_node, int64_t nn, hipStream_t s) {
   if (nb <= 0) return 0;
+  // lam_node holds pt_adjoint_node_scratch(nn) doubles, all zero on entry:
+  // the node cotangents, then the private copies of the hot ones
+  const int64_t nhot = std::min<int64_t>(nn, kHotNodes);
+  double* hot = lam_node + (int64_t)kNodeAdj * nn;
   adjoint_list_kernel<<<(unsigned)((nb + kAdjWaves - 1) / kAdjWaves), kAdjThreads, 0, s>>>(
       bk, nb, nodes, gn, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2], wx, wy,
-      wz, gx, gy, gz, gm, lam_node, nn);
+      wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
+  hot_reduce_kernel<<<grid_for(nhot * kNodeAdj), 256, 0, s>>>(hot, nhot, lam_node, nn);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -73,6 +73,7 @@ struct pt_ctx {
   DBuf<GravNode32> gn32;
   DBuf<Bucket> bk;
   DBuf<unsigned long long> cpart;  // per-block counter partials
+  DBuf<unsigned long long> wqueue;  // per-XCD work counters of the fused kernel
   bool lists_valid = false;
   // lists
   DBuf<int32_t> pp_cnt, pc_cnt, pp, pc;
@@ -302,6 +303,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
+  release(c->wqueue);
   release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
@@ -632,8 +634,9 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   double gf;
   if ((st = set_visitor(c, theta, offset, &gf))) return st;
   const int64_t nb = c->nb;
-  const int64_t nblk = std::max<int64_t>(pt_gravity_blocks(nb), 1);
+  const int64_t nblk = std::max<int64_t>(pt_gravity_blocks(nb, (int)precision), 1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
+  PT_TRY_HIP(c, ensure(c->wqueue, 8));
   for (int attempt = 0; attempt < 2; attempt++) {
     PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
@@ -642,7 +645,7 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
       PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
                                          c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, gf,
                                          c->offset, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                         c->cpart.p, c->derr.p, c->stream));
+                                         c->cpart.p, c->derr.p, c->wqueue.p, c->stream));
     }
     int32_t herr = 0;
     if ((st = finish_counters(c, nblk, &herr))) return st;
@@ -865,10 +868,10 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
   if ((st = pt_upward(c))) return st;
   if ((st = pt_traverse(c, theta, nullptr, nullptr))) return st;
   const int64_t nn = c->nn;
-  const int nf = pt_node_adjoint_fields();
-  PT_TRY_HIP(c, ensure(c->lam_node, nf * std::max<int64_t>(nn, 1)));
+  const int64_t nscr = pt_adjoint_node_scratch(std::max<int64_t>(nn, 1));
+  PT_TRY_HIP(c, ensure(c->lam_node, nscr));
   Timer t(c, PT_T_ADJOINT);
-  if ((st = zero(c, c->lam_node.p, nf * nn))) return st;
+  if ((st = zero(c, c->lam_node.p, nscr))) return st;
   if ((st = zero(c, v.gsx, 4 * std::max<int64_t>(n, 1)))) return st;  // gsx gsy gsz gsm
   PT_TRY_LAUNCH(c, pt_launch_gather3(c->perm.p, n, wx, wy, wz, v.wsx, v.wsy, v.wsz, c->stream));
   PT_TRY_LAUNCH(c, pt_launch_adjoint_lists(c->bk.p, c->nb, c->tnode.p, c->gn.p, c->xs.p, c->ys.p,

@@ -824,34 +824,26 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
   }
 }
 
+// One target bucket of the fused kernel: the walk for the bucket box with
+// the bucket's chunks evaluated as they fill, counters added to the wave's.
 template <bool F32, int STACK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 5 : 1, 8)))
-gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
-               const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
-               const double* __restrict__ px,
-               const double* __restrict__ py, const double* __restrict__ pz,
-               const double* __restrict__ pm, const double* __restrict__ pe, double gf,
-               double ox, double oy, double oz, double* __restrict__ ax,
-               double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
-               unsigned long long* __restrict__ partials, int32_t* __restrict__ err) {
+__device__ __forceinline__ void gravity_bucket(
+    WaveLds<F32, STACK>& W, int64_t bi, const Bucket* __restrict__ bk,
+    const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
+    const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pm,
+    const double* __restrict__ pe, double gf, double ox, double oy, double oz,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ pot, unsigned long long& c_node, unsigned long long& c_part,
+    unsigned long long& c_open, unsigned long long& c_close, bool& overflow) {
   using W_t = WaveLds<F32, STACK>;
-  __shared__ W_t lds[kWaves];
-  __shared__ unsigned long long s_cnt[kWaves][4];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
-  const int64_t blk = pt_xcd_block(blockIdx.x, gridDim.x);
-  const int64_t bi_raw = blk * kWaves + wave;
-  const bool live = bi_raw < nb;
-  const int64_t bi = live ? bi_raw : nb - 1;
-  W_t& W = lds[wave];
   const Bucket b = bk[bi];
   const int64_t nt = b.count;
   const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
                ccz = (b.loz + b.hiz) * 0.5;
   const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
-  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
-  bool overflow = false;
-  for (int g0 = 0; live && g0 < b.count; g0 += kGroupMax) {
+  for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
     const Group g = make_group(b, g0);
     const bool count = g0 == 0;  // the walk repeats per target group; count once
     Lanes<F32> L;
@@ -934,8 +926,56 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
     }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
   }
+}
+
+// Fused walk + interactions, one wave per target bucket.  FP32C: one
+// work-group per 4 buckets, mapped XCD-aware (pt_xcd_block).  FP64: a
+// resident grid (a multiple of 8 work-groups, dispatched round-robin over the
+// 8 XCDs) where XCD x owns the contiguous eighth [b_lo, b_hi) of the buckets
+// and its waves take the next one from the XCD's counter (queue[x], zero at
+// launch); every wave leaves once the counter has passed the range.  (The
+// loop form needs more registers, which the FP32C kernel pays for with
+// fewer resident waves; the FP64 kernel gains from the dynamic balance.)
+template <bool F32, int STACK>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 5 : 1, 8)))
+gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+               const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+               const double* __restrict__ px,
+               const double* __restrict__ py, const double* __restrict__ pz,
+               const double* __restrict__ pm, const double* __restrict__ pe, double gf,
+               double ox, double oy, double oz, double* __restrict__ ax,
+               double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
+               unsigned long long* __restrict__ partials, int32_t* __restrict__ err,
+               unsigned long long* __restrict__ queue) {
+  using W_t = WaveLds<F32, STACK>;
+  __shared__ W_t lds[kWaves];
+  __shared__ unsigned long long s_cnt[kWaves][4];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  W_t& W = lds[wave];
+  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
+  bool overflow = false;
+  if constexpr (F32) {
+    const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
+    if (bi < nb)
+      gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow);
+  } else {
+    const int xcd = blockIdx.x % 8;
+    const int64_t b_lo = nb * xcd / 8, b_hi = nb * (xcd + 1) / 8;
+    unsigned long long first = 0;
+    if (lane == 0) first = atomicAdd(&queue[xcd], 1ull);
+    for (int64_t bi = b_lo + (int64_t)__shfl(first, 0, PT_WAVE); bi < b_hi;) {
+      // the next index is fetched now and used after this bucket
+      unsigned long long next = 0;
+      if (lane == 0) next = atomicAdd(&queue[xcd], 1ull);
+      gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow);
+      bi = b_lo + (int64_t)__shfl(next, 0, PT_WAVE);
+    }
+  }
   if (lane == 0) {
-    if (live && overflow) atomicExch(err, 1);
+    if (overflow) atomicExch(err, 1);
     s_cnt[wave][0] = c_node;
     s_cnt[wave][1] = c_part;
     s_cnt[wave][2] = c_open;
@@ -945,7 +985,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
   if (threadIdx.x < 4) {
     unsigned long long v = 0;
     for (int w = 0; w < kWaves; w++) v += s_cnt[w][threadIdx.x];
-    partials[blk * 4 + threadIdx.x] = v;
+    partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
   }
 }
 
@@ -953,7 +993,24 @@ inline unsigned grid_for(int64_t nb) { return (unsigned)((nb + kWaves - 1) / kWa
 
 }  // namespace
 
-int64_t pt_gravity_blocks(int64_t nb) { return (nb + kWaves - 1) / kWaves; }
+// Work-groups of the fused kernel: FP32C one per 4 buckets; FP64 the
+// resident grid, 8 per CU (at least what fits on a CU; work-groups that do
+// not fit start as others finish -- none waits on another), a multiple of 8
+// (one share per XCD), capped by the bucket count.
+int64_t pt_gravity_blocks(int64_t nb, int precision) {
+  const int64_t need = (nb + kWaves - 1) / kWaves;
+  if (precision != 0) return need;
+  static int64_t cap = 0;
+  if (cap == 0) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    cap = (int64_t)(cus > 0 ? cus : 8) * 8;
+  }
+  int64_t g = need < cap ? need : cap;
+  g = ((g + 7) / 8) * 8;
+  return g < 8 ? 8 : g;
+}
 
 int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
                        const GravNode* gn, const GravNode32* gn32, const double* x, const double* y, const double* z,
@@ -980,13 +1037,15 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
                       const double* x, const double* y, const double* z, const double* m,
                       const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
-                      double* pot, unsigned long long* partials, int32_t* err, hipStream_t s) {
+                      double* pot, unsigned long long* partials, int32_t* err,
+                      unsigned long long* queue, hipStream_t s) {
   if (nb <= 0) return 0;
+  if (pt_launch_fill_u64(queue, 8, 0ull, s) != 0) return -1;
+  const unsigned grid = (unsigned)pt_gravity_blocks(nb, precision);
 #define PT_GRAVITY_LAUNCH(F32, STACK)                                                          \
-  gravity_kernel<F32, STACK><<<grid_for(nb), kThreads, 0, s>>>(                                \
+  gravity_kernel<F32, STACK><<<grid, kThreads, 0, s>>>(                                        \
       bk, nb, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot,      \
-      partials,                                                                                  \
-      err)
+      partials, err, queue)
   if (precision == 0) {
     if (big_stack) PT_GRAVITY_LAUNCH(false, kBigStack);
     else PT_GRAVITY_LAUNCH(false, kSmallStack);

@@ -74,13 +74,15 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
                        const int64_t* pc_off, const int32_t* pc, const double off[3], double* ax,
                        double* ay, double* az, double* pot, hipStream_t s);
 
-// Fused walk + interactions; partials holds pt_gravity_blocks(nb) x 4.
-int64_t pt_gravity_blocks(int64_t nb);
+// Fused walk + interactions (resident grid of pt_gravity_blocks work-groups;
+// partials holds 4 counters per work-group; queue holds 8 work counters).
+int64_t pt_gravity_blocks(int64_t nb, int precision);
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
                       const double* x, const double* y, const double* z, const double* m,
                       const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
-                      double* pot, unsigned long long* partials, int32_t* err, hipStream_t s);
+                      double* pot, unsigned long long* partials, int32_t* err,
+                      unsigned long long* queue, hipStream_t s);
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const double* x, const double* y,
                      const double* z, const double* m, const double* e, double* xs, double* ys,
@@ -100,6 +102,8 @@ int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, h
 // Adjoint of the interactions (FP64).  lam_node holds pt_node_adjoint_fields()
 // x nn doubles (field-major) and must be zero on entry; g* accumulate.
 int pt_node_adjoint_fields();
+// Doubles of node-cotangent scratch the list adjoint needs for nn nodes.
+int64_t pt_adjoint_node_scratch(int64_t nn);
 int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
                             const GravNode* gn, const double* x, const double* y,
                             const double* z, const double* m, const double* e,

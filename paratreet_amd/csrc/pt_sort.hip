@@ -6,11 +6,11 @@
 // which is the order the oracle defines.
 //
 // Per 8-bit digit pass:
-//   1. count_kernel   -- LDS histogram of each 4096-key tile, written
+//   1. count_kernel   -- LDS histogram of each 2048-key tile, written
 //                        digit-major so one exclusive scan yields every
 //                        (digit, tile) output offset;
 //   2. device-wide exclusive scan of the 256 x tiles table;
-//   3. scatter_kernel -- each wave ranks its 1024-key sub-tile in 16 rounds of
+//   3. scatter_kernel -- each wave ranks its 512-key sub-tile in 8 rounds of
 //                        64 with an 8-ballot digit match and per-wave LDS
 //                        counters; ranks become tile-local positions, keys are
 //                        staged through LDS and written out in digit runs so
@@ -22,8 +22,8 @@ namespace {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / PT_WAVE;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per block
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per block
 constexpr int kRadix = 256;
 
 __global__ void __launch_bounds__(kSortThreads)

@@ -186,6 +186,28 @@ def test_fused_path_matches_oracle(pair):
     assert rms <= FP32C_RMS_TOL, name
 
 
+def test_walk_stack_overflow_rerun():
+    """A deep walk (30k particles in 8 halos, bucket 4, theta 0.1: a CPU
+    replay of the walk order reaches 226 pending entries) overflows the small
+    LDS walk stack; the evaluation reruns with the big stack and still
+    matches the oracle exactly (counters) and to 1e-12 (FP64)."""
+    pos, _, m = ics.halo_mixture(30000, n_halos=8, seed=3)
+    e = np.full(len(pos), 0.05)
+    P = O.Pipeline(pos, m, e, bucket=4, theta=0.1)
+    d = _gpu(pos, m, e, 4, 0.1)
+    cnt = d.compute_forces(theta=0.1, precision=0)
+    acc, pot = d.accelerations()
+    oacc, opot, ocnt = P.gravity_unsorted()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
+    assert rel_err(acc, oacc).max() <= FP64_TOL
+    cnt32 = d.compute_forces(theta=0.1, precision=1)
+    a32, _ = d.accelerations()
+    d.close()
+    assert tuple(cnt32) == tuple(cnt)
+    e32 = rel_err(a32, acc)
+    assert float(np.sqrt(np.mean(e32 * e32))) <= FP32C_RMS_TOL
+
+
 def test_lambs_tree_error_vs_reference_direct():
     """Tree force error on the reference's accuracy fixture against its
     direct.acc: same order as the oracle's (quadrupole, theta 0.7)."""
