@@ -73,6 +73,13 @@ def _declare(L):
     L.or_intersect_box_point.restype = ctypes.c_int
     L.or_kick_drift.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _D]
     L.or_direct_vjp.argtypes = [_P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P]
+    L.or_hexa_moments.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P]
+    L.or_hexa_raw_moments.argtypes = [_P, _P, _P, _P, _I64, _P, _P]
+    L.or_hexa_reduce.argtypes = [_D, _P, _P, _P]
+    L.or_hexa_eval.argtypes = [_P, _D, _D, _D, _P, _P]
+    L.or_gravity_hexa.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _I64,
+                                  _P, _P, _P, _P, _P, ctypes.c_int]
+    L.or_lists_hexa.argtypes = [_P, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P]
 
 
 def _p(a):
@@ -133,40 +140,94 @@ def buckets(nodes):
     return out
 
 
+HEXA_FIELDS = ("m", "xx", "yy", "xy", "xz", "yz", "xxx", "xyy", "xxy", "yyy", "xxz", "yyz",
+               "xyz", "xxxx", "xyyy", "xxxy", "yyyy", "xxxz", "yyyz", "xxyy", "xxyz", "xyyz")
+
+
+def hexa_moments(nodes, data, x, y, z, m, raw=False):
+    """Reduced moments (nn x 22, HEXA_FIELDS order) of every node; with
+    raw=True also the raw third/fourth moments about each cm (nn x 25)."""
+    nodes = np.ascontiguousarray(nodes)
+    data = np.ascontiguousarray(data)
+    x, y, z, m = (_f64(a) for a in (x, y, z, m))
+    out = np.zeros((len(nodes), 22))
+    r = np.zeros((len(nodes), 25))
+    lib().or_hexa_moments(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m), _p(r),
+                          _p(out))
+    return (out, r) if raw else out
+
+
+def hexa_raw_moments(pos, m, cm):
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (_f64(pos[:, k]) for k in range(3))
+    out = np.zeros(25)
+    lib().or_hexa_raw_moments(_p(x), _p(y), _p(z), _p(_f64(m)), len(x), _p(_f64(cm)), _p(out))
+    return out
+
+
+def hexa_reduce(mass, q, raw):
+    out = np.zeros(22)
+    lib().or_hexa_reduce(float(mass), _p(_f64(q)), _p(_f64(raw)), _p(out))
+    return out
+
+
+def hexa_eval(h, r):
+    """(potential, acceleration) of reduced moments h at r = target - cm."""
+    h = _f64(h)
+    pot = ctypes.c_double()
+    acc = np.zeros(3)
+    lib().or_hexa_eval(_p(h), float(r[0]), float(r[1]), float(r[2]), ctypes.byref(pot), _p(acc))
+    return pot.value, acc
+
+
 def gravity(nodes, data, x, y, z, m, eps, theta, offset=(0.0, 0.0, 0.0),
-            bucket_sel=None, nthreads=0):
+            bucket_sel=None, nthreads=0, hexa=None):
     """Accelerations and potentials in sorted order + counters
-    {n_node_ints, n_part_ints, n_opens, n_closes}."""
+    {n_node_ints, n_part_ints, n_opens, n_closes}.  hexa: the nodes' reduced
+    moments (hexa_moments) for the HEXADECAPOLE build, None for quadrupole."""
     n = len(x)
     x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
     off = _f64(offset)
     ax, ay, az, pot = (np.zeros(n) for _ in range(4))
     cnt = np.zeros(4, dtype=np.uint64)
     sel = None if bucket_sel is None else np.ascontiguousarray(bucket_sel, dtype=np.int32)
-    lib().or_gravity(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m), _p(eps),
-                     float(theta), _p(off), _p(sel), 0 if sel is None else len(sel),
-                     _p(ax), _p(ay), _p(az), _p(pot), _p(cnt), int(nthreads))
+    if hexa is None:
+        lib().or_gravity(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m), _p(eps),
+                         float(theta), _p(off), _p(sel), 0 if sel is None else len(sel),
+                         _p(ax), _p(ay), _p(az), _p(pot), _p(cnt), int(nthreads))
+    else:
+        h = _f64(hexa)
+        lib().or_gravity_hexa(_p(nodes), len(nodes), _p(data), _p(h), _p(x), _p(y), _p(z),
+                              _p(m), _p(eps), float(theta), _p(off), _p(sel),
+                              0 if sel is None else len(sel), _p(ax), _p(ay), _p(az), _p(pot),
+                              _p(cnt), int(nthreads))
     return np.stack([ax, ay, az], axis=1), pot, cnt
 
 
-def lists(nodes, data, theta, offset=(0.0, 0.0, 0.0)):
+def lists(nodes, data, theta, offset=(0.0, 0.0, 0.0), hexa=None):
     """Per-bucket interaction lists in DFS order: (pp_off, pp, pc_off, pc) with
-    node ids into `nodes`."""
+    node ids into `nodes` (HEXADECAPOLE open() when hexa is given)."""
     nb = len(buckets(nodes))
     off = _f64(offset)
     ppc = np.zeros(nb, dtype=np.int64)
     pcc = np.zeros(nb, dtype=np.int64)
     L = lib()
-    L.or_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), _p(ppc), _p(pcc),
-               None, None, None, None)
+    if hexa is not None:
+        h = _f64(hexa)
+
+        def call(*a):
+            L.or_lists_hexa(_p(nodes), len(nodes), _p(data), _p(h), float(theta), _p(off), *a)
+    else:
+        def call(*a):
+            L.or_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), *a)
+    call(_p(ppc), _p(pcc), None, None, None, None)
     pp_off = np.zeros(nb + 1, dtype=np.int64)
     pc_off = np.zeros(nb + 1, dtype=np.int64)
     np.cumsum(ppc, out=pp_off[1:])
     np.cumsum(pcc, out=pc_off[1:])
     pp = np.zeros(max(1, pp_off[-1]), dtype=np.int32)
     pc = np.zeros(max(1, pc_off[-1]), dtype=np.int32)
-    L.or_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), _p(ppc), _p(pcc),
-               _p(pp_off), _p(pc_off), _p(pp), _p(pc))
+    call(_p(ppc), _p(pcc), _p(pp_off), _p(pc_off), _p(pp), _p(pc))
     return pp_off, pp[:pp_off[-1]], pc_off, pc[:pc_off[-1]]
 
 
@@ -223,7 +284,8 @@ class Pipeline:
     """Full oracle pipeline on one particle set (original order in, sorted
     order inside).  Attributes mirror the stages of the reference path."""
 
-    def __init__(self, pos, mass, eps, bucket, theta, offset=(0.0, 0.0, 0.0)):
+    def __init__(self, pos, mass, eps, bucket, theta, offset=(0.0, 0.0, 0.0),
+                 hexadecapole=False):
         pos = np.asarray(pos, dtype=np.float64)
         self.n = len(pos)
         self.bucket = bucket
@@ -240,10 +302,13 @@ class Pipeline:
         self.nodes = build_tree(self.sorted_keys, bucket)
         self.data = moments(self.nodes, self.x, self.y, self.z, self.m, self.eps)
         self.bucket_nodes = buckets(self.nodes)
+        self.hexa = (hexa_moments(self.nodes, self.data, self.x, self.y, self.z, self.m)
+                     if hexadecapole else None)
 
     def gravity(self, bucket_sel=None, nthreads=0):
         acc, pot, cnt = gravity(self.nodes, self.data, self.x, self.y, self.z, self.m,
-                                self.eps, self.theta, self.offset, bucket_sel, nthreads)
+                                self.eps, self.theta, self.offset, bucket_sel, nthreads,
+                                hexa=self.hexa)
         return acc, pot, cnt
 
     def gravity_unsorted(self, nthreads=0):
@@ -256,4 +321,4 @@ class Pipeline:
         return out_a, out_p, cnt
 
     def lists(self):
-        return lists(self.nodes, self.data, self.theta, self.offset)
+        return lists(self.nodes, self.data, self.theta, self.offset, hexa=self.hexa)

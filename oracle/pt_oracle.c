@@ -414,6 +414,231 @@ void or_moments(const or_node* nodes, int64_t nn,
 }
 
 /* ------------------------------------------------------------------------- */
+/* HEXADECAPOLE moments (the -DHEXADECAPOLE build, examples/Makefile:5)        */
+/* ------------------------------------------------------------------------- */
+/*
+ * The reference keeps pkdgrav-style reduced (symmetric trace-free) moments up
+ * to fourth order, scaled by the node radius (FMOMR, examples/moments.h; made
+ * per particle by momMakeFmomr and moved to a parent by momShiftFmomr and
+ * momScaledAddFmomr, examples/MultipoleMoments.h:244-252,317-330).  As written
+ * its leaf update runs only `if (radius > 0)`, and the radius is still 0 while
+ * the leaf constructor adds its particles (CentroidData.h:41-59 sets it after
+ * the loop), so every FMOMR stays zero.  This restatement builds the moments
+ * the code intends: for every node the trace-free parts of
+ *   R(a,b,c) = sum_j m_j dx^a dy^b dz^c,  d = x_j - cm,  a + b + c = 2, 3, 4,
+ * unscaled (Real is double, so the radius scaling changes nothing but
+ * rounding).  Leaves sum their particles in sorted order; a parent sums its
+ * children's raw moments moved to its centre of mass with the binomial
+ * expansion of (d + s)^a (d + s)^b (d + s)^c, s = cm_child - cm_parent, in
+ * child order (a child's dipole about its own centre of mass is zero).  Second
+ * moments are the quadrupole build's q (the same quantity, merged in the
+ * reference's order).  Field order of the 22 reduced values (FMOMR,
+ * examples/moments.h): m, xx yy xy xz yz, xxx xyy xxy yyy xxz yyz xyz,
+ * xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz.
+ */
+
+/* index of component (a, b, .) among the (l+1)(l+2)/2 of order l, ordered by
+ * a descending, then b descending */
+static int hx_index(int l, int a, int b) { return (l - a) * (l - a + 1) / 2 + (l - a - b); }
+
+static const double hx_binom[5][5] = {
+    {1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+
+/* raw moment of exponents (a, b, c) from (mass, q, R3 | R4) */
+static double hx_raw(double mass, const double q[6], const double* r, int a, int b, int c) {
+  const int l = a + b + c;
+  if (l == 0) return mass;
+  if (l == 1) return 0.0;
+  if (l == 2) return q[hx_index(2, a, b)];
+  if (l == 3) return r[hx_index(3, a, b)];
+  return r[10 + hx_index(4, a, b)];
+}
+
+static void hx_pow(double v, double p[5]) {
+  p[0] = 1.0;
+  p[1] = v;
+  p[2] = p[1] * v;
+  p[3] = p[2] * v;
+  p[4] = p[3] * v;
+}
+
+/* leaf: R3, R4 about cm from the particles [b, b + n) in order */
+static void hx_leaf(double r[25], const double cm[3], const double* x, const double* y,
+                    const double* z, const double* m, int64_t b, int64_t n) {
+  for (int k = 0; k < 25; k++) r[k] = 0.0;
+  for (int64_t i = b; i < b + n; i++) {
+    double px[5], py[5], pz[5];
+    hx_pow(x[i] - cm[0], px);
+    hx_pow(y[i] - cm[1], py);
+    hx_pow(z[i] - cm[2], pz);
+    for (int l = 3; l <= 4; l++)
+      for (int a = l; a >= 0; a--)
+        for (int bb = l - a; bb >= 0; bb--) {
+          const int c = l - a - bb;
+          r[(l == 3 ? 0 : 10) + hx_index(l, a, bb)] += ((m[i] * px[a]) * py[bb]) * pz[c];
+        }
+  }
+}
+
+/* parent += child moved by s = cm_child - cm_parent */
+static void hx_shift_add(double r[25], const double s[3], double mass, const double q[6],
+                         const double rc[25]) {
+  double px[5], py[5], pz[5];
+  hx_pow(s[0], px);
+  hx_pow(s[1], py);
+  hx_pow(s[2], pz);
+  for (int l = 3; l <= 4; l++)
+    for (int a = l; a >= 0; a--)
+      for (int b = l - a; b >= 0; b--) {
+        const int c = l - a - b;
+        double sum = 0.0;
+        for (int i = 0; i <= a; i++)
+          for (int j = 0; j <= b; j++)
+            for (int k = 0; k <= c; k++) {
+              if (i + j + k == 1) continue;
+              const double coef = hx_binom[a][i] * hx_binom[b][j] * hx_binom[c][k];
+              sum += (((coef * px[a - i]) * py[b - j]) * pz[c - k]) * hx_raw(mass, q, rc, i, j, k);
+            }
+        r[(l == 3 ? 0 : 10) + hx_index(l, a, b)] += sum;
+      }
+}
+
+/* trace-free (reduced) moments from mass, q and R3 | R4 */
+static void hx_reduce(double out[22], double mass, const double q[6], const double r[25]) {
+#define R3(a, b) r[hx_index(3, a, b)]
+#define R4(a, b) r[10 + hx_index(4, a, b)]
+  const double tr = q[0] + q[3] + q[5];
+  const double vx = R3(3, 0) + R3(1, 2) + R3(1, 0);
+  const double vy = R3(2, 1) + R3(0, 3) + R3(0, 1);
+  const double vz = R3(2, 0) + R3(0, 2) + R3(0, 0);
+  const double txx = R4(4, 0) + R4(2, 2) + R4(2, 0);
+  const double tyy = R4(2, 2) + R4(0, 4) + R4(0, 2);
+  const double tzz = R4(2, 0) + R4(0, 2) + R4(0, 0);
+  const double txy = R4(3, 1) + R4(1, 3) + R4(1, 1);
+  const double txz = R4(3, 0) + R4(1, 2) + R4(1, 0);
+  const double tyz = R4(2, 1) + R4(0, 3) + R4(0, 1);
+  const double ss = txx + tyy + tzz;
+  out[0] = mass;
+  out[1] = q[0] - tr * (1.0 / 3.0);  /* xx */
+  out[2] = q[3] - tr * (1.0 / 3.0);  /* yy */
+  out[3] = q[1];                     /* xy */
+  out[4] = q[2];                     /* xz */
+  out[5] = q[4];                     /* yz */
+  out[6] = R3(3, 0) - vx * (3.0 / 5.0);  /* xxx */
+  out[7] = R3(1, 2) - vx * (1.0 / 5.0);  /* xyy */
+  out[8] = R3(2, 1) - vy * (1.0 / 5.0);  /* xxy */
+  out[9] = R3(0, 3) - vy * (3.0 / 5.0);  /* yyy */
+  out[10] = R3(2, 0) - vz * (1.0 / 5.0); /* xxz */
+  out[11] = R3(0, 2) - vz * (1.0 / 5.0); /* yyz */
+  out[12] = R3(1, 1);                    /* xyz */
+  out[13] = (R4(4, 0) - txx * (6.0 / 7.0)) + ss * (3.0 / 35.0);  /* xxxx */
+  out[14] = R4(1, 3) - txy * (3.0 / 7.0);                        /* xyyy */
+  out[15] = R4(3, 1) - txy * (3.0 / 7.0);                        /* xxxy */
+  out[16] = (R4(0, 4) - tyy * (6.0 / 7.0)) + ss * (3.0 / 35.0);  /* yyyy */
+  out[17] = R4(3, 0) - txz * (3.0 / 7.0);                        /* xxxz */
+  out[18] = R4(0, 3) - tyz * (3.0 / 7.0);                        /* yyyz */
+  out[19] = (R4(2, 2) - (txx + tyy) * (1.0 / 7.0)) + ss * (1.0 / 35.0);  /* xxyy */
+  out[20] = R4(2, 1) - tyz * (1.0 / 7.0);                        /* xxyz */
+  out[21] = R4(1, 2) - txz * (1.0 / 7.0);                        /* xyyz */
+#undef R3
+#undef R4
+}
+
+void or_hexa_moments(const or_node* nodes, int64_t nn, const or_cdata* d, const double* x,
+                     const double* y, const double* z, const double* m, double* raw,
+                     double* out) {
+  if (nn <= 0) return;
+  double* r = raw ? raw : (double*)malloc(sizeof(double) * 25 * (size_t)nn);
+  /* pre-order: every child follows its parent, so a reverse sweep sees
+   * children first */
+  for (int64_t i = nn - 1; i >= 0; i--) {
+    double* ri = r + 25 * i;
+    if (nodes[i].type == OR_LEAF) {
+      hx_leaf(ri, d[i].cm, x, y, z, m, nodes[i].begin, nodes[i].count);
+    } else {
+      for (int k = 0; k < 25; k++) ri[k] = 0.0;
+      if (nodes[i].type == OR_INTERNAL && d[i].mass != 0.0) {
+        for (int c = 0; c < 8; c++) {
+          const int32_t ch = nodes[i].child[c];
+          if (d[ch].mass == 0.0) continue;
+          const double s[3] = {d[ch].cm[0] - d[i].cm[0], d[ch].cm[1] - d[i].cm[1],
+                               d[ch].cm[2] - d[i].cm[2]};
+          hx_shift_add(ri, s, d[ch].mass, d[ch].q, r + 25 * ch);
+        }
+      }
+    }
+    hx_reduce(out + 22 * i, d[i].mass, d[i].q, ri);
+  }
+  if (!raw) free(r);
+}
+
+void or_hexa_raw_moments(const double* x, const double* y, const double* z, const double* m,
+                         int64_t n, const double cm[3], double out[25]) {
+  hx_leaf(out, cm, x, y, z, m, 0, n);
+}
+
+void or_hexa_reduce(double mass, const double q[6], const double r[25], double out[22]) {
+  hx_reduce(out, mass, q, r);
+}
+
+/* Potential and acceleration at r = target - cm of the reduced moments h
+ * (GravityVisitor::node -> momEvalFmomrcm, examples/GravityVisitor.h:199-206,
+ * examples/moments.C:1469-1526).  With c_l = (2l-1)!!/l! (3/2, 5/2, 35/8) and
+ * v_l = M_l contracted with l-1 copies of r, s_l = v_l . r:
+ *   pot = -(m / r + sum_l c_l s_l / r^(2l+1))
+ *   acc = sum_l c_l (l v_l / r^(2l+1) - (2l+1) s_l r / r^(2l+3)) - m r / r^3. */
+void or_hexa_eval(const double h[22], double rx, double ry, double rz, double* pot,
+                  double acc[3]) {
+  const double m = h[0];
+  const double xx = h[1], yy = h[2], xy = h[3], xz = h[4], yz = h[5];
+  const double zz = -(xx + yy);
+  const double xxx = h[6], xyy = h[7], xxy = h[8], yyy = h[9], xxz = h[10], yyz = h[11],
+               xyz = h[12];
+  const double xzz = -(xxx + xyy), yzz = -(xxy + yyy), zzz = -(xxz + yyz);
+  const double xxxx = h[13], xyyy = h[14], xxxy = h[15], yyyy = h[16], xxxz = h[17],
+               yyyz = h[18], xxyy = h[19], xxyz = h[20], xyyz = h[21];
+  const double xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
+  const double xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
+  const double rsq = rx * rx + ry * ry + rz * rz;
+  const double ir = 1.0 / sqrt(rsq);
+  const double ir2 = ir * ir;
+  const double x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
+  const double xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
+  /* order 2 */
+  const double v2x = xx * rx + xy * ry + xz * rz;
+  const double v2y = xy * rx + yy * ry + yz * rz;
+  const double v2z = xz * rx + yz * ry + zz * rz;
+  const double s2 = v2x * rx + v2y * ry + v2z * rz;
+  /* order 3 */
+  const double v3x = xxx * x2 + xyy * y2 + xzz * z2 + 2.0 * (xxy * xyp + xxz * xzp + xyz * yzp);
+  const double v3y = xxy * x2 + yyy * y2 + yzz * z2 + 2.0 * (xyy * xyp + xyz * xzp + yyz * yzp);
+  const double v3z = xxz * x2 + yyz * y2 + zzz * z2 + 2.0 * (xyz * xyp + xzz * xzp + yzz * yzp);
+  const double s3 = v3x * rx + v3y * ry + v3z * rz;
+  /* order 4 */
+  const double x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz, xyz3 = xyp * rz;
+  const double v4x = xxxx * x3 + xyyy * y3 + xzzz * z3 +
+                     3.0 * (xxxy * x2 * ry + xxxz * x2 * rz + xxyy * rx * y2 + xxzz * rx * z2 +
+                            xyyz * y2 * rz + xyzz * ry * z2) +
+                     6.0 * xxyz * xyz3;
+  const double v4y = xxxy * x3 + yyyy * y3 + yzzz * z3 +
+                     3.0 * (xxyy * x2 * ry + xxyz * x2 * rz + xyyy * rx * y2 + xyzz * rx * z2 +
+                            yyyz * y2 * rz + yyzz * ry * z2) +
+                     6.0 * xyyz * xyz3;
+  const double v4z = xxxz * x3 + yyyz * y3 + zzzz * z3 +
+                     3.0 * (xxyz * x2 * ry + xxzz * x2 * rz + xyyz * rx * y2 + xzzz * rx * z2 +
+                            yyzz * y2 * rz + yzzz * ry * z2) +
+                     6.0 * xyzz * xyz3;
+  const double s4 = v4x * rx + v4y * ry + v4z * rz;
+  const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2,
+               ir11 = ir9 * ir2;
+  *pot = -(m * ir + 1.5 * s2 * ir5 + 2.5 * s3 * ir7 + 4.375 * s4 * ir9);
+  const double g = m * ir3 + 7.5 * s2 * ir7 + 17.5 * s3 * ir9 + 39.375 * s4 * ir11;
+  acc[0] = 3.0 * v2x * ir5 + 7.5 * v3x * ir7 + 17.5 * v4x * ir9 - rx * g;
+  acc[1] = 3.0 * v2y * ir5 + 7.5 * v3y * ir7 + 17.5 * v4y * ir9 - ry * g;
+  acc[2] = 3.0 * v2z * ir5 + 7.5 * v3z * ir7 + 17.5 * v4z * ir9 - rz * g;
+}
+
+/* ------------------------------------------------------------------------- */
 /* GravityVisitor                                                              */
 /* ------------------------------------------------------------------------- */
 
@@ -509,6 +734,8 @@ typedef struct {
   const or_cdata* d;
   const double *x, *y, *z, *m, *e;
   double gravity_factor;
+  double mono_factor;   /* GravityVisitor::monopole_gravity_factor */
+  const double* hexa;   /* reduced moments per node (22 each); NULL = quadrupole build */
   double off[3];
   double *ax, *ay, *az, *pot;
   /* lists mode */
@@ -532,6 +759,21 @@ static int gv_open_softening(const trav_ctx* t, const or_cdata* src, const or_cd
   const double sr = 2.0 * src->soft;
   if (intersect_sphere_sphere(sc, sr, tgt->cm, 2.0 * tgt->soft)) return 1;
   return or_intersect_box_point(tgt->lo, tgt->hi, sc, sr * sr);
+}
+
+/* GravityVisitor::open, HEXADECAPOLE branch (examples/GravityVisitor.h:
+ * 162-178).  The softening test is called as written there, with the target
+ * bucket's data in the source role: openSoftening(target.data, source.data). */
+static int gv_open_hexa(const trav_ctx* t, const or_cdata* src, const or_cdata* tgt) {
+  if (src->count <= 6) return 1;
+  const double dataRsq = src->radius * src->radius * t->gravity_factor;
+  const double p[3] = {src->cm[0] + t->off[0], src->cm[1] + t->off[1], src->cm[2] + t->off[2]};
+  if (!or_intersect_box_point(tgt->lo, tgt->hi, p, dataRsq)) {
+    if (!gv_open_softening(t, tgt, src)) return 0;
+    const double monoRad = src->radius * t->mono_factor;
+    return or_intersect_box_point(tgt->lo, tgt->hi, p, monoRad * monoRad);
+  }
+  return 1;
 }
 
 /* GravityVisitor::leaf (examples/GravityVisitor.h:144-160). */
@@ -599,6 +841,41 @@ static void gv_node(trav_ctx* t, const or_cdata* s, const or_node* tgt, const or
   }
 }
 
+/* GravityVisitor::node, HEXADECAPOLE branch (examples/GravityVisitor.h:
+ * 184-206): addGravity when the softening spheres meet, else the reduced
+ * moments evaluated at every target (potential and acceleration). */
+static void gv_node_hexa(trav_ctx* t, int32_t si, const or_node* tgt, const or_cdata* td) {
+  const or_cdata* s = &t->d[si];
+  if (s->count == 0) return;
+  if (gv_open_softening(t, s, td)) {
+    for (int64_t i = tgt->begin; i < tgt->begin + tgt->count; i++) {
+      const double dx = (s->cm[0] + t->off[0]) - t->x[i];
+      const double dy = (s->cm[1] + t->off[1]) - t->y[i];
+      const double dz = (s->cm[2] + t->off[2]) - t->z[i];
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      if (rsq != 0) {
+        const double f = s->mass / (rsq * sqrt(rsq));
+        t->ax[i] += dx * f;
+        t->ay[i] += dy * f;
+        t->az[i] += dz * f;
+      }
+    }
+    return;
+  }
+  const double* h = t->hexa + 22 * (int64_t)si;
+  for (int64_t i = tgt->begin; i < tgt->begin + tgt->count; i++) {
+    const double rx = (t->x[i] - s->cm[0]) - t->off[0];
+    const double ry = (t->y[i] - s->cm[1]) - t->off[1];
+    const double rz = (t->z[i] - s->cm[2]) - t->off[2];
+    double pot, acc[3];
+    or_hexa_eval(h, rx, ry, rz, &pot, acc);
+    t->pot[i] += pot;
+    t->ax[i] += acc[0];
+    t->ay[i] += acc[1];
+    t->az[i] += acc[2];
+  }
+}
+
 /* TransposedDownTraverser::recurse (src/Traverser.h:154-217) seen from one
  * target bucket: the sequence of visitor calls this bucket receives. */
 static void visit(trav_ctx* t, int32_t ni, int32_t bn, int lists) {
@@ -615,7 +892,8 @@ static void visit(trav_ctx* t, int32_t ni, int32_t bn, int lists) {
       t->cnt[1] += (uint64_t)(nd->count * tg->count);
       break;
     case OR_INTERNAL: {
-      const int open = gv_open(t, &t->d[ni], &t->d[bn]);
+      const int open = t->hexa ? gv_open_hexa(t, &t->d[ni], &t->d[bn])
+                               : gv_open(t, &t->d[ni], &t->d[bn]);
       t->cnt[open ? 2 : 3]++;
       if (open) {
         for (int c = 0; c < 8; c++) visit(t, nd->child[c], bn, lists);
@@ -623,6 +901,8 @@ static void visit(trav_ctx* t, int32_t ni, int32_t bn, int lists) {
         if (lists) {
           if (t->pc) t->pc[t->npc] = ni;
           t->npc++;
+        } else if (t->hexa) {
+          gv_node_hexa(t, ni, tg, &t->d[bn]);
         } else {
           gv_node(t, &t->d[ni], tg, &t->d[bn]);
         }
@@ -641,19 +921,25 @@ static double gravity_factor(double theta) {
   return opening_geometry_factor_squared / (theta * theta);
 }
 
-void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
-                const double* x, const double* y, const double* z,
-                const double* m, const double* eps,
-                double theta, const double offset[3],
-                const int32_t* bucket_sel, int64_t n_sel,
-                double* ax, double* ay, double* az, double* pot,
-                uint64_t counters[4], int nthreads) {
+static double mono_factor(double theta) {
+  /* GravityVisitor ctor (examples/GravityVisitor.h:16-19) */
+  const double opening_geometry_factor_squared = 4.0 / 3.0;
+  return sqrt(opening_geometry_factor_squared) / (theta * theta * theta * theta);
+}
+
+static void gravity_impl(const or_node* nodes, int64_t nn, const or_cdata* d,
+                         const double* hexa, const double* x, const double* y, const double* z,
+                         const double* m, const double* eps, double theta,
+                         const double offset[3], const int32_t* bucket_sel, int64_t n_sel,
+                         double* ax, double* ay, double* az, double* pot, uint64_t counters[4],
+                         int nthreads) {
   const int64_t nb = or_buckets(nodes, nn, NULL);
   int32_t* bn = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
   or_buckets(nodes, nn, bn);
   const int64_t ns = bucket_sel ? n_sel : nb;
   uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
   const double gf = gravity_factor(theta);
+  const double gm = mono_factor(theta);
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) reduction(+ : c0, c1, c2, c3)
@@ -666,6 +952,8 @@ void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
     t.d = d;
     t.x = x; t.y = y; t.z = z; t.m = m; t.e = eps;
     t.gravity_factor = gf;
+    t.mono_factor = gm;
+    t.hexa = hexa;
     for (int k = 0; k < 3; k++) t.off[k] = offset[k];
     t.ax = ax; t.ay = ay; t.az = az; t.pot = pot;
     for (int64_t i = nodes[b].begin; i < nodes[b].begin + nodes[b].count; i++) {
@@ -685,11 +973,30 @@ void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
   (void)nthreads;
 }
 
-void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
-              double theta, const double offset[3],
-              int64_t* pp_count, int64_t* pc_count,
-              const int64_t* pp_off, const int64_t* pc_off,
-              int32_t* pp, int32_t* pc) {
+void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
+                const double* x, const double* y, const double* z,
+                const double* m, const double* eps,
+                double theta, const double offset[3],
+                const int32_t* bucket_sel, int64_t n_sel,
+                double* ax, double* ay, double* az, double* pot,
+                uint64_t counters[4], int nthreads) {
+  gravity_impl(nodes, nn, d, NULL, x, y, z, m, eps, theta, offset, bucket_sel, n_sel, ax, ay, az,
+               pot, counters, nthreads);
+}
+
+void or_gravity_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
+                     const double* x, const double* y, const double* z, const double* m,
+                     const double* eps, double theta, const double offset[3],
+                     const int32_t* bucket_sel, int64_t n_sel, double* ax, double* ay,
+                     double* az, double* pot, uint64_t counters[4], int nthreads) {
+  gravity_impl(nodes, nn, d, hexa, x, y, z, m, eps, theta, offset, bucket_sel, n_sel, ax, ay, az,
+               pot, counters, nthreads);
+}
+
+static void lists_impl(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
+                       double theta, const double offset[3], int64_t* pp_count,
+                       int64_t* pc_count, const int64_t* pp_off, const int64_t* pc_off,
+                       int32_t* pp, int32_t* pc) {
   const int64_t nb = or_buckets(nodes, nn, NULL);
   int32_t* bn = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
   or_buckets(nodes, nn, bn);
@@ -703,6 +1010,8 @@ void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
     t.nodes = nodes;
     t.d = d;
     t.gravity_factor = gf;
+    t.mono_factor = mono_factor(theta);
+    t.hexa = hexa;
     for (int k = 0; k < 3; k++) t.off[k] = offset[k];
     if (pp && pc) {
       t.pp = pp + pp_off[s];
@@ -713,6 +1022,20 @@ void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
     pc_count[s] = t.npc;
   }
   free(bn);
+}
+
+void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
+              double theta, const double offset[3],
+              int64_t* pp_count, int64_t* pc_count,
+              const int64_t* pp_off, const int64_t* pc_off,
+              int32_t* pp, int32_t* pc) {
+  lists_impl(nodes, nn, d, NULL, theta, offset, pp_count, pc_count, pp_off, pc_off, pp, pc);
+}
+
+void or_lists_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
+                   double theta, const double offset[3], int64_t* pp_count, int64_t* pc_count,
+                   const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc) {
+  lists_impl(nodes, nn, d, hexa, theta, offset, pp_count, pc_count, pp_off, pc_off, pp, pc);
 }
 
 void or_direct(const double* x, const double* y, const double* z,

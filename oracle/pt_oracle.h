@@ -7,9 +7,10 @@
  * loads or falls back to it.
  *
  * Every function restates one piece of the reference (paratreet/paratreet) in
- * plain C with Real = double (the reference built with -DUSE_DOUBLE_FP, no
- * -DHEXADECAPOLE, Morton keys).  Arithmetic follows the reference's operation
- * order so results are reproducible bit for bit when compiled with
+ * plain C with Real = double (the reference built with -DUSE_DOUBLE_FP, Morton
+ * keys), without -DHEXADECAPOLE except the or_hexa_* and *_hexa functions.
+ * Arithmetic follows the reference's operation order so results are
+ * reproducible bit for bit when compiled with
  * -ffp-contract=off (see oracle/Makefile).
  *
  * Pieces whose source is NOT in the reference tree (the unvendored
@@ -120,6 +121,33 @@ void or_splineq(double invr, double r2, double twoh,
 /* Space::intersect(box, point, rsq) as defined in pt_oracle.c. */
 int or_intersect_box_point(const double lo[3], const double hi[3],
                            const double p[3], double rsq);
+
+/* ---- HEXADECAPOLE build (-DHEXADECAPOLE, examples/Makefile:5) ----
+ * Reduced (trace-free) moments to fourth order per node, 22 values in FMOMR
+ * order (examples/moments.h): m, xx yy xy xz yz, xxx xyy xxy yyy xxz yyz xyz,
+ * xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz; unscaled.  raw (optional,
+ * nn x 25): raw third and fourth moments about each node's cm. */
+void or_hexa_moments(const or_node* nodes, int64_t nn, const or_cdata* d, const double* x,
+                     const double* y, const double* z, const double* m, double* raw,
+                     double* out);
+/* raw third and fourth moments of n particles about cm (25 values) and their
+ * reduction with (mass, q) to the 22 FMOMR-order values */
+void or_hexa_raw_moments(const double* x, const double* y, const double* z, const double* m,
+                         int64_t n, const double cm[3], double out[25]);
+void or_hexa_reduce(double mass, const double q[6], const double r[25], double out[22]);
+/* potential and acceleration of reduced moments h at r = target - cm */
+void or_hexa_eval(const double h[22], double rx, double ry, double rz, double* pot,
+                  double acc[3]);
+/* or_gravity / or_lists of the HEXADECAPOLE build (open() with the softening
+ * and monopole tests, node() with the reduced moments) */
+void or_gravity_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
+                     const double* x, const double* y, const double* z, const double* m,
+                     const double* eps, double theta, const double offset[3],
+                     const int32_t* bucket_sel, int64_t n_sel, double* ax, double* ay,
+                     double* az, double* pot, uint64_t counters[4], int nthreads);
+void or_lists_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
+                   double theta, const double offset[3], int64_t* pp_count, int64_t* pc_count,
+                   const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc);
 
 /* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
 /* (da/dx)^T w and (da/dm)^T w of or_direct over all targets (test-only

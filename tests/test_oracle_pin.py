@@ -367,3 +367,192 @@ def test_direct_vjp_invariants():
     assert np.abs(g.sum(0)).max() < 1e-12 * scale * len(pos)
     wa = _dot_acc(pos, m, e, w)
     assert abs(np.dot(m, gm) - wa) < 1e-12 * abs(wa) * len(pos)
+
+
+# ---------------------------------------------------------------- HEXADECAPOLE
+
+def _stf_numpy(pos, m, cm):
+    """Trace-free moments of orders 2-4 by tensor algebra (einsum over the
+    full tensors, trace removal with Kronecker deltas), in the 22-value FMOMR
+    order -- an independent route from the oracle's component formulas."""
+    d = pos - cm
+    I = np.eye(3)
+    T2 = np.einsum("n,ni,nj->ij", m, d, d)
+    T3 = np.einsum("n,ni,nj,nk->ijk", m, d, d, d)
+    T4 = np.einsum("n,ni,nj,nk,nl->ijkl", m, d, d, d, d)
+    S2 = T2 - np.trace(T2) / 3 * I
+    V = np.einsum("ijj->i", T3)
+    S3 = T3 - (np.einsum("ij,k->ijk", I, V) + np.einsum("ik,j->ijk", I, V)
+               + np.einsum("jk,i->ijk", I, V)) / 5
+    T = np.einsum("iikl->kl", T4)
+    s = np.trace(T)
+    S4 = (T4 - (np.einsum("ij,kl->ijkl", I, T) + np.einsum("ik,jl->ijkl", I, T)
+                + np.einsum("il,jk->ijkl", I, T) + np.einsum("jk,il->ijkl", I, T)
+                + np.einsum("jl,ik->ijkl", I, T) + np.einsum("kl,ij->ijkl", I, T)) / 7
+          + s / 35 * (np.einsum("ij,kl->ijkl", I, I) + np.einsum("ik,jl->ijkl", I, I)
+                      + np.einsum("il,jk->ijkl", I, I)))
+    assert np.allclose(np.einsum("iikl->kl", S4), 0, atol=1e-12 * np.abs(T4).max())
+    x, y, z = 0, 1, 2
+    return np.array([m.sum(), S2[x, x], S2[y, y], S2[x, y], S2[x, z], S2[y, z],
+                     S3[x, x, x], S3[x, y, y], S3[x, x, y], S3[y, y, y], S3[x, x, z],
+                     S3[y, y, z], S3[x, y, z], S4[x, x, x, x], S4[x, y, y, y], S4[x, x, x, y],
+                     S4[y, y, y, y], S4[x, x, x, z], S4[y, y, y, z], S4[x, x, y, y],
+                     S4[x, x, y, z], S4[x, y, y, z]])
+
+
+def _cluster(seed=5, n=60):
+    rng = np.random.default_rng(seed)
+    pos = rng.standard_normal((n, 3)) * [0.12, 0.08, 0.05] + [0.3, -0.2, 0.1]
+    m = rng.uniform(0.5, 1.5, n)
+    cm = (m[:, None] * pos).sum(0) / m.sum()
+    return pos, m, cm
+
+
+def _hexa_of(pos, m, cm):
+    d = pos - cm
+    q = np.array([(m * d[:, a] * d[:, b]).sum() for a, b in
+                  ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))])
+    return O.hexa_reduce(m.sum(), q, O.hexa_raw_moments(pos, m, cm))
+
+
+def test_hexa_reduced_moments_match_tensor_algebra():
+    pos, m, cm = _cluster()
+    h = _hexa_of(pos, m, cm)
+    ref = _stf_numpy(pos, m, cm)
+    scale = m.sum() * np.abs(pos - cm).max() ** np.array([0] + [2] * 5 + [3] * 7 + [4] * 9)
+    assert np.all(np.abs(h - ref) <= 1e-14 * scale)
+
+
+def test_hexa_potential_matches_legendre_series():
+    """pot(r) = -sum_j m_j sum_{l=0,2,3,4} |d_j|^l P_l(cos g) / r^(l+1) (the
+    dipole vanishes about the centre of mass): pins the (2l-1)!!/l! weights."""
+    from numpy.polynomial import legendre
+    pos, m, cm = _cluster()
+    h = _hexa_of(pos, m, cm)
+    d = pos - cm
+    for r in ([1.0, 0.3, -0.2], [0.2, -0.7, 0.9], [-2.0, 1.0, 0.5]):
+        r = np.array(r)
+        rn = np.linalg.norm(r)
+        dn = np.linalg.norm(d, axis=1)
+        c = d @ r / (dn * rn)
+        ref = 0.0
+        for l in (0, 2, 3, 4):
+            ref -= (m * dn ** l * legendre.legval(c, [0] * l + [1])).sum() / rn ** (l + 1)
+        pot, _ = O.hexa_eval(h, r)
+        assert abs(pot - ref) <= 1e-14 * abs(ref)
+
+
+def test_hexa_acceleration_is_minus_gradient_of_potential():
+    pos, m, cm = _cluster()
+    h = _hexa_of(pos, m, cm)
+    for r in ([1.0, 0.3, -0.2], [-0.4, 0.9, 0.7]):
+        r = np.array(r)
+        _, acc = O.hexa_eval(h, r)
+        g = np.zeros(3)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-5
+            g[k] = -(O.hexa_eval(h, r + e)[0] - O.hexa_eval(h, r - e)[0]) / 2e-5
+        assert np.abs(acc - g).max() <= 1e-8 * np.abs(acc).max()
+
+
+def test_hexa_truncation_error_falls_as_fifth_order():
+    """Against the exact point-mass sum, the acceleration error of the order-4
+    expansion falls as (size/distance)^5 relative (x32 per doubling); the
+    quadrupole truncation would fall as ^3."""
+    pos, m, cm = _cluster()
+    h = _hexa_of(pos, m, cm)
+    u = np.array([0.6, -0.48, 0.64])
+    errs = []
+    for dist in (1.5, 3.0, 6.0):
+        r = u * dist
+        _, acc = O.hexa_eval(h, r)
+        dd = r - (pos - cm)
+        ref = -(m[:, None] * dd / (np.linalg.norm(dd, axis=1) ** 3)[:, None]).sum(0)
+        errs.append(np.linalg.norm(acc - ref) / np.linalg.norm(ref))
+    assert 20 < errs[0] / errs[1] < 50 and 20 < errs[1] / errs[2] < 50
+
+
+def test_hexa_tree_moments_match_closed_form_per_node():
+    """Leaf sums and the binomial shifts of the upward pass give, for every
+    node, the trace-free moments of all its particles about its centre of
+    mass (tensor algebra on the particles)."""
+    pos, _, m = ics.plummer(2000, seed=21)
+    P = O.Pipeline(pos, m, np.full(2000, 0.01), bucket=8, theta=0.7, hexadecapole=True)
+    px = np.stack([P.x, P.y, P.z], axis=1)
+    for i in range(len(P.nodes)):
+        b, c = int(P.nodes[i]["begin"]), int(P.nodes[i]["count"])
+        if c == 0:
+            continue
+        cm = P.data[i]["cm"]
+        ref = _stf_numpy(px[b:b + c], P.m[b:b + c], cm)
+        ext = np.abs(px[b:b + c] - cm).max()
+        scale = P.m[b:b + c].sum() * ext ** np.array([0] + [2] * 5 + [3] * 7 + [4] * 9)
+        assert np.all(np.abs(P.hexa[i] - ref) <= 1e-12 * scale + 1e-300), i
+
+
+def test_hexa_lists_respect_the_hexadecapole_opening():
+    """GravityVisitor::open, HEXADECAPOLE branch, by brute force: with large
+    softening the softening/monopole branch decides many nodes; every listed
+    P-C node fails open() and all its ancestors pass it; the sets differ from
+    the quadrupole build's."""
+    pos, _, m = ics.plummer(3000, seed=4)
+    eps = np.full(3000, 0.15)
+    theta = 0.6
+    P = O.Pipeline(pos, m, eps, bucket=8, theta=theta, hexadecapole=True)
+    Q = O.Pipeline(pos, m, eps, bucket=8, theta=theta)
+    pp_off, pp, pc_off, pc = P.lists()
+    nodes, data = P.nodes, P.data
+    gf = (4.0 / 3.0) / theta ** 2
+    gm = np.sqrt(4.0 / 3.0) / theta ** 4
+
+    def soft_meet(a_cm, a_soft, b):
+        # openSoftening(source = a, target = b): spheres, then b's box
+        r1 = 2.0 * a_soft
+        dsq = ((a_cm - data[b]["cm"]) ** 2).sum()
+        if dsq <= (r1 + 2.0 * data[b]["soft"]) ** 2:
+            return True
+        return O.intersect_box_point(data[b]["lo"], data[b]["hi"], a_cm, r1 * r1)
+
+    def opens(src, tgt):
+        if data[src]["count"] <= 6:
+            return True
+        lo, hi, cm = data[tgt]["lo"], data[tgt]["hi"], data[src]["cm"]
+        if O.intersect_box_point(lo, hi, cm, data[src]["radius"] ** 2 * gf):
+            return True
+        if not soft_meet(data[tgt]["cm"], data[tgt]["soft"], src):
+            return False
+        return O.intersect_box_point(lo, hi, cm, (data[src]["radius"] * gm) ** 2)
+
+    n = len(P.x)
+    for bi in range(0, len(P.bucket_nodes), 5):
+        bn = P.bucket_nodes[bi]
+        cover = np.zeros(n, dtype=np.int64)
+        for s in pp[pp_off[bi]:pp_off[bi + 1]]:
+            cover[nodes[s]["begin"]:nodes[s]["begin"] + nodes[s]["count"]] += 1
+        for s in pc[pc_off[bi]:pc_off[bi + 1]]:
+            assert nodes[s]["type"] == O.OR_INTERNAL and not opens(s, bn)
+            cover[nodes[s]["begin"]:nodes[s]["begin"] + nodes[s]["count"]] += 1
+        assert np.all(cover == 1)
+        for s in np.concatenate([pp[pp_off[bi]:pp_off[bi + 1]], pc[pc_off[bi]:pc_off[bi + 1]]]):
+            a = nodes[s]["parent"]
+            while a >= 0:
+                assert opens(a, bn)
+                a = nodes[a]["parent"]
+    assert len(pc) != len(Q.lists()[3])
+
+
+def test_hexa_tree_force_meets_reference_accuracy_statement():
+    """The reference's accuracy test (tests/gravity/acc_test.sh:5) states, for
+    its HEXADECAPOLE build on this fixture with its defaults (bucket 12,
+    theta 0.7): RMS relative force error .0008 and maximum .03 against
+    direct.acc.  The quadrupole build misses the maximum."""
+    pos, m, e, ref = _lambs()
+    P = O.Pipeline(pos, m, e, bucket=12, theta=0.7, hexadecapole=True)
+    acc, _, _ = P.gravity_unsorted()
+    r = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.sqrt(np.mean(r * r)) < 8e-4 and r.max() < 0.03
+    Q = O.Pipeline(pos, m, e, bucket=12, theta=0.7)
+    acc_q, _, _ = Q.gravity_unsorted()
+    rq = np.linalg.norm(acc_q - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert rq.max() > 0.03 and np.sqrt(np.mean(r * r)) < np.sqrt(np.mean(rq * rq)) / 3
