@@ -72,6 +72,7 @@ struct Configuration {
   std::string input_file;          // -f achInputFile
   std::string output_file;         // -v achOutputFile
   int periodic = 0;                // bPeriodic
+  bool hexadecapole = false;       // the -DHEXADECAPOLE build (examples/Makefile:5)
 
   int branchFactor() const {
     switch (tree_type) {
@@ -132,6 +133,7 @@ struct Configuration {
     c.dsoft = dSoft;
     c.has_dsoft = dSoft_set ? 1 : 0;
     c.periodic = periodic;
+    c.hexadecapole = hexadecapole ? 1 : 0;
     return c;
   }
 

@@ -68,6 +68,11 @@ typedef struct pt_config {
   double dsoft;                   /* '-e' dSoft; applied to every particle if has_dsoft */
   int32_t has_dsoft;              /* Configuration::origin_of("dSoft") != Unknown */
   int32_t periodic;               /* bPeriodic; must be 0 (non-periodic path) */
+  int32_t hexadecapole;           /* the -DHEXADECAPOLE build (examples/Makefile:5):
+                                     moments to fourth order, GravityVisitor::open/node
+                                     HEXADECAPOLE branches (examples/GravityVisitor.h:
+                                     162-178,199-206); FP64 only, list path, no adjoint.
+                                     0 = the quadrupole build */
 } pt_config;
 
 /* Interaction counters, src/ThreadStateHolder.h:10-13 (COUNT_INTERACTIONS). */
@@ -197,6 +202,12 @@ pt_status pt_leapfrog_adjoint(pt_ctx* ctx, const double* lam_x, const double* la
                               double* grad_x, double* grad_v, double* grad_m, pt_memory where);
 
 /* ---- read-back (host buffers, synchronous) ---- */
+
+/* HEXADECAPOLE contexts: per node (level order, as pt_get_nodes) the mass and
+ * the 21 reduced moments in FMOMR order (examples/moments.h: xx yy xy xz yz,
+ * xxx xyy xxy yyy xxz yyz xyz, xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz),
+ * unscaled; out: nn x 22. */
+pt_status pt_get_node_hexadecapole(pt_ctx* ctx, double* out);
 pt_status pt_get_universe(pt_ctx* ctx, double box[6]);
 pt_status pt_get_keys(pt_ctx* ctx, uint64_t* keys);                 /* original order */
 pt_status pt_get_sorted(pt_ctx* ctx, uint64_t* sorted_keys, uint32_t* perm);

@@ -27,7 +27,7 @@ EXPORTS = (
     "pt_get_keys", "pt_get_sorted", "pt_get_tree_size", "pt_get_nodes", "pt_get_node_data",
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
-    "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint",
+    "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint", "pt_get_node_hexadecapole",
 )
 
 
@@ -42,6 +42,7 @@ class PtConfig(ctypes.Structure):
         ("dsoft", ctypes.c_double),
         ("has_dsoft", ctypes.c_int32),
         ("periodic", ctypes.c_int32),
+        ("hexadecapole", ctypes.c_int32),
     ]
 
 
@@ -123,6 +124,7 @@ def load():
         "pt_get_tree_size": (I32, [P, P, P, P]),
         "pt_get_nodes": (I32, [P, P, P, P, P, P, P, P]),
         "pt_get_node_data": (I32, [P, P]),
+        "pt_get_node_hexadecapole": (I32, [P, P]),
         "pt_get_buckets": (I32, [P, P]),
         "pt_get_lists_size": (I32, [P, P, P]),
         "pt_get_lists": (I32, [P, P, P, P, P]),

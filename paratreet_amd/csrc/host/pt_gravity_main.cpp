@@ -30,13 +30,14 @@ int main(int argc, char** argv) {
       if (rest[i] == "-j" && i + 1 < rest.size()) max_timestep = std::atof(rest[++i].c_str());
       else if (rest[i] == "-theta" && i + 1 < rest.size()) theta = std::atof(rest[++i].c_str());
       else if (rest[i] == "-fp32c") prec = PT_FP32C;
+      else if (rest[i] == "-hexadecapole") conf.hexadecapole = true;  // -DHEXADECAPOLE build
       else {
         std::fprintf(stderr, "unsupported option %s\n", rest[i].c_str());
         return 2;
       }
     }
     if (conf.input_file.empty()) {
-      std::fprintf(stderr, "usage: %s -f <tipsy> [-l B] [-e dSoft] [-i it] [-v out] [-theta t] [-fp32c]\n",
+      std::fprintf(stderr, "usage: %s -f <tipsy> [-l B] [-e dSoft] [-i it] [-v out] [-theta t] [-fp32c] [-hexadecapole]\n",
                    argv[0]);
       return 2;
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -72,6 +73,11 @@ struct pt_ctx {
   DBuf<GravNode> gn;
   DBuf<GravNode32> gn32;
   DBuf<Bucket> bk;
+  // HEXADECAPOLE build: raw third/fourth moments (SoA), node() and open() records
+  DBuf<double> hraw;
+  DBuf<HexNode> hn;
+  DBuf<HexOpen> ho;
+  double gm = 0.0;  // GravityVisitor::monopole_gravity_factor
   DBuf<unsigned long long> cpart;  // per-block counter partials
   DBuf<unsigned long long> wqueue;  // per-XCD work counters of the fused kernel
   bool lists_valid = false;
@@ -246,6 +252,7 @@ void pt_config_default(pt_config* cfg) {
   cfg->dsoft = 0.0;
   cfg->has_dsoft = 0;
   cfg->periodic = 0;
+  cfg->hexadecapole = 0;
 }
 
 pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx** out) {
@@ -534,6 +541,22 @@ pt_status pt_upward(pt_ctx* c) {
   }
   PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tnode.p, c->gn.p,
                                   c->gn32.p, c->bk.p, c->stream));
+  if (c->cfg.hexadecapole) {
+    // third and fourth moments once every centre of mass is final
+    PT_TRY_HIP(c, ensure(c->hraw, (int64_t)PT_HEXA_RAW * nn));
+    PT_TRY_HIP(c, ensure(c->hn, nn));
+    PT_TRY_HIP(c, ensure(c->ho, nn));
+    PT_TRY_LAUNCH(c, pt_launch_hexa_leaf(tr, c->bucket_node.p, c->nb, c->xs.p, c->ys.p, c->zs.p,
+                                         c->ms.p, c->nd.p, nn, c->hraw.p, c->stream));
+    for (int d = nlev - 1; d >= 0; d--) {
+      const int64_t a = c->level_off[d], b = c->level_off[d + 1];
+      if (b > a)
+        PT_TRY_LAUNCH(c, pt_launch_hexa_level(tr, c->level_nodes.p + a, (int32_t)(b - a),
+                                              c->nd.p, nn, c->hraw.p, c->stream));
+    }
+    PT_TRY_LAUNCH(c, pt_launch_hexa_pack(tr, c->nd.p, c->hraw.p, nn, c->hn.p, c->ho.p,
+                                         c->stream));
+  }
   c->stage = S_MOMENTS;
   c->lists_valid = false;
   return PT_OK;
@@ -546,6 +569,7 @@ static pt_status set_visitor(pt_ctx* c, double theta, const double offset[3], do
   // GravityVisitor(offset, theta) (examples/GravityVisitor.h:16-19)
   const double opening_geometry_factor_squared = 4.0 / 3.0;
   *gf = opening_geometry_factor_squared / (theta * theta);
+  c->gm = std::sqrt(opening_geometry_factor_squared) / (theta * theta * theta * theta);
   return PT_OK;
 }
 
@@ -577,9 +601,10 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
     if ((st = ensure_scratch(c, pt_scan_scratch_bytes(nb + 1)))) return st;
     PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, std::max<int64_t>(nblk, 1) * 4, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
-    PT_TRY_LAUNCH(c, pt_launch_traverse(false, c->tnode.p, c->bk.p, nb, gf, c->offset, c->pp_cnt.p,
-                                        c->pc_cnt.p, nullptr, nullptr, nullptr, nullptr,
-                                        c->cpart.p, c->derr.p, c->stream));
+    const HexOpen* ho = c->cfg.hexadecapole ? c->ho.p : nullptr;
+    PT_TRY_LAUNCH(c, pt_launch_traverse(false, c->tnode.p, ho, c->bk.p, nb, gf, c->gm, c->offset,
+                                        c->pp_cnt.p, c->pc_cnt.p, nullptr, nullptr, nullptr,
+                                        nullptr, c->cpart.p, c->derr.p, c->stream));
     PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pp_cnt.p, c->pp_off.p, nb, c->scratch.p,
                                                   c->pp_off.p + nb, c->stream));
     PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pc_cnt.p, c->pc_off.p, nb, c->scratch.p,
@@ -594,9 +619,9 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
     c->n_pc = tot[1];
     PT_TRY_HIP(c, ensure(c->pp, std::max<int64_t>(c->n_pp, 1)));
     PT_TRY_HIP(c, ensure(c->pc, std::max<int64_t>(c->n_pc, 1)));
-    PT_TRY_LAUNCH(c, pt_launch_traverse(true, c->tnode.p, c->bk.p, nb, gf, c->offset, nullptr,
-                                        nullptr, c->pp_off.p, c->pc_off.p, c->pp.p, c->pc.p,
-                                        c->cpart.p, c->derr.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_traverse(true, c->tnode.p, ho, c->bk.p, nb, gf, c->gm, c->offset,
+                                        nullptr, nullptr, c->pp_off.p, c->pc_off.p, c->pp.p,
+                                        c->pc.p, c->cpart.p, c->derr.p, c->stream));
   }
   if (counters) *counters = c->h_counters;
   c->lists_valid = true;
@@ -608,7 +633,15 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
   if (st) return st;
   if (!c->lists_valid) return fail(c, PT_ERR_STATE, "pt_interact called before pt_traverse");
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
-  {
+  if (c->cfg.hexadecapole && precision != PT_FP64)
+    return fail(c, PT_ERR_UNSUPPORTED, "the hexadecapole build evaluates node() in FP64 only");
+  if (c->cfg.hexadecapole) {
+    Timer t(c, PT_T_FORCE);
+    PT_TRY_LAUNCH(c, pt_launch_interact_hexa(c->bk.p, c->nb, c->tnode.p, c->hn.p, c->xs.p, c->ys.p,
+                                             c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
+                                             c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
+                                             c->azs.p, c->pots.p, c->stream));
+  } else {
     Timer t(c, PT_T_FORCE);
     PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
                                         c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
@@ -631,6 +664,15 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (st) return st;
   if ((st = need(c, S_MOMENTS, "pt_compute_forces"))) return st;
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  if (c->cfg.hexadecapole) {
+    // the fused kernel is quadrupole-only; the hexadecapole build runs the
+    // traversal lists and their interaction kernel
+    pt_counters cnt;
+    if ((st = pt_traverse(c, theta, offset, &cnt))) return st;
+    if ((st = pt_interact(c, precision))) return st;
+    if (counters) *counters = cnt;
+    return PT_OK;
+  }
   double gf;
   if ((st = set_visitor(c, theta, offset, &gf))) return st;
   const int64_t nb = c->nb;
@@ -901,6 +943,8 @@ pt_status pt_force_vjp(pt_ctx* c, double theta, pt_force_mode mode, const double
   if (!w) return fail(c, PT_ERR_ARG, "missing cotangent");
   if (c->n == 0) return PT_OK;
   if (mode == PT_FORCE_TREE && !(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
+  if (mode == PT_FORCE_TREE && c->cfg.hexadecapole)
+    return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the hexadecapole node()");
   AdjViews v;
   if ((st = adj_views(c, &v))) return st;
   // the sorted-order block (gsx..) doubles as the host staging area before
@@ -924,6 +968,8 @@ pt_status pt_leapfrog_forward(pt_ctx* c, int32_t steps, double dt, double theta,
   if (mode != PT_FORCE_TREE && mode != PT_FORCE_DIRECT) return fail(c, PT_ERR_ARG, "bad force mode");
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
   if (c->n == 0) return fail(c, PT_ERR_ARG, "no particles");
+  if (mode == PT_FORCE_TREE && c->cfg.hexadecapole)
+    return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the hexadecapole node()");
   const int32_t every = checkpoint_every > 0 ? checkpoint_every : std::max<int32_t>(steps, 1);
   const int64_t nseg = std::max<int64_t>(1, (steps + every - 1) / every);
   const int64_t n = c->n;
@@ -1075,6 +1121,21 @@ pt_status pt_get_node_data(pt_ctx* c, double* out) {
   if ((st = d2h(c, soa.data(), c->nd.p, soa.size() * sizeof(double)))) return st;
   for (int64_t i = 0; i < nn; i++)
     for (int f = 0; f < PT_CD_NFIELDS; f++) out[i * PT_CD_NFIELDS + f] = soa[(size_t)f * nn + i];
+  return PT_OK;
+}
+
+pt_status pt_get_node_hexadecapole(pt_ctx* c, double* out) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_MOMENTS, "pt_get_node_hexadecapole"))) return st;
+  if (!c->cfg.hexadecapole) return fail(c, PT_ERR_STATE, "context not created with hexadecapole");
+  const int64_t nn = c->nn;
+  std::vector<HexNode> h((size_t)nn);
+  if ((st = d2h(c, h.data(), c->hn.p, h.size() * sizeof(HexNode)))) return st;
+  for (int64_t i = 0; i < nn; i++) {
+    out[i * 22] = h[i].mass;
+    for (int k = 0; k < PT_HEXA_MOMS; k++) out[i * 22 + 1 + k] = h[i].h[k];
+  }
   return PT_OK;
 }
 

@@ -49,6 +49,22 @@ struct __align__(32) GravNode {
 struct __align__(32) GravNode32 {
   float mass, soft, qxx, qxy, qxz, qyy, qyz, qzz;
 };
+// HEXADECAPOLE build (-DHEXADECAPOLE, examples/Makefile:5).  Source record
+// of node(): centre of mass, mass, softening and the 21 reduced (trace-free)
+// moments of orders 2-4 in FMOMR order (examples/moments.h): xx yy xy xz yz,
+// xxx xyy xxy yyy xxz yyz xyz, xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz.
+#define PT_HEXA_MOMS 21
+#define PT_HEXA_RAW 25  // raw third (10) and fourth (15) moments about the cm
+struct __align__(64) HexNode {
+  double cx, cy, cz, mass, soft;
+  double h[PT_HEXA_MOMS];
+  double pad[6];
+};
+// Source data read by the HEXADECAPOLE open(): the node's box and softening.
+struct __align__(64) HexOpen {
+  double lox, loy, loz, hix, hiy, hiz, soft, pad;
+};
+
 // Bucket (target leaf) record: particle range, box, centre of mass, softening.
 struct __align__(32) Bucket {
   double lox, loy, loz, hix;

@@ -824,6 +824,177 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// HEXADECAPOLE build: node() with the reduced moments to fourth order
+// (GravityVisitor::node, examples/GravityVisitor.h:184-206 -> momEvalFmomrcm,
+// examples/moments.C:1469-1526), over the per-bucket lists, FP64.  leaf() is
+// the quadrupole build's.
+// ---------------------------------------------------------------------------
+
+// Staged HEXADECAPOLE source: centre of mass, mass, monopole flag (1 when
+// openSoftening sends the node to addGravity) and the 21 reduced moments.
+struct SrcH {
+  double cx, cy, cz, mass, mono;
+  double h[PT_HEXA_MOMS];
+};
+
+struct __align__(16) HexaLds {
+  static constexpr int kStage = PT_WAVE * sizeof(SrcH);
+  static_assert(kSlots * sizeof(SrcP64) <= kStage, "particle chunk");
+  static_assert(64 * 4 * sizeof(double) <= kStage, "reduction area");
+  unsigned char stage[kStage];
+  int32_t fl_mark[PT_WAVE];
+};
+
+// Potential and acceleration of reduced moments h at r = target - cm: with
+// c_l = (2l-1)!!/l! (3/2, 5/2, 35/8), v_l = M_l contracted with l-1 copies of
+// r and s_l = v_l . r,
+//   pot = -(m / r + sum_l c_l s_l / r^(2l+1)),
+//   acc = sum_l c_l (l v_l / r^(2l+1) - (2l+1) s_l r / r^(2l+3)) - m r / r^3
+// (the trace-free moments give the Legendre series of the potential).
+__device__ __forceinline__ void hexa_eval(const double* __restrict__ h, double m, double rx,
+                                          double ry, double rz, double& pot, double& gx,
+                                          double& gy, double& gz) {
+  const double xx = h[0], yy = h[1], xy = h[2], xz = h[3], yz = h[4];
+  const double zz = -(xx + yy);
+  const double xxx = h[5], xyy = h[6], xxy = h[7], yyy = h[8], xxz = h[9], yyz = h[10],
+               xyz = h[11];
+  const double xzz = -(xxx + xyy), yzz = -(xxy + yyy), zzz = -(xxz + yyz);
+  const double xxxx = h[12], xyyy = h[13], xxxy = h[14], yyyy = h[15], xxxz = h[16],
+               yyyz = h[17], xxyy = h[18], xxyz = h[19], xyyz = h[20];
+  const double xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
+  const double xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
+  const double rsq = rx * rx + ry * ry + rz * rz;
+  const double ir = 1.0 / sqrt(rsq);
+  const double ir2 = ir * ir;
+  const double x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
+  const double xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
+  const double v2x = xx * rx + xy * ry + xz * rz;
+  const double v2y = xy * rx + yy * ry + yz * rz;
+  const double v2z = xz * rx + yz * ry + zz * rz;
+  const double s2 = v2x * rx + v2y * ry + v2z * rz;
+  const double v3x = xxx * x2 + xyy * y2 + xzz * z2 + 2.0 * (xxy * xyp + xxz * xzp + xyz * yzp);
+  const double v3y = xxy * x2 + yyy * y2 + yzz * z2 + 2.0 * (xyy * xyp + xyz * xzp + yyz * yzp);
+  const double v3z = xxz * x2 + yyz * y2 + zzz * z2 + 2.0 * (xyz * xyp + xzz * xzp + yzz * yzp);
+  const double s3 = v3x * rx + v3y * ry + v3z * rz;
+  const double x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz, xyz3 = xyp * rz;
+  const double v4x = xxxx * x3 + xyyy * y3 + xzzz * z3 +
+                     3.0 * (xxxy * x2 * ry + xxxz * x2 * rz + xxyy * rx * y2 + xxzz * rx * z2 +
+                            xyyz * y2 * rz + xyzz * ry * z2) +
+                     6.0 * xxyz * xyz3;
+  const double v4y = xxxy * x3 + yyyy * y3 + yzzz * z3 +
+                     3.0 * (xxyy * x2 * ry + xxyz * x2 * rz + xyyy * rx * y2 + xyzz * rx * z2 +
+                            yyyz * y2 * rz + yyzz * ry * z2) +
+                     6.0 * xyyz * xyz3;
+  const double v4z = xxxz * x3 + yyyz * y3 + zzzz * z3 +
+                     3.0 * (xxyz * x2 * ry + xxzz * x2 * rz + xyyz * rx * y2 + xzzz * rx * z2 +
+                            yyzz * y2 * rz + yzzz * ry * z2) +
+                     6.0 * xyzz * xyz3;
+  const double s4 = v4x * rx + v4y * ry + v4z * rz;
+  const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2,
+               ir11 = ir9 * ir2;
+  pot = -(m * ir + 1.5 * s2 * ir5 + 2.5 * s3 * ir7 + 4.375 * s4 * ir9);
+  const double g = m * ir3 + 7.5 * s2 * ir7 + 17.5 * s3 * ir9 + 39.375 * s4 * ir11;
+  gx = 3.0 * v2x * ir5 + 7.5 * v3x * ir7 + 17.5 * v4x * ir9 - rx * g;
+  gy = 3.0 * v2y * ir5 + 7.5 * v3y * ir7 + 17.5 * v4y * ir9 - ry * g;
+  gz = 3.0 * v2z * ir5 + 7.5 * v3z * ir7 + 17.5 * v4z * ir9 - rz * g;
+}
+
+__device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double ox, double oy,
+                                          double oz) {
+  if (s.mono != 0.0) {
+    // addGravity: unsoftened monopole, no potential
+    const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    double f = s.mass / (rsq * sqrt(rsq));
+    f = rsq != 0 ? f : 0.0;
+    L.ax += dx * f;
+    L.ay += dy * f;
+    L.az += dz * f;
+  } else {
+    const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
+    double p, gx, gy, gz;
+    hexa_eval(s.h, s.mass, rx, ry, rz, p, gx, gy, gz);
+    L.ap += p;
+    L.ax += gx;
+    L.ay += gy;
+    L.az += gz;
+  }
+}
+
+// Stages up to 64 closed nodes (one per lane) and evaluates them for the
+// group's targets (slot s of a lane takes sources s, s + S, ...).
+__device__ __forceinline__ void do_nodes_hexa(HexaLds& W, int32_t id, int nsrc, const Group& g,
+                                              Lanes<false>& L, const HexNode* __restrict__ hn,
+                                              const Bucket& b, double ox, double oy, double oz) {
+  const int lane = __lane_id();
+  SrcH* st = reinterpret_cast<SrcH*>(W.stage);
+  if (lane < nsrc) {
+    const HexNode& s = hn[id];
+    SrcH r;
+    r.cx = s.cx;
+    r.cy = s.cy;
+    r.cz = s.cz;
+    r.mass = s.mass;
+    r.mono = pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, b) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = s.h[k];
+    st[lane] = r;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int kend = ceil_div_s(nsrc, g);
+  for (int k = 0; k < kend; k++) {
+    const int i = g.s + k * g.S;
+    if (i < nsrc) hexa_pair(st[i], L, ox, oy, oz);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(kThreads)
+interact_hexa_kernel(const Bucket* __restrict__ bk, int64_t nb,
+                     const TreeNode* __restrict__ nodes, const HexNode* __restrict__ hn,
+                     const double* __restrict__ px, const double* __restrict__ py,
+                     const double* __restrict__ pz, const double* __restrict__ pm,
+                     const double* __restrict__ pe, const int64_t* __restrict__ pp_off,
+                     const int32_t* __restrict__ pp, const int64_t* __restrict__ pc_off,
+                     const int32_t* __restrict__ pc, double ox, double oy, double oz,
+                     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+                     double* __restrict__ pot) {
+  __shared__ HexaLds lds[kWaves];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  if (bi >= nb) return;
+  HexaLds& W = lds[wave];
+  const Bucket b = bk[bi];
+  const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+               ccz = (b.loz + b.hiz) * 0.5;
+  const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
+  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
+  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
+  for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
+    const Group g = make_group(b, g0);
+    Lanes<false> L;
+    load_targets<false>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+    for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
+      const int nsrc = (pc1 - e0) < PT_WAVE ? (int)(pc1 - e0) : PT_WAVE;
+      const int32_t id = lane < nsrc ? pc[e0 + lane] : 0;
+      do_nodes_hexa(W, id, nsrc, g, L, hn, b, ox, oy, oz);
+    }
+    for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
+      int32_t beg = 0, cnt = 0;
+      if (l0 + lane < pp1) {
+        const TreeNode nd = nodes[pp[l0 + lane]];
+        beg = nd.begin;
+        cnt = nd.count;
+      }
+      do_leaves<false, HexaLds>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
+                                ccz);
+    }
+    finish_group<false, HexaLds>(W, g, L, ax, ay, az, pot);
+  }
+}
+
 // One target bucket of the fused kernel: the walk for the bucket box with
 // the bucket's chunks evaluated as they fill, counters added to the wave's.
 template <bool F32, int STACK>
@@ -1027,6 +1198,19 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
                                                             e, pp_off, pp, pc_off, pc, off[0], off[1],
                                                             off[2], ax, ay, az, pot);
   }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_interact_hexa(const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const HexNode* hn, const double* x, const double* y, const double* z,
+                            const double* m, const double* e, const int64_t* pp_off,
+                            const int32_t* pp, const int64_t* pc_off, const int32_t* pc,
+                            const double off[3], double* ax, double* ay, double* az, double* pot,
+                            hipStream_t s) {
+  if (nb <= 0) return 0;
+  interact_hexa_kernel<<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, hn, x, y, z, m, e, pp_off,
+                                                         pp, pc_off, pc, off[0], off[1], off[2],
+                                                         ax, ay, az, pot);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

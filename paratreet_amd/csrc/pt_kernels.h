@@ -48,6 +48,15 @@ int pt_launch_buckets(const PtTree& t, int64_t nn, int64_t n, uint32_t* head, ui
                       void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node, hipStream_t s);
 
 int pt_launch_moments_init(double* nd, int64_t nn, hipStream_t s);
+// HEXADECAPOLE moments: raw third/fourth moments (SoA, PT_HEXA_RAW x nn) of
+// the buckets, then of each internal level (deepest first), then the records.
+int pt_launch_hexa_leaf(const PtTree& t, const int32_t* bucket_node, int64_t nb, const double* x,
+                        const double* y, const double* z, const double* m, const double* nd,
+                        int64_t nn, double* raw, hipStream_t s);
+int pt_launch_hexa_level(const PtTree& t, const int32_t* level_nodes, int32_t nl,
+                         const double* nd, int64_t nn, double* raw, hipStream_t s);
+int pt_launch_hexa_pack(const PtTree& t, const double* nd, const double* raw, int64_t nn,
+                        HexNode* hn, HexOpen* ho, hipStream_t s);
 int pt_launch_leaf_moments(const PtTree& t, const int32_t* bucket_node, int64_t nb,
                            const double* x, const double* y, const double* z, const double* m,
                            const double* e, double* nd, int64_t nn, hipStream_t s);
@@ -60,10 +69,11 @@ int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t*
 // List-producing traversal (count pass when fill is false).  partials holds
 // pt_traverse_blocks(nb) x 4 counters, reduced by pt_launch_reduce_counters.
 int64_t pt_traverse_blocks(int64_t nb);
-int pt_launch_traverse(bool fill, const TreeNode* nodes, const Bucket* bk, int64_t nb, double gf,
-                       const double off[3], int32_t* pp_cnt, int32_t* pc_cnt,
-                       const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc,
-                       unsigned long long* partials, int32_t* err, hipStream_t s);
+// ho != nullptr: HEXADECAPOLE open() (gm = monopole_gravity_factor).
+int pt_launch_traverse(bool fill, const TreeNode* nodes, const HexOpen* ho, const Bucket* bk,
+                       int64_t nb, double gf, double gm, const double off[3], int32_t* pp_cnt,
+                       int32_t* pc_cnt, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                       int32_t* pc, unsigned long long* partials, int32_t* err, hipStream_t s);
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
                               unsigned long long* out, hipStream_t s);
 
@@ -76,6 +86,13 @@ int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNo
 
 // Fused walk + interactions (resident grid of pt_gravity_blocks work-groups;
 // partials holds 4 counters per work-group; queue holds 8 work counters).
+// node() / leaf() of the HEXADECAPOLE build over the lists (FP64).
+int pt_launch_interact_hexa(const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const HexNode* hn, const double* x, const double* y, const double* z,
+                            const double* m, const double* e, const int64_t* pp_off,
+                            const int32_t* pp, const int64_t* pc_off, const int32_t* pc,
+                            const double off[3], double* ax, double* ay, double* az, double* pot,
+                            hipStream_t s);
 int64_t pt_gravity_blocks(int64_t nb, int precision);
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,

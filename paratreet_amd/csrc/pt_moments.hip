@@ -370,6 +370,211 @@ pack_buckets_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
   bk[i] = b;
 }
 
+// ---------------------------------------------------------------------------
+// HEXADECAPOLE moments (the -DHEXADECAPOLE build, examples/Makefile:5).
+//
+// The reference keeps reduced (trace-free) moments to fourth order per node
+// (FMOMR, examples/moments.h), built per particle and shifted to parents
+// (examples/MultipoleMoments.h:244-252,317-330).  As written its leaf update
+// runs only while the node radius is positive, and the leaf constructor sets
+// the radius after adding the particles (examples/CentroidData.h:41-59), so
+// every FMOMR stays zero.  These kernels compute what the code intends: the
+// raw third and fourth moments R(a,b,c) = sum m dx^a dy^b dz^c about each
+// node's centre of mass (leaves sum their particles in sorted order; a parent
+// sums its children's moments moved to its centre of mass by the binomial
+// expansion, children in index order), then the trace-free parts together
+// with the second moments q of the quadrupole pass.  Same operations, in the
+// same order, as the oracle's restatement.
+// ---------------------------------------------------------------------------
+
+// index of component (a, b, .) among those of order l (a descending, then b)
+__device__ __forceinline__ int hx_index(int l, int a, int b) {
+  return (l - a) * (l - a + 1) / 2 + (l - a - b);
+}
+
+__device__ __forceinline__ double hx_binom(int n, int k) {
+  // rows 0-4 of Pascal's triangle
+  constexpr double t[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0},
+                              {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+  return t[n][k];
+}
+
+__device__ __forceinline__ void hx_pow(double v, double p[5]) {
+  p[0] = 1.0;
+  p[1] = v;
+  p[2] = p[1] * v;
+  p[3] = p[2] * v;
+  p[4] = p[3] * v;
+}
+
+// One thread per bucket: raw moments about the leaf's centre of mass.
+__global__ void __launch_bounds__(256)
+hexa_leaf_kernel(const PtTree t, const int32_t* __restrict__ bucket_node, int64_t nb,
+                 const double* __restrict__ x, const double* __restrict__ y,
+                 const double* __restrict__ z, const double* __restrict__ m,
+                 const double* __restrict__ nd, int64_t nn, double* __restrict__ raw) {
+  const int64_t bi = blockIdx.x * 256ll + threadIdx.x;
+  if (bi >= nb) return;
+  const int32_t id = bucket_node[bi];
+  const double cx = nd[(PT_CD_CM + 0) * nn + id], cy = nd[(PT_CD_CM + 1) * nn + id],
+               cz = nd[(PT_CD_CM + 2) * nn + id];
+  double r[PT_HEXA_RAW];
+#pragma unroll
+  for (int k = 0; k < PT_HEXA_RAW; k++) r[k] = 0.0;
+  const int64_t b = t.begin[id], n = t.count[id];
+  for (int64_t i = b; i < b + n; i++) {
+    double px[5], py[5], pz[5];
+    hx_pow(x[i] - cx, px);
+    hx_pow(y[i] - cy, py);
+    hx_pow(z[i] - cz, pz);
+    const double mi = m[i];
+#pragma unroll
+    for (int l = 3; l <= 4; l++)
+#pragma unroll
+      for (int a = l; a >= 0; a--)
+#pragma unroll
+        for (int bb = l - a; bb >= 0; bb--) {
+          const int c = l - a - bb;
+          r[(l == 3 ? 0 : 10) + hx_index(l, a, bb)] += ((mi * px[a]) * py[bb]) * pz[c];
+        }
+  }
+#pragma unroll
+  for (int k = 0; k < PT_HEXA_RAW; k++) raw[k * nn + id] = r[k];
+}
+
+// raw moment of exponents (i, j, k) of a child: mass, second moments q
+// (xx xy xz yy yz zz), third/fourth from its raw record
+__device__ __forceinline__ double hx_child(const double* __restrict__ nd,
+                                           const double* __restrict__ raw, int64_t nn,
+                                           int64_t ch, double mass, int i, int j, int k) {
+  const int l = i + j + k;
+  if (l == 0) return mass;
+  if (l == 2) return nd[(PT_CD_Q + hx_index(2, i, j)) * nn + ch];
+  if (l == 3) return raw[hx_index(3, i, j) * nn + ch];
+  return raw[(10 + hx_index(4, i, j)) * nn + ch];
+}
+
+// One thread per internal node of one level (deepest level first, after the
+// quadrupole pass has fixed every centre of mass).
+__global__ void __launch_bounds__(256)
+hexa_up_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
+               const double* __restrict__ nd, int64_t nn, double* __restrict__ raw) {
+  const int64_t li = blockIdx.x * 256ll + threadIdx.x;
+  if (li >= nl) return;
+  const int32_t p = level_nodes[li];
+  double r[PT_HEXA_RAW];
+#pragma unroll
+  for (int k = 0; k < PT_HEXA_RAW; k++) r[k] = 0.0;
+  const double pmass = nd[PT_CD_MASS * nn + p];
+  if (pmass != 0.0) {
+    const double pc[3] = {nd[(PT_CD_CM + 0) * nn + p], nd[(PT_CD_CM + 1) * nn + p],
+                          nd[(PT_CD_CM + 2) * nn + p]};
+    const int32_t fc = t.first_child[p];
+    for (int c = 0; c < 8; c++) {
+      const int64_t ch = fc + c;
+      const double cm = nd[PT_CD_MASS * nn + ch];
+      if (cm == 0.0) continue;
+      double px[5], py[5], pz[5];
+      hx_pow(nd[(PT_CD_CM + 0) * nn + ch] - pc[0], px);
+      hx_pow(nd[(PT_CD_CM + 1) * nn + ch] - pc[1], py);
+      hx_pow(nd[(PT_CD_CM + 2) * nn + ch] - pc[2], pz);
+#pragma unroll
+      for (int l = 3; l <= 4; l++)
+#pragma unroll
+        for (int a = l; a >= 0; a--)
+#pragma unroll
+          for (int b = l - a; b >= 0; b--) {
+            const int cc = l - a - b;
+            double sum = 0.0;
+#pragma unroll
+            for (int i = 0; i <= a; i++)
+#pragma unroll
+              for (int j = 0; j <= b; j++)
+#pragma unroll
+                for (int k = 0; k <= cc; k++) {
+                  if (i + j + k == 1) continue;
+                  const double coef = hx_binom(a, i) * hx_binom(b, j) * hx_binom(cc, k);
+                  sum += (((coef * px[a - i]) * py[b - j]) * pz[cc - k]) *
+                         hx_child(nd, raw, nn, ch, cm, i, j, k);
+                }
+            r[(l == 3 ? 0 : 10) + hx_index(l, a, b)] += sum;
+          }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PT_HEXA_RAW; k++) raw[k * nn + p] = r[k];
+}
+
+// Trace-free moments and the HEXADECAPOLE records of every node.
+__global__ void __launch_bounds__(256)
+hexa_pack_kernel(const PtTree t, const double* __restrict__ nd, const double* __restrict__ raw,
+                 int64_t nn, HexNode* __restrict__ hn, HexOpen* __restrict__ ho) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= nn) return;
+  const int ty = t.type[i];
+  double r[PT_HEXA_RAW];
+#pragma unroll
+  for (int k = 0; k < PT_HEXA_RAW; k++) r[k] = ty == PT_NODE_EMPTY ? 0.0 : raw[k * nn + i];
+  double q[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) q[k] = nd[(PT_CD_Q + k) * nn + i];
+#define R3(a, b) r[hx_index(3, a, b)]
+#define R4(a, b) r[10 + hx_index(4, a, b)]
+  const double tr = q[0] + q[3] + q[5];
+  const double vx = R3(3, 0) + R3(1, 2) + R3(1, 0);
+  const double vy = R3(2, 1) + R3(0, 3) + R3(0, 1);
+  const double vz = R3(2, 0) + R3(0, 2) + R3(0, 0);
+  const double txx = R4(4, 0) + R4(2, 2) + R4(2, 0);
+  const double tyy = R4(2, 2) + R4(0, 4) + R4(0, 2);
+  const double tzz = R4(2, 0) + R4(0, 2) + R4(0, 0);
+  const double txy = R4(3, 1) + R4(1, 3) + R4(1, 1);
+  const double txz = R4(3, 0) + R4(1, 2) + R4(1, 0);
+  const double tyz = R4(2, 1) + R4(0, 3) + R4(0, 1);
+  const double ss = txx + tyy + tzz;
+  HexNode h;
+  h.cx = nd[(PT_CD_CM + 0) * nn + i];
+  h.cy = nd[(PT_CD_CM + 1) * nn + i];
+  h.cz = nd[(PT_CD_CM + 2) * nn + i];
+  h.mass = nd[PT_CD_MASS * nn + i];
+  h.soft = nd[PT_CD_SOFT * nn + i];
+  h.h[0] = q[0] - tr * (1.0 / 3.0);
+  h.h[1] = q[3] - tr * (1.0 / 3.0);
+  h.h[2] = q[1];
+  h.h[3] = q[2];
+  h.h[4] = q[4];
+  h.h[5] = R3(3, 0) - vx * (3.0 / 5.0);
+  h.h[6] = R3(1, 2) - vx * (1.0 / 5.0);
+  h.h[7] = R3(2, 1) - vy * (1.0 / 5.0);
+  h.h[8] = R3(0, 3) - vy * (3.0 / 5.0);
+  h.h[9] = R3(2, 0) - vz * (1.0 / 5.0);
+  h.h[10] = R3(0, 2) - vz * (1.0 / 5.0);
+  h.h[11] = R3(1, 1);
+  h.h[12] = (R4(4, 0) - txx * (6.0 / 7.0)) + ss * (3.0 / 35.0);
+  h.h[13] = R4(1, 3) - txy * (3.0 / 7.0);
+  h.h[14] = R4(3, 1) - txy * (3.0 / 7.0);
+  h.h[15] = (R4(0, 4) - tyy * (6.0 / 7.0)) + ss * (3.0 / 35.0);
+  h.h[16] = R4(3, 0) - txz * (3.0 / 7.0);
+  h.h[17] = R4(0, 3) - tyz * (3.0 / 7.0);
+  h.h[18] = (R4(2, 2) - (txx + tyy) * (1.0 / 7.0)) + ss * (1.0 / 35.0);
+  h.h[19] = R4(2, 1) - tyz * (1.0 / 7.0);
+  h.h[20] = R4(1, 2) - txz * (1.0 / 7.0);
+#undef R3
+#undef R4
+#pragma unroll
+  for (int k = 0; k < 6; k++) h.pad[k] = 0.0;
+  hn[i] = h;
+  HexOpen o;
+  o.lox = nd[(PT_CD_LO + 0) * nn + i];
+  o.loy = nd[(PT_CD_LO + 1) * nn + i];
+  o.loz = nd[(PT_CD_LO + 2) * nn + i];
+  o.hix = nd[(PT_CD_HI + 0) * nn + i];
+  o.hiy = nd[(PT_CD_HI + 1) * nn + i];
+  o.hiz = nd[(PT_CD_HI + 2) * nn + i];
+  o.soft = h.soft;
+  o.pad = 0.0;
+  ho[i] = o;
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -400,5 +605,27 @@ int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t*
                    hipStream_t s) {
   if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tnode, gn, gn32);
   if (nb > 0) pack_buckets_kernel<<<grid_for(nb), 256, 0, s>>>(t, nd, nn, bucket_node, nb, bk);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_hexa_leaf(const PtTree& t, const int32_t* bucket_node, int64_t nb, const double* x,
+                        const double* y, const double* z, const double* m, const double* nd,
+                        int64_t nn, double* raw, hipStream_t s) {
+  if (nb <= 0) return 0;
+  hexa_leaf_kernel<<<grid_for(nb), 256, 0, s>>>(t, bucket_node, nb, x, y, z, m, nd, nn, raw);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_hexa_level(const PtTree& t, const int32_t* level_nodes, int32_t nl,
+                         const double* nd, int64_t nn, double* raw, hipStream_t s) {
+  if (nl <= 0) return 0;
+  hexa_up_kernel<<<grid_for(nl), 256, 0, s>>>(t, level_nodes, nl, nd, nn, raw);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_hexa_pack(const PtTree& t, const double* nd, const double* raw, int64_t nn,
+                        HexNode* hn, HexOpen* ho, hipStream_t s) {
+  if (nn <= 0) return 0;
+  hexa_pack_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, raw, nn, hn, ho);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -18,10 +18,11 @@ namespace {
 constexpr int kTravWaves = 4;
 constexpr int kTravThreads = kTravWaves * PT_WAVE;
 
-template <bool FILL>
+template <bool FILL, bool HEXA>
 __global__ void __launch_bounds__(kTravThreads)
-traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ bk, int64_t nb,
-                double gf, double ox, double oy, double oz, int32_t* __restrict__ pp_cnt,
+traverse_kernel(const TreeNode* __restrict__ nodes, const HexOpen* __restrict__ ho,
+                const Bucket* __restrict__ bk, int64_t nb, double gf, double gm, double ox,
+                double oy, double oz, int32_t* __restrict__ pp_cnt,
                 int32_t* __restrict__ pc_cnt, const int64_t* __restrict__ pp_off,
                 const int64_t* __restrict__ pc_off, int32_t* __restrict__ pp,
                 int32_t* __restrict__ pc, unsigned long long* __restrict__ partials,
@@ -46,7 +47,7 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
   int32_t* pcw = FILL ? pc + pc_off[bi] : nullptr;
   if (live) {
     WalkCounts wc;
-    const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, st, sp, wc);
+    const WalkLane r = pt_walk_root<HEXA>(nodes, b, gf, ox, oy, oz, st, sp, wc, ho, gm);
     if (r.is_leaf) {
       if (FILL && lane == 0) ppw[0] = 0;
       npp = 1;
@@ -60,8 +61,8 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ b
     c_part += (unsigned long long)wc.leaf_parts * nt;
     while (sp > 0) {
       unsigned long long leafm, closem;
-      const WalkLane r2 = pt_walk_step<PT_WALK_STACK>(nodes, b, gf, ox, oy, oz, st, sp, overflow,
-                                                      wc, leafm, closem);
+      const WalkLane r2 = pt_walk_step<PT_WALK_STACK, HEXA>(nodes, b, gf, ox, oy, oz, st, sp,
+                                                            overflow, wc, leafm, closem, ho, gm);
       if (FILL) {
         if (r2.is_leaf) ppw[npp + pt_rank_in(leafm)] = r2.node;
         if (r2.is_closed) pcw[npc + pt_rank_in(closem)] = r2.node;
@@ -132,20 +133,24 @@ int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_traverse(bool fill, const TreeNode* nodes, const Bucket* bk, int64_t nb, double gf,
-                       const double off[3], int32_t* pp_cnt, int32_t* pc_cnt,
-                       const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc,
-                       unsigned long long* partials, int32_t* err, hipStream_t s) {
+// ho != nullptr selects the HEXADECAPOLE open() (gm = monopole_gravity_factor).
+int pt_launch_traverse(bool fill, const TreeNode* nodes, const HexOpen* ho, const Bucket* bk,
+                       int64_t nb, double gf, double gm, const double off[3], int32_t* pp_cnt,
+                       int32_t* pc_cnt, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                       int32_t* pc, unsigned long long* partials, int32_t* err, hipStream_t s) {
   if (nb <= 0) return 0;
   const unsigned grid = (unsigned)pt_traverse_blocks(nb);
-  if (fill) {
-    traverse_kernel<true><<<grid, kTravThreads, 0, s>>>(nodes, bk, nb, gf, off[0], off[1], off[2],
-                                                        pp_cnt, pc_cnt, pp_off, pc_off, pp, pc,
-                                                        partials, err);
+#define PT_TRAV(F, H)                                                                        \
+  traverse_kernel<F, H><<<grid, kTravThreads, 0, s>>>(nodes, ho, bk, nb, gf, gm, off[0],     \
+                                                      off[1], off[2], pp_cnt, pc_cnt, pp_off, \
+                                                      pc_off, pp, pc, partials, err)
+  if (ho) {
+    if (fill) PT_TRAV(true, true);
+    else PT_TRAV(false, true);
   } else {
-    traverse_kernel<false><<<grid, kTravThreads, 0, s>>>(nodes, bk, nb, gf, off[0], off[1],
-                                                         off[2], pp_cnt, pc_cnt, pp_off, pc_off,
-                                                         pp, pc, partials, err);
+    if (fill) PT_TRAV(true, false);
+    else PT_TRAV(false, false);
   }
+#undef PT_TRAV
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

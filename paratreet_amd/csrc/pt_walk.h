@@ -71,6 +71,58 @@ __device__ __forceinline__ bool pt_open_softening(double sx, double sy, double s
   return pt_intersect_box_point(b, sx, sy, sz, sr * sr);
 }
 
+// GravityVisitor::open, HEXADECAPOLE branch (examples/GravityVisitor.h:
+// 162-178): closed when the opening sphere misses the bucket box, unless the
+// softening spheres meet and the (larger) monopole sphere reaches the box.
+// The softening test is called as written there, openSoftening(target.data,
+// source.data): the bucket's centre (+ offset) and softening take the source
+// role, the node's sphere and box the target role.  gm is
+// monopole_gravity_factor.
+__device__ __forceinline__ bool pt_gv_open_hexa(const TreeNode& s, const HexOpen& h,
+                                                const Bucket& b, double gf, double gm, double ox,
+                                                double oy, double oz) {
+  if (s.count <= PT_MIN_PARTICLE_NODE) return true;
+  const double dataRsq = s.radius * s.radius * gf;
+  const double px = s.cx + ox, py = s.cy + oy, pz = s.cz + oz;
+  if (pt_intersect_box_point(b, px, py, pz, dataRsq)) return true;
+  const double bx = b.cx + ox, by = b.cy + oy, bz = b.cz + oz;
+  const double sr = 2.0 * b.soft;
+  bool soft_meet;
+  {
+    const double dx = bx - s.cx, dy = by - s.cy, dz = bz - s.cz;
+    const double dsq = dx * dx + dy * dy + dz * dz;
+    const double rs = sr + 2.0 * h.soft;
+    soft_meet = dsq <= rs * rs;
+  }
+  if (!soft_meet) {
+    Bucket nb;
+    nb.lox = h.lox;
+    nb.loy = h.loy;
+    nb.loz = h.loz;
+    nb.hix = h.hix;
+    nb.hiy = h.hiy;
+    nb.hiz = h.hiz;
+    soft_meet = pt_intersect_box_point(nb, bx, by, bz, sr * sr);
+  }
+  if (!soft_meet) return false;
+  const double monoRad = s.radius * gm;
+  return pt_intersect_box_point(b, px, py, pz, monoRad * monoRad);
+}
+
+// The opening test of the build: quadrupole (ho == nullptr) or HEXADECAPOLE.
+template <bool HEXA>
+__device__ __forceinline__ bool pt_open_any(const TreeNode& s, int32_t id,
+                                            const HexOpen* __restrict__ ho, const Bucket& b,
+                                            double gf, double gm, double ox, double oy,
+                                            double oz) {
+  if constexpr (HEXA) {
+    if (s.count <= PT_MIN_PARTICLE_NODE) return true;
+    return pt_gv_open_hexa(s, ho[id], b, gf, gm, ox, oy, oz);
+  } else {
+    return pt_gv_open(s, b, gf, ox, oy, oz);
+  }
+}
+
 // Result of one walk step for the calling lane.
 struct WalkLane {
   int32_t node;      // visited node id (valid if active)
@@ -89,11 +141,14 @@ struct WalkCounts {
 
 // Visits the root (node 0): pushes its first child when it is opened.  sp and
 // the counts are wave-uniform; lane 0 owns the stack write.
+template <bool HEXA = false>
 __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ nodes,
                                                  const Bucket& b, double gf, double ox,
                                                  double oy, double oz,
                                                  int32_t* __restrict__ stack, int& sp,
-                                                 WalkCounts& wc) {
+                                                 WalkCounts& wc,
+                                                 const HexOpen* __restrict__ ho = nullptr,
+                                                 double gm = 0.0) {
   WalkLane r{0, 0, 0, false, false};
   wc = WalkCounts{0, 0, 0};
   const TreeNode root = nodes[0];
@@ -103,7 +158,7 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
     r.is_leaf = true;
     wc.leaf_parts = root.count;
   } else if (root.type == PT_NODE_INTERNAL) {
-    if (pt_gv_open(root, b, gf, ox, oy, oz)) {
+    if (pt_open_any<HEXA>(root, 0, ho, b, gf, gm, ox, oy, oz)) {
       if (__lane_id() == 0) stack[0] = root.first_child;
       sp = 1;
       wc.n_open = 1;
@@ -119,14 +174,16 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
 // this lane's child and its classification; pushes opened children.  A push
 // beyond STACK entries sets overflow and ends the walk (the caller reports
 // it; nothing is silently dropped).
-template <int STACK>
+template <int STACK, bool HEXA = false>
 __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ nodes,
                                                  const Bucket& b, double gf, double ox,
                                                  double oy, double oz,
                                                  int32_t* __restrict__ stack, int& sp,
                                                  bool& overflow, WalkCounts& wc,
                                                  unsigned long long& leafm,
-                                                 unsigned long long& closem) {
+                                                 unsigned long long& closem,
+                                                 const HexOpen* __restrict__ ho = nullptr,
+                                                 double gm = 0.0) {
   const int lane = __lane_id();
   const int k = sp < 8 ? sp : 8;
   sp -= k;
@@ -144,7 +201,7 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
     if (nd.type == PT_NODE_LEAF) {
       r.is_leaf = true;
     } else if (nd.type == PT_NODE_INTERNAL) {
-      op = pt_gv_open(nd, b, gf, ox, oy, oz);
+      op = pt_open_any<HEXA>(nd, r.node, ho, b, gf, gm, ox, oy, oz);
       r.is_closed = !op;
     }
   }

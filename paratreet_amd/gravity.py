@@ -51,6 +51,9 @@ class Configuration:
     input_file: str = ""                # -f
     output_file: str = ""               # -v
     periodic: int = 0
+    # the reference's -DHEXADECAPOLE build (examples/Makefile:5): a build flag
+    # there, a context option here
+    hexadecapole: bool = False
     origin: dict = field(default_factory=dict)
 
     LETTERS = {"n": ("min_n_subtrees", int), "p": ("min_n_partitions", int),
@@ -112,6 +115,7 @@ class Configuration:
         c.dsoft = self.dSoft
         c.has_dsoft = 1 if "dSoft" in self.origin else 0
         c.periodic = self.periodic
+        c.hexadecapole = 1 if self.hexadecapole else 0
         return c
 
 
@@ -332,6 +336,14 @@ class Driver:
         nn, _, _ = self.tree_size()
         d = np.zeros((nn, 18))
         self._check(self.L.pt_get_node_data(self.ctx, _ptr(d)))
+        return d
+
+    def node_hexadecapole(self):
+        """(n_nodes, 22): mass and the reduced moments to fourth order (FMOMR
+        order) of a hexadecapole context."""
+        nn, _, _ = self.tree_size()
+        d = np.zeros((nn, 22))
+        self._check(self.L.pt_get_node_hexadecapole(self.ctx, _ptr(d)))
         return d
 
     def buckets(self):
