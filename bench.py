@@ -198,6 +198,45 @@ def adjoint_bench(name, fd_coords, direct):
     print(json.dumps(out))
 
 
+def hexadecapole_bench(args):
+    """The HEXADECAPOLE build (moments to fourth order, FP64 node() over the
+    traversal lists) on the same workload: one JSON line of its own."""
+    import torch
+    from paratreet_amd.gravity import Configuration, Driver, PT_FP64
+    pos, vel, m, eps, bucket, theta = make_input(args.config)
+    if args.theta is not None:
+        theta = args.theta
+    n = len(pos)
+    d = Driver(Configuration(max_particles_per_leaf=bucket, hexadecapole=True))
+    d.load(pos, m, eps, vel)
+    ms, cnt, f_ms, st = time_path(d, theta, PT_FP64, args.steps, args.warmup)
+    acc, _ = d.accelerations()
+    ints = float(cnt[0] + cnt[1])
+    err = {}
+    if not args.no_error:
+        rng = np.random.default_rng(99)
+        tgt = np.sort(rng.choice(n, 4096, replace=False))
+        ref = d.direct_sum(tgt)
+        e = np.linalg.norm(acc[tgt] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        err = {"vs_direct_rms": float(np.sqrt(np.mean(e * e))),
+               "vs_direct_p99": float(np.percentile(e, 99)), "vs_direct_max": float(e.max())}
+    d.close()
+    out = {
+        "metric": "interactions/sec, HEXADECAPOLE build (FP64), %s" % args.config,
+        "value": ints / (ms / 1000.0), "unit": "interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": "%s: n=%d, bucket=%d, theta=%.2f, eps=%g, moments to fourth "
+                               "order, full force evaluation per step (lists + hexadecapole "
+                               "node())" % (args.config, n, bucket, theta, eps[0]),
+                   "n": n, "bucket": bucket, "theta": theta, "parallelism": "single GPU"},
+        "interactions_per_step": ints, "n_part_ints": int(cnt[1]), "n_node_ints": int(cnt[0]),
+        "stage_ms": st, "errors": err,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +252,8 @@ def main():
     ap.add_argument("--fd-coords", type=int, default=64)
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the direct-sum gradient leg of --adjoint")
+    ap.add_argument("--hexadecapole", action="store_true",
+                    help="time the HEXADECAPOLE build (FP64) on the workload instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -226,6 +267,9 @@ def main():
 
     if args.adjoint:
         adjoint_bench(args.adjoint, args.fd_coords, not args.no_direct)
+        return
+    if args.hexadecapole:
+        hexadecapole_bench(args)
         return
 
     pos, vel, m, eps, bucket, theta = make_input(args.config)

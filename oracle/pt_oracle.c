@@ -5,9 +5,10 @@
  * baseline.  Nothing in the product (libpt_gravity.so, paratreet_amd) links or
  * calls this file.
  *
- * Reference configuration restated: Real = double (-DUSE_DOUBLE_FP), no
- * -DHEXADECAPOLE, Morton keys (examples/Main.C:66-67 '-m'), TreeType::eOct with
- * one subtree (tp_key = 1), GravityVisitor with gconst = 1.
+ * Reference configuration restated: Real = double (-DUSE_DOUBLE_FP), Morton
+ * keys (examples/Main.C:66-67 '-m'), TreeType::eOct with one subtree
+ * (tp_key = 1), GravityVisitor with gconst = 1; the quadrupole build and,
+ * in the or_hexa_* and *_hexa functions, the -DHEXADECAPOLE build.
  *
  * How each function is pinned (tests/test_oracle_*.py):
  *   or_spline / or_direct   -- tests/gravity/direct.acc of the reference
@@ -27,6 +28,13 @@
  *                              small sets (positions and masses, every
  *                              softening branch), and the symmetry identity
  *                              sum_i g_i = 0 for translation invariance.
+ *   or_hexa_*, *_hexa       -- reduced moments against tensor algebra on
+ *                              the particles of every node, the potential
+ *                              against the Legendre series, acceleration
+ *                              against its gradient, fifth-order truncation,
+ *                              the hexadecapole open() by brute force, and
+ *                              the reference's stated accuracy on lambs 30K
+ *                              (tests/gravity/acc_test.sh:5).
  *   or_gravity / or_lists   -- interaction sets against a brute-force
  *                              enumeration of the opening predicate; forces
  *                              against or_direct (tree error ~1e-3 at
