@@ -63,6 +63,7 @@ struct pt_ctx {
   DBuf<uint64_t> nkey;
   DBuf<int32_t> nbegin, ncount, ndepth, ntype, nparent, nfirst;
   DBuf<int32_t> level_nodes, level_cnt;
+  DBuf<int64_t> tstat;  // build read-back: {nn, nb, overflow, level counts}
   std::vector<int64_t> level_off;  // internal nodes of depth d: [level_off[d], level_off[d+1])
   DBuf<uint32_t> flag, fpos, head, head_pos, u32_total;
   DBuf<int32_t> bucket_node;
@@ -308,6 +309,8 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->nparent); release(c->nfirst); release(c->level_nodes);
   release(c->level_cnt);
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
+  release(c->tstat);
+  release(c->hraw); release(c->hn); release(c->ho);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
   release(c->wqueue);
@@ -462,32 +465,32 @@ pt_status pt_build_tree(pt_ctx* c) {
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_root(c->n, B, tr, c->level_cnt.p, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->level_nodes.p, 1, 0u, c->stream));
-    int32_t cnt[PT_MAX_DEPTH + 2] = {0};
-    int32_t herr = 0;
+    // every level is launched (a level past the tree's depth is empty and its
+    // threads exit at once); the buckets follow on the device and one
+    // read-back returns counts, sizes and the overflow flag
     int64_t bound = 1;  // upper bound of the level's internal-node count
-    int depth = 0;
-    for (; depth <= PT_MAX_DEPTH - 1; depth++) {
+    for (int depth = 0; depth <= PT_MAX_DEPTH - 1; depth++) {
       PT_TRY_LAUNCH(c, pt_launch_split_level(c->skeys.p, c->level_nodes.p, c->level_cnt.p, depth,
                                              bound, B, tr, node_cap, level_cap, c->derr.p,
                                              c->stream));
       bound = std::min<int64_t>(bound * 8, per_level);
-      if (depth % 4 == 3) {
-        PT_TRY_HIP(c, hipMemcpyAsync(cnt, c->level_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost,
-                                     c->stream));
-        PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-        if (cnt[depth + 1] == 0) break;
-      }
     }
-    PT_TRY_HIP(c, hipMemcpyAsync(cnt, c->level_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost,
+    if ((st = ensure_scratch(c, pt_scan_scratch_bytes(c->n)))) return st;
+    PT_TRY_HIP(c, ensure(c->tstat, 3 + PT_MAX_DEPTH + 2));
+    PT_TRY_LAUNCH(c, pt_launch_buckets(tr, node_cap, c->n, c->level_cnt.p, c->derr.p, c->head.p,
+                                       c->head_pos.p, c->scratch.p, c->u32_total.p,
+                                       c->bucket_node.p, c->tstat.p, c->stream));
+    int64_t stat[3 + PT_MAX_DEPTH + 2] = {0};
+    PT_TRY_HIP(c, hipMemcpyAsync(stat, c->tstat.p, sizeof(stat), hipMemcpyDeviceToHost,
                                  c->stream));
-    PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
     PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    if (herr) {
+    if (stat[2] || stat[0] == 0) {
       if (attempt >= 3) return fail(c, PT_ERR_CAPACITY, "octree node table overflow");
       level_cap *= 4;
       continue;
     }
     // level table: internal nodes of depth d are [level_off[d], level_off[d+1])
+    const int64_t* cnt = stat + 3;
     c->level_off.assign(1, 0);
     c->max_depth = 0;
     int64_t internal = 0;
@@ -496,19 +499,10 @@ pt_status pt_build_tree(pt_ctx* c) {
       c->level_off.push_back(internal);
       if (cnt[d] > 0) c->max_depth = d + 1;
     }
-    c->nn = 1 + 8 * internal;
+    c->nn = stat[0];
+    c->nb = stat[1];
     break;
   }
-  // buckets: non-empty leaves in particle (= DFS) order
-  if ((st = ensure_scratch(c, pt_scan_scratch_bytes(c->n)))) return st;
-  PtTree tr = tree_of(c);
-  PT_TRY_LAUNCH(c, pt_launch_buckets(tr, c->nn, c->n, c->head.p, c->head_pos.p, c->scratch.p,
-                                     c->u32_total.p, c->bucket_node.p, c->stream));
-  uint32_t nb = 0;
-  PT_TRY_HIP(c, hipMemcpyAsync(&nb, c->u32_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               c->stream));
-  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-  c->nb = nb;
   c->stage = S_TREE;
   return PT_OK;
 }
@@ -527,18 +521,30 @@ pt_status pt_upward(pt_ctx* c) {
   PT_TRY_HIP(c, ensure(c->gn32, nn));
   PT_TRY_HIP(c, ensure(c->bk, std::max<int64_t>(c->nb, 1)));
   PtTree tr = tree_of(c);
-  PT_TRY_LAUNCH(c, pt_launch_moments_init(c->nd.p, nn, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_leaf_moments(tr, c->bucket_node.p, c->nb, c->xs.p, c->ys.p, c->zs.p,
-                                          c->ms.p, c->es.p, c->nd.p, nn, c->stream));
-  // internal nodes, deepest level first
+  PT_TRY_LAUNCH(c, pt_launch_leaf_moments(tr, nn, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p,
+                                          c->nd.p, c->stream));
+  // internal nodes, deepest level first: a level of more than kRunNodes
+  // nodes gets its own launch, consecutive smaller ones share one work-group
+  constexpr int64_t kRunNodes = 256;
   const int nlev = (int)c->level_off.size() - 1;
+  PtLevelRun run{};
   for (int d = nlev - 1; d >= 0; d--) {
     const int64_t a = c->level_off[d], b = c->level_off[d + 1];
-    if (b > a) {
-      PT_TRY_LAUNCH(c, pt_launch_upward_level(tr, c->level_nodes.p + a, (int32_t)(b - a), c->nd.p, nn,
-                                              c->ord_h.p, c->ord_b.p, c->stream));
+    if (b <= a) continue;
+    if (b - a <= kRunNodes && run.nlev < PT_MAX_RUN) {
+      run.off[run.nlev] = (int32_t)a;
+      run.cnt[run.nlev] = (int32_t)(b - a);
+      run.nlev++;
+      continue;
     }
+    PT_TRY_LAUNCH(c, pt_launch_upward_run(tr, c->level_nodes.p, run, c->nd.p, nn, c->ord_h.p,
+                                          c->ord_b.p, c->stream));
+    run.nlev = 0;
+    PT_TRY_LAUNCH(c, pt_launch_upward_level(tr, c->level_nodes.p + a, (int32_t)(b - a), c->nd.p,
+                                            nn, c->ord_h.p, c->ord_b.p, c->stream));
   }
+  PT_TRY_LAUNCH(c, pt_launch_upward_run(tr, c->level_nodes.p, run, c->nd.p, nn, c->ord_h.p,
+                                        c->ord_b.p, c->stream));
   PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tnode.p, c->gn.p,
                                   c->gn32.p, c->bk.p, c->stream));
   if (c->cfg.hexadecapole) {

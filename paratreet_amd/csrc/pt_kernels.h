@@ -44,10 +44,11 @@ int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, i
                     int bucket, int child_depth, const PtTree& t, uint32_t* flag, hipStream_t s);
 int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
                       int32_t* next, hipStream_t s);
-int pt_launch_buckets(const PtTree& t, int64_t nn, int64_t n, uint32_t* head, uint32_t* head_pos,
-                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node, hipStream_t s);
+int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, const int32_t* level_cnt,
+                      const int32_t* overflow, uint32_t* head, uint32_t* head_pos,
+                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node,
+                      int64_t* status, hipStream_t s);
 
-int pt_launch_moments_init(double* nd, int64_t nn, hipStream_t s);
 // HEXADECAPOLE moments: raw third/fourth moments (SoA, PT_HEXA_RAW x nn) of
 // the buckets, then of each internal level (deepest first), then the records.
 int pt_launch_hexa_leaf(const PtTree& t, const int32_t* bucket_node, int64_t nb, const double* x,
@@ -57,9 +58,20 @@ int pt_launch_hexa_level(const PtTree& t, const int32_t* level_nodes, int32_t nl
                          const double* nd, int64_t nn, double* raw, hipStream_t s);
 int pt_launch_hexa_pack(const PtTree& t, const double* nd, const double* raw, int64_t nn,
                         HexNode* hn, HexOpen* ho, hipStream_t s);
-int pt_launch_leaf_moments(const PtTree& t, const int32_t* bucket_node, int64_t nb,
-                           const double* x, const double* y, const double* z, const double* m,
-                           const double* e, double* nd, int64_t nn, hipStream_t s);
+// Leaf CentroidData of every leaf and the default record of every empty leaf.
+int pt_launch_leaf_moments(const PtTree& t, int64_t nn, const double* x, const double* y,
+                           const double* z, const double* m, const double* e, double* nd,
+                           hipStream_t s);
+// Consecutive small levels of the upward pass (deepest first) in one
+// work-group: off/cnt of each level within level_nodes.
+#define PT_MAX_RUN 24
+struct PtLevelRun {
+  int32_t nlev;
+  int32_t off[PT_MAX_RUN];
+  int32_t cnt[PT_MAX_RUN];
+};
+int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLevelRun& run,
+                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
                            int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
 int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,

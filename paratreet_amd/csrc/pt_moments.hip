@@ -170,43 +170,37 @@ __device__ __forceinline__ void cd_merge(CData& d, const CData& c) {
   }
 }
 
+// One thread per node: a leaf runs the sequential particle recurrence of the
+// CentroidData leaf constructor in sorted particle order; an empty leaf stores
+// the default CentroidData; internal nodes are left to the upward pass.
 __global__ void __launch_bounds__(256)
-init_kernel(double* __restrict__ nd, int64_t nn) {
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i >= nn) return;
+leaf_kernel(const PtTree t, int64_t nn, const double* __restrict__ x,
+            const double* __restrict__ y, const double* __restrict__ z,
+            const double* __restrict__ m, const double* __restrict__ e, double* __restrict__ nd) {
+  const int64_t id = blockIdx.x * 256ll + threadIdx.x;
+  if (id >= nn) return;
+  const int ty = t.type[id];
+  if (ty == PT_NODE_INTERNAL) return;
   CData d;
   cd_default(d);
-  cd_store(d, nd, nn, i);
-}
-
-// One thread per bucket: the sequential particle recurrence of the
-// CentroidData leaf constructor, in sorted particle order.
-__global__ void __launch_bounds__(256)
-leaf_kernel(const PtTree t, const int32_t* __restrict__ bucket_node, int64_t nb,
-            const double* __restrict__ x, const double* __restrict__ y,
-            const double* __restrict__ z, const double* __restrict__ m,
-            const double* __restrict__ e, double* __restrict__ nd, int64_t nn) {
-  const int64_t bi = blockIdx.x * 256ll + threadIdx.x;
-  if (bi >= nb) return;
-  const int32_t id = bucket_node[bi];
-  const int64_t b = t.begin[id];
-  const int64_t n = t.count[id];
-  CData d;
-  cd_default(d);
-  for (int64_t i = b; i < b + n; i++) {
-    const double px = x[i], py = y[i], pz = z[i];
-    mom_add_particle(d, px, py, pz, m[i], e[i]);
-    const double p[3] = {px, py, pz};
-    for (int k = 0; k < 3; k++) {
-      if (p[k] < d.lo[k]) d.lo[k] = p[k];
-      if (p[k] > d.hi[k]) d.hi[k] = p[k];
+  if (ty == PT_NODE_LEAF) {
+    const int64_t b = t.begin[id];
+    const int64_t n = t.count[id];
+    for (int64_t i = b; i < b + n; i++) {
+      const double px = x[i], py = y[i], pz = z[i];
+      mom_add_particle(d, px, py, pz, m[i], e[i]);
+      const double p[3] = {px, py, pz};
+      for (int k = 0; k < 3; k++) {
+        if (p[k] < d.lo[k]) d.lo[k] = p[k];
+        if (p[k] > d.hi[k]) d.hi[k] = p[k];
+      }
     }
-  }
-  d.count = n;
-  if (d.count > 1) {
-    d.radius = 0.5 * len3(d.hi[0] - d.lo[0], d.hi[1] - d.lo[1], d.hi[2] - d.lo[2]);
-  } else {
-    d.radius = 1.0;
+    d.count = n;
+    if (d.count > 1) {
+      d.radius = 0.5 * len3(d.hi[0] - d.lo[0], d.hi[1] - d.lo[1], d.hi[2] - d.lo[2]);
+    } else {
+      d.radius = 1.0;
+    }
   }
   cd_store(d, nd, nn, id);
 }
@@ -216,16 +210,12 @@ __device__ __forceinline__ uint64_t left_align(uint64_t k) {
   return k << (PT_KEY_BITS - (3 * depth + 1));
 }
 
-// One thread per internal node of one level (deepest level first).  The
-// children's pop order is built in registers (fully unrolled loops over the
-// 8 children, the order packed as 8 nibbles) so nothing spills to scratch.
-__global__ void __launch_bounds__(256)
-upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
-              double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
-              uint64_t* __restrict__ ord_b) {
-  const int64_t li = blockIdx.x * 256ll + threadIdx.x;
-  if (li >= nl) return;
-  const int32_t p = level_nodes[li];
+// One internal node (its children are final).  The children's pop order is
+// built in registers (fully unrolled loops over the 8 children, the order
+// packed as 8 nibbles) so nothing spills to scratch.
+__device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* __restrict__ nd,
+                                            int64_t nn, int32_t* __restrict__ ord_h,
+                                            uint64_t* __restrict__ ord_b) {
   const int32_t fc = t.first_child[p];
   int ctype[8];
   int32_t h[8];
@@ -287,16 +277,43 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
     ord_h[p] = hmax + 1;
     ord_b[p] = bmax;
   }
+  // the parent starts from the default CentroidData; an empty child is the
+  // default too and is not read
   CData d;
-  cd_load(d, nd, nn, p, 0);
-  d.count = 0;
+  cd_default(d);
   for (int j = 0; j < 8; j++) {
     const int c = (order >> (4 * j)) & 15;
     CData cd;
-    cd_load(cd, nd, nn, fc + c, t.count[fc + c]);
+    if (ctype[c] == PT_NODE_EMPTY) cd_default(cd);
+    else cd_load(cd, nd, nn, fc + c, t.count[fc + c]);
     cd_merge(d, cd);
   }
   cd_store(d, nd, nn, p);
+}
+
+// One thread per internal node of one level (deepest level first).
+__global__ void __launch_bounds__(256)
+upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
+              double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
+              uint64_t* __restrict__ ord_b) {
+  const int64_t li = blockIdx.x * 256ll + threadIdx.x;
+  if (li >= nl) return;
+  upward_node(t, level_nodes[li], nd, nn, ord_h, ord_b);
+}
+
+// A run of consecutive small levels in one work-group, deepest first, with a
+// work-group barrier between levels (a level reads what the one below wrote;
+// all waves of the group share the CU's caches).
+__global__ void __launch_bounds__(256)
+upward_run_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, PtLevelRun run,
+                  double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
+                  uint64_t* __restrict__ ord_b) {
+  for (int k = 0; k < run.nlev; k++) {
+    const int32_t off = run.off[k], cnt = run.cnt[k];
+    for (int32_t li = threadIdx.x; li < cnt; li += 256)
+      upward_node(t, level_nodes[off + li], nd, nn, ord_h, ord_b);
+    __syncthreads();
+  }
 }
 
 // Packs the walk and interaction records from the SoA node table.
@@ -579,17 +596,18 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-int pt_launch_moments_init(double* nd, int64_t nn, hipStream_t s) {
+int pt_launch_leaf_moments(const PtTree& t, int64_t nn, const double* x, const double* y,
+                           const double* z, const double* m, const double* e, double* nd,
+                           hipStream_t s) {
   if (nn <= 0) return 0;
-  init_kernel<<<grid_for(nn), 256, 0, s>>>(nd, nn);
+  leaf_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, x, y, z, m, e, nd);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_leaf_moments(const PtTree& t, const int32_t* bucket_node, int64_t nb,
-                           const double* x, const double* y, const double* z, const double* m,
-                           const double* e, double* nd, int64_t nn, hipStream_t s) {
-  if (nb <= 0) return 0;
-  leaf_kernel<<<grid_for(nb), 256, 0, s>>>(t, bucket_node, nb, x, y, z, m, e, nd, nn);
+int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLevelRun& run,
+                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s) {
+  if (run.nlev <= 0) return 0;
+  upward_run_kernel<<<1, 256, 0, s>>>(t, level_nodes, run, nd, nn, ord_h, ord_b);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -102,13 +102,15 @@ split_level_kernel(const uint64_t* __restrict__ keys, int32_t* __restrict__ leve
   if (gid >= nf * 8) return;
   const int64_t f = gid >> 3;
   const int c = (int)(gid & 7);
-  const int32_t p = level_nodes[f0 + f];
   const int64_t base = 1 + 8 * f0;
   const int64_t id = base + gid;
-  if (id >= node_cap) {
+  // after an overflow at a shallower level the counts run past what was
+  // recorded: nothing beyond the capacities is read or written
+  if (id >= node_cap || f0 + f >= level_cap) {
     atomicExch(overflow, 1);
     return;
   }
+  const int32_t p = level_nodes[f0 + f];
   const int64_t b = t.begin[p];
   const int64_t e = b + t.count[p];
   const uint64_t child_key = (t.key[p] << 3) | (uint64_t)c;
@@ -140,17 +142,42 @@ compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ p
   if (i < m && flag[i]) next[pos[i]] = base + (int32_t)i;
 }
 
+// Node count of the finished levels (one thread): status[0] = 1 + 8 * (all
+// internal nodes), or 0 when the build overflowed (nothing downstream runs).
+__global__ void count_nodes_kernel(const int32_t* __restrict__ level_cnt,
+                                   const int32_t* __restrict__ overflow, int64_t node_cap,
+                                   int64_t* __restrict__ status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t internal = 0;
+  for (int l = 0; l <= PT_MAX_DEPTH + 1; l++) internal += level_cnt[l];
+  const int64_t nn = 1 + 8 * internal;
+  status[0] = (*overflow || nn > node_cap) ? 0 : nn;
+}
+
+// Collects what the host reads back after the build in one place:
+// {nn, nb, overflow, level counts}.
+__global__ void tree_status_kernel(const int32_t* __restrict__ level_cnt,
+                                   const int32_t* __restrict__ overflow,
+                                   const uint32_t* __restrict__ nb, int64_t* __restrict__ status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  status[1] = status[0] > 0 ? (int64_t)*nb : 0;
+  status[2] = *overflow;
+  for (int l = 0; l <= PT_MAX_DEPTH + 1; l++) status[3 + l] = level_cnt[l];
+}
+
+// Grids cover the node capacity; the live count is status[0].
 __global__ void __launch_bounds__(256)
-leaf_heads_kernel(const PtTree t, int64_t nn, uint32_t* __restrict__ head) {
+leaf_heads_kernel(const PtTree t, const int64_t* __restrict__ status,
+                  uint32_t* __restrict__ head) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i < nn && t.type[i] == PT_NODE_LEAF) head[t.begin[i]] = 1u;
+  if (i < status[0] && t.type[i] == PT_NODE_LEAF) head[t.begin[i]] = 1u;
 }
 
 __global__ void __launch_bounds__(256)
-bucket_ids_kernel(const PtTree t, int64_t nn, const uint32_t* __restrict__ head_pos,
-                  int32_t* __restrict__ bucket_node) {
+bucket_ids_kernel(const PtTree t, const int64_t* __restrict__ status,
+                  const uint32_t* __restrict__ head_pos, int32_t* __restrict__ bucket_node) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i < nn && t.type[i] == PT_NODE_LEAF) bucket_node[head_pos[t.begin[i]]] = (int32_t)i;
+  if (i < status[0] && t.type[i] == PT_NODE_LEAF) bucket_node[head_pos[t.begin[i]]] = (int32_t)i;
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -188,13 +215,22 @@ int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int3
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_buckets(const PtTree& t, int64_t nn, int64_t n, uint32_t* head, uint32_t* head_pos,
+// Buckets of a device-built tree without a host round trip: the node count
+// comes from the level counts on the device (status[0]); afterwards status =
+// {nn, nb, overflow, level counts [PT_MAX_DEPTH + 2]} for one read-back.
+int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, const int32_t* level_cnt,
+                      const int32_t* overflow, uint32_t* head, uint32_t* head_pos,
                       void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node,
-                      hipStream_t s) {
-  if (n <= 0) return 0;
-  if (pt_launch_fill_u32(head, n, 0u, s) != 0) return -1;
-  leaf_heads_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, head);
-  if (pt_exclusive_scan_u32(head, head_pos, n, scan_scratch, nb_out, s) != 0) return -1;
-  bucket_ids_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, head_pos, bucket_node);
+                      int64_t* status, hipStream_t s) {
+  count_nodes_kernel<<<1, 64, 0, s>>>(level_cnt, overflow, node_cap, status);
+  if (n > 0) {
+    if (pt_launch_fill_u32(head, n, 0u, s) != 0) return -1;
+    leaf_heads_kernel<<<grid_for(node_cap), 256, 0, s>>>(t, status, head);
+    if (pt_exclusive_scan_u32(head, head_pos, n, scan_scratch, nb_out, s) != 0) return -1;
+    bucket_ids_kernel<<<grid_for(node_cap), 256, 0, s>>>(t, status, head_pos, bucket_node);
+  } else {
+    if (pt_launch_fill_u32(nb_out, 1, 0u, s) != 0) return -1;
+  }
+  tree_status_kernel<<<1, 64, 0, s>>>(level_cnt, overflow, nb_out, status);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
