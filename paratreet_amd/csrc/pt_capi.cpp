@@ -53,6 +53,7 @@ struct pt_ctx {
   // keys and sorted copies
   DBuf<uint64_t> keys, skeys;
   DBuf<uint32_t> perm;
+  DBuf<ParticleRec> prec;  // x y z m e per particle, original order (sort gather)
   DBuf<double> xs, ys, zs, ms, es, axs, ays, azs, pots;
   // universe
   DBuf<double> box, bbox, box_parts;
@@ -310,6 +311,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->level_cnt);
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->tstat);
+  release(c->prec);
   release(c->hraw); release(c->hn); release(c->ho);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
@@ -421,8 +423,9 @@ pt_status pt_assign_keys(pt_ctx* c) {
   if (st) return st;
   if ((st = need(c, S_UNIVERSE, "pt_assign_keys"))) return st;
   Timer t(c, PT_T_KEYS);
-  PT_TRY_LAUNCH(c, pt_launch_keys(c->x.p, c->y.p, c->z.p, c->n, c->box.p, c->keys.p, c->perm.p,
-                                  c->stream));
+  PT_TRY_HIP(c, ensure(c->prec, std::max<int64_t>(c->n, 1)));
+  PT_TRY_LAUNCH(c, pt_launch_keys(c->x.p, c->y.p, c->z.p, c->m.p, c->e.p, c->n, c->box.p,
+                                  c->keys.p, c->prec.p, c->stream));
   c->stage = S_KEYS;
   return PT_OK;
 }
@@ -433,12 +436,10 @@ pt_status pt_sort(pt_ctx* c) {
   if ((st = need(c, S_KEYS, "pt_sort"))) return st;
   if ((st = ensure_scratch(c, pt_sort_scratch_bytes(c->n)))) return st;
   Timer t(c, PT_T_SORT);
-  PT_TRY_HIP(c, hipMemcpyAsync(c->skeys.p, c->keys.p, (size_t)c->n * 8, hipMemcpyDeviceToDevice,
-                               c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_iota(c->perm.p, c->n, c->stream));
-  PT_TRY_LAUNCH(c, pt_radix_sort(c->skeys.p, c->perm.p, c->n, 0, 64, c->scratch.p, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->x.p, c->y.p, c->z.p, c->m.p, c->e.p,
-                                    c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_radix_sort(c->keys.p, c->skeys.p, c->perm.p, c->n, 0, 64, c->scratch.p,
+                                 c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->prec.p, c->xs.p, c->ys.p, c->zs.p,
+                                    c->ms.p, c->es.p, c->stream));
   c->stage = S_SORTED;
   return PT_OK;
 }

@@ -65,6 +65,12 @@ struct __align__(64) HexOpen {
   double lox, loy, loz, hix, hiy, hiz, soft, pad;
 };
 
+// One particle's inputs in one 64-byte line (original order), so the sort's
+// permutation gathers one line per particle instead of five scattered words.
+struct __align__(64) ParticleRec {
+  double x, y, z, m, e, pad[3];
+};
+
 // Bucket (target leaf) record: particle range, box, centre of mass, softening.
 struct __align__(32) Bucket {
   double lox, loy, loz, hix;

@@ -28,12 +28,15 @@ int pt_exclusive_scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, voi
 int pt_launch_universe(const double* x, const double* y, const double* z, int64_t n,
                        double* scratch_parts, int max_parts, double* bbox_out, double* box,
                        hipStream_t s);
-int pt_launch_keys(const double* x, const double* y, const double* z, int64_t n,
-                   const double* box, uint64_t* keys, uint32_t* idx, hipStream_t s);
+// Morton keys and the 64-byte particle records the sort's gather reads.
+int pt_launch_keys(const double* x, const double* y, const double* z, const double* m,
+                   const double* e, int64_t n, const double* box, uint64_t* keys,
+                   ParticleRec* rec, hipStream_t s);
 
 int64_t pt_sort_scratch_bytes(int64_t n);
-int pt_radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int end_bit,
-                  void* scratch, hipStream_t s);
+// Stable LSD sort of src (read only) into keys, with vals = input index.
+int pt_radix_sort(const uint64_t* src, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                  int end_bit, void* scratch, hipStream_t s);
 int pt_launch_iota(uint32_t* v, int64_t n, hipStream_t s);
 
 int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s);
@@ -113,9 +116,8 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
                       double* pot, unsigned long long* partials, int32_t* err,
                       unsigned long long* queue, hipStream_t s);
 
-int pt_launch_gather(const uint32_t* perm, int64_t n, const double* x, const double* y,
-                     const double* z, const double* m, const double* e, double* xs, double* ys,
-                     double* zs, double* ms, double* es, hipStream_t s);
+int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
+                     double* ys, double* zs, double* ms, double* es, hipStream_t s);
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,
                      const double* a2, const double* a3, double* o0, double* o1, double* o2,
                      double* o3, hipStream_t s);

@@ -123,16 +123,25 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v) {
 
 __global__ void __launch_bounds__(256)
 keys_kernel(const double* __restrict__ x, const double* __restrict__ y,
-            const double* __restrict__ z, int64_t n, const double* __restrict__ box,
-            uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+            const double* __restrict__ z, const double* __restrict__ m,
+            const double* __restrict__ e, int64_t n, const double* __restrict__ box,
+            uint64_t* __restrict__ keys, ParticleRec* __restrict__ rec) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= n) return;
-  const uint64_t qx = quantise(x[i], box[0], box[3]);
-  const uint64_t qy = quantise(y[i], box[1], box[4]);
-  const uint64_t qz = quantise(z[i], box[2], box[5]);
+  const double px = x[i], py = y[i], pz = z[i];
+  const uint64_t qx = quantise(px, box[0], box[3]);
+  const uint64_t qy = quantise(py, box[1], box[4]);
+  const uint64_t qz = quantise(pz, box[2], box[5]);
   const uint64_t k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
   keys[i] = k | (1ull << (PT_KEY_BITS - 1));
-  if (idx) idx[i] = (uint32_t)i;
+  ParticleRec r;
+  r.x = px;
+  r.y = py;
+  r.z = pz;
+  r.m = m[i];
+  r.e = e[i];
+  r.pad[0] = r.pad[1] = r.pad[2] = 0.0;
+  rec[i] = r;
 }
 
 }  // namespace
@@ -148,10 +157,11 @@ int pt_launch_universe(const double* x, const double* y, const double* z, int64_
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_keys(const double* x, const double* y, const double* z, int64_t n,
-                   const double* box, uint64_t* keys, uint32_t* idx, hipStream_t s) {
+int pt_launch_keys(const double* x, const double* y, const double* z, const double* m,
+                   const double* e, int64_t n, const double* box, uint64_t* keys,
+                   ParticleRec* rec, hipStream_t s) {
   if (n <= 0) return 0;
   const int64_t nb = (n + 255) / 256;
-  keys_kernel<<<(unsigned)nb, 256, 0, s>>>(x, y, z, n, box, keys, idx);
+  keys_kernel<<<(unsigned)nb, 256, 0, s>>>(x, y, z, m, e, n, box, keys, rec);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -9,19 +9,17 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 // Sorted copies of the particle arrays (the reference sorts whole Particle
 // structs, src/Subtree.h:281; here only the fields the path reads move).
 __global__ void __launch_bounds__(256)
-gather_kernel(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ x,
-              const double* __restrict__ y, const double* __restrict__ z,
-              const double* __restrict__ m, const double* __restrict__ e,
+gather_kernel(const uint32_t* __restrict__ perm, int64_t n, const ParticleRec* __restrict__ rec,
               double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ zs,
               double* __restrict__ ms, double* __restrict__ es) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= n) return;
-  const uint32_t j = perm[i];
-  xs[i] = x[j];
-  ys[i] = y[j];
-  zs[i] = z[j];
-  ms[i] = m[j];
-  es[i] = e[j];
+  const ParticleRec r = rec[perm[i]];
+  xs[i] = r.x;
+  ys[i] = r.y;
+  zs[i] = r.z;
+  ms[i] = r.m;
+  es[i] = r.e;
 }
 
 __global__ void __launch_bounds__(256)
@@ -132,11 +130,10 @@ fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long lon
 
 }  // namespace
 
-int pt_launch_gather(const uint32_t* perm, int64_t n, const double* x, const double* y,
-                     const double* z, const double* m, const double* e, double* xs, double* ys,
-                     double* zs, double* ms, double* es, hipStream_t s) {
+int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
+                     double* ys, double* zs, double* ms, double* es, hipStream_t s) {
   if (n <= 0) return 0;
-  gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, x, y, z, m, e, xs, ys, zs, ms, es);
+  gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, rec, xs, ys, zs, ms, es);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

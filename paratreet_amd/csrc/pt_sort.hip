@@ -42,6 +42,8 @@ count_kernel(const uint64_t* __restrict__ keys, int64_t n, int shift, int ntiles
   counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// FIRST: the payload is the key's input index (vals_in unused).
+template <bool FIRST>
 __global__ void __launch_bounds__(kSortThreads)
 scatter_kernel(const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                int64_t n, int shift, int ntiles, const uint32_t* __restrict__ offsets,
@@ -69,7 +71,7 @@ scatter_kernel(const uint64_t* __restrict__ keys_in, const uint32_t* __restrict_
     const int64_t i = wave_base + (int64_t)r * PT_WAVE + lane;
     const bool valid = i < n;
     k[r] = valid ? keys_in[i] : ~0ull;
-    v[r] = valid ? vals_in[i] : 0u;
+    v[r] = valid ? (FIRST ? (uint32_t)i : vals_in[i]) : 0u;
     const uint32_t d = (uint32_t)(k[r] >> shift) & (kRadix - 1);
     // lanes holding the same digit (invalid lanes never match a valid one:
     // they are excluded from the initial mask)
@@ -151,11 +153,18 @@ int64_t pt_sort_scratch_bytes(int64_t n) {
   return table * 4 + pt_scan_scratch_bytes(table) + n * 8 + n * 4 + 1024;
 }
 
-// Sorts (keys, vals) in place over n items using scratch of
-// pt_sort_scratch_bytes(n) bytes.  begin_bit/end_bit bound the digit passes.
-int pt_radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int end_bit,
-                  void* scratch, hipStream_t s) {
-  if (n <= 1) return 0;
+// Sorts src (n keys, read only) into (keys, vals): vals = input index of
+// each sorted key.  Scratch: pt_sort_scratch_bytes(n) bytes; begin_bit /
+// end_bit bound the digit passes.
+int pt_radix_sort(const uint64_t* src, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                  int end_bit, void* scratch, hipStream_t s) {
+  if (n <= 0) return 0;
+  if (n == 1 || end_bit <= begin_bit) {
+    if (n > 1) return -1;
+    if (hipMemcpyAsync(keys, src, 8, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+    iota_kernel<<<1, 256, 0, s>>>(vals, 1);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
   if (n > 0x7fffffffll) return -1;
   const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
   const int64_t table = (int64_t)ntiles * kRadix;
@@ -168,24 +177,27 @@ int pt_radix_sort(uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int 
   p += ((n * 8 + 255) / 256) * 256;
   uint32_t* v2 = (uint32_t*)p;
 
-  uint64_t* kin = keys;
-  uint32_t* vin = vals;
-  uint64_t* kout = k2;
-  uint32_t* vout = v2;
-  int passes = 0;
-  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+  // pass p writes (keys, vals) when an even number of passes follow it, the
+  // scratch pair otherwise, so the last pass lands in (keys, vals)
+  const int passes = (end_bit - begin_bit + 7) / 8;
+  const uint64_t* kin = src;
+  const uint32_t* vin = nullptr;
+  for (int p = 0; p < passes; p++) {
+    const int shift = begin_bit + 8 * p;
+    const bool to_final = ((passes - 1 - p) % 2) == 0;
+    uint64_t* kout = to_final ? keys : k2;
+    uint32_t* vout = to_final ? vals : v2;
     count_kernel<<<ntiles, kSortThreads, 0, s>>>(kin, n, shift, ntiles, counts);
     if (pt_exclusive_scan_u32(counts, counts, table, scan_scratch, nullptr, s) != 0) return -1;
-    scatter_kernel<<<ntiles, kSortThreads, 0, s>>>(kin, vin, n, shift, ntiles, counts, kout, vout);
-    uint64_t* tk = kin; kin = kout; kout = tk;
-    uint32_t* tv = vin; vin = vout; vout = tv;
-    passes++;
+    if (p == 0)
+      scatter_kernel<true><<<ntiles, kSortThreads, 0, s>>>(kin, vin, n, shift, ntiles, counts, kout,
+                                                           vout);
+    else
+      scatter_kernel<false><<<ntiles, kSortThreads, 0, s>>>(kin, vin, n, shift, ntiles, counts,
+                                                            kout, vout);
+    kin = kout;
+    vin = vout;
   }
-  if (kin != keys) {
-    if (hipMemcpyAsync(keys, kin, n * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
-    if (hipMemcpyAsync(vals, vin, n * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
-  }
-  (void)passes;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
