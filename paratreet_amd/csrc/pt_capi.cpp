@@ -286,7 +286,7 @@ pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx**
     (void)hipEventCreate(&c->ev[i][1]);
   }
   if (ensure(c->box, 6) != hipSuccess || ensure(c->bbox, 6) != hipSuccess ||
-      ensure(c->box_parts, 6 * kMaxBoxParts) != hipSuccess || ensure(c->counters, 4) != hipSuccess ||
+      ensure(c->box_parts, 6 * kMaxBoxParts) != hipSuccess || ensure(c->counters, 5) != hipSuccess ||
       ensure(c->derr, 1) != hipSuccess || ensure(c->u32_total, 4) != hipSuccess ||
       ensure(c->i64_total, 4) != hipSuccess) {
     pt_ctx_destroy(c);
@@ -581,12 +581,23 @@ static pt_status set_visitor(pt_ctx* c, double theta, const double offset[3], do
 }
 
 // Reads back the reduced counters and the overflow flag (synchronises).
-static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr) {
-  PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblocks, c->counters.p, c->stream));
-  PT_TRY_HIP(c, hipMemcpyAsync(herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
-  PT_TRY_HIP(c, hipMemcpyAsync(&c->h_counters, c->counters.p, sizeof(pt_counters),
-                               hipMemcpyDeviceToHost, c->stream));
+// Reduces the counter partials (with the overflow flag alongside) and reads
+// both back with one synchronise; with `unsort` the results are first put
+// back in caller order (the fused path).
+static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr, bool unsort = false) {
+  PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblocks, c->counters.p, c->derr.p,
+                                             c->stream));
+  if (unsort)
+    PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
+                                      c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+  unsigned long long h[5];
+  PT_TRY_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  c->h_counters.n_node_ints = h[0];
+  c->h_counters.n_part_ints = h[1];
+  c->h_counters.n_opens = h[2];
+  c->h_counters.n_closes = h[3];
+  *herr = (int32_t)h[4];
   return PT_OK;
 }
 
@@ -696,14 +707,13 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
                                          c->offset, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
                                          c->cpart.p, c->derr.p, c->wqueue.p, c->stream));
     }
+    // the results go back to caller order before the one read-back; a rerun
+    // (stack overflow, rare) repeats both
     int32_t herr = 0;
-    if ((st = finish_counters(c, nblk, &herr))) return st;
+    if ((st = finish_counters(c, nblk, &herr, true))) return st;
     if (!herr) break;
     if (attempt == 1) return fail(c, PT_ERR_CAPACITY, "walk stack overflow at 1536 entries");
   }
-  PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                    c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
-  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   if (counters) *counters = c->h_counters;
   c->lists_valid = false;
   c->stage = S_FORCES;

@@ -89,8 +89,9 @@ int pt_launch_traverse(bool fill, const TreeNode* nodes, const HexOpen* ho, cons
                        int64_t nb, double gf, double gm, const double off[3], int32_t* pp_cnt,
                        int32_t* pc_cnt, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
                        int32_t* pc, unsigned long long* partials, int32_t* err, hipStream_t s);
+// out[0..3] = summed counters, out[4] = *err (err may be null).
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
-                              unsigned long long* out, hipStream_t s);
+                              unsigned long long* out, const int32_t* err, hipStream_t s);
 
 // Interactions over the lists (precision 0 = FP64, 1 = FP32C).
 int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,

@@ -104,7 +104,9 @@ traverse_kernel(const TreeNode* __restrict__ nodes, const HexOpen* __restrict__ 
 // adds it to `out` (zeroed beforehand) with four atomics.
 __global__ void __launch_bounds__(256)
 reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblocks,
-                       unsigned long long* __restrict__ out) {
+                       unsigned long long* __restrict__ out, const int32_t* __restrict__ err) {
+  // out[4]: the walk's overflow flag, so one read-back returns both
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[4] = err ? (unsigned long long)*err : 0ull;
   __shared__ unsigned long long s[4][256];
   unsigned long long v[4] = {0, 0, 0, 0};
   for (int64_t b = blockIdx.x * 256ll + threadIdx.x; b < nblocks; b += (int64_t)gridDim.x * 256)
@@ -124,12 +126,12 @@ reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblo
 int64_t pt_traverse_blocks(int64_t nb) { return (nb + kTravWaves - 1) / kTravWaves; }
 
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
-                              unsigned long long* out, hipStream_t s) {
+                              unsigned long long* out, const int32_t* err, hipStream_t s) {
   if (pt_launch_fill_u64(out, 4, 0ull, s) != 0) return -1;
   int64_t grid = (nblocks + 255) / 256;
   if (grid > 256) grid = 256;
   if (grid < 1) grid = 1;
-  reduce_counters_kernel<<<(unsigned)grid, 256, 0, s>>>(part, nblocks, out);
+  reduce_counters_kernel<<<(unsigned)grid, 256, 0, s>>>(part, nblocks, out, err);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
