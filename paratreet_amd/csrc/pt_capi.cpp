@@ -66,6 +66,7 @@ struct pt_ctx {
   DBuf<int32_t> level_nodes, level_cnt;
   DBuf<int64_t> tstat;  // build read-back: {nn, nb, overflow, level counts}
   std::vector<int64_t> level_off;  // internal nodes of depth d: [level_off[d], level_off[d+1])
+  int32_t split_levels = PT_MAX_DEPTH;  // split launches of the next build (last depth + 1)
   DBuf<uint32_t> flag, fpos, head, head_pos, u32_total;
   DBuf<int32_t> bucket_node;
   DBuf<double> nd;
@@ -457,6 +458,7 @@ pt_status pt_build_tree(pt_ctx* c) {
   const int64_t per_level = c->n / (B + 1) + 1;
   int64_t level_cap = std::max<int64_t>(64, 2 * per_level);
   PT_TRY_HIP(c, ensure(c->level_cnt, PT_MAX_DEPTH + 2));
+  int capacity_retries = 0;
   for (int attempt = 0;; attempt++) {
     const int64_t node_cap = 1 + 8 * level_cap;
     c->nn = 0;
@@ -466,11 +468,13 @@ pt_status pt_build_tree(pt_ctx* c) {
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_root(c->n, B, tr, c->level_cnt.p, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->level_nodes.p, 1, 0u, c->stream));
-    // every level is launched (a level past the tree's depth is empty and its
-    // threads exit at once); the buckets follow on the device and one
-    // read-back returns counts, sizes and the overflow flag
+    // the splits of the previous build's depth plus one level (all levels the
+    // first time; a deeper tree is detected below and rebuilt with all);
+    // the buckets follow on the device and one read-back returns counts,
+    // sizes and the overflow flag
+    const int levels = c->split_levels;
     int64_t bound = 1;  // upper bound of the level's internal-node count
-    for (int depth = 0; depth <= PT_MAX_DEPTH - 1; depth++) {
+    for (int depth = 0; depth < levels; depth++) {
       PT_TRY_LAUNCH(c, pt_launch_split_level(c->skeys.p, c->level_nodes.p, c->level_cnt.p, depth,
                                              bound, B, tr, node_cap, level_cap, c->derr.p,
                                              c->stream));
@@ -478,16 +482,23 @@ pt_status pt_build_tree(pt_ctx* c) {
     }
     if ((st = ensure_scratch(c, pt_scan_scratch_bytes(c->n)))) return st;
     PT_TRY_HIP(c, ensure(c->tstat, 3 + PT_MAX_DEPTH + 2));
-    PT_TRY_LAUNCH(c, pt_launch_buckets(tr, node_cap, c->n, c->level_cnt.p, c->derr.p, c->head.p,
-                                       c->head_pos.p, c->scratch.p, c->u32_total.p,
+    PT_TRY_LAUNCH(c, pt_launch_buckets(tr, node_cap, c->n, levels, c->level_cnt.p, c->derr.p,
+                                       c->head.p, c->head_pos.p, c->scratch.p, c->u32_total.p,
                                        c->bucket_node.p, c->tstat.p, c->stream));
     int64_t stat[3 + PT_MAX_DEPTH + 2] = {0};
     PT_TRY_HIP(c, hipMemcpyAsync(stat, c->tstat.p, sizeof(stat), hipMemcpyDeviceToHost,
                                  c->stream));
     PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    if (stat[2] || stat[0] == 0) {
-      if (attempt >= 3) return fail(c, PT_ERR_CAPACITY, "octree node table overflow");
+    if (stat[2]) {
+      if (++capacity_retries > 3) return fail(c, PT_ERR_CAPACITY, "octree node table overflow");
       level_cap *= 4;
+      continue;
+    }
+    if (stat[0] == 0) {
+      // deeper than predicted: all levels
+      if (levels >= PT_MAX_DEPTH || attempt > 8)
+        return fail(c, PT_ERR_CAPACITY, "octree build did not complete");
+      c->split_levels = PT_MAX_DEPTH;
       continue;
     }
     // level table: internal nodes of depth d are [level_off[d], level_off[d+1])
@@ -502,6 +513,7 @@ pt_status pt_build_tree(pt_ctx* c) {
     }
     c->nn = stat[0];
     c->nb = stat[1];
+    c->split_levels = std::min<int32_t>(PT_MAX_DEPTH, c->max_depth + 1);
     break;
   }
   c->stage = S_TREE;

@@ -47,10 +47,11 @@ int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, i
                     int bucket, int child_depth, const PtTree& t, uint32_t* flag, hipStream_t s);
 int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
                       int32_t* next, hipStream_t s);
-int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, const int32_t* level_cnt,
-                      const int32_t* overflow, uint32_t* head, uint32_t* head_pos,
-                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node,
-                      int64_t* status, hipStream_t s);
+// levels: the split launches done (depths 0 .. levels-1).
+int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, int levels,
+                      const int32_t* level_cnt, const int32_t* overflow, uint32_t* head,
+                      uint32_t* head_pos, void* scan_scratch, uint32_t* nb_out,
+                      int32_t* bucket_node, int64_t* status, hipStream_t s);
 
 // HEXADECAPOLE moments: raw third/fourth moments (SoA, PT_HEXA_RAW x nn) of
 // the buckets, then of each internal level (deepest first), then the records.

@@ -143,15 +143,18 @@ compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ p
 }
 
 // Node count of the finished levels (one thread): status[0] = 1 + 8 * (all
-// internal nodes), or 0 when the build overflowed (nothing downstream runs).
+// internal nodes), or 0 when the build overflowed or is incomplete (internal
+// nodes at depth `levels`, which no launched split handled): nothing
+// downstream runs on such a tree.
 __global__ void count_nodes_kernel(const int32_t* __restrict__ level_cnt,
                                    const int32_t* __restrict__ overflow, int64_t node_cap,
-                                   int64_t* __restrict__ status) {
+                                   int levels, int64_t* __restrict__ status) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int64_t internal = 0;
   for (int l = 0; l <= PT_MAX_DEPTH + 1; l++) internal += level_cnt[l];
   const int64_t nn = 1 + 8 * internal;
-  status[0] = (*overflow || nn > node_cap) ? 0 : nn;
+  const bool incomplete = levels <= PT_MAX_DEPTH && level_cnt[levels] > 0;
+  status[0] = (*overflow || nn > node_cap || incomplete) ? 0 : nn;
 }
 
 // Collects what the host reads back after the build in one place:
@@ -218,11 +221,11 @@ int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int3
 // Buckets of a device-built tree without a host round trip: the node count
 // comes from the level counts on the device (status[0]); afterwards status =
 // {nn, nb, overflow, level counts [PT_MAX_DEPTH + 2]} for one read-back.
-int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, const int32_t* level_cnt,
-                      const int32_t* overflow, uint32_t* head, uint32_t* head_pos,
-                      void* scan_scratch, uint32_t* nb_out, int32_t* bucket_node,
-                      int64_t* status, hipStream_t s) {
-  count_nodes_kernel<<<1, 64, 0, s>>>(level_cnt, overflow, node_cap, status);
+int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, int levels,
+                      const int32_t* level_cnt, const int32_t* overflow, uint32_t* head,
+                      uint32_t* head_pos, void* scan_scratch, uint32_t* nb_out,
+                      int32_t* bucket_node, int64_t* status, hipStream_t s) {
+  count_nodes_kernel<<<1, 64, 0, s>>>(level_cnt, overflow, node_cap, levels, status);
   if (n > 0) {
     if (pt_launch_fill_u32(head, n, 0u, s) != 0) return -1;
     leaf_heads_kernel<<<grid_for(node_cap), 256, 0, s>>>(t, status, head);

@@ -273,3 +273,25 @@ def test_leapfrog_matches_oracle():
     opos, ovel = O.kick_drift(pos, vel, acc, 1e-3)
     assert np.array_equal(bits(gpos), bits(opos))
     assert np.array_equal(bits(gvel), bits(ovel))
+
+
+def test_deeper_tree_in_same_context():
+    """The build launches the previous tree's depth + 1 levels of splits; a
+    particle set whose tree is deeper is detected and rebuilt with every
+    level (counters and FP64 forces still match the oracle)."""
+    rng = np.random.default_rng(8)
+    pos0 = rng.uniform(-1, 1, (2000, 3))
+    d = _gpu(pos0, np.full(2000, 1.0 / 2000), np.full(2000, 0.01), 12, 0.7)
+    d.compute_forces(theta=0.7, precision=0)
+    pos, _, m = ics.halo_mixture(30000, n_halos=8, seed=3)
+    e = np.full(len(pos), 0.01)
+    d.load(pos, m, e)
+    cnt = d.compute_forces(theta=0.7, precision=0)
+    acc, _ = d.accelerations()
+    P = O.Pipeline(pos, m, e, bucket=12, theta=0.7)
+    oacc, _, ocnt = P.gravity_unsorted()
+    nn, nb, depth = d.tree_size()
+    d.close()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
+    assert nn == len(P.nodes) and nb == len(P.bucket_nodes)
+    assert rel_err(acc, oacc).max() <= FP64_TOL
