@@ -37,7 +37,7 @@ CONFIGS = {
 # against an 8 MB device copy: factor 1.0).  Most of the write traffic is the
 # kernel's register spill area (28 B per lane per bucket under the 5-wave
 # register budget); the algorithmic output is 32 B per particle.
-TRAFFIC_BYTES_FP32C = (105543.6 + 276476.1) * 1024
+TRAFFIC_BYTES_FP32C = (105570.8 + 276059.0) * 1024
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
