@@ -107,6 +107,19 @@ __device__ __forceinline__ void splineq(R dir, R r2, R twoh, R& a, R& b, R& c, R
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// 1/sqrt(x), x > 0, in FP64: the hardware estimate refined by two Newton
+// steps, within a few ulp of the correctly rounded 1.0 / sqrt(x) (which
+// costs a square-root and a divide expansion).
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double t = x * y;
+  double e = fma(-t, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  t = x * y;
+  e = fma(-t, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // Compensated (Kahan) FP32 accumulation.
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   const float y = v - c;
@@ -121,10 +134,10 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 // Closed-node records: soft < 0 marks a node that openSoftening sends to
 // addGravity (unsoftened monopole); tr = 0.5 * trace(Q).
-struct SrcC64 {  // closed node, FP64
+struct __align__(16) SrcC64 {  // closed node, FP64
   double cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, tr;
 };
-struct SrcP64 {  // source particle, FP64
+struct __align__(16) SrcP64 {  // source particle, FP64
   double x, y, z, m, e, pad;
 };
 struct SrcC32 {  // closed node, FP32, centre in the bucket frame
@@ -269,6 +282,15 @@ __device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
   g.t = g.act ? lane - s * g.ng : 0;
   g.first = (int64_t)b.begin + g0;
   return g;
+}
+
+// A wave-uniform value moved to a scalar register when U (scalar loop control
+// for the FP64 kernel; the FP32C kernel, at its 96-register budget, measured
+// faster without it).
+template <bool U>
+__device__ __forceinline__ int uniform_if(int v) {
+  if constexpr (U) return __builtin_amdgcn_readfirstlane(v);
+  else return v;
 }
 
 // ceil(n / S) for 0 <= n <= 128 without an integer division: with
@@ -491,7 +513,8 @@ __device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& 
     if (s.soft < 0.0) {
       const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-      double f = s.mass / (rsq * sqrt(rsq));
+      const double ir = rsqrt64(rsq);
+      double f = s.mass * (ir * ir * ir);
       f = rsq != 0 ? f : 0.0;
       px = fma(dx, f, px);
       py = fma(dy, f, py);
@@ -499,7 +522,7 @@ __device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& 
     } else {
       const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
       const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-      const double dir = 1.0 / sqrt(rsq);
+      const double dir = rsqrt64(rsq);
       double a, bb, c, d;
       splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
       const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
@@ -549,7 +572,7 @@ __device__ __forceinline__ void node_pair_far(
   } else {
     const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
     const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-    const double a = 1.0 / sqrt(rsq);
+    const double a = rsqrt64(rsq);
     const double bb = a * a * a;
     const double c = 3.0 * bb * a * a;
     const double d = 5.0 * c * a * a;
@@ -650,7 +673,7 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       const SrcP64 s = src[g.s + k * g.S];
       const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double a = 1.0 / sqrt(rsq);
+      const double a = rsqrt64(rsq);
       const double f = (a * a * a) * s.m;
       L.ax = fma(dx, f, L.ax);
       L.ay = fma(dy, f, L.ay);
@@ -674,7 +697,7 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
                                           double ccz) {
   const int lane = __lane_id();
   const int incl = pt_wave_incl_scan(cnt);
-  const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
+  const int total = uniform_if<!F32>(__shfl(incl, PT_WAVE - 1, PT_WAVE));
   const int exc = incl - cnt;
   int carry = 0;
   for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
@@ -707,7 +730,8 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
     pad_chunk<F32, false, W_t>(W, nsrc, g);
     __builtin_amdgcn_wave_barrier();
-    eval_particles<F32, W_t>(W, g, L, ceil_div_s(nnear, g), ceil_div_s(nsrc, g), ox, oy, oz);
+    eval_particles<F32, W_t>(W, g, L, uniform_if<!F32>(ceil_div_s(nnear, g)),
+                             uniform_if<!F32>(ceil_div_s(nsrc, g)), ox, oy, oz);
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -728,7 +752,8 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
   if (act) reinterpret_cast<SrcC<F32>*>(W.stage)[nr ? pt_rank_in(mn) : nnear + pt_rank_in(mf)] = r;
   pad_chunk<F32, true, W_t>(W, nsrc, g);
   __builtin_amdgcn_wave_barrier();
-  eval_nodes<F32, W_t>(W, g, L, ceil_div_s(nnear, g), ceil_div_s(nsrc, g), ox, oy, oz);
+  eval_nodes<F32, W_t>(W, g, L, uniform_if<!F32>(ceil_div_s(nnear, g)),
+                       uniform_if<!F32>(ceil_div_s(nsrc, g)), ox, oy, oz);
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -865,7 +890,7 @@ __device__ __forceinline__ void hexa_eval(const double* __restrict__ h, double m
   const double xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
   const double xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
   const double rsq = rx * rx + ry * ry + rz * rz;
-  const double ir = 1.0 / sqrt(rsq);
+  const double ir = rsqrt64(rsq);
   const double ir2 = ir * ir;
   const double x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
   const double xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
@@ -906,7 +931,8 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
     // addGravity: unsoftened monopole, no potential
     const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    double f = s.mass / (rsq * sqrt(rsq));
+    const double ir = rsqrt64(rsq);
+    double f = s.mass * (ir * ir * ir);
     f = rsq != 0 ? f : 0.0;
     L.ax += dx * f;
     L.ay += dy * f;
