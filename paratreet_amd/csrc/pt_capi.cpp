@@ -694,19 +694,14 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (st) return st;
   if ((st = need(c, S_MOMENTS, "pt_compute_forces"))) return st;
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
-  if (c->cfg.hexadecapole) {
-    // the fused kernel is quadrupole-only; the hexadecapole build runs the
-    // traversal lists and their interaction kernel
-    pt_counters cnt;
-    if ((st = pt_traverse(c, theta, offset, &cnt))) return st;
-    if ((st = pt_interact(c, precision))) return st;
-    if (counters) *counters = cnt;
-    return PT_OK;
-  }
+  if (c->cfg.hexadecapole && precision != PT_FP64)
+    return fail(c, PT_ERR_UNSUPPORTED, "the hexadecapole build evaluates node() in FP64 only");
   double gf;
   if ((st = set_visitor(c, theta, offset, &gf))) return st;
   const int64_t nb = c->nb;
-  const int64_t nblk = std::max<int64_t>(pt_gravity_blocks(nb, (int)precision), 1);
+  const bool hexa = c->cfg.hexadecapole != 0;
+  const int64_t nblk = std::max<int64_t>(
+      hexa ? pt_hexa_gravity_blocks(nb) : pt_gravity_blocks(nb, (int)precision), 1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
   PT_TRY_HIP(c, ensure(c->wqueue, 8));
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -714,10 +709,18 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
     {
       Timer t(c, PT_T_FORCE);
-      PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
-                                         c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p, gf,
-                                         c->offset, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                         c->cpart.p, c->derr.p, c->wqueue.p, c->stream));
+      if (hexa)
+        PT_TRY_LAUNCH(c, pt_launch_hexa_gravity(attempt == 1, c->bk.p, nb, c->tnode.p, c->ho.p,
+                                                c->hn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
+                                                c->es.p, gf, c->gm, c->offset, c->axs.p, c->ays.p,
+                                                c->azs.p, c->pots.p, c->cpart.p, c->derr.p,
+                                                c->stream));
+      else
+        PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
+                                           c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
+                                           c->es.p, gf, c->offset, c->axs.p, c->ays.p, c->azs.p,
+                                           c->pots.p, c->cpart.p, c->derr.p, c->wqueue.p,
+                                           c->stream));
     }
     // the results go back to caller order before the one read-back; a rerun
     // (stack overflow, rare) repeats both

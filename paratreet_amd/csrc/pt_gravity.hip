@@ -880,6 +880,8 @@ struct __align__(16) HexaLds {
 __device__ __forceinline__ void hexa_eval(const double* __restrict__ h, double m, double rx,
                                           double ry, double rz, double& pot, double& gx,
                                           double& gy, double& gz) {
+  // every multiply-add is an explicit fma (this file is compiled without
+  // contraction)
   const double xx = h[0], yy = h[1], xy = h[2], xz = h[3], yz = h[4];
   const double zz = -(xx + yy);
   const double xxx = h[5], xyy = h[6], xxy = h[7], yyy = h[8], xxz = h[9], yyz = h[10],
@@ -889,40 +891,47 @@ __device__ __forceinline__ void hexa_eval(const double* __restrict__ h, double m
                yyyz = h[17], xxyy = h[18], xxyz = h[19], xyyz = h[20];
   const double xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
   const double xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
-  const double rsq = rx * rx + ry * ry + rz * rz;
+  const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
   const double ir = rsqrt64(rsq);
   const double ir2 = ir * ir;
   const double x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
   const double xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
-  const double v2x = xx * rx + xy * ry + xz * rz;
-  const double v2y = xy * rx + yy * ry + yz * rz;
-  const double v2z = xz * rx + yz * ry + zz * rz;
-  const double s2 = v2x * rx + v2y * ry + v2z * rz;
-  const double v3x = xxx * x2 + xyy * y2 + xzz * z2 + 2.0 * (xxy * xyp + xxz * xzp + xyz * yzp);
-  const double v3y = xxy * x2 + yyy * y2 + yzz * z2 + 2.0 * (xyy * xyp + xyz * xzp + yyz * yzp);
-  const double v3z = xxz * x2 + yyz * y2 + zzz * z2 + 2.0 * (xyz * xyp + xzz * xzp + yzz * yzp);
-  const double s3 = v3x * rx + v3y * ry + v3z * rz;
-  const double x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz, xyz3 = xyp * rz;
-  const double v4x = xxxx * x3 + xyyy * y3 + xzzz * z3 +
-                     3.0 * (xxxy * x2 * ry + xxxz * x2 * rz + xxyy * rx * y2 + xxzz * rx * z2 +
-                            xyyz * y2 * rz + xyzz * ry * z2) +
-                     6.0 * xxyz * xyz3;
-  const double v4y = xxxy * x3 + yyyy * y3 + yzzz * z3 +
-                     3.0 * (xxyy * x2 * ry + xxyz * x2 * rz + xyyy * rx * y2 + xyzz * rx * z2 +
-                            yyyz * y2 * rz + yyzz * ry * z2) +
-                     6.0 * xyyz * xyz3;
-  const double v4z = xxxz * x3 + yyyz * y3 + zzzz * z3 +
-                     3.0 * (xxyz * x2 * ry + xxzz * x2 * rz + xyyz * rx * y2 + xzzz * rx * z2 +
-                            yyzz * y2 * rz + yzzz * ry * z2) +
-                     6.0 * xyzz * xyz3;
-  const double s4 = v4x * rx + v4y * ry + v4z * rz;
+  // order 2: v2 = M2 r, s2 = r.M2.r
+  const double v2x = fma(xx, rx, fma(xy, ry, xz * rz));
+  const double v2y = fma(xy, rx, fma(yy, ry, yz * rz));
+  const double v2z = fma(xz, rx, fma(yz, ry, zz * rz));
+  const double s2 = fma(v2x, rx, fma(v2y, ry, v2z * rz));
+  // order 3: v3_i = M3_ijk r_j r_k
+  const double v3x = fma(xxx, x2, fma(xyy, y2, fma(xzz, z2, 2.0 * fma(xxy, xyp, fma(xxz, xzp, xyz * yzp)))));
+  const double v3y = fma(xxy, x2, fma(yyy, y2, fma(yzz, z2, 2.0 * fma(xyy, xyp, fma(xyz, xzp, yyz * yzp)))));
+  const double v3z = fma(xxz, x2, fma(yyz, y2, fma(zzz, z2, 2.0 * fma(xyz, xyp, fma(xzz, xzp, yzz * yzp)))));
+  const double s3 = fma(v3x, rx, fma(v3y, ry, v3z * rz));
+  // order 4: v4_i = M4_ijkl r_j r_k r_l over the 10 cubic monomials
+  const double x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz;
+  const double x2y = x2 * ry, x2z = x2 * rz, xy2 = rx * y2, xz2 = rx * z2, y2z = y2 * rz,
+               yz2 = ry * z2, xyzm = xyp * rz;
+  const double v4x =
+      fma(6.0 * xxyz, xyzm,
+          fma(3.0, fma(xxxy, x2y, fma(xxxz, x2z, fma(xxyy, xy2, fma(xxzz, xz2, fma(xyyz, y2z, xyzz * yz2))))),
+              fma(xxxx, x3, fma(xyyy, y3, xzzz * z3))));
+  const double v4y =
+      fma(6.0 * xyyz, xyzm,
+          fma(3.0, fma(xxyy, x2y, fma(xxyz, x2z, fma(xyyy, xy2, fma(xyzz, xz2, fma(yyyz, y2z, yyzz * yz2))))),
+              fma(xxxy, x3, fma(yyyy, y3, yzzz * z3))));
+  const double v4z =
+      fma(6.0 * xyzz, xyzm,
+          fma(3.0, fma(xxyz, x2y, fma(xxzz, x2z, fma(xyyz, xy2, fma(xzzz, xz2, fma(yyzz, y2z, yzzz * yz2))))),
+              fma(xxxz, x3, fma(yyyz, y3, zzzz * z3))));
+  const double s4 = fma(v4x, rx, fma(v4y, ry, v4z * rz));
   const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2,
                ir11 = ir9 * ir2;
-  pot = -(m * ir + 1.5 * s2 * ir5 + 2.5 * s3 * ir7 + 4.375 * s4 * ir9);
-  const double g = m * ir3 + 7.5 * s2 * ir7 + 17.5 * s3 * ir9 + 39.375 * s4 * ir11;
-  gx = 3.0 * v2x * ir5 + 7.5 * v3x * ir7 + 17.5 * v4x * ir9 - rx * g;
-  gy = 3.0 * v2y * ir5 + 7.5 * v3y * ir7 + 17.5 * v4y * ir9 - ry * g;
-  gz = 3.0 * v2z * ir5 + 7.5 * v3z * ir7 + 17.5 * v4z * ir9 - rz * g;
+  // pot = -(m/r + 3/2 s2/r^5 + 5/2 s3/r^7 + 35/8 s4/r^9)
+  pot = -fma(m, ir, fma(1.5 * s2, ir5, fma(2.5 * s3, ir7, 4.375 * s4 * ir9)));
+  const double g = fma(m, ir3, fma(7.5 * s2, ir7, fma(17.5 * s3, ir9, 39.375 * s4 * ir11)));
+  const double c2 = 3.0 * ir5, c3 = 7.5 * ir7, c4 = 17.5 * ir9;
+  gx = fma(c2, v2x, fma(c3, v3x, fma(c4, v4x, -rx * g)));
+  gy = fma(c2, v2y, fma(c3, v3y, fma(c4, v4y, -ry * g)));
+  gz = fma(c2, v2z, fma(c3, v3z, fma(c4, v4z, -rz * g)));
 }
 
 __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double ox, double oy,
@@ -950,7 +959,8 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
 
 // Stages up to 64 closed nodes (one per lane) and evaluates them for the
 // group's targets (slot s of a lane takes sources s, s + S, ...).
-__device__ __forceinline__ void do_nodes_hexa(HexaLds& W, int32_t id, int nsrc, const Group& g,
+template <typename W_t>
+__device__ __forceinline__ void do_nodes_hexa(W_t& W, int32_t id, int nsrc, const Group& g,
                                               Lanes<false>& L, const HexNode* __restrict__ hn,
                                               const Bucket& b, double ox, double oy, double oz) {
   const int lane = __lane_id();
@@ -1005,7 +1015,7 @@ interact_hexa_kernel(const Bucket* __restrict__ bk, int64_t nb,
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
       const int nsrc = (pc1 - e0) < PT_WAVE ? (int)(pc1 - e0) : PT_WAVE;
       const int32_t id = lane < nsrc ? pc[e0 + lane] : 0;
-      do_nodes_hexa(W, id, nsrc, g, L, hn, b, ox, oy, oz);
+      do_nodes_hexa<HexaLds>(W, id, nsrc, g, L, hn, b, ox, oy, oz);
     }
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       int32_t beg = 0, cnt = 0;
@@ -1018,6 +1028,144 @@ interact_hexa_kernel(const Bucket* __restrict__ bk, int64_t nb,
                                 ccz);
     }
     finish_group<false, HexaLds>(W, g, L, ax, ay, az, pot);
+  }
+}
+
+// Fused walk + interactions of the HEXADECAPOLE build (FP64): the walk with
+// the hexadecapole open(), closed nodes evaluated with the reduced moments,
+// leaves with leaf(); one wave per bucket, chunks evaluated as they fill.
+template <int STACK>
+struct __align__(16) HexaFusedLds {
+  static constexpr int kStage = HexaLds::kStage;
+  unsigned char stage[kStage];
+  int32_t stack[STACK];
+  int32_t pcbuf[2 * PT_WAVE];
+  int32_t ppbeg[2 * PT_WAVE];
+  int32_t ppcnt[2 * PT_WAVE];
+  int32_t fl_mark[PT_WAVE];
+};
+
+template <int STACK>
+__global__ void __launch_bounds__(kThreads)
+hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+                    const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn,
+                    const double* __restrict__ px, const double* __restrict__ py,
+                    const double* __restrict__ pz, const double* __restrict__ pm,
+                    const double* __restrict__ pe, double gf, double gm, double ox, double oy,
+                    double oz, double* __restrict__ ax, double* __restrict__ ay,
+                    double* __restrict__ az, double* __restrict__ pot,
+                    unsigned long long* __restrict__ partials, int32_t* __restrict__ err) {
+  using W_t = HexaFusedLds<STACK>;
+  __shared__ W_t lds[kWaves];
+  __shared__ unsigned long long s_cnt[kWaves][4];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  W_t& W = lds[wave];
+  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
+  bool overflow = false;
+  const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  if (bi < nb) {
+    const Bucket b = bk[bi];
+    const int64_t nt = b.count;
+    const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+                 ccz = (b.loz + b.hiz) * 0.5;
+    const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
+    for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
+      const Group g = make_group(b, g0);
+      const bool count = g0 == 0;  // the walk repeats per target group; count once
+      Lanes<false> L;
+      load_targets<false>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+      int sp = 0;
+      int npc = 0, npp = 0;
+      WalkCounts wc;
+      {
+        const WalkLane r = pt_walk_root<true>(nodes, b, gf, ox, oy, oz, W.stack, sp, wc, ho, gm);
+        if (r.is_closed) {
+          if (lane == 0) W.pcbuf[0] = 0;
+          npc = 1;
+        } else if (r.is_leaf) {
+          if (lane == 0) {
+            W.ppbeg[0] = r.begin;
+            W.ppcnt[0] = r.count;
+          }
+          npp = 1;
+        }
+        if (count) {
+          c_open += wc.n_open;
+          c_close += wc.n_close;
+          c_node += (unsigned long long)wc.n_close * nt;
+          c_part += (unsigned long long)wc.leaf_parts * nt;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      while (sp > 0) {
+        unsigned long long leafm, closem;
+        const WalkLane r = pt_walk_step<STACK, true>(nodes, b, gf, ox, oy, oz, W.stack, sp,
+                                                     overflow, wc, leafm, closem, ho, gm);
+        if (count) {
+          c_open += wc.n_open;
+          c_close += wc.n_close;
+          c_node += (unsigned long long)wc.n_close * nt;
+          c_part += (unsigned long long)wc.leaf_parts * nt;
+        }
+        if (r.is_closed) W.pcbuf[npc + pt_rank_in(closem)] = r.node;
+        if (r.is_leaf) {
+          const int k = npp + pt_rank_in(leafm);
+          W.ppbeg[k] = r.begin;
+          W.ppcnt[k] = r.count;
+        }
+        npc += __popcll(closem);
+        npp += __popcll(leafm);
+        __builtin_amdgcn_wave_barrier();
+        if (npc >= PT_WAVE) {
+          do_nodes_hexa<W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, hn, b, ox, oy, oz);
+          const int rest = npc - PT_WAVE;
+          const int32_t v = lane < rest ? W.pcbuf[PT_WAVE + lane] : 0;
+          __builtin_amdgcn_wave_barrier();
+          if (lane < rest) W.pcbuf[lane] = v;
+          npc = rest;
+          __builtin_amdgcn_wave_barrier();
+        }
+        if (npp >= PT_WAVE) {
+          do_leaves<false, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox,
+                                oy, oz, ccx, ccy, ccz);
+          const int rest = npp - PT_WAVE;
+          const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
+          const int32_t vc = lane < rest ? W.ppcnt[PT_WAVE + lane] : 0;
+          __builtin_amdgcn_wave_barrier();
+          if (lane < rest) {
+            W.ppbeg[lane] = vb;
+            W.ppcnt[lane] = vc;
+          }
+          npp = rest;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      if (npc > 0) {
+        const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
+        do_nodes_hexa<W_t>(W, id, npc, g, L, hn, b, ox, oy, oz);
+      }
+      if (npp > 0) {
+        const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
+        const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
+        do_leaves<false, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
+                              ccz);
+      }
+      finish_group<false, W_t>(W, g, L, ax, ay, az, pot);
+    }
+  }
+  if (lane == 0) {
+    if (overflow) atomicExch(err, 1);
+    s_cnt[wave][0] = c_node;
+    s_cnt[wave][1] = c_part;
+    s_cnt[wave][2] = c_open;
+    s_cnt[wave][3] = c_close;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kWaves; w++) v += s_cnt[w][threadIdx.x];
+    partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
   }
 }
 
@@ -1237,6 +1385,28 @@ int pt_launch_interact_hexa(const Bucket* bk, int64_t nb, const TreeNode* nodes,
   interact_hexa_kernel<<<grid_for(nb), kThreads, 0, s>>>(bk, nb, nodes, hn, x, y, z, m, e, pp_off,
                                                          pp, pc_off, pc, off[0], off[1], off[2],
                                                          ax, ay, az, pot);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Fused HEXADECAPOLE evaluation: one work-group per 4 buckets.
+int64_t pt_hexa_gravity_blocks(int64_t nb) { return (nb + kWaves - 1) / kWaves; }
+
+int pt_launch_hexa_gravity(bool big_stack, const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                           const HexOpen* ho, const HexNode* hn, const double* x, const double* y,
+                           const double* z, const double* m, const double* e, double gf,
+                           double gm, const double off[3], double* ax, double* ay, double* az,
+                           double* pot, unsigned long long* partials, int32_t* err,
+                           hipStream_t s) {
+  if (nb <= 0) return 0;
+  const unsigned grid = (unsigned)pt_hexa_gravity_blocks(nb);
+  if (big_stack)
+    hexa_gravity_kernel<kBigStack><<<grid, kThreads, 0, s>>>(
+        bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
+        partials, err);
+  else
+    hexa_gravity_kernel<kSmallStack><<<grid, kThreads, 0, s>>>(
+        bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
+        partials, err);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

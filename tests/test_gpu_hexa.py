@@ -134,3 +134,19 @@ def test_hexa_rejects_fp32c_and_adjoint():
         d.force_vjp(np.ones((500, 3)), theta=0.7)
     assert ex.value.status == _lib.PT_ERR_UNSUPPORTED
     d.close()
+
+
+def test_hexa_walk_stack_overflow_rerun():
+    """The fused hexadecapole evaluation reruns with the big walk stack when
+    the small one overflows (deep halos, bucket 4, theta 0.1) and still
+    matches the oracle."""
+    pos, _, m = ics.halo_mixture(30000, n_halos=8, seed=3)
+    e = np.full(len(pos), 0.05)
+    P = O.Pipeline(pos, m, e, bucket=4, theta=0.1, hexadecapole=True)
+    d = _gpu(pos, m, e, 4)
+    cnt = d.compute_forces(theta=0.1, precision=0)
+    acc, _ = d.accelerations()
+    d.close()
+    oacc, _, ocnt = P.gravity_unsorted()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
+    assert rel_err(acc, oacc).max() <= FP64_TOL
