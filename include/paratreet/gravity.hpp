@@ -72,6 +72,8 @@ struct Configuration {
   std::string input_file;          // -f achInputFile
   std::string output_file;         // -v achOutputFile
   int periodic = 0;                // bPeriodic
+  double fPeriod[3] = {0, 0, 0};   // dxPeriod dyPeriod dzPeriod
+  int nReplicas = 0;               // nReplicas
   bool hexadecapole = false;       // the -DHEXADECAPOLE build (examples/Makefile:5)
 
   int branchFactor() const {
@@ -118,6 +120,12 @@ struct Configuration {
                                        {"achOutputFile", "v"}};
       for (auto& nm : names)
         if (k == nm[0]) setLetter(nm[1][0], v);
+      // fields registered without a command-line letter (src/Configuration.h:145-149)
+      if (k == "bPeriodic") periodic = std::atoi(v.c_str());
+      if (k == "dxPeriod") fPeriod[0] = std::atof(v.c_str());
+      if (k == "dyPeriod") fPeriod[1] = std::atof(v.c_str());
+      if (k == "dzPeriod") fPeriod[2] = std::atof(v.c_str());
+      if (k == "nReplicas") nReplicas = std::atoi(v.c_str());
     }
   }
 
@@ -134,6 +142,8 @@ struct Configuration {
     c.has_dsoft = dSoft_set ? 1 : 0;
     c.periodic = periodic;
     c.hexadecapole = hexadecapole ? 1 : 0;
+    for (int k = 0; k < 3; k++) c.period[k] = fPeriod[k];
+    c.n_replicas = nReplicas;
     return c;
   }
 

@@ -67,12 +67,16 @@ typedef struct pt_config {
   int32_t num_iterations;         /* '-i'; used by pt_run */
   double dsoft;                   /* '-e' dSoft; applied to every particle if has_dsoft */
   int32_t has_dsoft;              /* Configuration::origin_of("dSoft") != Unknown */
-  int32_t periodic;               /* bPeriodic; must be 0 (non-periodic path) */
+  int32_t periodic;               /* bPeriodic: pt_compute_forces / pt_run sum the
+                                     (2 n_replicas + 1)^3 replicas of the period box
+                                     (examples/Gravity.C:35-46) */
   int32_t hexadecapole;           /* the -DHEXADECAPOLE build (examples/Makefile:5):
                                      moments to fourth order, GravityVisitor::open/node
                                      HEXADECAPOLE branches (examples/GravityVisitor.h:
                                      162-178,199-206); FP64 only, list path, no adjoint.
                                      0 = the quadrupole build */
+  double period[3];               /* dxPeriod dyPeriod dzPeriod (used when periodic) */
+  int32_t n_replicas;             /* nReplicas, 0..16 (used when periodic) */
 } pt_config;
 
 /* Interaction counters, src/ThreadStateHolder.h:10-13 (COUNT_INTERACTIONS). */

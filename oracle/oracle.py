@@ -320,5 +320,30 @@ class Pipeline:
         out_p[self.perm] = pot
         return out_a, out_p, cnt
 
+    def gravity_replicas(self, n_replicas, period, nthreads=0):
+        """ExMain::traversalFn with periodic (examples/Gravity.C:35-46): the sum
+        of the traversals with offsets (X, Y, Z) * period, X, Y, Z in
+        [-N, N] (X outermost), in the caller's original order, with the
+        counters summed over the replicas."""
+        n = len(self.x)
+        acc = np.zeros((n, 3))
+        pot = np.zeros(n)
+        cnt = np.zeros(4, dtype=np.uint64)
+        r = range(-n_replicas, n_replicas + 1)
+        for X in r:
+            for Y in r:
+                for Z in r:
+                    off = (X * period[0], Y * period[1], Z * period[2])
+                    a, p, c = gravity(self.nodes, self.data, self.x, self.y, self.z, self.m,
+                                      self.eps, self.theta, off, None, nthreads, hexa=self.hexa)
+                    acc += a
+                    pot += p
+                    cnt += c
+        out_a = np.empty_like(acc)
+        out_p = np.empty_like(pot)
+        out_a[self.perm] = acc
+        out_p[self.perm] = pot
+        return out_a, out_p, cnt
+
     def lists(self):
         return lists(self.nodes, self.data, self.theta, self.offset, hexa=self.hexa)

@@ -43,6 +43,8 @@ class PtConfig(ctypes.Structure):
         ("has_dsoft", ctypes.c_int32),
         ("periodic", ctypes.c_int32),
         ("hexadecapole", ctypes.c_int32),
+        ("period", ctypes.c_double * 3),
+        ("n_replicas", ctypes.c_int32),
     ]
 
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -256,6 +257,8 @@ void pt_config_default(pt_config* cfg) {
   cfg->has_dsoft = 0;
   cfg->periodic = 0;
   cfg->hexadecapole = 0;
+  cfg->period[0] = cfg->period[1] = cfg->period[2] = 0.0;
+  cfg->n_replicas = 0;
 }
 
 pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx** out) {
@@ -263,7 +266,12 @@ pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx**
   *out = nullptr;
   if (cfg->tree_type != PT_TREE_OCT || cfg->decomp_type != PT_DECOMP_OCT) return PT_ERR_UNSUPPORTED;
   if (cfg->max_particles_per_leaf < 1) return PT_ERR_ARG;
-  if (cfg->periodic) return PT_ERR_UNSUPPORTED;
+  if (cfg->periodic) {
+    // replicas of the period box (examples/Gravity.C:35-46)
+    if (cfg->n_replicas < 0 || cfg->n_replicas > 16) return PT_ERR_ARG;
+    for (int k = 0; k < 3; k++)
+      if (!(cfg->period[k] > 0.0) || !std::isfinite(cfg->period[k])) return PT_ERR_ARG;
+  }
   pt_ctx* c = new pt_ctx();
   c->cfg = *cfg;
   c->device = device;
@@ -596,9 +604,10 @@ static pt_status set_visitor(pt_ctx* c, double theta, const double offset[3], do
 // Reduces the counter partials (with the overflow flag alongside) and reads
 // both back with one synchronise; with `unsort` the results are first put
 // back in caller order (the fused path).
-static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr, bool unsort = false) {
+static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr, bool unsort = false,
+                                 bool accumulate = false) {
   PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblocks, c->counters.p, c->derr.p,
-                                             c->stream));
+                                             c->stream, accumulate));
   if (unsort)
     PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
                                       c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
@@ -704,28 +713,47 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
       hexa ? pt_hexa_gravity_blocks(nb) : pt_gravity_blocks(nb, (int)precision), 1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
   PT_TRY_HIP(c, ensure(c->wqueue, 8));
+  // the visitor offsets: one, or the (2N+1)^3 period replicas in the order of
+  // ExMain::traversalFn (examples/Gravity.C:35-46), each added to `offset`;
+  // every replica's traversal adds to the accelerations and the counters
+  std::vector<std::array<double, 3>> offs;
+  const int nr = c->cfg.periodic ? c->cfg.n_replicas : 0;
+  for (int X = -nr; X <= nr; X++)
+    for (int Y = -nr; Y <= nr; Y++)
+      for (int Z = -nr; Z <= nr; Z++)
+        offs.push_back({c->offset[0] + X * c->cfg.period[0], c->offset[1] + Y * c->cfg.period[1],
+                        c->offset[2] + Z * c->cfg.period[2]});
   for (int attempt = 0; attempt < 2; attempt++) {
-    PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
-    {
-      Timer t(c, PT_T_FORCE);
-      if (hexa)
-        PT_TRY_LAUNCH(c, pt_launch_hexa_gravity(attempt == 1, c->bk.p, nb, c->tnode.p, c->ho.p,
-                                                c->hn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
-                                                c->es.p, gf, c->gm, c->offset, c->axs.p, c->ays.p,
-                                                c->azs.p, c->pots.p, c->cpart.p, c->derr.p,
-                                                c->stream));
+    for (size_t r = 0; r < offs.size(); r++) {
+      const bool acc = r > 0;
+      const double* off = offs[r].data();
+      PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
+      {
+        Timer t(c, PT_T_FORCE);
+        if (hexa)
+          PT_TRY_LAUNCH(c, pt_launch_hexa_gravity(attempt == 1, c->bk.p, nb, c->tnode.p, c->ho.p,
+                                                  c->hn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
+                                                  c->es.p, gf, c->gm, off, c->axs.p, c->ays.p,
+                                                  c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc,
+                                                  c->stream));
+        else
+          PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb,
+                                             c->tnode.p, c->gn.p, c->gn32.p, c->xs.p, c->ys.p,
+                                             c->zs.p, c->ms.p, c->es.p, gf, off, c->axs.p,
+                                             c->ays.p, c->azs.p, c->pots.p, c->cpart.p,
+                                             c->derr.p, c->wqueue.p, acc, c->stream));
+      }
+      if (r + 1 < offs.size())
+        PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblk, c->counters.p, c->derr.p,
+                                                   c->stream, acc));
       else
-        PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb, c->tnode.p,
-                                           c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
-                                           c->es.p, gf, c->offset, c->axs.p, c->ays.p, c->azs.p,
-                                           c->pots.p, c->cpart.p, c->derr.p, c->wqueue.p,
-                                           c->stream));
+        break;
     }
-    // the results go back to caller order before the one read-back; a rerun
-    // (stack overflow, rare) repeats both
+    // the last replica's counters, the unsort to caller order and the one
+    // read-back; a rerun (stack overflow, rare) repeats every replica
     int32_t herr = 0;
-    if ((st = finish_counters(c, nblk, &herr, true))) return st;
+    if ((st = finish_counters(c, nblk, &herr, true, offs.size() > 1))) return st;
     if (!herr) break;
     if (attempt == 1) return fail(c, PT_ERR_CAPACITY, "walk stack overflow at 1536 entries");
   }
@@ -977,6 +1005,8 @@ pt_status pt_force_vjp(pt_ctx* c, double theta, pt_force_mode mode, const double
   if (mode == PT_FORCE_TREE && !(theta > 0.0)) return fail(c, PT_ERR_ARG, "theta must be positive");
   if (mode == PT_FORCE_TREE && c->cfg.hexadecapole)
     return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the hexadecapole node()");
+  if (mode == PT_FORCE_TREE && c->cfg.periodic)
+    return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the periodic replicas");
   AdjViews v;
   if ((st = adj_views(c, &v))) return st;
   // the sorted-order block (gsx..) doubles as the host staging area before
@@ -1002,6 +1032,8 @@ pt_status pt_leapfrog_forward(pt_ctx* c, int32_t steps, double dt, double theta,
   if (c->n == 0) return fail(c, PT_ERR_ARG, "no particles");
   if (mode == PT_FORCE_TREE && c->cfg.hexadecapole)
     return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the hexadecapole node()");
+  if (mode == PT_FORCE_TREE && c->cfg.periodic)
+    return fail(c, PT_ERR_UNSUPPORTED, "no adjoint of the periodic replicas");
   const int32_t every = checkpoint_every > 0 ? checkpoint_every : std::max<int32_t>(steps, 1);
   const int64_t nseg = std::max<int64_t>(1, (steps + every - 1) / every);
   const int64_t n = c->n;

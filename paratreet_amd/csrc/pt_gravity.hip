@@ -761,7 +761,8 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
 template <bool F32, typename W_t>
 __device__ __forceinline__ void finish_group(W_t& W, const Group& g, const Lanes<F32>& L,
                                              double* __restrict__ ax, double* __restrict__ ay,
-                                             double* __restrict__ az, double* __restrict__ pot) {
+                                             double* __restrict__ az, double* __restrict__ pot,
+                                             bool acc = false) {
   double* red = reinterpret_cast<double*>(W.stage);
   const int lane = __lane_id();
   double v0, v1, v2, v3;
@@ -791,10 +792,18 @@ __device__ __forceinline__ void finish_group(W_t& W, const Group& g, const Lanes
       s3 += red[l * 4 + 3];
     }
     const int64_t i = g.first + lane;
-    ax[i] = s0;
-    ay[i] = s1;
-    az[i] = s2;
-    pot[i] = s3;
+    if (acc) {
+      // another periodic replica: added to what the earlier ones wrote
+      ax[i] += s0;
+      ay[i] += s1;
+      az[i] += s2;
+      pot[i] += s3;
+    } else {
+      ax[i] = s0;
+      ay[i] = s1;
+      az[i] = s2;
+      pot[i] = s3;
+    }
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -1054,7 +1063,7 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
                     const double* __restrict__ pe, double gf, double gm, double ox, double oy,
                     double oz, double* __restrict__ ax, double* __restrict__ ay,
                     double* __restrict__ az, double* __restrict__ pot,
-                    unsigned long long* __restrict__ partials, int32_t* __restrict__ err) {
+                    unsigned long long* __restrict__ partials, int32_t* __restrict__ err, int acc) {
   using W_t = HexaFusedLds<STACK>;
   __shared__ W_t lds[kWaves];
   __shared__ unsigned long long s_cnt[kWaves][4];
@@ -1151,7 +1160,7 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         do_leaves<false, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
                               ccz);
       }
-      finish_group<false, W_t>(W, g, L, ax, ay, az, pot);
+      finish_group<false, W_t>(W, g, L, ax, ay, az, pot, acc != 0);
     }
   }
   if (lane == 0) {
@@ -1180,7 +1189,7 @@ __device__ __forceinline__ void gravity_bucket(
     const double* __restrict__ pe, double gf, double ox, double oy, double oz,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     double* __restrict__ pot, unsigned long long& c_node, unsigned long long& c_part,
-    unsigned long long& c_open, unsigned long long& c_close, bool& overflow) {
+    unsigned long long& c_open, unsigned long long& c_close, bool& overflow, bool acc) {
   using W_t = WaveLds<F32, STACK>;
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const Bucket b = bk[bi];
@@ -1269,7 +1278,7 @@ __device__ __forceinline__ void gravity_bucket(
       const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
       do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
     }
-    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot);
+    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
   }
 }
 
@@ -1291,7 +1300,7 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
                double ox, double oy, double oz, double* __restrict__ ax,
                double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
                unsigned long long* __restrict__ partials, int32_t* __restrict__ err,
-               unsigned long long* __restrict__ queue) {
+               unsigned long long* __restrict__ queue, int acc) {
   using W_t = WaveLds<F32, STACK>;
   __shared__ W_t lds[kWaves];
   __shared__ unsigned long long s_cnt[kWaves][4];
@@ -1304,7 +1313,8 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
     const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
     if (bi < nb)
       gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
-                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow);
+                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
+                                 acc != 0);
   } else {
     const int xcd = blockIdx.x % 8;
     const int64_t b_lo = nb * xcd / 8, b_hi = nb * (xcd + 1) / 8;
@@ -1315,7 +1325,8 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
       unsigned long long next = 0;
       if (lane == 0) next = atomicAdd(&queue[xcd], 1ull);
       gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
-                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow);
+                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
+                                 acc != 0);
       bi = b_lo + (int64_t)__shfl(next, 0, PT_WAVE);
     }
   }
@@ -1396,17 +1407,17 @@ int pt_launch_hexa_gravity(bool big_stack, const Bucket* bk, int64_t nb, const T
                            const double* z, const double* m, const double* e, double gf,
                            double gm, const double off[3], double* ax, double* ay, double* az,
                            double* pot, unsigned long long* partials, int32_t* err,
-                           hipStream_t s) {
+                           bool accumulate, hipStream_t s) {
   if (nb <= 0) return 0;
   const unsigned grid = (unsigned)pt_hexa_gravity_blocks(nb);
   if (big_stack)
     hexa_gravity_kernel<kBigStack><<<grid, kThreads, 0, s>>>(
         bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
-        partials, err);
+        partials, err, accumulate ? 1 : 0);
   else
     hexa_gravity_kernel<kSmallStack><<<grid, kThreads, 0, s>>>(
         bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
-        partials, err);
+        partials, err, accumulate ? 1 : 0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1418,14 +1429,14 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
                       const double* x, const double* y, const double* z, const double* m,
                       const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
                       double* pot, unsigned long long* partials, int32_t* err,
-                      unsigned long long* queue, hipStream_t s) {
+                      unsigned long long* queue, bool accumulate, hipStream_t s) {
   if (nb <= 0) return 0;
   if (pt_launch_fill_u64(queue, 8, 0ull, s) != 0) return -1;
   const unsigned grid = (unsigned)pt_gravity_blocks(nb, precision);
 #define PT_GRAVITY_LAUNCH(F32, STACK)                                                          \
   gravity_kernel<F32, STACK><<<grid, kThreads, 0, s>>>(                                        \
       bk, nb, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot,      \
-      partials, err, queue)
+      partials, err, queue, accumulate ? 1 : 0)
   if (precision == 0) {
     if (big_stack) PT_GRAVITY_LAUNCH(false, kBigStack);
     else PT_GRAVITY_LAUNCH(false, kSmallStack);

@@ -90,9 +90,11 @@ int pt_launch_traverse(bool fill, const TreeNode* nodes, const HexOpen* ho, cons
                        int64_t nb, double gf, double gm, const double off[3], int32_t* pp_cnt,
                        int32_t* pc_cnt, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
                        int32_t* pc, unsigned long long* partials, int32_t* err, hipStream_t s);
-// out[0..3] = summed counters, out[4] = *err (err may be null).
+// out[0..3] = summed counters (added to out with accumulate), out[4] = *err
+// (err may be null).
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
-                              unsigned long long* out, const int32_t* err, hipStream_t s);
+                              unsigned long long* out, const int32_t* err, hipStream_t s,
+                              bool accumulate = false);
 
 // Interactions over the lists (precision 0 = FP64, 1 = FP32C).
 int pt_launch_interact(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
@@ -118,14 +120,14 @@ int pt_launch_hexa_gravity(bool big_stack, const Bucket* bk, int64_t nb, const T
                            const double* z, const double* m, const double* e, double gf,
                            double gm, const double off[3], double* ax, double* ay, double* az,
                            double* pot, unsigned long long* partials, int32_t* err,
-                           hipStream_t s);
+                           bool accumulate, hipStream_t s);
 int64_t pt_gravity_blocks(int64_t nb, int precision);
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,
                       const double* x, const double* y, const double* z, const double* m,
                       const double* e, double gf, const double off[3], double* ax, double* ay, double* az,
                       double* pot, unsigned long long* partials, int32_t* err,
-                      unsigned long long* queue, hipStream_t s);
+                      unsigned long long* queue, bool accumulate, hipStream_t s);
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
                      double* ys, double* zs, double* ms, double* es, hipStream_t s);

@@ -126,8 +126,9 @@ reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblo
 int64_t pt_traverse_blocks(int64_t nb) { return (nb + kTravWaves - 1) / kTravWaves; }
 
 int pt_launch_reduce_counters(const unsigned long long* part, int64_t nblocks,
-                              unsigned long long* out, const int32_t* err, hipStream_t s) {
-  if (pt_launch_fill_u64(out, 4, 0ull, s) != 0) return -1;
+                              unsigned long long* out, const int32_t* err, hipStream_t s,
+                              bool accumulate) {
+  if (!accumulate && pt_launch_fill_u64(out, 4, 0ull, s) != 0) return -1;
   int64_t grid = (nblocks + 255) / 256;
   if (grid > 256) grid = 256;
   if (grid < 1) grid = 1;

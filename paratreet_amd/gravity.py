@@ -50,7 +50,9 @@ class Configuration:
     dSoft: float = 0.0                  # -e
     input_file: str = ""                # -f
     output_file: str = ""               # -v
-    periodic: int = 0
+    periodic: int = 0                   # bPeriodic (parameter file)
+    fPeriod: tuple = (0.0, 0.0, 0.0)    # dxPeriod dyPeriod dzPeriod
+    nReplicas: int = 0                  # nReplicas
     # the reference's -DHEXADECAPOLE build (examples/Makefile:5): a build flag
     # there, a context option here
     hexadecapole: bool = False
@@ -92,6 +94,14 @@ class Configuration:
                 k, v = (s.strip() for s in line.split("=", 1))
                 if k in self.NAMES:
                     self._set(self.NAMES[k], v, "file")
+                elif k == "bPeriodic":
+                    self.periodic = int(v)
+                elif k in ("dxPeriod", "dyPeriod", "dzPeriod"):
+                    p = list(self.fPeriod)
+                    p["xyz".index(k[1])] = float(v)
+                    self.fPeriod = tuple(p)
+                elif k == "nReplicas":
+                    self.nReplicas = int(v)
 
     def _set(self, letter, value, origin):
         name, typ = self.LETTERS[letter]
@@ -116,6 +126,9 @@ class Configuration:
         c.has_dsoft = 1 if "dSoft" in self.origin else 0
         c.periodic = self.periodic
         c.hexadecapole = 1 if self.hexadecapole else 0
+        for k in range(3):
+            c.period[k] = float(self.fPeriod[k])
+        c.n_replicas = int(self.nReplicas)
         return c
 
 

@@ -61,10 +61,30 @@ def test_invalid_bucket_and_periodic_rejected():
     c.max_particles_per_leaf = 0
     ctx = ctypes.c_void_p()
     assert L.pt_ctx_create(ctypes.byref(c), 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_ARG
+    # periodic needs a positive finite period and 0 <= nReplicas <= 16
     L.pt_config_default(ctypes.byref(c))
     c.periodic = 1
-    assert L.pt_ctx_create(ctypes.byref(c), 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_UNSUPPORTED
+    assert L.pt_ctx_create(ctypes.byref(c), 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_ARG
+    c.period[0] = c.period[1] = c.period[2] = 4.0
+    c.n_replicas = 17
+    assert L.pt_ctx_create(ctypes.byref(c), 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_ARG
+    c.n_replicas = 1
+    c.period[2] = float("inf")
+    assert L.pt_ctx_create(ctypes.byref(c), 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_ARG
     assert L.pt_ctx_create(None, 0, None, ctypes.byref(ctx)) == _lib.PT_ERR_ARG
+
+
+def test_periodic_parameter_file(tmp_path):
+    """bPeriodic / dxPeriod / dyPeriod / dzPeriod / nReplicas are parameter-file
+    fields without command-line letters (src/Configuration.h:145-149)."""
+    from paratreet_amd.gravity import Configuration
+    f = tmp_path / "p.param"
+    f.write_text("bPeriodic = 1\ndxPeriod = 2.5\ndyPeriod = 3\ndzPeriod = 4\nnReplicas = 2\n")
+    cfg = Configuration()
+    cfg.parse(["-x", str(f)])
+    c = cfg.to_c()
+    assert c.periodic == 1 and c.n_replicas == 2
+    assert tuple(c.period) == (2.5, 3.0, 4.0)
 
 
 def test_null_context_calls_fail_cleanly():

@@ -556,3 +556,34 @@ def test_hexa_tree_force_meets_reference_accuracy_statement():
     acc_q, _, _ = Q.gravity_unsorted()
     rq = np.linalg.norm(acc_q - ref, axis=1) / np.linalg.norm(ref, axis=1)
     assert rq.max() > 0.03 and np.sqrt(np.mean(r * r)) < np.sqrt(np.mean(rq * rq)) / 3
+
+
+# ---------------------------------------------------------------- periodic replicas
+
+def test_replica_sum_matches_direct_sum_over_replicated_images():
+    """ExMain::traversalFn with periodic (examples/Gravity.C:35-46): the tree
+    sum over the 27 replica offsets (nReplicas = 1) against the direct sum of
+    the leaf() kernel over an explicitly replicated particle set (targets in
+    the central copy): the offsets, their sign and the self-image pairs
+    (rsq = period^2, included) agree up to the tree's truncation error."""
+    rng = np.random.default_rng(21)
+    n = 400
+    pos = rng.uniform(0.0, 1.0, (n, 3))
+    m = np.full(n, 1.0 / n)
+    e = np.full(n, 0.01)
+    period = (1.0, 1.0, 1.0)
+    P = O.Pipeline(pos, m, e, bucket=8, theta=0.3)
+    acc, _, cnt = P.gravity_replicas(1, period)
+    reps = [np.array([X, Y, Z], dtype=np.float64) for X in (-1, 0, 1) for Y in (-1, 0, 1)
+            for Z in (-1, 0, 1)]
+    allpos = np.concatenate([pos + r for r in reps])
+    allm = np.tile(m, len(reps))
+    alle = np.tile(e, len(reps))
+    centre = 13  # (0, 0, 0) in the X-outer order
+    t = np.arange(centre * n, (centre + 1) * n)
+    ref = O.direct(allpos[:, 0], allpos[:, 1], allpos[:, 2], allm, alle, t)
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.sqrt(np.mean(rel ** 2)) < 2e-3
+    # the replicas add node() and leaf() calls to the single box's
+    _, _, c0 = P.gravity()
+    assert cnt[0] > c0[0] and cnt[1] > c0[1] and cnt[3] > c0[3]
