@@ -80,6 +80,9 @@ def _declare(L):
     L.or_gravity_hexa.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _I64,
                                   _P, _P, _P, _P, _P, ctypes.c_int]
     L.or_lists_hexa.argtypes = [_P, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P]
+    L.or_ewald.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _P, _P, _P, _P]
+    L.or_ewald_general.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _D, _D, _D,
+                                   _P, _P, _P, _P]
 
 
 def _p(a):
@@ -253,6 +256,27 @@ def direct_vjp(pos, m, eps, w):
     return np.stack([gx, gy, gz], axis=1), gm
 
 
+def ewald(pos, mass, cm, q, L, n_replicas, a_fac=None, ew_cut=2.6, h_cut=2.8):
+    """LeafEwaldFn's quadrupole-form lattice correction for the particles at
+    pos (n x 3) from a root (mass, cm, raw second moments q: xx xy xz yy yz
+    zz): (acc n x 3, pot n).  a_fac=None uses the reference's constants."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (_f64(pos[:, k]) for k in range(3))
+    n = len(x)
+    ax, ay, az, pot = (np.zeros(n) for _ in range(4))
+    if a_fac is None:
+        lib().or_ewald(_p(x), _p(y), _p(z), n, float(mass), _p(_f64(cm)), _p(_f64(q)), float(L),
+                       int(n_replicas), _p(ax), _p(ay), _p(az), _p(pot))
+    else:
+        lib().or_ewald_general(_p(x), _p(y), _p(z), n, float(mass), _p(_f64(cm)), _p(_f64(q)),
+                               float(L), int(n_replicas), float(a_fac), float(ew_cut),
+                               float(h_cut), _p(ax), _p(ay), _p(az), _p(pot))
+    return np.stack([ax, ay, az], axis=1), pot
+
+
+_ewald_fn = ewald
+
+
 def spline(r2, twoh):
     a, b = ctypes.c_double(), ctypes.c_double()
     lib().or_spline(float(r2), float(twoh), ctypes.byref(a), ctypes.byref(b))
@@ -320,11 +344,12 @@ class Pipeline:
         out_p[self.perm] = pot
         return out_a, out_p, cnt
 
-    def gravity_replicas(self, n_replicas, period, nthreads=0):
+    def gravity_replicas(self, n_replicas, period, nthreads=0, ewald=False):
         """ExMain::traversalFn with periodic (examples/Gravity.C:35-46): the sum
         of the traversals with offsets (X, Y, Z) * period, X, Y, Z in
         [-N, N] (X outermost), in the caller's original order, with the
-        counters summed over the replicas."""
+        counters summed over the replicas; with ewald, plus LeafEwaldFn's
+        lattice correction from the root (quadrupole form, L = period[0])."""
         n = len(self.x)
         acc = np.zeros((n, 3))
         pot = np.zeros(n)
@@ -339,6 +364,12 @@ class Pipeline:
                     acc += a
                     pot += p
                     cnt += c
+        if ewald:
+            root = self.data[0]
+            ea, ep = _ewald_fn(np.stack([self.x, self.y, self.z], axis=1), root["mass"],
+                               root["cm"], root["q"], period[0], n_replicas)
+            acc += ea
+            pot += ep
         out_a = np.empty_like(acc)
         out_p = np.empty_like(pot)
         out_a[self.perm] = acc

@@ -1128,6 +1128,125 @@ void or_direct_vjp(const double* x, const double* y, const double* z,
   }
 }
 
+/* ------------------------------------------------------------------------- */
+/* Ewald sum of the periodic build (quadrupole form)                           */
+/* ------------------------------------------------------------------------- */
+/*
+ * LeafEwaldFn with EwaldData::EwaldInit (examples/Ewald.h, examples/Ewald.C,
+ * non-HEXADECAPOLE branches): after the explicit replica traversals, every
+ * particle gets the rest of the infinite lattice of the root's monopole +
+ * quadrupole (mass M, centre cm, raw second moments q) by the Ewald split
+ * with alpha = a_fac / L: real-space erfc terms over the lattice vectors
+ * within ew_cut * L (those of the explicit replicas -- the "hole" --
+ * contribute -erf instead, removing the bare 1/r part the tree already
+ * summed; a Taylor series replaces the closed forms below r^2 = 1.2e-3 L^2),
+ * and the Fourier terms of the integer vectors 0 < |h| <= h_cut.  The
+ * reference uses a_fac = 2, ew_cut = 2.6, h_cut = 2.8 and L = dxPeriod.
+ * Adds to ax/ay/az/pot.
+ */
+void or_ewald_general(const double* x, const double* y, const double* z, int64_t n,
+                      double mass, const double cm[3], const double q[6], double L,
+                      int n_replicas, double a_fac, double ew_cut, double h_cut, double* ax,
+                      double* ay, double* az, double* pot) {
+  const double pi = 3.14159265358979323846;
+  const double alpha = a_fac / L, alpha2 = alpha * alpha;
+  const double k1 = pi / (alpha2 * L * L * L);
+  const double ka = 2.0 * alpha / sqrt(pi);
+  const double q2 = 0.5 * (q[0] + q[3] + q[5]);
+  int nrep = (int)ceil(ew_cut);
+  if (n_replicas > nrep) nrep = n_replicas;
+  const double cut2 = ew_cut * ew_cut * L * L, inner2 = 1.2e-3 * L * L;
+  /* Fourier table: C(h) = -(e^{-k4 h^2} / (pi h^2 L)) (M - (2 pi / L)^2 h.q.h / 2); the sine
+   * coefficients vanish at quadrupole order */
+  const int hr = (int)ceil(h_cut);
+  const double k4 = pi * pi / (alpha2 * L * L);
+  int nh = 0;
+  double* hv = (double*)malloc(sizeof(double) * 4 * (size_t)(2 * hr + 1) * (2 * hr + 1) * (2 * hr + 1));
+  for (int hx = -hr; hx <= hr; hx++)
+    for (int hy = -hr; hy <= hr; hy++)
+      for (int hz = -hr; hz <= hr; hz++) {
+        const int h2 = hx * hx + hy * hy + hz * hz;
+        if (h2 == 0 || h2 > h_cut * h_cut) continue;
+        const double g0 = exp(-k4 * h2) / (pi * h2 * L);
+        const double w = 2.0 * pi / L;
+        const double hqh = 0.5 * (hx * (q[0] * hx + q[1] * hy + q[2] * hz) +
+                                  hy * (q[1] * hx + q[3] * hy + q[4] * hz) +
+                                  hz * (q[2] * hx + q[4] * hy + q[5] * hz));
+        hv[4 * nh + 0] = w * hx;
+        hv[4 * nh + 1] = w * hy;
+        hv[4 * nh + 2] = w * hz;
+        hv[4 * nh + 3] = -(g0 * mass - w * w * g0 * hqh);
+        nh++;
+      }
+  for (int64_t i = 0; i < n; i++) {
+    double p = mass * k1, fx = 0.0, fy = 0.0, fz = 0.0;
+    const double dx = x[i] - cm[0], dy = y[i] - cm[1], dz = z[i] - cm[2];
+    for (int ix = -nrep; ix <= nrep; ix++) {
+      const int hole_x = ix >= -n_replicas && ix <= n_replicas;
+      const double rx = dx + ix * L;
+      for (int iy = -nrep; iy <= nrep; iy++) {
+        const int hole_xy = hole_x && iy >= -n_replicas && iy <= n_replicas;
+        const double ry = dy + iy * L;
+        for (int iz = -nrep; iz <= nrep; iz++) {
+          const int hole = hole_xy && iz >= -n_replicas && iz <= n_replicas;
+          const double rz = dz + iz * L;
+          double r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 > cut2 && !hole) continue;
+          double g0, g1, g2, g3;
+          if (r2 < inner2) {
+            /* series of -erf(alpha r)/r and its derivatives about r = 0 */
+            const double s2 = r2 * alpha2;
+            double an = ka;
+            g0 = an * ((1.0 / 3.0) * s2 - 1.0);
+            an *= 2.0 * alpha2;
+            g1 = an * ((1.0 / 5.0) * s2 - (1.0 / 3.0));
+            an *= 2.0 * alpha2;
+            g2 = an * ((1.0 / 7.0) * s2 - (1.0 / 5.0));
+            an *= 2.0 * alpha2;
+            g3 = an * ((1.0 / 9.0) * s2 - (1.0 / 7.0));
+          } else {
+            const double r = sqrt(r2);
+            const double ir2 = 1.0 / r2;
+            const double a = exp(-r2 * alpha2) * ka * ir2;
+            g0 = (hole ? -erf(alpha * r) : erfc(alpha * r)) / r;
+            g1 = g0 * ir2 + a;
+            g2 = 3.0 * g1 * ir2 + 2.0 * alpha2 * a;
+            g3 = 5.0 * g2 * ir2 + 4.0 * alpha2 * alpha2 * a;
+          }
+          const double qx = q[0] * rx + q[1] * ry + q[2] * rz;
+          const double qy = q[1] * rx + q[3] * ry + q[4] * rz;
+          const double qz = q[2] * rx + q[4] * ry + q[5] * rz;
+          const double qrr = 0.5 * (qx * rx + qy * ry + qz * rz);
+          const double qta = g1 * mass - g2 * q2 + g3 * qrr;
+          p -= g0 * mass - g1 * q2 + g2 * qrr;
+          fx += g2 * qx - rx * qta;
+          fy += g2 * qy - ry * qta;
+          fz += g2 * qz - rz * qta;
+        }
+      }
+    }
+    for (int k = 0; k < nh; k++) {
+      const double hd = hv[4 * k] * dx + hv[4 * k + 1] * dy + hv[4 * k + 2] * dz;
+      const double c = cos(hd), sn = sin(hd), cf = hv[4 * k + 3];
+      p += cf * c;
+      fx += hv[4 * k] * (cf * sn);
+      fy += hv[4 * k + 1] * (cf * sn);
+      fz += hv[4 * k + 2] * (cf * sn);
+    }
+    ax[i] += fx;
+    ay[i] += fy;
+    az[i] += fz;
+    pot[i] += p;
+  }
+  free(hv);
+}
+
+void or_ewald(const double* x, const double* y, const double* z, int64_t n, double mass,
+              const double cm[3], const double q[6], double L, int n_replicas, double* ax,
+              double* ay, double* az, double* pot) {
+  or_ewald_general(x, y, z, n, mass, cm, q, L, n_replicas, 2.0, 2.6, 2.8, ax, ay, az, pot);
+}
+
 void or_kick_drift(double* x, double* y, double* z,
                    double* vx, double* vy, double* vz,
                    const double* ax, const double* ay, const double* az,

@@ -149,6 +149,19 @@ void or_lists_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const do
                    double theta, const double offset[3], int64_t* pp_count, int64_t* pc_count,
                    const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc);
 
+/* LeafEwaldFn + EwaldData::EwaldInit, quadrupole form (examples/Ewald.h,
+ * examples/Ewald.C): the lattice of the root's monopole + quadrupole beyond
+ * the explicit replicas, added to ax/ay/az/pot.  or_ewald uses the
+ * reference's constants (alpha = 2/L, real cut 2.6 L, Fourier cut 2.8);
+ * or_ewald_general takes them as parameters. */
+void or_ewald(const double* x, const double* y, const double* z, int64_t n, double mass,
+              const double cm[3], const double q[6], double L, int n_replicas, double* ax,
+              double* ay, double* az, double* pot);
+void or_ewald_general(const double* x, const double* y, const double* z, int64_t n,
+                      double mass, const double cm[3], const double q[6], double L,
+                      int n_replicas, double a_fac, double ew_cut, double h_cut, double* ax,
+                      double* ay, double* az, double* pot);
+
 /* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
 /* (da/dx)^T w and (da/dm)^T w of or_direct over all targets (test-only
  * reference for the adjoint). */

@@ -82,6 +82,8 @@ struct pt_ctx {
   DBuf<HexNode> hn;
   DBuf<HexOpen> ho;
   double gm = 0.0;  // GravityVisitor::monopole_gravity_factor
+  DBuf<double> ewt;  // Ewald Fourier table (periodic build)
+  DBuf<int32_t> ewcount;
   DBuf<unsigned long long> cpart;  // per-block counter partials
   DBuf<unsigned long long> wqueue;  // per-XCD work counters of the fused kernel
   bool lists_valid = false;
@@ -321,6 +323,8 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->flag); release(c->fpos); release(c->head); release(c->head_pos); release(c->u32_total);
   release(c->tstat);
   release(c->prec);
+  release(c->ewt);
+  release(c->ewcount);
   release(c->hraw); release(c->hn); release(c->ho);
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
@@ -749,6 +753,16 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
                                                    c->stream, acc));
       else
         break;
+    }
+    if (c->cfg.periodic) {
+      // LeafEwaldFn after the replicas (examples/Gravity.C:47-50)
+      Timer t(c, PT_T_FORCE);
+      PT_TRY_HIP(c, ensure(c->ewt, 4 * pt_ewald_table_entries()));
+      PT_TRY_HIP(c, ensure(c->ewcount, 1));
+      PT_TRY_LAUNCH(c, pt_launch_ewald(c->n, c->xs.p, c->ys.p, c->zs.p, c->nd.p, c->nn,
+                                       c->cfg.period[0], c->cfg.n_replicas, c->ewt.p,
+                                       c->ewcount.p, c->axs.p, c->ays.p, c->azs.p,
+                                       c->pots.p, c->stream));
     }
     // the last replica's counters, the unsort to caller order and the one
     // read-back; a rerun (stack overflow, rare) repeats every replica

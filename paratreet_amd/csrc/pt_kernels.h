@@ -129,6 +129,15 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
                       double* pot, unsigned long long* partials, int32_t* err,
                       unsigned long long* queue, bool accumulate, hipStream_t s);
 
+// Ewald lattice correction of the periodic build (quadrupole form) from the
+// root's moments (node 0 of nd), added to the sorted results.  table holds
+// 4 * pt_ewald_table_entries() doubles, count one int.
+int64_t pt_ewald_table_entries();
+int pt_launch_ewald(int64_t n, const double* x, const double* y, const double* z,
+                    const double* nd, int64_t nn, double L, int n_replicas, double* table,
+                    int32_t* count, double* ax, double* ay, double* az, double* pot,
+                    hipStream_t s);
+
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
                      double* ys, double* zs, double* ms, double* es, hipStream_t s);
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,

@@ -128,6 +128,137 @@ fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long lon
   if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Ewald sum of the periodic build, quadrupole form: LeafEwaldFn with
+// EwaldData::EwaldInit (examples/Ewald.h, examples/Ewald.C, non-HEXADECAPOLE
+// branches).  After the replica traversals every particle gets the rest of
+// the lattice of the root's monopole + quadrupole (mass M, centre cm, raw
+// second moments q): erfc-damped real-space terms over the lattice vectors
+// within 2.6 L (the explicit replicas -- the hole -- contribute -erf, which
+// removes the bare 1/r part the tree summed; a two-term series replaces the
+// closed forms below r^2 = 1.2e-3 L^2) and the Fourier terms of the integer
+// vectors 0 < |h| <= 2.8, with alpha = 2 / L, L = dxPeriod.  Same
+// operations, in the same order, as the oracle's restatement.
+// ---------------------------------------------------------------------------
+
+constexpr double kEwAlphaFac = 2.0;  // alpha = 2 / L
+constexpr double kEwCut = 2.6;       // real-space radius, in L
+constexpr double kEwHCut = 2.8;      // Fourier radius, in units of 2 pi / L
+constexpr double kPi = 3.14159265358979323846;
+
+// One thread: the Fourier table {2 pi h / L, C(h)} in the lattice order of the
+// loops (h_x outermost); C(h) = -(e^{-k4 h^2}/(pi h^2 L)) (M - (2 pi/L)^2 h.q.h/2)
+// (the sine coefficients vanish at quadrupole order).
+__global__ void ewald_table_kernel(const double* __restrict__ nd, int64_t nn, double L,
+                                   double* __restrict__ table, int32_t* __restrict__ count) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double mass = nd[PT_CD_MASS * nn];
+  double q[6];
+  for (int k = 0; k < 6; k++) q[k] = nd[(PT_CD_Q + k) * nn];
+  const double alpha = kEwAlphaFac / L, alpha2 = alpha * alpha;
+  const double k4 = kPi * kPi / (alpha2 * L * L);
+  const int hr = (int)ceil(kEwHCut);
+  int nh = 0;
+  for (int hx = -hr; hx <= hr; hx++)
+    for (int hy = -hr; hy <= hr; hy++)
+      for (int hz = -hr; hz <= hr; hz++) {
+        const int h2 = hx * hx + hy * hy + hz * hz;
+        if (h2 == 0 || h2 > kEwHCut * kEwHCut) continue;
+        const double g0 = exp(-k4 * h2) / (kPi * h2 * L);
+        const double w = 2.0 * kPi / L;
+        const double hqh = 0.5 * (hx * (q[0] * hx + q[1] * hy + q[2] * hz) +
+                                  hy * (q[1] * hx + q[3] * hy + q[4] * hz) +
+                                  hz * (q[2] * hx + q[4] * hy + q[5] * hz));
+        table[4 * nh + 0] = w * hx;
+        table[4 * nh + 1] = w * hy;
+        table[4 * nh + 2] = w * hz;
+        table[4 * nh + 3] = -(g0 * mass - w * w * g0 * hqh);
+        nh++;
+      }
+  *count = nh;
+}
+
+// One thread per particle (sorted order); adds to the accelerations and
+// potentials the replica traversals left.
+__global__ void __launch_bounds__(256)
+ewald_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+             const double* __restrict__ z, const double* __restrict__ nd, int64_t nn, double L,
+             int n_replicas, const double* __restrict__ table, const int32_t* __restrict__ count,
+             double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+             double* __restrict__ pot) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const double mass = nd[PT_CD_MASS * nn];
+  const double cm[3] = {nd[PT_CD_CM * nn], nd[(PT_CD_CM + 1) * nn], nd[(PT_CD_CM + 2) * nn]};
+  double q[6];
+  for (int k = 0; k < 6; k++) q[k] = nd[(PT_CD_Q + k) * nn];
+  const double alpha = kEwAlphaFac / L, alpha2 = alpha * alpha;
+  const double k1 = kPi / (alpha2 * L * L * L);
+  const double ka = 2.0 * alpha / sqrt(kPi);
+  const double q2 = 0.5 * (q[0] + q[3] + q[5]);
+  int nrep = (int)ceil(kEwCut);
+  if (n_replicas > nrep) nrep = n_replicas;
+  const double cut2 = kEwCut * kEwCut * L * L, inner2 = 1.2e-3 * L * L;
+  double p = mass * k1, fx = 0.0, fy = 0.0, fz = 0.0;
+  const double dx = x[i] - cm[0], dy = y[i] - cm[1], dz = z[i] - cm[2];
+  for (int ix = -nrep; ix <= nrep; ix++) {
+    const bool hole_x = ix >= -n_replicas && ix <= n_replicas;
+    const double rx = dx + ix * L;
+    for (int iy = -nrep; iy <= nrep; iy++) {
+      const bool hole_xy = hole_x && iy >= -n_replicas && iy <= n_replicas;
+      const double ry = dy + iy * L;
+      for (int iz = -nrep; iz <= nrep; iz++) {
+        const bool hole = hole_xy && iz >= -n_replicas && iz <= n_replicas;
+        const double rz = dz + iz * L;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > cut2 && !hole) continue;
+        double g0, g1, g2, g3;
+        if (r2 < inner2) {
+          const double s2 = r2 * alpha2;
+          double an = ka;
+          g0 = an * ((1.0 / 3.0) * s2 - 1.0);
+          an *= 2.0 * alpha2;
+          g1 = an * ((1.0 / 5.0) * s2 - (1.0 / 3.0));
+          an *= 2.0 * alpha2;
+          g2 = an * ((1.0 / 7.0) * s2 - (1.0 / 5.0));
+          an *= 2.0 * alpha2;
+          g3 = an * ((1.0 / 9.0) * s2 - (1.0 / 7.0));
+        } else {
+          const double r = sqrt(r2);
+          const double ir2 = 1.0 / r2;
+          const double a = exp(-r2 * alpha2) * ka * ir2;
+          g0 = (hole ? -erf(alpha * r) : erfc(alpha * r)) / r;
+          g1 = g0 * ir2 + a;
+          g2 = 3.0 * g1 * ir2 + 2.0 * alpha2 * a;
+          g3 = 5.0 * g2 * ir2 + 4.0 * alpha2 * alpha2 * a;
+        }
+        const double qx = q[0] * rx + q[1] * ry + q[2] * rz;
+        const double qy = q[1] * rx + q[3] * ry + q[4] * rz;
+        const double qz = q[2] * rx + q[4] * ry + q[5] * rz;
+        const double qrr = 0.5 * (qx * rx + qy * ry + qz * rz);
+        const double qta = g1 * mass - g2 * q2 + g3 * qrr;
+        p -= g0 * mass - g1 * q2 + g2 * qrr;
+        fx += g2 * qx - rx * qta;
+        fy += g2 * qy - ry * qta;
+        fz += g2 * qz - rz * qta;
+      }
+    }
+  }
+  const int nh = *count;
+  for (int k = 0; k < nh; k++) {
+    const double hd = table[4 * k] * dx + table[4 * k + 1] * dy + table[4 * k + 2] * dz;
+    const double c = cos(hd), sn = sin(hd), cf = table[4 * k + 3];
+    p += cf * c;
+    fx += table[4 * k] * (cf * sn);
+    fy += table[4 * k + 1] * (cf * sn);
+    fz += table[4 * k + 2] * (cf * sn);
+  }
+  ax[i] += fx;
+  ay[i] += fy;
+  az[i] += fz;
+  pot[i] += p;
+}
+
 }  // namespace
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
@@ -170,5 +301,20 @@ int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
   if (n <= 0) return 0;
   fill_u64_kernel<<<grid_for(n), 256, 0, s>>>(p, n, v);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Fourier table entries the Ewald kernel can hold: the integer vectors of a
+// 7^3 cube (the cut 2.8 keeps 80 of them).
+int64_t pt_ewald_table_entries() { return 343; }
+
+int pt_launch_ewald(int64_t n, const double* x, const double* y, const double* z,
+                    const double* nd, int64_t nn, double L, int n_replicas, double* table,
+                    int32_t* count, double* ax, double* ay, double* az, double* pot,
+                    hipStream_t s) {
+  if (n <= 0) return 0;
+  ewald_table_kernel<<<1, 64, 0, s>>>(nd, nn, L, table, count);
+  ewald_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, nd, nn, L, n_replicas,
+                                                           table, count, ax, ay, az, pot);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

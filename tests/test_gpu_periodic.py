@@ -1,8 +1,8 @@
-"""GPU parity of the periodic replica sum (ExMain::traversalFn with
-bPeriodic, examples/Gravity.C:35-46): pt_compute_forces sums the (2N+1)^3
-replica traversals; counters exact and FP64 forces within 1e-12 of the
-oracle's replica sum, FP32C within 1e-5 RMS of FP64, for the quadrupole and
-hexadecapole builds."""
+"""GPU parity of the periodic build (ExMain::traversalFn with bPeriodic,
+examples/Gravity.C:35-50): pt_compute_forces sums the (2N+1)^3 replica
+traversals and adds LeafEwaldFn's lattice correction (quadrupole form);
+counters exact and FP64 forces within 1e-12 of the oracle, FP32C within 1e-5
+RMS of FP64, for the quadrupole and hexadecapole builds."""
 import numpy as np
 import pytest
 
@@ -31,7 +31,7 @@ def test_periodic_replicas_match_oracle(hexa):
     e = np.full(n, 0.02)
     period = (2.0, 2.0, 2.0)
     P = O.Pipeline(pos, m, e, bucket=8, theta=0.7, hexadecapole=hexa)
-    oacc, opot, ocnt = P.gravity_replicas(1, period)
+    oacc, opot, ocnt = P.gravity_replicas(1, period, ewald=True)
     d = _driver(8, period, 1, hexa)
     d.load(pos, m, e)
     cnt = d.compute_forces(theta=0.7, precision=0)
@@ -53,7 +53,7 @@ def test_periodic_replicas_match_oracle(hexa):
     d.close()
 
 
-def test_periodic_zero_replicas_is_the_single_box():
+def test_periodic_zero_replicas_is_the_single_box_plus_ewald():
     pos, _, m = ics.plummer(3000, seed=9)
     e = np.full(3000, 0.01)
     d = _driver(12, (25.0, 25.0, 25.0), 0)
@@ -62,6 +62,6 @@ def test_periodic_zero_replicas_is_the_single_box():
     acc, _ = d.accelerations()
     d.close()
     P = O.Pipeline(pos, m, e, bucket=12, theta=0.7)
-    oacc, _, ocnt = P.gravity_unsorted()
+    oacc, _, ocnt = P.gravity_replicas(0, (25.0, 25.0, 25.0), ewald=True)
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
     assert rel_err(acc, oacc).max() <= FP64_TOL

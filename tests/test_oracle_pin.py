@@ -587,3 +587,92 @@ def test_replica_sum_matches_direct_sum_over_replicated_images():
     # the replicas add node() and leaf() calls to the single box's
     _, _, c0 = P.gravity()
     assert cnt[0] > c0[0] and cnt[1] > c0[1] and cnt[3] > c0[3]
+
+
+def _ewald_setup(seed=3, n=300):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0.0, 1.0, (n, 3))
+    pos[: n // 3] = 0.4 + 0.05 * rng.standard_normal((n // 3, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    M = m.sum()
+    cm = (m[:, None] * pos).sum(0) / M
+    d = pos - cm
+    q = np.array([(m * d[:, a] * d[:, b]).sum() for a, b in
+                  ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))])
+    return pos, M, cm, q
+
+
+@pytest.mark.parametrize("n_rep", [0, 1])
+def test_ewald_is_independent_of_the_split(n_rep):
+    """The Ewald split parameter alpha only moves terms between the real-space
+    (erfc / -erf) and Fourier sums: alpha = 2/L with the reference's cuts and
+    alpha = 3/L with wider cuts give the same correction (a wrong coefficient,
+    sign or hole rule in either sum breaks this)."""
+    pos, M, cm, q = _ewald_setup()
+    t = pos[::30]
+    a1, p1 = O.ewald(t, M, cm, q, 1.0, n_rep)
+    # (the two-term series below r^2 = 1.2e-3 L^2 is accurate to ~(alpha r)^4,
+    # so alpha stays near 2/L; the reference's cuts truncate at ~1e-7)
+    a2, p2 = O.ewald(t, M, cm, q, 1.0, n_rep, a_fac=2.0, ew_cut=4.0, h_cut=5.0)
+    a3, p3 = O.ewald(t, M, cm, q, 1.0, n_rep, a_fac=1.5, ew_cut=4.0, h_cut=3.0)
+    scale = np.abs(a1).max()
+    assert np.abs(a1 - a2).max() < 1e-6 * scale and np.abs(a1 - a3).max() < 1e-6 * scale
+    assert np.abs(p1 - p2).max() < 1e-6 * np.abs(p1).max()
+    assert np.abs(p1 - p3).max() < 1e-6 * np.abs(p1).max()
+
+
+def test_ewald_small_r_series_is_continuous():
+    """Below r^2 = 1.2e-3 L^2 from the root's centre the closed forms give way
+    to their two-term Taylor series; both sides of the switch agree to the
+    series' truncation, ~(alpha r)^4 of the leading terms (a wrong
+    coefficient would open a gap of order 1e-2)."""
+    _, M, cm, q = _ewald_setup()
+    r0 = np.sqrt(1.2e-3)
+    u = np.array([0.6, 0.48, 0.64])
+    pts = np.array([cm + u * r0 * (1 - 1e-7), cm + u * r0 * (1 + 1e-7)])
+    a, p = O.ewald(pts, M, cm, q, 1.0, 1)
+    assert np.abs(a[0] - a[1]).max() < 1e-5 * np.abs(a).max()
+    assert abs(p[0] - p[1]) < 1e-6 * abs(p[0])
+
+
+def test_ewald_of_a_lone_mass_is_symmetric():
+    """One mass in the box: the lattice of its images pulls it nowhere (each
+    image has a mirror image), and points mirrored through it feel opposite
+    forces."""
+    cm = np.array([0.3, 0.7, 0.5])
+    q = np.zeros(6)
+    a, _ = O.ewald(cm[None, :], 2.0, cm, q, 1.0, 1)
+    assert np.abs(a).max() < 1e-12
+    d = np.array([0.11, -0.07, 0.23])
+    a2, p2 = O.ewald(np.array([cm + d, cm - d]), 2.0, cm, q, 1.0, 1)
+    assert np.abs(a2[0] + a2[1]).max() < 1e-12 * np.abs(a2).max()
+    assert abs(p2[0] - p2[1]) < 1e-12 * abs(p2[0])
+
+
+def test_periodic_build_approaches_the_exact_periodic_force():
+    """Replicas (nReplicas = 1, tree at theta 0.1) + LeafEwaldFn's correction
+    from the root's monopole + quadrupole against the exact periodic force:
+    the replicated direct sum plus each source's own point-mass Ewald lattice.
+    The root expansion leaves ~1% (RMS 6e-3 here); the replicas alone miss by
+    a factor of a few, so the correction is what makes the sum periodic."""
+    rng = np.random.default_rng(4)
+    n, L = 64, 1.0
+    pos = rng.uniform(0, 1, (n, 3))
+    m = rng.uniform(0.5, 1.5, n) / n
+    e = np.full(n, 1e-7)
+    P = O.Pipeline(pos, m, e, bucket=4, theta=0.1)
+    acc, _, _ = P.gravity_replicas(1, (L, L, L), ewald=True)
+    acc0, _, _ = P.gravity_replicas(1, (L, L, L), ewald=False)
+    reps = [np.array([X, Y, Z], float) for X in (-1, 0, 1) for Y in (-1, 0, 1) for Z in (-1, 0, 1)]
+    allpos = np.concatenate([pos + r for r in reps])
+    exact = O.direct(allpos[:, 0], allpos[:, 1], allpos[:, 2], np.tile(m, 27), np.tile(e, 27),
+                     np.arange(13 * n, 14 * n))
+    for j in range(n):
+        a, _ = O.ewald(pos, m[j], pos[j], np.zeros(6), L, 1)
+        exact += a
+    rel = np.linalg.norm(acc - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    rel0 = np.linalg.norm(acc0 - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    assert np.sqrt(np.mean(rel ** 2)) < 2e-2 and rel.max() < 5e-2
+    assert np.sqrt(np.mean(rel0 ** 2)) > 0.5
+    # the exact periodic forces conserve momentum
+    assert np.abs((m[:, None] * exact).sum(0)).max() < 1e-12
