@@ -550,7 +550,7 @@ pt_status pt_upward(pt_ctx* c) {
                                           c->nd.p, c->stream));
   // internal nodes, deepest level first: a level of more than kRunNodes
   // nodes gets its own launch, consecutive smaller ones share one work-group
-  constexpr int64_t kRunNodes = 256;
+  constexpr int64_t kRunNodes = 64;
   const int nlev = (int)c->level_off.size() - 1;
   PtLevelRun run{};
   for (int d = nlev - 1; d >= 0; d--) {

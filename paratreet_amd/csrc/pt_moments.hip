@@ -48,20 +48,6 @@ __device__ __forceinline__ void cd_default(CData& d) {
   d.count = 0;
 }
 
-__device__ __forceinline__ void cd_load(CData& d, const double* __restrict__ nd, int64_t nn,
-                                        int64_t i, int32_t count) {
-  d.radius = nd[PT_CD_RADIUS * nn + i];
-  d.soft = nd[PT_CD_SOFT * nn + i];
-  d.mass = nd[PT_CD_MASS * nn + i];
-  for (int k = 0; k < 3; k++) {
-    d.cm[k] = nd[(PT_CD_CM + k) * nn + i];
-    d.lo[k] = nd[(PT_CD_LO + k) * nn + i];
-    d.hi[k] = nd[(PT_CD_HI + k) * nn + i];
-  }
-  for (int k = 0; k < 6; k++) d.q[k] = nd[(PT_CD_Q + k) * nn + i];
-  d.count = count;
-}
-
 __device__ __forceinline__ void cd_store(const CData& d, double* __restrict__ nd, int64_t nn,
                                          int64_t i) {
   nd[PT_CD_RADIUS * nn + i] = d.radius;
@@ -213,10 +199,33 @@ __device__ __forceinline__ uint64_t left_align(uint64_t k) {
 // One internal node (its children are final).  The children's pop order is
 // built in registers (fully unrolled loops over the 8 children, the order
 // packed as 8 nibbles) so nothing spills to scratch.
+// A child's CentroidData fields as the merge reads them: soft mass cm[3]
+// q[6] lo[3] hi[3] (its radius is not used: the merge recomputes it).
+constexpr int kUpFields = 17;
+constexpr int kUpThreads = 64;  // threads (= parents) per work-group of the upward pass
+
+// The thread's 8 children's fields are first loaded together into its LDS
+// slab (every load in flight at once), then merged from there.
 __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* __restrict__ nd,
                                             int64_t nn, int32_t* __restrict__ ord_h,
-                                            uint64_t* __restrict__ ord_b) {
+                                            uint64_t* __restrict__ ord_b,
+                                            double (*__restrict__ slab)[kUpFields]) {
   const int32_t fc = t.first_child[p];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    if (t.type[fc + c] == PT_NODE_EMPTY) continue;
+    const int64_t ch = fc + c;
+    slab[c][0] = nd[PT_CD_SOFT * nn + ch];
+    slab[c][1] = nd[PT_CD_MASS * nn + ch];
+#pragma unroll
+    for (int k = 0; k < 3; k++) slab[c][2 + k] = nd[(PT_CD_CM + k) * nn + ch];
+#pragma unroll
+    for (int k = 0; k < 6; k++) slab[c][5 + k] = nd[(PT_CD_Q + k) * nn + ch];
+#pragma unroll
+    for (int k = 0; k < 3; k++) slab[c][11 + k] = nd[(PT_CD_LO + k) * nn + ch];
+#pragma unroll
+    for (int k = 0; k < 3; k++) slab[c][14 + k] = nd[(PT_CD_HI + k) * nn + ch];
+  }
   int ctype[8];
   int32_t h[8];
   uint64_t b[8];
@@ -284,34 +293,45 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
   for (int j = 0; j < 8; j++) {
     const int c = (order >> (4 * j)) & 15;
     CData cd;
-    if (ctype[c] == PT_NODE_EMPTY) cd_default(cd);
-    else cd_load(cd, nd, nn, fc + c, t.count[fc + c]);
+    cd_default(cd);
+    if (ctype[c] != PT_NODE_EMPTY) {
+      const double* f = slab[c];
+      cd.soft = f[0];
+      cd.mass = f[1];
+      for (int k = 0; k < 3; k++) cd.cm[k] = f[2 + k];
+      for (int k = 0; k < 6; k++) cd.q[k] = f[5 + k];
+      for (int k = 0; k < 3; k++) cd.lo[k] = f[11 + k];
+      for (int k = 0; k < 3; k++) cd.hi[k] = f[14 + k];
+      cd.count = t.count[fc + c];
+    }
     cd_merge(d, cd);
   }
   cd_store(d, nd, nn, p);
 }
 
 // One thread per internal node of one level (deepest level first).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kUpThreads)
 upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
               double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
               uint64_t* __restrict__ ord_b) {
-  const int64_t li = blockIdx.x * 256ll + threadIdx.x;
+  __shared__ double slab[kUpThreads][8][kUpFields];
+  const int64_t li = (int64_t)blockIdx.x * kUpThreads + threadIdx.x;
   if (li >= nl) return;
-  upward_node(t, level_nodes[li], nd, nn, ord_h, ord_b);
+  upward_node(t, level_nodes[li], nd, nn, ord_h, ord_b, slab[threadIdx.x]);
 }
 
 // A run of consecutive small levels in one work-group, deepest first, with a
 // work-group barrier between levels (a level reads what the one below wrote;
 // all waves of the group share the CU's caches).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kUpThreads)
 upward_run_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, PtLevelRun run,
                   double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
                   uint64_t* __restrict__ ord_b) {
+  __shared__ double slab[kUpThreads][8][kUpFields];
   for (int k = 0; k < run.nlev; k++) {
     const int32_t off = run.off[k], cnt = run.cnt[k];
-    for (int32_t li = threadIdx.x; li < cnt; li += 256)
-      upward_node(t, level_nodes[off + li], nd, nn, ord_h, ord_b);
+    for (int32_t li = threadIdx.x; li < cnt; li += kUpThreads)
+      upward_node(t, level_nodes[off + li], nd, nn, ord_h, ord_b, slab[threadIdx.x]);
     __syncthreads();
   }
 }
@@ -607,14 +627,15 @@ int pt_launch_leaf_moments(const PtTree& t, int64_t nn, const double* x, const d
 int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLevelRun& run,
                          double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s) {
   if (run.nlev <= 0) return 0;
-  upward_run_kernel<<<1, 256, 0, s>>>(t, level_nodes, run, nd, nn, ord_h, ord_b);
+  upward_run_kernel<<<1, kUpThreads, 0, s>>>(t, level_nodes, run, nd, nn, ord_h, ord_b);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
                            int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s) {
   if (nl <= 0) return 0;
-  upward_kernel<<<grid_for(nl), 256, 0, s>>>(t, level_nodes, nl, nd, nn, ord_h, ord_b);
+  upward_kernel<<<(unsigned)((nl + kUpThreads - 1) / kUpThreads), kUpThreads, 0, s>>>(
+      t, level_nodes, nl, nd, nn, ord_h, ord_b);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
