@@ -684,8 +684,9 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
 
 // Flattens up to 64 leaf sources (begin/count in the calling lane, count 0 =
 // none) into particles and evaluates them 64 at a time.  Each window marks
-// the slot where every leaf starts and an inclusive max-scan over the 64
-// slots gives each slot its leaf.
+// the slot where every leaf starts (the leaf's lane in LDS, a bit in a
+// ballot); a slot's leaf is the mark at the highest start bit at or below
+// it, or the previous window's last leaf when there is none.
 template <bool F32, typename W_t>
 __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, const Group& g,
                                           Lanes<F32>& L, const double* __restrict__ px,
@@ -705,14 +706,10 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
     __builtin_amdgcn_wave_barrier();
     if (cnt > 0 && exc >= s0 && exc < s0 + PT_WAVE) W.fl_mark[exc - s0] = lane;
     __builtin_amdgcn_wave_barrier();
-    int leaf = W.fl_mark[lane];
-#pragma unroll
-    for (int d = 1; d < PT_WAVE; d <<= 1) {
-      const int o = __shfl_up(leaf, d, PT_WAVE);
-      if (lane >= d) leaf = leaf > o ? leaf : o;
-    }
-    leaf = leaf > carry ? leaf : carry;
-    carry = __shfl(leaf, PT_WAVE - 1, PT_WAVE);
+    const unsigned long long starts = __ballot(W.fl_mark[lane] >= 0);
+    const unsigned long long upto = starts & (~0ull >> (PT_WAVE - 1 - lane));
+    const int leaf = upto ? W.fl_mark[PT_WAVE - 1 - __clzll(upto)] : carry;
+    carry = __builtin_amdgcn_readlane(leaf, PT_WAVE - 1);
     const int lbeg = __shfl(beg, leaf, PT_WAVE);
     const int lexc = __shfl(exc, leaf, PT_WAVE);
     const int slot = s0 + lane;
