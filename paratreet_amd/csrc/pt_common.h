@@ -88,14 +88,23 @@ struct PtCounters {
 __device__ __forceinline__ int pt_lane() { return __lane_id(); }
 
 // Wave-wide inclusive prefix sum of a 32-bit value.
+// Inclusive prefix sum over the 64 lanes: shifts within each row of 16 lanes
+// (row_shr 1, 2, 4, 8), then each row's total broadcast into the rows above
+// (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3).  DPP
+// moves, no LDS round trips; every lane must be active.
 __device__ __forceinline__ int pt_wave_incl_scan(int v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < PT_WAVE; d <<= 1) {
-    const int o = __shfl_up(v, d, PT_WAVE);
-    if (lane >= d) v += o;
-  }
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
   return v;
+}
+
+// Sum over the 64 lanes (all active), as a wave-uniform value.
+__device__ __forceinline__ int pt_wave_sum_int(int v) {
+  return __builtin_amdgcn_readlane(pt_wave_incl_scan(v), PT_WAVE - 1);
 }
 
 __device__ __forceinline__ double pt_wave_sum(double v) {

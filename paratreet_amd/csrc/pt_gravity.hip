@@ -698,7 +698,7 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
                                           double ccz) {
   const int lane = __lane_id();
   const int incl = pt_wave_incl_scan(cnt);
-  const int total = uniform_if<!F32>(__shfl(incl, PT_WAVE - 1, PT_WAVE));
+  const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
   const int exc = incl - cnt;
   int carry = 0;
   for (int s0 = 0; s0 < total; s0 += PT_WAVE) {

@@ -221,12 +221,9 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   if (op) stack[sp + pt_rank_in(openm)] = fc;
   __builtin_amdgcn_wave_barrier();
   sp += n_open;
-  int lc = r.is_leaf ? r.count : 0;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) lc += __shfl_xor(lc, d, PT_WAVE);
   wc.n_open = n_open;
   wc.n_close = __popcll(closem);
-  wc.leaf_parts = __builtin_amdgcn_readfirstlane(lc);
+  wc.leaf_parts = pt_wave_sum_int(r.is_leaf ? r.count : 0);
   return r;
 }
 
