@@ -267,7 +267,7 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         beg = nd.begin;
       }
       const int incl = pt_wave_incl_scan(cnt);
-      const int total = __shfl(incl, PT_WAVE - 1, PT_WAVE);
+      const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
       __builtin_amdgcn_wave_barrier();
       s_beg[wave][lane] = beg;
       s_exc[wave][lane] = incl - cnt;

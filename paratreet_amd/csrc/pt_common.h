@@ -102,6 +102,20 @@ __device__ __forceinline__ int pt_wave_incl_scan(int v) {
   return v;
 }
 
+// Maximum over the 64 lanes (all active), as a wave-uniform value: the same
+// row shifts and broadcasts as pt_wave_incl_scan with max, lanes without a
+// source reading -infinity.
+__device__ __forceinline__ float pt_wave_max_f32(float v) {
+  const int ninf = __float_as_int(-INFINITY);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x111, 0xf, 0xf, false)));
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x112, 0xf, 0xf, false)));
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x114, 0xf, 0xf, false)));
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x118, 0xf, 0xf, false)));
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x142, 0xa, 0xf, false)));
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(ninf, __float_as_int(v), 0x143, 0xc, 0xf, false)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), PT_WAVE - 1));
+}
+
 // Sum over the 64 lanes (all active), as a wave-uniform value.
 __device__ __forceinline__ int pt_wave_sum_int(int v) {
   return __builtin_amdgcn_readlane(pt_wave_incl_scan(v), PT_WAVE - 1);

@@ -254,11 +254,6 @@ __device__ __forceinline__ float frame32(double v, double o, double cc) {
   return (float)((v + o) - cc);
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, PT_WAVE));
-  return v;
-}
 
 struct Group {
   int ng;      // targets in this pass
@@ -322,7 +317,7 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
     L.te = pe[i];
   }
   L.ax = L.ay = L.az = L.ap = 0;
-  L.emax = wave_max((float)L.te);
+  L.emax = pt_wave_max_f32((float)L.te);
 }
 
 // ---- staging (one lane per source) ----------------------------------------
