@@ -87,7 +87,6 @@ struct PtCounters {
 
 __device__ __forceinline__ int pt_lane() { return __lane_id(); }
 
-// Wave-wide inclusive prefix sum of a 32-bit value.
 // Inclusive prefix sum over the 64 lanes: shifts within each row of 16 lanes
 // (row_shr 1, 2, 4, 8), then each row's total broadcast into the rows above
 // (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3).  DPP
