@@ -864,8 +864,16 @@ struct SrcH {
   double h[PT_HEXA_MOMS];
 };
 
+// Closed nodes are staged kHexaStage at a time (half a chunk), which keeps a
+// wave's LDS near 9 KiB, and the fused kernel is held to kHexaWaves waves per
+// SIMD (168 VGPRs, a small spill): 1M Plummer, theta 0.7, 13.8 ms against
+// 17.7 ms with a 64-entry stage at two waves per SIMD (four waves spill 264
+// bytes per lane and take 21.5 ms).
+constexpr int kHexaStage = 32;
+constexpr int kHexaWaves = 3;
+
 struct __align__(16) HexaLds {
-  static constexpr int kStage = PT_WAVE * sizeof(SrcH);
+  static constexpr int kStage = kHexaStage * sizeof(SrcH);
   static_assert(kSlots * sizeof(SrcP64) <= kStage, "particle chunk");
   static_assert(64 * 4 * sizeof(double) <= kStage, "reduction area");
   unsigned char stage[kStage];
@@ -958,33 +966,38 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
   }
 }
 
-// Stages up to 64 closed nodes (one per lane) and evaluates them for the
-// group's targets (slot s of a lane takes sources s, s + S, ...).
+// Stages up to 64 closed nodes (node id in the calling lane), kHexaStage at a
+// time, and evaluates them for the group's targets (slot s of a lane takes
+// sources s, s + S, ... of each part; the S partial sums are combined in
+// finish_group).
 template <typename W_t>
 __device__ __forceinline__ void do_nodes_hexa(W_t& W, int32_t id, int nsrc, const Group& g,
                                               Lanes<false>& L, const HexNode* __restrict__ hn,
                                               const Bucket& b, double ox, double oy, double oz) {
   const int lane = __lane_id();
   SrcH* st = reinterpret_cast<SrcH*>(W.stage);
-  if (lane < nsrc) {
-    const HexNode& s = hn[id];
-    SrcH r;
-    r.cx = s.cx;
-    r.cy = s.cy;
-    r.cz = s.cz;
-    r.mass = s.mass;
-    r.mono = pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, b) ? 1.0 : 0.0;
+  for (int h = 0; h < nsrc; h += kHexaStage) {
+    if (lane >= h && lane < h + kHexaStage && lane < nsrc) {
+      const HexNode& s = hn[id];
+      SrcH r;
+      r.cx = s.cx;
+      r.cy = s.cy;
+      r.cz = s.cz;
+      r.mass = s.mass;
+      r.mono = pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, b) ? 1.0 : 0.0;
 #pragma unroll
-    for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = s.h[k];
-    st[lane] = r;
+      for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = s.h[k];
+      st[lane - h] = r;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int n = (nsrc - h) < kHexaStage ? (nsrc - h) : kHexaStage;
+    const int kend = ceil_div_s(n, g);
+    for (int k = 0; k < kend; k++) {
+      const int i = g.s + k * g.S;
+      if (i < n) hexa_pair(st[i], L, ox, oy, oz);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-  __builtin_amdgcn_wave_barrier();
-  const int kend = ceil_div_s(nsrc, g);
-  for (int k = 0; k < kend; k++) {
-    const int i = g.s + k * g.S;
-    if (i < nsrc) hexa_pair(st[i], L, ox, oy, oz);
-  }
-  __builtin_amdgcn_wave_barrier();
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -1047,7 +1060,7 @@ struct __align__(16) HexaFusedLds {
 };
 
 template <int STACK>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(STACK == kSmallStack ? kHexaWaves : 1, 8)))
 hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
                     const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn,
                     const double* __restrict__ px, const double* __restrict__ py,
