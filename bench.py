@@ -113,9 +113,11 @@ ADJOINT_CONFIGS = {
 
 
 def _loss_grad(d, pos, vel, m, eps, steps, dt, theta, precision, mode, every):
-    """Forward + reverse sweep of L = sum m |x_T - x_0|^2; returns the
-    gradients (x0, v0, m) and the (forward, adjoint) wall times in ms."""
+    """Forward + reverse sweep of L = sum m |x_T - x_0|^2 with the adjoint in
+    the forward's precision; returns the gradients (x0, v0, m) and the
+    (forward, adjoint) wall times in ms."""
     import torch
+    d.set_adjoint_precision(precision)
     d.load(pos, m, eps, vel)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -154,13 +156,17 @@ def adjoint_bench(name, fd_coords, direct):
     hbm = free0 - torch.cuda.mem_get_info()[0]
     out = {"workload": name, "n": n, "bucket": bucket, "theta": theta, "eps": eps0,
            "steps": steps, "dt": dt, "checkpoint_every": every or steps,
-           "forward_precision": prec, "adjoint_arithmetic": "f64",
+           "forward_precision": prec,
+           "adjoint_arithmetic": "f32 pairs, f64 sums" if prec == "fp32c" else "f64",
            "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "device_bytes_in_use": int(hbm)}
     if precision == PT_FP32C:
-        g64, _, _ = _loss_grad(d, pos, vel, m, eps, steps, dt, theta, PT_FP64, FORCE_TREE, every)
+        # the same trajectory with FP64 forward and FP64 adjoint
+        g64, f64_ms, a64_ms = _loss_grad(d, pos, vel, m, eps, steps, dt, theta, PT_FP64,
+                                         FORCE_TREE, every)
         out["fp32c_vs_fp64_gradient_rel_l2"] = {
             k: float(np.linalg.norm(a - b) / np.linalg.norm(b))
             for k, a, b in zip(("x0", "v0", "m"), g, g64)}
+        out["fp64_ms"] = {"forward": f64_ms, "adjoint": a64_ms}
     if precision == PT_FP32C:
         # FP32 force rounding (~1e-7 of |a| per step) swamps an h = 1e-6
         # difference of the loss; the FD check belongs to the FP64 workload

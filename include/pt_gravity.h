@@ -187,6 +187,13 @@ pt_status pt_direct_sum(pt_ctx* ctx, const int64_t* targets, int64_t nt, double*
 pt_status pt_force_vjp(pt_ctx* ctx, double theta, pt_force_mode mode, const double* w,
                        double* grad_x, double* grad_m, pt_memory where);
 
+/* Arithmetic of the tree-mode adjoint in pt_force_vjp and pt_leapfrog_adjoint
+ * (default PT_FP64).  PT_FP32C evaluates the pair expressions in FP32 on
+ * coordinates shifted to each bucket's box centre, with every sum over pairs
+ * carried in FP64 (the form of the compensated forward).  The direct-sum
+ * adjoint is always FP64. */
+pt_status pt_set_adjoint_precision(pt_ctx* ctx, pt_precision precision);
+
 /* pt_run (Driver::run, src/Driver.h:183-288, with Partition::kick/perturb,
  * src/Particle.C:7-21) from the current state, keeping a checkpoint of
  * positions and velocities every `checkpoint_every` steps (<= 0: one
@@ -201,7 +208,8 @@ pt_status pt_leapfrog_forward(pt_ctx* ctx, int32_t steps, double dt, double thet
  * zero) returns the gradients with respect to the initial positions,
  * velocities (n x 3) and masses (n).  Steps between checkpoints are
  * recomputed with the forward's force model and precision; the adjoint
- * arithmetic is FP64.  The final state is restored afterwards. */
+ * arithmetic is the context's adjoint precision (pt_set_adjoint_precision).
+ * The final state is restored afterwards. */
 pt_status pt_leapfrog_adjoint(pt_ctx* ctx, const double* lam_x, const double* lam_v,
                               double* grad_x, double* grad_v, double* grad_m, pt_memory where);
 

@@ -28,6 +28,7 @@ EXPORTS = (
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
     "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint", "pt_get_node_hexadecapole",
+    "pt_set_adjoint_precision",
 )
 
 
@@ -136,6 +137,7 @@ def load():
         "pt_set_timing": (I32, [P, ctypes.c_int]),
         "pt_get_timings": (I32, [P, P]),
         "pt_force_vjp": (I32, [P, D, ctypes.c_int, P, P, P, ctypes.c_int]),
+        "pt_set_adjoint_precision": (I32, [P, ctypes.c_int]),
         "pt_leapfrog_forward": (I32, [P, I32, D, D, ctypes.c_int, ctypes.c_int, I32, P]),
         "pt_leapfrog_adjoint": (I32, [P, P, P, P, P, P, ctypes.c_int]),
     }

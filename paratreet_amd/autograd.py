@@ -51,7 +51,8 @@ class LeapfrogFunction(torch.autograd.Function):
     backward is pt_leapfrog_adjoint on the same Driver."""
 
     @staticmethod
-    def forward(ctx, x0, v0, m, soft, driver, steps, dt, theta, precision, mode, every):
+    def forward(ctx, x0, v0, m, soft, driver, steps, dt, theta, precision, mode, every,
+                adjoint_precision):
         device = x0.device
         if device.type != "cuda":
             raise ValueError("trajectory tensors must be on a GPU device")
@@ -66,6 +67,7 @@ class LeapfrogFunction(torch.autograd.Function):
         driver.load_device(n, xs[0].data_ptr(), xs[1].data_ptr(), xs[2].data_ptr(),
                            md.data_ptr(), None if sd is None else sd.data_ptr(),
                            vs[0].data_ptr(), vs[1].data_ptr(), vs[2].data_ptr())
+        driver.set_adjoint_precision(adjoint_precision)
         driver.leapfrog_forward(steps, dt, theta, precision, mode, every)
         view = driver.device_view()
         out = torch.empty((6, n), dtype=torch.float64, device=device)
@@ -95,16 +97,20 @@ class LeapfrogFunction(torch.autograd.Function):
             driver.ctx, None if lx is None else lx.data_ptr(),
             None if lv is None else lv.data_ptr(), grad_x.data_ptr(), grad_v.data_ptr(),
             grad_m.data_ptr(), _lib.PT_DEVICE))
-        return grad_x, grad_v, grad_m, None, None, None, None, None, None, None, None
+        return grad_x, grad_v, grad_m, None, None, None, None, None, None, None, None, None
 
 
 def trajectory(x0, v0, m, soft=None, *, steps, dt, theta=0.7, precision="fp64", mode="tree",
-               checkpoint_every=0, driver=None, bucket=12):
+               checkpoint_every=0, driver=None, bucket=12, adjoint_precision=None):
     """Final (positions, velocities) after `steps` kick-drift steps; both
     differentiable with respect to x0, v0 and m.  `driver` (a gravity.Driver
-    on the tensors' GPU) is created with bucket size `bucket` if not given."""
+    on the tensors' GPU) is created with bucket size `bucket` if not given.
+    The backward's arithmetic is `adjoint_precision` ("fp64" or "fp32c"),
+    by default the forward's `precision`."""
     if driver is None:
         index = x0.device.index if x0.device.index is not None else 0
         driver = Driver(Configuration(max_particles_per_leaf=bucket), device=index)
+    adj = precision if adjoint_precision is None else adjoint_precision
     return LeapfrogFunction.apply(x0, v0, m, soft, driver, int(steps), float(dt), float(theta),
-                                  _PRECISIONS[precision], _MODES[mode], int(checkpoint_every))
+                                  _PRECISIONS[precision], _MODES[mode], int(checkpoint_every),
+                                  _PRECISIONS[adj])

@@ -1,4 +1,5 @@
-// Adjoint (vector-Jacobian products) of the gravity interactions, FP64.
+// Adjoint (vector-Jacobian products) of the gravity interactions, FP64 and
+// FP32 (shifted coordinates, FP64 sums).
 //
 // For a cotangent w_i on every particle's acceleration these kernels compute
 //   g_j  = sum_i (da_i/dx_j)^T w_i      (positions)
@@ -339,6 +340,304 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 form: the same pair expressions in FP32 on coordinates shifted to the
+// bucket's box centre (in FP64, then rounded), with the hardware reciprocal
+// and reciprocal square root; every sum that spans pairs (target gradients,
+// node and source-particle cotangents) is carried in FP64.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// spline_b_db in FP32; ir = 1 / r.
+__device__ __forceinline__ void spline_b_db32(float r, float ir, float twoh, float& b, float& db) {
+  if (r < twoh) {
+    const float dih = 2.0f * rcp32(twoh);
+    const float u = r * dih;
+    const float dih3 = dih * dih * dih;
+    if (u < 1.0f) {
+      b = dih3 * (4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u);
+      db = dih3 * dih * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
+    } else {
+      b = -1.0f / 15.0f * ir * ir * ir +
+          dih3 * (8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u - 1.0f / 6.0f * u * u * u);
+      db = 1.0f / 5.0f * ir * ir * ir * ir +
+           dih3 * dih * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
+    }
+  } else {
+    b = ir * ir * ir;
+    db = -3.0f * b * ir;
+  }
+}
+
+struct SplineQ32 {
+  float b, c, d, br, cr, dr_, bt, ct, dt_;
+};
+
+// splineq_grad in FP32; ir = 1 / r.
+__device__ __forceinline__ SplineQ32 splineq_grad32(float r, float ir, float r2, float twoh) {
+  SplineQ32 s;
+  const float dir = ir;
+  if (r2 < twoh * twoh) {
+    const float dih = 2.0f * rcp32(twoh);
+    const float u = dih * r;
+    const float dih2 = dih * dih, dih3 = dih2 * dih, dih4 = dih3 * dih, dih5 = dih4 * dih,
+                dih6 = dih5 * dih;
+    float bD, cD, dD;
+    if (u < 1.0f) {
+      s.b = dih3 * 4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u;
+      s.c = dih5 * 12.0f / 5.0f - 3.0f / 2.0f * u;
+      s.d = 3.0f / 2.0f * dih6 * dir;
+      s.br = dih * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
+      s.cr = -3.0f / 2.0f * dih;
+      s.dr_ = -3.0f / 2.0f * dih6 * dir * dir;
+      bD = 4.0f * dih2 + r * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
+      cD = 12.0f * dih4 - 3.0f / 2.0f * r;
+      dD = 9.0f * dih5 * dir;
+    } else {
+      const float dir2 = dir * dir, dir3 = dir2 * dir;
+      s.b = -1.0f / 15.0f * dir3 + dih3 * 8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u -
+            1.0f / 6.0f * u * u * u;
+      s.c = -1.0f / 5.0f * dir3 * dir2 + 3.0f * dih4 * dir - dih5 * 12.0f / 5.0f + 1.0f / 2.0f * u;
+      s.d = -dir3 * dir3 * dir + 3.0f * dih4 * dir3 - 1.0f / 2.0f * dih6 * dir;
+      s.br = 1.0f / 5.0f * dir2 * dir2 + dih * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
+      s.cr = dir3 * dir3 - 3.0f * dih4 * dir2 + 1.0f / 2.0f * dih;
+      s.dr_ = 7.0f * dir3 * dir3 * dir2 - 9.0f * dih4 * dir2 * dir2 + 1.0f / 2.0f * dih6 * dir2;
+      bD = 8.0f * dih2 + r * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
+      cD = 12.0f * dih3 * dir - 12.0f * dih4 + 1.0f / 2.0f * r;
+      dD = 12.0f * dih3 * dir3 - 3.0f * dih5 * dir;
+    }
+    const float dd_dt = -0.5f * dih2;
+    s.bt = bD * dd_dt;
+    s.ct = cD * dd_dt;
+    s.dt_ = dD * dd_dt;
+  } else {
+    const float a = dir;
+    s.b = a * a * a;
+    s.c = 3.0f * s.b * a * a;
+    s.d = 5.0f * s.c * a * a;
+    s.br = -3.0f * s.b * a;
+    s.cr = -5.0f * s.c * a;
+    s.dr_ = -7.0f * s.d * a;
+    s.bt = s.ct = s.dt_ = 0.0f;
+  }
+  return s;
+}
+
+struct TgtA32 {
+  float x, y, z, e, wx, wy, wz, pad;
+};
+
+// adjoint_list_kernel in the FP32 form (same work mapping; node moments from
+// the FP32 copies, node centres and softening decisions from FP64).
+__global__ void __launch_bounds__(kAdjThreads)
+adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
+                      const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
+                      const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+                      const double* __restrict__ py, const double* __restrict__ pz,
+                      const double* __restrict__ pm, const double* __restrict__ pe,
+                      const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
+                      const int64_t* __restrict__ pc_off, const int32_t* __restrict__ pc,
+                      double ox, double oy, double oz, const double* __restrict__ wx,
+                      const double* __restrict__ wy, const double* __restrict__ wz,
+                      double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
+                      double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn,
+                      double* __restrict__ hot, int64_t nhot) {
+  __shared__ TgtA32 s_t[kAdjWaves][kANT];
+  __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
+  __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t bi = (int64_t)blockIdx.x * kAdjWaves + wave;
+  if (bi >= nb) return;
+  const Bucket b = bk[bi];
+  const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+               ccz = (b.loz + b.hiz) * 0.5;
+  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
+  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
+  for (int g0 = 0; g0 < b.count; g0 += kANT) {
+    const int ng = (b.count - g0) < kANT ? (b.count - g0) : kANT;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < ng) {
+      const int64_t i = (int64_t)b.begin + g0 + lane;
+      s_t[wave][lane] = TgtA32{(float)(px[i] - ccx), (float)(py[i] - ccy), (float)(pz[i] - ccz),
+                               (float)pe[i], (float)wx[i], (float)wy[i], (float)wz[i], 0.0f};
+    }
+    __builtin_amdgcn_wave_barrier();
+    double tg[kANT][3];
+#pragma unroll
+    for (int t = 0; t < kANT; t++) tg[t][0] = tg[t][1] = tg[t][2] = 0.0;
+
+    // ---- node() sources
+    for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
+      const int64_t e = e0 + lane;
+      if (e < pc1) {
+        const int32_t sid = pc[e];
+        const GravNode s64 = gn[sid];
+        const GravNode32 s = gn32[sid];
+        const double cxd = s64.cx + ox, cyd = s64.cy + oy, czd = s64.cz + oz;
+        const bool mono = pt_open_softening(cxd, cyd, czd, s64.soft, b);
+        const float cx = (float)(cxd - ccx), cy = (float)(cyd - ccy), cz = (float)(czd - ccz);
+        const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
+        float lam[kNodeAdj];
+#pragma unroll
+        for (int k = 0; k < kNodeAdj; k++) lam[k] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < kANT; t++) {
+          if (t < ng) {
+            const TgtA32 tt = s_t[wave][t];
+            if (mono) {
+              const float dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 != 0.0f) {
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float ir3 = ir * ir * ir;
+                const float dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+                const float f = -3.0f * s.mass * ir3 * ir * ir * dw;
+                const float Tx = s.mass * ir3 * tt.wx + f * dx;
+                const float Ty = s.mass * ir3 * tt.wy + f * dy;
+                const float Tz = s.mass * ir3 * tt.wz + f * dz;
+                tg[t][0] -= (double)Tx;
+                tg[t][1] -= (double)Ty;
+                tg[t][2] -= (double)Tz;
+                lam[0] += Tx;
+                lam[1] += Ty;
+                lam[2] += Tz;
+                lam[3] += dw * ir3;
+              }
+            } else {
+              const float rx = tt.x - cx, ry = tt.y - cy, rz = tt.z - cz;
+              const float r2 = rx * rx + ry * ry + rz * rz;
+              const float ir = __builtin_amdgcn_rsqf(r2);
+              const float r = r2 * ir;
+              const SplineQ32 q = splineq_grad32(r, ir, r2, s.soft + tt.e);
+              const float qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
+              const float qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
+              const float qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
+              const float qir = 0.5f * (qx * rx + qy * ry + qz * rz);
+              const float qir3 = q.b * s.mass + q.d * qir - q.c * tr;
+              const float rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
+              const float qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
+              const float Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
+              const float Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
+              const float Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
+              const float k1 = (s.mass * q.br + qir * q.dr_ - tr * q.cr) * ir;
+              const float k2 = q.cr * ir;
+              const float Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
+                               (k2 * qw) * rx;
+              const float Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
+                               (k2 * qw) * ry;
+              const float Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
+                               (k2 * qw) * rz;
+              tg[t][0] += (double)Jx;
+              tg[t][1] += (double)Jy;
+              tg[t][2] += (double)Jz;
+              lam[0] -= Jx;
+              lam[1] -= Jy;
+              lam[2] -= Jz;
+              lam[3] -= q.b * rw;
+              const float hd = -0.5f * q.d * rw, hc = 0.5f * q.c * rw;
+              lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
+              lam[5] += 2.0f * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
+              lam[6] += 2.0f * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
+              lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
+              lam[8] += 2.0f * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
+              lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
+              lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * tr) * rw + q.ct * qw;
+            }
+          }
+        }
+        double* dst = sid < nhot
+                          ? hot + (int64_t)(blockIdx.x % kHotParts) * kNodeAdj * nhot + sid
+                          : lam_node + sid;
+        const int64_t stride = sid < nhot ? nhot : nn;
+#pragma unroll
+        for (int k = 0; k < kNodeAdj; k++)
+          if (lam[k] != 0.0f) atomicAdd(dst + (int64_t)k * stride, (double)lam[k]);
+      }
+    }
+
+    // ---- leaf() sources, flattened to particles
+    for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
+      const int64_t l = l0 + lane;
+      int32_t cnt = 0, beg = 0;
+      if (l < pp1) {
+        const TreeNode nd = nodes[pp[l]];
+        cnt = nd.count;
+        beg = nd.begin;
+      }
+      const int incl = pt_wave_incl_scan(cnt);
+      const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
+      __builtin_amdgcn_wave_barrier();
+      s_beg[wave][lane] = beg;
+      s_exc[wave][lane] = incl - cnt;
+      __builtin_amdgcn_wave_barrier();
+      for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+        const int slot = s0 + lane;
+        if (slot < total) {
+          int lo = 0, hi = PT_WAVE - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_exc[wave][mid] <= slot) lo = mid;
+            else hi = mid - 1;
+          }
+          const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
+          const float jx = (float)((px[j] + ox) - ccx), jy = (float)((py[j] + oy) - ccy),
+                      jz = (float)((pz[j] + oz) - ccz);
+          const float jm = (float)pm[j], je = (float)pe[j];
+          float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sgm = 0.0f;
+#pragma unroll
+          for (int t = 0; t < kANT; t++) {
+            if (t < ng) {
+              const TgtA32 tt = s_t[wave][t];
+              const float dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 != 0.0f) {
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * ir;
+                float bb, db;
+                spline_b_db32(r, ir, je + tt.e, bb, db);
+                const float dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+                const float f = db * ir * dw;
+                const float Kx = jm * (bb * tt.wx + f * dx);
+                const float Ky = jm * (bb * tt.wy + f * dy);
+                const float Kz = jm * (bb * tt.wz + f * dz);
+                tg[t][0] -= (double)Kx;
+                tg[t][1] -= (double)Ky;
+                tg[t][2] -= (double)Kz;
+                sgx += Kx;
+                sgy += Ky;
+                sgz += Kz;
+                sgm += bb * dw;
+              }
+            }
+          }
+          atomicAdd(&gx[j], (double)sgx);
+          atomicAdd(&gy[j], (double)sgy);
+          atomicAdd(&gz[j], (double)sgz);
+          atomicAdd(&gmass[j], (double)sgm);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+
+#pragma unroll
+    for (int t = 0; t < kANT; t++) {
+      if (t < ng) {
+        const double rx = pt_wave_sum(tg[t][0]);
+        const double ry = pt_wave_sum(tg[t][1]);
+        const double rz = pt_wave_sum(tg[t][2]);
+        if (lane == t) {
+          const int64_t i = (int64_t)b.begin + g0 + t;
+          atomicAdd(&gx[i], rx);
+          atomicAdd(&gy[i], ry);
+          atomicAdd(&gz[i], rz);
+        }
+      }
+    }
+  }
+}
+
 // Pushes node cotangents to every particle through the closed-form moments of
 // its leaf and each ancestor (parent links up to the root).  head/head_pos
 // give the particle's bucket (head_pos[j] + head[j] - 1).
@@ -568,9 +867,9 @@ int64_t pt_adjoint_node_scratch(int64_t nn) {
 
 int pt_node_adjoint_fields() { return kNodeAdj; }
 
-int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                            const GravNode* gn, const double* x, const double* y,
-                            const double* z, const double* m, const double* e,
+int pt_launch_adjoint_lists(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const GravNode* gn, const GravNode32* gn32, const double* x,
+                            const double* y, const double* z, const double* m, const double* e,
                             const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
                             const int32_t* pc, const double off[3], const double* wx,
                             const double* wy, const double* wz, double* gx, double* gy,
@@ -580,9 +879,15 @@ int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
   // the node cotangents, then the private copies of the hot ones
   const int64_t nhot = std::min<int64_t>(nn, kHotNodes);
   double* hot = lam_node + (int64_t)kNodeAdj * nn;
-  adjoint_list_kernel<<<(unsigned)((nb + kAdjWaves - 1) / kAdjWaves), kAdjThreads, 0, s>>>(
-      bk, nb, nodes, gn, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2], wx, wy,
-      wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
+  const unsigned grid = (unsigned)((nb + kAdjWaves - 1) / kAdjWaves);
+  if (precision != 0)  // the compensated-FP32 precision
+    adjoint_list32_kernel<<<grid, kAdjThreads, 0, s>>>(
+        bk, nb, nodes, gn, gn32, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2],
+        wx, wy, wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
+  else
+    adjoint_list_kernel<<<grid, kAdjThreads, 0, s>>>(
+        bk, nb, nodes, gn, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2], wx, wy,
+        wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
   hot_reduce_kernel<<<grid_for(nhot * kNodeAdj), 256, 0, s>>>(hot, nhot, lam_node, nn);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

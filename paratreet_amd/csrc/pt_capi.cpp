@@ -104,6 +104,7 @@ struct pt_ctx {
   bool traj_valid = false;
   int32_t traj_steps = 0, traj_every = 1;
   int traj_mode = 0, traj_prec = 0;
+  int adj_prec = PT_FP64;  // arithmetic of the tree adjoint (pt_set_adjoint_precision)
   double traj_dt = 0.0, traj_theta = 0.7;
   int64_t traj_n = 0;
   // scratch
@@ -990,7 +991,8 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
   if ((st = zero(c, c->lam_node.p, nscr))) return st;
   if ((st = zero(c, v.gsx, 4 * std::max<int64_t>(n, 1)))) return st;  // gsx gsy gsz gsm
   PT_TRY_LAUNCH(c, pt_launch_gather3(c->perm.p, n, wx, wy, wz, v.wsx, v.wsy, v.wsz, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_adjoint_lists(c->bk.p, c->nb, c->tnode.p, c->gn.p, c->xs.p, c->ys.p,
+  PT_TRY_LAUNCH(c, pt_launch_adjoint_lists(c->adj_prec, c->bk.p, c->nb, c->tnode.p, c->gn.p,
+                                           c->gn32.p, c->xs.p, c->ys.p,
                                            c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
                                            c->pc_off.p, c->pc.p, c->offset, v.wsx, v.wsy, v.wsz,
                                            v.gsx, v.gsy, v.gsz, v.gsm, c->lam_node.p, nn,
@@ -1007,6 +1009,14 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
 }  // namespace
 
 extern "C" {
+
+pt_status pt_set_adjoint_precision(pt_ctx* c, pt_precision precision) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  c->adj_prec = precision;
+  return PT_OK;
+}
 
 pt_status pt_force_vjp(pt_ctx* c, double theta, pt_force_mode mode, const double* w,
                        double* grad_x, double* grad_m, pt_memory where) {

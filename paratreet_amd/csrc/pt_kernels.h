@@ -152,14 +152,16 @@ int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const 
 int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 
-// Adjoint of the interactions (FP64).  lam_node holds pt_node_adjoint_fields()
-// x nn doubles (field-major) and must be zero on entry; g* accumulate.
+// Adjoint of the interactions (precision 0: FP64; otherwise FP32 pair
+// arithmetic on bucket-centre-shifted coordinates with FP64 sums).  lam_node
+// holds pt_node_adjoint_fields() x nn doubles (field-major) and must be zero
+// on entry; g* accumulate.
 int pt_node_adjoint_fields();
 // Doubles of node-cotangent scratch the list adjoint needs for nn nodes.
 int64_t pt_adjoint_node_scratch(int64_t nn);
-int pt_launch_adjoint_lists(const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                            const GravNode* gn, const double* x, const double* y,
-                            const double* z, const double* m, const double* e,
+int pt_launch_adjoint_lists(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
+                            const GravNode* gn, const GravNode32* gn32, const double* x,
+                            const double* y, const double* z, const double* m, const double* e,
                             const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
                             const int32_t* pc, const double off[3], const double* wx,
                             const double* wy, const double* wz, double* gx, double* gy,

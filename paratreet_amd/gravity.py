@@ -282,6 +282,11 @@ class Driver:
         return out
 
     # ---- differentiation (no counterpart in the reference) -----------
+    def set_adjoint_precision(self, precision=PT_FP64):
+        """Arithmetic of the tree adjoint: PT_FP64 or PT_FP32C (FP32 pairs on
+        bucket-centre-shifted coordinates, FP64 sums)."""
+        self._check(self.L.pt_set_adjoint_precision(self.ctx, int(precision)))
+
     def force_vjp(self, w, theta=0.7, mode=FORCE_TREE):
         """(da/dx)^T w and (da/dm)^T w at the current positions (host arrays)."""
         w = _f64(w).reshape(self.n, 3)

@@ -117,6 +117,34 @@ def test_tree_vjp_identities_and_direct():
     assert rx < 2e-2 and rm < 2e-2
 
 
+def test_tree_vjp_fp32c_against_fp64():
+    """The FP32 adjoint (FP32 pairs on bucket-centre-shifted coordinates, FP64
+    sums) against the FP64 adjoint on the same lists, which finite
+    differences pin above; the identities hold to FP32 rounding."""
+    from paratreet_amd import _lib
+    from paratreet_amd.gravity import FORCE_TREE
+    pos, _, m, e = _plummer(30000, 23, 0.005)
+    w = np.random.default_rng(5).standard_normal(pos.shape)
+    d = _driver(16)
+    a = _tree_acc(d, pos, m, e, 0.7)
+    g64, gm64 = d.force_vjp(w, theta=0.7, mode=FORCE_TREE)
+    d.set_adjoint_precision(_lib.PT_FP32C)
+    g32, gm32 = d.force_vjp(w, theta=0.7, mode=FORCE_TREE)
+    d.set_adjoint_precision(_lib.PT_FP64)
+    rx = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    rm = np.linalg.norm(gm32 - gm64) / np.linalg.norm(gm64)
+    print("fp32c vs fp64 tree VJP: positions %.2e masses %.2e" % (rx, rm))
+    assert rx < 1e-5 and rm < 1e-5
+    assert np.abs(g32.sum(0)).max() < 1e-6 * np.abs(g32).sum()
+    assert abs(np.dot(m, gm32) - np.sum(w * a)) < 1e-6 * np.sum(np.abs(w * a))
+    # the setting is back to FP64 (equal up to the order of the atomic sums)
+    g2, gm2 = d.force_vjp(w, theta=0.7, mode=FORCE_TREE)
+    assert np.linalg.norm(g2 - g64) <= 1e-12 * np.linalg.norm(g64)
+    assert np.linalg.norm(gm2 - gm64) <= 1e-12 * np.linalg.norm(gm64)
+    with pytest.raises(_lib.PtError):
+        d.set_adjoint_precision(7)
+
+
 # ---------------------------------------------------------------- trajectory
 
 STEPS, DT = 10, 1e-3
@@ -231,6 +259,28 @@ def test_tree_gradient_close_to_direct_gradient():
     errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(t, r)]
     print("tree vs direct gradient: x %.2e v %.2e m %.2e" % tuple(errs))
     assert max(errs) < 1e-2
+
+
+def test_leapfrog_fp32c_forward_and_adjoint():
+    """FP32C forward and FP32C adjoint against the FP64 forward and adjoint
+    (config-5 form at 20k particles, checkpoints every 3 steps)."""
+    from paratreet_amd import _lib
+    from paratreet_amd.gravity import FORCE_TREE
+    pos, vel, m, e = _plummer(20000, 36, 0.005)
+    d = _driver(16)
+    ref = _gradients(d, pos, vel, m, e, FORCE_TREE, every=3)
+    d.load(pos, m, e, vel=vel)
+    d.set_adjoint_precision(_lib.PT_FP32C)
+    d.leapfrog_forward(STEPS, DT, theta=0.7, precision=_lib.PT_FP32C, mode=FORCE_TREE,
+                       checkpoint_every=3)
+    xT = d.state()[0]
+    lam = 2.0 * m[:, None] * (xT - pos)
+    gx, gv, gm = d.leapfrog_adjoint(lam_x=lam)
+    d.set_adjoint_precision(_lib.PT_FP64)
+    got = (gx - lam, gv, gm + np.sum((xT - pos) ** 2, axis=1))
+    errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(got, ref)]
+    print("fp32c vs fp64 trajectory gradient: x %.2e v %.2e m %.2e" % tuple(errs))
+    assert max(errs) < 1e-5
 
 
 def test_autograd_function():
