@@ -571,7 +571,8 @@ pt_status pt_upward(pt_ctx* c) {
   }
   PT_TRY_LAUNCH(c, pt_launch_upward_run(tr, c->level_nodes.p, run, c->nd.p, nn, c->ord_h.p,
                                         c->ord_b.p, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->bucket_node.p, c->nb, c->tnode.p, c->gn.p,
+  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->es.p, c->bucket_node.p, c->nb, c->tnode.p,
+                                  c->gn.p,
                                   c->gn32.p, c->bk.p, c->stream));
   if (c->cfg.hexadecapole) {
     // third and fourth moments once every centre of mass is final

@@ -35,7 +35,11 @@ struct __align__(64) TreeNode {
   int32_t first_child;  // -1 for leaves
   int32_t type;
   int32_t begin;
-  int32_t pad[4];
+  // leaves: upper bounds of |x_j - cm| (the farthest corner of the particle
+  // box) and of the particles' softening, rounded up to FP32; 0 otherwise
+  float leaf_reach;
+  float leaf_soft;
+  int32_t pad[2];
 };
 // Packed source record for particle-cell interactions (FP64 path).
 // tr = 0.5 * (qxx + qyy + qzz) as GravityVisitor::node computes it.

@@ -250,6 +250,35 @@ __device__ __forceinline__ bool near32(float sx, float sy, float sz, float se, c
   return !(db > reach * reach);
 }
 
+// True when every particle of a leaf source is beyond softening reach of
+// every target of the group: the leaf's centre of mass (s, bucket frame) lies
+// farther from the bucket box than the leaf's reach bound plus the largest
+// source and target softenings, with the margins of near32 (the particles'
+// coordinates are bounded by |s| + reach).  All its particles then take the
+// branch-free far loop.
+__device__ __forceinline__ bool leaf_far32(float sx, float sy, float sz, float lreach,
+                                           float lsoft, const BucketF& f, float emax) {
+  const float eps = 1e-6f * (fabsf(sx) + fabsf(sy) + fabsf(sz) + f.ext + 3.0f * lreach);
+  const float reach = lreach + (lsoft + emax) * 1.00001f + eps;
+  const float ex = fmaxf(fmaxf(f.lox - sx, sx - f.hix), 0.f);
+  const float ey = fmaxf(fmaxf(f.loy - sy, sy - f.hiy), 0.f);
+  const float ez = fmaxf(fmaxf(f.loz - sz, sz - f.hiz), 0.f);
+  const float db = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+  return db > reach * reach;
+}
+
+// A leaf source's count with the far flag of leaf_far32 in bit 30.
+constexpr int32_t kFarLeaf = 1 << 30;
+
+__device__ __forceinline__ int32_t leaf_entry(int32_t count, double cx, double cy, double cz,
+                                              float lreach, float lsoft, const BucketF& f,
+                                              float emax, double ox, double oy, double oz,
+                                              double ccx, double ccy, double ccz) {
+  const bool far = leaf_far32((float)((cx + ox) - ccx), (float)((cy + oy) - ccy),
+                              (float)((cz + oz) - ccz), lreach, lsoft, f, emax);
+  return far ? (count | kFarLeaf) : count;
+}
+
 __device__ __forceinline__ float frame32(double v, double o, double cc) {
   return (float)((v + o) - cc);
 }
@@ -677,13 +706,15 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
   }
 }
 
-// Flattens up to 64 leaf sources (begin/count in the calling lane, count 0 =
-// none) into particles and evaluates them 64 at a time.  Each window marks
-// the slot where every leaf starts (the leaf's lane in LDS, a bit in a
-// ballot); a slot's leaf is the mark at the highest start bit at or below
-// it, or the previous window's last leaf when there is none.
+// Flattens up to 64 leaf sources (begin and flagged count, leaf_entry, in
+// the calling lane; count 0 = none) into particles and evaluates them 64 at a
+// time.  Leaves not flagged far come first, so each window holds its near
+// particles in its first slots and the rest take the branch-free loop.  Each
+// window marks the slot where every leaf starts (the leaf's lane in LDS, a
+// bit in a ballot); a slot's leaf is the mark at the highest start bit at or
+// below it, or the previous window's last leaf when there is none.
 template <bool F32, typename W_t>
-__device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, const Group& g,
+__device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cntf, const Group& g,
                                           Lanes<F32>& L, const double* __restrict__ px,
                                           const double* __restrict__ py,
                                           const double* __restrict__ pz,
@@ -692,9 +723,14 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
                                           double ox, double oy, double oz, double ccx, double ccy,
                                           double ccz) {
   const int lane = __lane_id();
-  const int incl = pt_wave_incl_scan(cnt);
-  const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
-  const int exc = incl - cnt;
+  const bool far = (cntf & kFarLeaf) != 0;
+  const int cnt = cntf & ~kFarLeaf;
+  const int cn = far ? 0 : cnt, cf = far ? cnt : 0;
+  const int incl_n = pt_wave_incl_scan(cn);
+  const int incl_f = pt_wave_incl_scan(cf);
+  const int near_total = __builtin_amdgcn_readlane(incl_n, PT_WAVE - 1);
+  const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
+  const int exc = far ? near_total + incl_f - cf : incl_n - cn;
   int carry = 0;
   for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
     W.fl_mark[lane] = -1;
@@ -708,18 +744,15 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cnt, cons
     const int lbeg = __shfl(beg, leaf, PT_WAVE);
     const int lexc = __shfl(exc, leaf, PT_WAVE);
     const int slot = s0 + lane;
-    const bool act = slot < total;
-    bool nr = false;
-    SrcP<F32> r{};
-    if (act) {
+    if (slot < total) {
       const int64_t j = (int64_t)lbeg + (slot - lexc);
-      r = particle_record<F32>(j, px, py, pz, pm, pe, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
+      bool nr;
+      reinterpret_cast<SrcP<F32>*>(W.stage)[lane] = particle_record<F32>(
+          j, px, py, pz, pm, pe, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
     }
-    // near sources first, then far ones
-    const unsigned long long mn = pt_ballot(act && nr), mf = pt_ballot(act && !nr);
-    const int nnear = __popcll(mn);
-    if (act) reinterpret_cast<SrcP<F32>*>(W.stage)[nr ? pt_rank_in(mn) : nnear + pt_rank_in(mf)] = r;
     const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
+    const int nn = near_total - s0;
+    const int nnear = nn < 0 ? 0 : (nn < nsrc ? nn : nsrc);
     pad_chunk<F32, false, W_t>(W, nsrc, g);
     __builtin_amdgcn_wave_barrier();
     eval_particles<F32, W_t>(W, g, L, uniform_if<!F32>(ceil_div_s(nnear, g)),
@@ -842,7 +875,8 @@ interact_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __res
       if (l0 + lane < pp1) {
         const TreeNode nd = nodes[pp[l0 + lane]];
         beg = nd.begin;
-        cnt = nd.count;
+        cnt = leaf_entry(nd.count, nd.cx, nd.cy, nd.cz, nd.leaf_reach, nd.leaf_soft, bf, L.emax,
+                         ox, oy, oz, ccx, ccy, ccz);
       }
       do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
     }
@@ -1036,7 +1070,8 @@ interact_hexa_kernel(const Bucket* __restrict__ bk, int64_t nb,
       if (l0 + lane < pp1) {
         const TreeNode nd = nodes[pp[l0 + lane]];
         beg = nd.begin;
-        cnt = nd.count;
+        cnt = leaf_entry(nd.count, nd.cx, nd.cy, nd.cz, nd.leaf_reach, nd.leaf_soft, bf, L.emax,
+                         ox, oy, oz, ccx, ccy, ccz);
       }
       do_leaves<false, HexaLds>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
                                 ccz);
@@ -1126,7 +1161,8 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         if (r.is_leaf) {
           const int k = npp + pt_rank_in(leafm);
           W.ppbeg[k] = r.begin;
-          W.ppcnt[k] = r.count;
+          W.ppcnt[k] = leaf_entry(r.count, r.cx, r.cy, r.cz, r.leaf_reach, r.leaf_soft, bf,
+                                  L.emax, ox, oy, oz, ccx, ccy, ccz);
         }
         npc += __popcll(closem);
         npp += __popcll(leafm);
@@ -1244,7 +1280,8 @@ __device__ __forceinline__ void gravity_bucket(
       if (r.is_leaf) {
         const int k = npp + pt_rank_in(leafm);
         W.ppbeg[k] = r.begin;
-        W.ppcnt[k] = r.count;
+        W.ppcnt[k] = leaf_entry(r.count, r.cx, r.cy, r.cz, r.leaf_reach, r.leaf_soft, bf, L.emax,
+                                ox, oy, oz, ccx, ccy, ccz);
       }
       npc += __popcll(closem);
       npp += __popcll(leafm);

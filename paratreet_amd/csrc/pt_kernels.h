@@ -78,9 +78,10 @@ int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLe
                          double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
                            int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
-int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
-                   int64_t nb, TreeNode* tnode, GravNode* gn, GravNode32* gn32, Bucket* bk,
-                   hipStream_t s);
+// e: the sorted softenings (for the leaves' softening bounds).
+int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const double* e,
+                   const int32_t* bucket_node, int64_t nb, TreeNode* tnode, GravNode* gn,
+                   GravNode32* gn32, Bucket* bk, hipStream_t s);
 
 // List-producing traversal (count pass when fill is false).  partials holds
 // pt_traverse_blocks(nb) x 4 counters, reduced by pt_launch_reduce_counters.

@@ -337,10 +337,17 @@ upward_run_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, PtLev
 }
 
 // Packs the walk and interaction records from the SoA node table.
+// The smallest float >= v (v >= 0, finite).
+__device__ __forceinline__ float round_up_f32(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = __int_as_float(__float_as_int(f) + 1);
+  return f;
+}
+
 __global__ void __launch_bounds__(256)
 pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
-                  TreeNode* __restrict__ tnode, GravNode* __restrict__ gn,
-                  GravNode32* __restrict__ gn32) {
+                  const double* __restrict__ pe, TreeNode* __restrict__ tnode,
+                  GravNode* __restrict__ gn, GravNode32* __restrict__ gn32) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= nn) return;
   TreeNode a;
@@ -352,7 +359,26 @@ pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
   a.first_child = t.first_child[i];
   a.type = t.type[i];
   a.begin = t.begin[i];
-  a.pad[0] = a.pad[1] = a.pad[2] = a.pad[3] = 0;
+  a.leaf_reach = 0.f;
+  a.leaf_soft = 0.f;
+  if (a.type == PT_NODE_LEAF) {
+    const double cm[3] = {a.cx, a.cy, a.cz};
+    double r2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const double d1 = cm[k] - nd[(PT_CD_LO + k) * nn + i];
+      const double d2 = nd[(PT_CD_HI + k) * nn + i] - cm[k];
+      const double d = d1 > d2 ? d1 : d2;
+      r2 += d * d;
+    }
+    double smax = 0.0;
+    for (int j = 0; j < a.count; j++) {
+      const double e = pe[a.begin + j];
+      smax = e > smax ? e : smax;
+    }
+    a.leaf_reach = round_up_f32(sqrt(r2));
+    a.leaf_soft = round_up_f32(smax);
+  }
+  a.pad[0] = a.pad[1] = 0;
   tnode[i] = a;
   GravNode g;
   g.cx = a.cx;
@@ -639,10 +665,10 @@ int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const int32_t* bucket_node,
-                   int64_t nb, TreeNode* tnode, GravNode* gn, GravNode32* gn32, Bucket* bk,
-                   hipStream_t s) {
-  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, tnode, gn, gn32);
+int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const double* e,
+                   const int32_t* bucket_node, int64_t nb, TreeNode* tnode, GravNode* gn,
+                   GravNode32* gn32, Bucket* bk, hipStream_t s) {
+  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, e, tnode, gn, gn32);
   if (nb > 0) pack_buckets_kernel<<<grid_for(nb), 256, 0, s>>>(t, nd, nn, bucket_node, nb, bk);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

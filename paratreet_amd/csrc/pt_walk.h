@@ -130,6 +130,8 @@ struct WalkLane {
   int32_t begin;     // its first particle
   bool is_leaf;      // -> leaf() (P-P)
   bool is_closed;    // -> node() (P-C)
+  double cx, cy, cz;        // its centre of mass
+  float leaf_reach, leaf_soft;  // leaf bounds (TreeNode)
 };
 
 // Wave-uniform per-step counts for the interaction counters.
@@ -149,11 +151,16 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
                                                  WalkCounts& wc,
                                                  const HexOpen* __restrict__ ho = nullptr,
                                                  double gm = 0.0) {
-  WalkLane r{0, 0, 0, false, false};
+  WalkLane r{0, 0, 0, false, false, 0.0, 0.0, 0.0, 0.f, 0.f};
   wc = WalkCounts{0, 0, 0};
   const TreeNode root = nodes[0];
   r.count = root.count;
   r.begin = root.begin;
+  r.cx = root.cx;
+  r.cy = root.cy;
+  r.cz = root.cz;
+  r.leaf_reach = root.leaf_reach;
+  r.leaf_soft = root.leaf_soft;
   if (root.type == PT_NODE_LEAF) {
     r.is_leaf = true;
     wc.leaf_parts = root.count;
@@ -189,7 +196,7 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   sp -= k;
   const int slot = lane >> 3;
   const bool active = slot < k;
-  WalkLane r{-1, 0, 0, false, false};
+  WalkLane r{-1, 0, 0, false, false, 0.0, 0.0, 0.0, 0.f, 0.f};
   bool op = false;
   int32_t fc = -1;
   if (active) {
@@ -197,6 +204,11 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
     const TreeNode nd = nodes[r.node];
     r.count = nd.count;
     r.begin = nd.begin;
+    r.cx = nd.cx;
+    r.cy = nd.cy;
+    r.cz = nd.cz;
+    r.leaf_reach = nd.leaf_reach;
+    r.leaf_soft = nd.leaf_soft;
     fc = nd.first_child;
     if (nd.type == PT_NODE_LEAF) {
       r.is_leaf = true;
