@@ -35,9 +35,9 @@ CONFIGS = {
 # HBM bytes per dispatch of gravity_kernel<FP32C> on this workload: FETCH_SIZE
 # + WRITE_SIZE from two rocprofv3 counter passes (write counter calibrated
 # against an 8 MB device copy: factor 1.0).  Most of the write traffic is the
-# kernel's register spill area (28 B per lane per bucket under the 5-wave
-# register budget); the algorithmic output is 32 B per particle.
-TRAFFIC_BYTES_FP32C = (105570.8 + 276059.0) * 1024
+# kernel's register spill area (44 B per lane under the 6-wave register
+# budget); the algorithmic output is 32 B per particle.
+TRAFFIC_BYTES_FP32C = (129238.5 + 500326.4) * 1024
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
