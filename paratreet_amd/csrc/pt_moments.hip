@@ -134,8 +134,11 @@ __device__ __forceinline__ double len3(double x, double y, double z) {
   return sqrt(x * x + y * y + z * z);
 }
 
-// CentroidData::operator+= (examples/CentroidData.h:61-73) with
-// calculateRadiusFarthestCorner (examples/MultipoleMoments.h:458-469).
+// CentroidData::operator+= (examples/CentroidData.h:61-73) without its last
+// step, the radius (cd_radius): no merge reads the radius it sets (a merge
+// into a massless node copies the child's and the next radius overwrites
+// it), so after a sequence of merges one cd_radius gives the value of the
+// last merge bit for bit.
 __device__ __forceinline__ void cd_merge(CData& d, const CData& c) {
   for (int k = 0; k < 3; k++) {
     if (c.lo[k] < d.lo[k]) d.lo[k] = c.lo[k];
@@ -143,6 +146,11 @@ __device__ __forceinline__ void cd_merge(CData& d, const CData& c) {
   }
   mom_merge(d, c);
   d.count += c.count;
+}
+
+// The radius step of CentroidData::operator+=: calculateRadiusFarthestCorner
+// (examples/MultipoleMoments.h:458-469) when count > 1, else 1.0.
+__device__ __forceinline__ void cd_radius(CData& d) {
   if (d.count > 1) {
     double dl[3];
     for (int k = 0; k < 3; k++) {
@@ -306,6 +314,7 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
     }
     cd_merge(d, cd);
   }
+  cd_radius(d);
   cd_store(d, nd, nn, p);
 }
 
