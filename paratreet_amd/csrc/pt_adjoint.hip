@@ -45,7 +45,7 @@ constexpr int kNodeAdj = 11;  // node cotangent: cm(3), M, Q(6: xx xy xz yy yz z
 // of most buckets; their cotangents are accumulated into kHotParts private
 // copies (chosen by work-group) and summed afterwards, so concurrent atomic
 // adds to one address come from 1/kHotParts of the work-groups.
-constexpr int64_t kHotNodes = 4096;
+constexpr int64_t kHotNodes = 16384;
 constexpr int kHotParts = 32;
 
 // GravityVisitor::SPLINE force factor b and db/dr (examples/GravityVisitor.h:
