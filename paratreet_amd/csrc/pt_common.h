@@ -124,10 +124,31 @@ __device__ __forceinline__ int pt_wave_sum_int(int v) {
   return __builtin_amdgcn_readlane(pt_wave_incl_scan(v), PT_WAVE - 1);
 }
 
+// A double moved by one DPP control (both 32-bit halves); lanes without a
+// source read 0.0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double pt_dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+
+// Sum over the 64 lanes (all active) with the row shifts and broadcasts of
+// pt_wave_incl_scan, as a wave-uniform value (lane 63's inclusive sum).
 __device__ __forceinline__ double pt_wave_sum(double v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, PT_WAVE);
-  return v;
+  v += pt_dpp_f64<0x111, 0xf>(v);
+  v += pt_dpp_f64<0x112, 0xf>(v);
+  v += pt_dpp_f64<0x114, 0xf>(v);
+  v += pt_dpp_f64<0x118, 0xf>(v);
+  v += pt_dpp_f64<0x142, 0xa>(v);
+  v += pt_dpp_f64<0x143, 0xc>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, PT_WAVE - 1);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), PT_WAVE - 1);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
 }
 
 __device__ __forceinline__ unsigned long long pt_ballot(bool p) {
