@@ -473,10 +473,9 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
       const int64_t e = e0 + lane;
       if (e < pc1) {
         const int32_t sid = pc[e];
-        const GravNode s64 = gn[sid];
         const GravNode32 s = gn32[sid];
-        const double cxd = s64.cx + ox, cyd = s64.cy + oy, czd = s64.cz + oz;
-        const bool mono = pt_open_softening(cxd, cyd, czd, s64.soft, b);
+        const double cxd = s.cx + ox, cyd = s.cy + oy, czd = s.cz + oz;
+        const bool mono = pt_open_softening(cxd, cyd, czd, s.soft64, b);
         const float cx = (float)(cxd - ccx), cy = (float)(cyd - ccy), cz = (float)(czd - ccz);
         const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
         float lam[kNodeAdj];

@@ -48,9 +48,12 @@ struct __align__(32) GravNode {
   double mass, qxx, qxy, qxz;
   double qyy, qyz, qzz, tr;
 };
-// FP32 copy of a node's moments for the compensated-FP32 interactions
-// (the centre stays FP64: it is shifted per target bucket).
-struct __align__(32) GravNode32 {
+// Source record of the compensated-FP32 interactions in one 64-byte line: the
+// centre of mass and softening in FP64 (the centre is shifted per target
+// bucket, the softening decides openSoftening exactly) and FP32 copies of the
+// mass, softening and quadrupole.
+struct __align__(64) GravNode32 {
+  double cx, cy, cz, soft64;
   float mass, soft, qxx, qxy, qxz, qyy, qyz, qzz;
 };
 // HEXADECAPOLE build (-DHEXADECAPOLE, examples/Makefile:5).  Source record

@@ -359,10 +359,10 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
                                                  const Bucket& b, const BucketF& bf, float emax,
                                                  double ox, double oy, double oz, double ccx,
                                                  double ccy, double ccz, bool& near) {
-  const double cx = gn[id].cx + ox, cy = gn[id].cy + oy, cz = gn[id].cz + oz;
-  const double soft = gn[id].soft;
   if constexpr (F32) {
     const GravNode32 h = gn32[id];
+    const double cx = h.cx + ox, cy = h.cy + oy, cz = h.cz + oz;
+    const double soft = h.soft64;
     SrcC32 r;
     r.cx = (float)(cx - ccx);
     r.cy = (float)(cy - ccy);
@@ -383,6 +383,8 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
     return r;
   } else {
     const GravNode s = gn[id];
+    const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
+    const double soft = s.soft;
     SrcC64 r;
     r.cx = s.cx;
     r.cy = s.cy;

@@ -404,6 +404,10 @@ pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
   g.tr = 0.5 * (g.qxx + g.qyy + g.qzz);
   gn[i] = g;
   GravNode32 h;
+  h.cx = g.cx;
+  h.cy = g.cy;
+  h.cz = g.cz;
+  h.soft64 = g.soft;
   h.mass = (float)g.mass;
   h.soft = (float)g.soft;
   h.qxx = (float)g.qxx;
