@@ -43,10 +43,6 @@ int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, h
 int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
                           int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
                           int64_t level_cap, int32_t* overflow, hipStream_t s);
-int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, int32_t base,
-                    int bucket, int child_depth, const PtTree& t, uint32_t* flag, hipStream_t s);
-int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
-                      int32_t* next, hipStream_t s);
 // levels: the split launches done (depths 0 .. levels-1).
 int pt_launch_buckets(const PtTree& t, int64_t node_cap, int64_t n, int levels,
                       const int32_t* level_cnt, const int32_t* overflow, uint32_t* head,

@@ -57,33 +57,6 @@ __global__ void root_kernel(int64_t n, int bucket, PtTree t, int32_t* __restrict
   t.first_child[0] = -1;
 }
 
-__global__ void __launch_bounds__(256)
-split_kernel(const uint64_t* __restrict__ keys, const int32_t* __restrict__ frontier,
-             int32_t nf, int32_t base, int bucket, int child_depth, PtTree t,
-             uint32_t* __restrict__ internal_flag) {
-  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
-  if (gid >= (int64_t)nf * 8) return;
-  const int f = (int)(gid >> 3);
-  const int c = (int)(gid & 7);
-  const int32_t p = frontier[f];
-  const int64_t b = t.begin[p];
-  const int64_t e = b + t.count[p];
-  const uint64_t child_key = (t.key[p] << 3) | (uint64_t)c;
-  const int64_t lo = (c == 0) ? b : first_ge(keys, b, e, left_align(child_key));
-  const int64_t hi = (c == 7) ? e : first_ge(keys, b, e, left_align(child_key + 1));
-  const int32_t id = base + (int32_t)gid;
-  const int ty = node_type(hi - lo, bucket, child_depth);
-  t.key[id] = child_key;
-  t.begin[id] = (int32_t)lo;
-  t.count[id] = (int32_t)(hi - lo);
-  t.depth[id] = child_depth;
-  t.type[id] = ty;
-  t.parent[id] = p;
-  t.first_child[id] = -1;
-  if (c == 0) t.first_child[p] = base + 8 * f;
-  internal_flag[gid] = (ty == PT_NODE_INTERNAL) ? 1u : 0u;
-}
-
 // Device-driven level split: level d's internal nodes are
 // level_nodes[off_d, off_d + level_cnt[d]) with off_d = sum of the counts of
 // the levels above; their children get ids 1 + 8 * off_d + 8 f + c, and the
@@ -133,13 +106,6 @@ split_level_kernel(const uint64_t* __restrict__ keys, int32_t* __restrict__ leve
     }
     level_nodes[slot] = (int32_t)id;
   }
-}
-
-__global__ void __launch_bounds__(256)
-compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t m,
-               int32_t base, int32_t* __restrict__ next) {
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i < m && flag[i]) next[pos[i]] = base + (int32_t)i;
 }
 
 // Node count of the finished levels (one thread): status[0] = 1 + 8 * (all
@@ -192,15 +158,6 @@ int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, h
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_split(const uint64_t* keys, const int32_t* frontier, int32_t nf, int32_t base,
-                    int bucket, int child_depth, const PtTree& t, uint32_t* flag,
-                    hipStream_t s) {
-  if (nf <= 0) return 0;
-  split_kernel<<<grid_for((int64_t)nf * 8), 256, 0, s>>>(keys, frontier, nf, base, bucket,
-                                                          child_depth, t, flag);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
 int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
                           int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
                           int64_t level_cap, int32_t* overflow, hipStream_t s) {
@@ -211,12 +168,6 @@ int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* l
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_compact(const uint32_t* flag, const uint32_t* pos, int64_t m, int32_t base,
-                      int32_t* next, hipStream_t s) {
-  if (m <= 0) return 0;
-  compact_kernel<<<grid_for(m), 256, 0, s>>>(flag, pos, m, base, next);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
 
 // Buckets of a device-built tree without a host round trip: the node count
 // comes from the level counts on the device (status[0]); afterwards status =
