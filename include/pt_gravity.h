@@ -73,7 +73,8 @@ typedef struct pt_config {
   int32_t hexadecapole;           /* the -DHEXADECAPOLE build (examples/Makefile:5):
                                      moments to fourth order, GravityVisitor::open/node
                                      HEXADECAPOLE branches (examples/GravityVisitor.h:
-                                     162-178,199-206); FP64 only, no adjoint.
+                                     162-178,199-206); pt_compute_forces in FP64 or
+                                     PT_FP32C, the list path FP64 only, no adjoint.
                                      0 = the quadrupole build */
   double period[3];               /* dxPeriod dyPeriod dzPeriod (used when periodic) */
   int32_t n_replicas;             /* nReplicas, 0..16 (used when periodic) */

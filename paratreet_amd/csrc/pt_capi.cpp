@@ -679,7 +679,9 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
   if (!c->lists_valid) return fail(c, PT_ERR_STATE, "pt_interact called before pt_traverse");
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
   if (c->cfg.hexadecapole && precision != PT_FP64)
-    return fail(c, PT_ERR_UNSUPPORTED, "the hexadecapole build evaluates node() in FP64 only");
+    return fail(c, PT_ERR_UNSUPPORTED,
+                "the hexadecapole list path evaluates node() in FP64 only (pt_compute_forces "
+                "has the compensated-FP32 form)");
   if (c->cfg.hexadecapole) {
     Timer t(c, PT_T_FORCE);
     PT_TRY_LAUNCH(c, pt_launch_interact_hexa(c->bk.p, c->nb, c->tnode.p, c->hn.p, c->xs.p, c->ys.p,
@@ -709,8 +711,6 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (st) return st;
   if ((st = need(c, S_MOMENTS, "pt_compute_forces"))) return st;
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
-  if (c->cfg.hexadecapole && precision != PT_FP64)
-    return fail(c, PT_ERR_UNSUPPORTED, "the hexadecapole build evaluates node() in FP64 only");
   double gf;
   if ((st = set_visitor(c, theta, offset, &gf))) return st;
   const int64_t nb = c->nb;
@@ -738,7 +738,8 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
       {
         Timer t(c, PT_T_FORCE);
         if (hexa)
-          PT_TRY_LAUNCH(c, pt_launch_hexa_gravity(attempt == 1, c->bk.p, nb, c->tnode.p, c->ho.p,
+          PT_TRY_LAUNCH(c, pt_launch_hexa_gravity((int)precision, attempt == 1, c->bk.p, nb,
+                                                  c->tnode.p, c->ho.p,
                                                   c->hn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
                                                   c->es.p, gf, c->gm, off, c->axs.p, c->ays.p,
                                                   c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc,

@@ -899,6 +899,12 @@ struct SrcH {
   double cx, cy, cz, mass, mono;
   double h[PT_HEXA_MOMS];
 };
+// The same in the compensated-FP32 form: centre in the bucket frame, FP32
+// mass and moments (a 64-entry stage takes the bytes of a 32-entry SrcH one).
+struct SrcH32 {
+  float cx, cy, cz, mass, mono;
+  float h[PT_HEXA_MOMS];
+};
 
 // Closed nodes are staged kHexaStage at a time (half a chunk), which keeps a
 // wave's LDS near 9 KiB, and the fused kernel is held to kHexaWaves waves per
@@ -907,6 +913,7 @@ struct SrcH {
 // bytes per lane and take 21.5 ms).
 constexpr int kHexaStage = 32;
 constexpr int kHexaWaves = 3;
+constexpr int kHexaWaves32 = 4;  // the compensated-FP32 form (LDS allows four)
 
 struct __align__(16) HexaLds {
   static constexpr int kStage = kHexaStage * sizeof(SrcH);
@@ -922,61 +929,69 @@ struct __align__(16) HexaLds {
 //   pot = -(m / r + sum_l c_l s_l / r^(2l+1)),
 //   acc = sum_l c_l (l v_l / r^(2l+1) - (2l+1) s_l r / r^(2l+3)) - m r / r^3
 // (the trace-free moments give the Legendre series of the potential).
-__device__ __forceinline__ void hexa_eval(const double* __restrict__ h, double m, double rx,
-                                          double ry, double rz, double& pot, double& gx,
-                                          double& gy, double& gz) {
+template <typename R>
+__device__ __forceinline__ R fma_r(R a, R b, R c) {
+  if constexpr (std::is_same<R, float>::value) return fmaf(a, b, c);
+  else return fma(a, b, c);
+}
+
+template <typename R>
+__device__ __forceinline__ void hexa_eval(const R* __restrict__ h, R m, R rx, R ry, R rz, R& pot,
+                                          R& gx, R& gy, R& gz) {
   // every multiply-add is an explicit fma (this file is compiled without
   // contraction)
-  const double xx = h[0], yy = h[1], xy = h[2], xz = h[3], yz = h[4];
-  const double zz = -(xx + yy);
-  const double xxx = h[5], xyy = h[6], xxy = h[7], yyy = h[8], xxz = h[9], yyz = h[10],
-               xyz = h[11];
-  const double xzz = -(xxx + xyy), yzz = -(xxy + yyy), zzz = -(xxz + yyz);
-  const double xxxx = h[12], xyyy = h[13], xxxy = h[14], yyyy = h[15], xxxz = h[16],
-               yyyz = h[17], xxyy = h[18], xxyz = h[19], xyyz = h[20];
-  const double xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
-  const double xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
-  const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-  const double ir = rsqrt64(rsq);
-  const double ir2 = ir * ir;
-  const double x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
-  const double xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
+  const R xx = h[0], yy = h[1], xy = h[2], xz = h[3], yz = h[4];
+  const R zz = -(xx + yy);
+  const R xxx = h[5], xyy = h[6], xxy = h[7], yyy = h[8], xxz = h[9], yyz = h[10],
+          xyz = h[11];
+  const R xzz = -(xxx + xyy), yzz = -(xxy + yyy), zzz = -(xxz + yyz);
+  const R xxxx = h[12], xyyy = h[13], xxxy = h[14], yyyy = h[15], xxxz = h[16],
+          yyyz = h[17], xxyy = h[18], xxyz = h[19], xyyz = h[20];
+  const R xxzz = -(xxxx + xxyy), yyzz = -(xxyy + yyyy), xyzz = -(xxxy + xyyy);
+  const R xzzz = -(xxxz + xyyz), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
+  const R rsq = fma_r(rx, rx, fma_r(ry, ry, rz * rz));
+  R ir;
+  if constexpr (std::is_same<R, float>::value) ir = rsqrt32(rsq);
+  else ir = rsqrt64(rsq);
+  const R ir2 = ir * ir;
+  const R x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
+  const R xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
   // order 2: v2 = M2 r, s2 = r.M2.r
-  const double v2x = fma(xx, rx, fma(xy, ry, xz * rz));
-  const double v2y = fma(xy, rx, fma(yy, ry, yz * rz));
-  const double v2z = fma(xz, rx, fma(yz, ry, zz * rz));
-  const double s2 = fma(v2x, rx, fma(v2y, ry, v2z * rz));
+  const R v2x = fma_r(xx, rx, fma_r(xy, ry, xz * rz));
+  const R v2y = fma_r(xy, rx, fma_r(yy, ry, yz * rz));
+  const R v2z = fma_r(xz, rx, fma_r(yz, ry, zz * rz));
+  const R s2 = fma_r(v2x, rx, fma_r(v2y, ry, v2z * rz));
   // order 3: v3_i = M3_ijk r_j r_k
-  const double v3x = fma(xxx, x2, fma(xyy, y2, fma(xzz, z2, 2.0 * fma(xxy, xyp, fma(xxz, xzp, xyz * yzp)))));
-  const double v3y = fma(xxy, x2, fma(yyy, y2, fma(yzz, z2, 2.0 * fma(xyy, xyp, fma(xyz, xzp, yyz * yzp)))));
-  const double v3z = fma(xxz, x2, fma(yyz, y2, fma(zzz, z2, 2.0 * fma(xyz, xyp, fma(xzz, xzp, yzz * yzp)))));
-  const double s3 = fma(v3x, rx, fma(v3y, ry, v3z * rz));
+  const R v3x = fma_r(xxx, x2, fma_r(xyy, y2, fma_r(xzz, z2, R(2.0) * fma_r(xxy, xyp, fma_r(xxz, xzp, xyz * yzp)))));
+  const R v3y = fma_r(xxy, x2, fma_r(yyy, y2, fma_r(yzz, z2, R(2.0) * fma_r(xyy, xyp, fma_r(xyz, xzp, yyz * yzp)))));
+  const R v3z = fma_r(xxz, x2, fma_r(yyz, y2, fma_r(zzz, z2, R(2.0) * fma_r(xyz, xyp, fma_r(xzz, xzp, yzz * yzp)))));
+  const R s3 = fma_r(v3x, rx, fma_r(v3y, ry, v3z * rz));
   // order 4: v4_i = M4_ijkl r_j r_k r_l over the 10 cubic monomials
-  const double x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz;
-  const double x2y = x2 * ry, x2z = x2 * rz, xy2 = rx * y2, xz2 = rx * z2, y2z = y2 * rz,
-               yz2 = ry * z2, xyzm = xyp * rz;
-  const double v4x =
-      fma(6.0 * xxyz, xyzm,
-          fma(3.0, fma(xxxy, x2y, fma(xxxz, x2z, fma(xxyy, xy2, fma(xxzz, xz2, fma(xyyz, y2z, xyzz * yz2))))),
-              fma(xxxx, x3, fma(xyyy, y3, xzzz * z3))));
-  const double v4y =
-      fma(6.0 * xyyz, xyzm,
-          fma(3.0, fma(xxyy, x2y, fma(xxyz, x2z, fma(xyyy, xy2, fma(xyzz, xz2, fma(yyyz, y2z, yyzz * yz2))))),
-              fma(xxxy, x3, fma(yyyy, y3, yzzz * z3))));
-  const double v4z =
-      fma(6.0 * xyzz, xyzm,
-          fma(3.0, fma(xxyz, x2y, fma(xxzz, x2z, fma(xyyz, xy2, fma(xzzz, xz2, fma(yyzz, y2z, yzzz * yz2))))),
-              fma(xxxz, x3, fma(yyyz, y3, zzzz * z3))));
-  const double s4 = fma(v4x, rx, fma(v4y, ry, v4z * rz));
-  const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2,
-               ir11 = ir9 * ir2;
+  const R x3 = x2 * rx, y3 = y2 * ry, z3 = z2 * rz;
+  const R x2y = x2 * ry, x2z = x2 * rz, xy2 = rx * y2, xz2 = rx * z2, y2z = y2 * rz,
+          yz2 = ry * z2, xyzm = xyp * rz;
+  const R v4x =
+      fma_r(R(6.0) * xxyz, xyzm,
+            fma_r(R(3.0), fma_r(xxxy, x2y, fma_r(xxxz, x2z, fma_r(xxyy, xy2, fma_r(xxzz, xz2, fma_r(xyyz, y2z, xyzz * yz2))))),
+                  fma_r(xxxx, x3, fma_r(xyyy, y3, xzzz * z3))));
+  const R v4y =
+      fma_r(R(6.0) * xyyz, xyzm,
+            fma_r(R(3.0), fma_r(xxyy, x2y, fma_r(xxyz, x2z, fma_r(xyyy, xy2, fma_r(xyzz, xz2, fma_r(yyyz, y2z, yyzz * yz2))))),
+                  fma_r(xxxy, x3, fma_r(yyyy, y3, yzzz * z3))));
+  const R v4z =
+      fma_r(R(6.0) * xyzz, xyzm,
+            fma_r(R(3.0), fma_r(xxyz, x2y, fma_r(xxzz, x2z, fma_r(xyyz, xy2, fma_r(xzzz, xz2, fma_r(yyzz, y2z, yzzz * yz2))))),
+                  fma_r(xxxz, x3, fma_r(yyyz, y3, zzzz * z3))));
+  const R s4 = fma_r(v4x, rx, fma_r(v4y, ry, v4z * rz));
+  const R ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2,
+          ir11 = ir9 * ir2;
   // pot = -(m/r + 3/2 s2/r^5 + 5/2 s3/r^7 + 35/8 s4/r^9)
-  pot = -fma(m, ir, fma(1.5 * s2, ir5, fma(2.5 * s3, ir7, 4.375 * s4 * ir9)));
-  const double g = fma(m, ir3, fma(7.5 * s2, ir7, fma(17.5 * s3, ir9, 39.375 * s4 * ir11)));
-  const double c2 = 3.0 * ir5, c3 = 7.5 * ir7, c4 = 17.5 * ir9;
-  gx = fma(c2, v2x, fma(c3, v3x, fma(c4, v4x, -rx * g)));
-  gy = fma(c2, v2y, fma(c3, v3y, fma(c4, v4y, -ry * g)));
-  gz = fma(c2, v2z, fma(c3, v3z, fma(c4, v4z, -rz * g)));
+  pot = -fma_r(m, ir, fma_r(R(1.5) * s2, ir5, fma_r(R(2.5) * s3, ir7, R(4.375) * s4 * ir9)));
+  const R g = fma_r(m, ir3, fma_r(R(7.5) * s2, ir7, fma_r(R(17.5) * s3, ir9, R(39.375) * s4 * ir11)));
+  const R c2 = R(3.0) * ir5, c3 = R(7.5) * ir7, c4 = R(17.5) * ir9;
+  gx = fma_r(c2, v2x, fma_r(c3, v3x, fma_r(c4, v4x, -rx * g)));
+  gy = fma_r(c2, v2y, fma_r(c3, v3y, fma_r(c4, v4y, -ry * g)));
+  gz = fma_r(c2, v2z, fma_r(c3, v3z, fma_r(c4, v4z, -rz * g)));
 }
 
 __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double ox, double oy,
@@ -994,7 +1009,7 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
   } else {
     const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
     double p, gx, gy, gz;
-    hexa_eval(s.h, s.mass, rx, ry, rz, p, gx, gy, gz);
+    hexa_eval<double>(s.h, s.mass, rx, ry, rz, p, gx, gy, gz);
     L.ap += p;
     L.ax += gx;
     L.ay += gy;
@@ -1034,6 +1049,67 @@ __device__ __forceinline__ void do_nodes_hexa(W_t& W, int32_t id, int nsrc, cons
     }
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+// One HEXADECAPOLE pair in FP32 (bucket frame), added to the chunk partials.
+__device__ __forceinline__ void hexa_pair32(const SrcH32& s, const Lanes<true>& L, float& px,
+                                            float& py, float& pz, float& pp) {
+  if (s.mono != 0.f) {
+    // addGravity: unsoftened monopole, no potential (rsq == 0 only for a
+    // coincident point, where dx = dy = dz = 0 contribute nothing)
+    const float dx = s.cx - L.tx, dy = s.cy - L.ty, dz = s.cz - L.tz;
+    const float rsq = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ri = fminf(rsqrt32(rsq), 1e10f);
+    const float f = s.mass * (ri * ri * ri);
+    px = fmaf(dx, f, px);
+    py = fmaf(dy, f, py);
+    pz = fmaf(dz, f, pz);
+  } else {
+    float p, gx, gy, gz;
+    hexa_eval<float>(s.h, s.mass, L.tx - s.cx, L.ty - s.cy, L.tz - s.cz, p, gx, gy, gz);
+    pp += p;
+    px += gx;
+    py += gy;
+    pz += gz;
+  }
+}
+
+// do_nodes_hexa in the compensated-FP32 form: up to 64 closed nodes staged at
+// once (centres shifted to the bucket frame in FP64, the openSoftening
+// decision in FP64), each chunk's FP32 partial sums added to the Kahan
+// accumulators.
+template <typename W_t>
+__device__ __forceinline__ void do_nodes_hexa32(W_t& W, int32_t id, int nsrc, const Group& g,
+                                                Lanes<true>& L, const HexNode* __restrict__ hn,
+                                                const Bucket& b, double ox, double oy, double oz,
+                                                double ccx, double ccy, double ccz) {
+  const int lane = __lane_id();
+  SrcH32* st = reinterpret_cast<SrcH32*>(W.stage);
+  if (lane < nsrc) {
+    const HexNode& s = hn[id];
+    const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
+    SrcH32 r;
+    r.cx = (float)(cx - ccx);
+    r.cy = (float)(cy - ccy);
+    r.cz = (float)(cz - ccz);
+    r.mass = (float)s.mass;
+    r.mono = pt_open_softening(cx, cy, cz, s.soft, b) ? 1.f : 0.f;
+#pragma unroll
+    for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = (float)s.h[k];
+    st[lane] = r;
+  }
+  __builtin_amdgcn_wave_barrier();
+  float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
+  const int kend = ceil_div_s(nsrc, g);
+  for (int k = 0; k < kend; k++) {
+    const int i = g.s + k * g.S;
+    if (i < nsrc) hexa_pair32(st[i], L, px, py, pz, pp);
+  }
+  kahan_add(L.ax, L.cx, px);
+  kahan_add(L.ay, L.cy, py);
+  kahan_add(L.az, L.cz, pz);
+  kahan_add(L.ap, L.cp, pp);
+  __builtin_amdgcn_wave_barrier();
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -1088,6 +1164,7 @@ interact_hexa_kernel(const Bucket* __restrict__ bk, int64_t nb,
 template <int STACK>
 struct __align__(16) HexaFusedLds {
   static constexpr int kStage = HexaLds::kStage;
+  static_assert(PT_WAVE * sizeof(SrcH32) <= kStage, "FP32 node chunk");
   unsigned char stage[kStage];
   int32_t stack[STACK];
   int32_t pcbuf[2 * PT_WAVE];
@@ -1096,8 +1173,8 @@ struct __align__(16) HexaFusedLds {
   int32_t fl_mark[PT_WAVE];
 };
 
-template <int STACK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(STACK == kSmallStack ? kHexaWaves : 1, 8)))
+template <int STACK, bool F32>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(STACK == kSmallStack ? (F32 ? kHexaWaves32 : kHexaWaves) : 1, 8)))
 hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
                     const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn,
                     const double* __restrict__ px, const double* __restrict__ py,
@@ -1124,8 +1201,8 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
     for (int g0 = 0; g0 < b.count; g0 += kGroupMax) {
       const Group g = make_group(b, g0);
       const bool count = g0 == 0;  // the walk repeats per target group; count once
-      Lanes<false> L;
-      load_targets<false>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+      Lanes<F32> L;
+      load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
       int sp = 0;
       int npc = 0, npp = 0;
       WalkCounts wc;
@@ -1170,7 +1247,11 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         npp += __popcll(leafm);
         __builtin_amdgcn_wave_barrier();
         if (npc >= PT_WAVE) {
-          do_nodes_hexa<W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, hn, b, ox, oy, oz);
+          if constexpr (F32)
+            do_nodes_hexa32<W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, hn, b, ox, oy, oz, ccx, ccy,
+                                 ccz);
+          else
+            do_nodes_hexa<W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, hn, b, ox, oy, oz);
           const int rest = npc - PT_WAVE;
           const int32_t v = lane < rest ? W.pcbuf[PT_WAVE + lane] : 0;
           __builtin_amdgcn_wave_barrier();
@@ -1179,7 +1260,7 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
           __builtin_amdgcn_wave_barrier();
         }
         if (npp >= PT_WAVE) {
-          do_leaves<false, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox,
+          do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox,
                                 oy, oz, ccx, ccy, ccz);
           const int rest = npp - PT_WAVE;
           const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
@@ -1195,15 +1276,18 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
       }
       if (npc > 0) {
         const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
-        do_nodes_hexa<W_t>(W, id, npc, g, L, hn, b, ox, oy, oz);
+        if constexpr (F32)
+          do_nodes_hexa32<W_t>(W, id, npc, g, L, hn, b, ox, oy, oz, ccx, ccy, ccz);
+        else
+          do_nodes_hexa<W_t>(W, id, npc, g, L, hn, b, ox, oy, oz);
       }
       if (npp > 0) {
         const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
         const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
-        do_leaves<false, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
+        do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy,
                               ccz);
       }
-      finish_group<false, W_t>(W, g, L, ax, ay, az, pot, acc != 0);
+      finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc != 0);
     }
   }
   if (lane == 0) {
@@ -1448,22 +1532,27 @@ int pt_launch_interact_hexa(const Bucket* bk, int64_t nb, const TreeNode* nodes,
 // Fused HEXADECAPOLE evaluation: one work-group per 4 buckets.
 int64_t pt_hexa_gravity_blocks(int64_t nb) { return (nb + kWaves - 1) / kWaves; }
 
-int pt_launch_hexa_gravity(bool big_stack, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                           const HexOpen* ho, const HexNode* hn, const double* x, const double* y,
-                           const double* z, const double* m, const double* e, double gf,
-                           double gm, const double off[3], double* ax, double* ay, double* az,
-                           double* pot, unsigned long long* partials, int32_t* err,
-                           bool accumulate, hipStream_t s) {
+int pt_launch_hexa_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
+                           const TreeNode* nodes, const HexOpen* ho, const HexNode* hn,
+                           const double* x, const double* y, const double* z, const double* m,
+                           const double* e, double gf, double gm, const double off[3],
+                           double* ax, double* ay, double* az, double* pot,
+                           unsigned long long* partials, int32_t* err, bool accumulate,
+                           hipStream_t s) {
   if (nb <= 0) return 0;
   const unsigned grid = (unsigned)pt_hexa_gravity_blocks(nb);
-  if (big_stack)
-    hexa_gravity_kernel<kBigStack><<<grid, kThreads, 0, s>>>(
-        bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
-        partials, err, accumulate ? 1 : 0);
-  else
-    hexa_gravity_kernel<kSmallStack><<<grid, kThreads, 0, s>>>(
-        bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,
-        partials, err, accumulate ? 1 : 0);
+#define PT_HEXA_LAUNCH(STACK, F32)                                                               \
+  hexa_gravity_kernel<STACK, F32><<<grid, kThreads, 0, s>>>(                                   \
+      bk, nb, nodes, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], off[2], ax, ay, az, pot,    \
+      partials, err, accumulate ? 1 : 0)
+  if (precision == 0) {
+    if (big_stack) PT_HEXA_LAUNCH(kBigStack, false);
+    else PT_HEXA_LAUNCH(kSmallStack, false);
+  } else {
+    if (big_stack) PT_HEXA_LAUNCH(kBigStack, true);
+    else PT_HEXA_LAUNCH(kSmallStack, true);
+  }
+#undef PT_HEXA_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -112,12 +112,14 @@ int pt_launch_interact_hexa(const Bucket* bk, int64_t nb, const TreeNode* nodes,
 // Fused walk + interactions of the HEXADECAPOLE build (FP64), counters as
 // partials of pt_hexa_gravity_blocks(nb) work-groups.
 int64_t pt_hexa_gravity_blocks(int64_t nb);
-int pt_launch_hexa_gravity(bool big_stack, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                           const HexOpen* ho, const HexNode* hn, const double* x, const double* y,
-                           const double* z, const double* m, const double* e, double gf,
-                           double gm, const double off[3], double* ax, double* ay, double* az,
-                           double* pot, unsigned long long* partials, int32_t* err,
-                           bool accumulate, hipStream_t s);
+// precision 0: FP64; otherwise the compensated-FP32 form of node() and leaf().
+int pt_launch_hexa_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
+                           const TreeNode* nodes, const HexOpen* ho, const HexNode* hn,
+                           const double* x, const double* y, const double* z, const double* m,
+                           const double* e, double gf, double gm, const double off[3],
+                           double* ax, double* ay, double* az, double* pot,
+                           unsigned long long* partials, int32_t* err, bool accumulate,
+                           hipStream_t s);
 int64_t pt_gravity_blocks(int64_t nb, int precision);
 int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t nb,
                       const TreeNode* nodes, const GravNode* gn, const GravNode32* gn32,

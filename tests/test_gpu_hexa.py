@@ -123,12 +123,33 @@ def test_hexa_lambs_meets_reference_accuracy():
     assert rms < 8e-4 and r.max() < 0.03
 
 
-def test_hexa_rejects_fp32c_and_adjoint():
+def test_hexa_fp32c_against_fp64(pair):
+    """The compensated-FP32 form of the hexadecapole build (fused path): the
+    same interactions (counters exact) within 1e-5 RMS of the FP64 GPU
+    forces, whose agreement with the oracle is pinned above."""
+    name, P, d, _ = pair
+    c64 = d.compute_forces(theta=P.theta, precision=0)
+    a64, p64 = d.accelerations()
+    c32 = d.compute_forces(theta=P.theta, precision=1)
+    a32, p32 = d.accelerations()
+    assert tuple(int(v) for v in c32) == tuple(int(v) for v in c64), name
+    r = rel_err(a32, a64)
+    rms = float(np.sqrt(np.mean(r * r)))
+    rp = np.abs(p32 - p64) / np.abs(p64)
+    print("%s hexadecapole fp32c vs fp64: rms %.3g p99 %.3g pot rms %.3g" % (
+        name, rms, np.percentile(r, 99), float(np.sqrt(np.mean(rp * rp)))))
+    assert rms <= 1e-5 and float(np.sqrt(np.mean(rp * rp))) <= 1e-5, name
+
+
+def test_hexa_rejects_fp32c_lists_and_adjoint():
     from paratreet_amd import _lib
     pos, _, m = ics.plummer(500, seed=2)
     d = _gpu(pos, m, np.full(500, 0.01), 12)
+    d.compute_forces(theta=0.7, precision=0)
+    from paratreet_amd.gravity import GravityVisitor
+    d.traverse(GravityVisitor((0.0, 0.0, 0.0), 0.7))
     with pytest.raises(_lib.PtError) as ex:
-        d.compute_forces(theta=0.7, precision=1)
+        d.interact(1)
     assert ex.value.status == _lib.PT_ERR_UNSUPPORTED
     with pytest.raises(_lib.PtError) as ex:
         d.force_vjp(np.ones((500, 3)), theta=0.7)
