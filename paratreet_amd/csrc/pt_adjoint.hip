@@ -429,11 +429,12 @@ struct TgtA32 {
 };
 
 // adjoint_list_kernel in the FP32 form (same work mapping; node moments from
-// the FP32 copies, node centres and softening decisions from FP64).
+// the FP32 copies, node centres and softening decisions from the FP64 fields
+// of the same records).
 __global__ void __launch_bounds__(kAdjThreads)
 adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
-                      const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
-                      const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+                      const TreeNode* __restrict__ nodes, const GravNode32* __restrict__ gn32,
+                      const double* __restrict__ px,
                       const double* __restrict__ py, const double* __restrict__ pz,
                       const double* __restrict__ pm, const double* __restrict__ pe,
                       const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
@@ -881,7 +882,7 @@ int pt_launch_adjoint_lists(int precision, const Bucket* bk, int64_t nb, const T
   const unsigned grid = (unsigned)((nb + kAdjWaves - 1) / kAdjWaves);
   if (precision != 0)  // the compensated-FP32 precision
     adjoint_list32_kernel<<<grid, kAdjThreads, 0, s>>>(
-        bk, nb, nodes, gn, gn32, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2],
+        bk, nb, nodes, gn32, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2],
         wx, wy, wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
   else
     adjoint_list_kernel<<<grid, kAdjThreads, 0, s>>>(

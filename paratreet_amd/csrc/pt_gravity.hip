@@ -769,7 +769,7 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
                                          const GravNode32* __restrict__ gn32, const Bucket& b,
                                          const BucketF& bf, double ox, double oy, double oz,
                                          double ccx, double ccy, double ccz) {
-  const bool act = __lane_id() < nsrc;
+  const bool act = (int)__lane_id() < nsrc;
   bool nr = false;
   SrcC<F32> r{};
   if (act) r = node_record<F32>(id, gn, gn32, b, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);

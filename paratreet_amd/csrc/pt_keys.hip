@@ -37,7 +37,7 @@ bbox_partial_kernel(const double* __restrict__ x, const double* __restrict__ y,
   }
   __syncthreads();
   for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+    if ((int)threadIdx.x < w) {
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const double a = s[k][threadIdx.x + w];
@@ -73,7 +73,7 @@ universe_kernel(const double* __restrict__ part, int nparts, double* __restrict_
   }
   __syncthreads();
   for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+    if ((int)threadIdx.x < w) {
       for (int k = 0; k < 3; k++) {
         const double a = s[k][threadIdx.x + w];
         if (a < s[k][threadIdx.x]) s[k][threadIdx.x] = a;

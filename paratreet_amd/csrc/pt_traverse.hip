@@ -114,7 +114,7 @@ reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblo
   for (int k = 0; k < 4; k++) s[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+    if ((int)threadIdx.x < w)
       for (int k = 0; k < 4; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
     __syncthreads();
   }
