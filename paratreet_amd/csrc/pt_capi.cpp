@@ -68,6 +68,11 @@ struct pt_ctx {
   DBuf<int64_t> tstat;  // build read-back: {nn, nb, overflow, level counts}
   std::vector<int64_t> level_off;  // internal nodes of depth d: [level_off[d], level_off[d+1])
   int32_t split_levels = PT_MAX_DEPTH;  // split launches of the next build (last depth + 1)
+  // key bits the last sort ordered (64: all; fewer: the top bits only, the
+  // build then puts each leaf in full order) and those the next force
+  // evaluation's sort will order (what the last tree's splits used)
+  int32_t sort_bits = 64;
+  int32_t sort_bits_next = 64;
   DBuf<uint32_t> flag, fpos, head, head_pos, u32_total;
   DBuf<int32_t> bucket_node;
   DBuf<double> nd;
@@ -444,19 +449,38 @@ pt_status pt_assign_keys(pt_ctx* c) {
   return PT_OK;
 }
 
-pt_status pt_sort(pt_ctx* c) {
+}  // extern "C"
+
+namespace {
+
+// Sorts on the top `bits` key bits (64: the full key, then the sorted copies
+// of the particles are gathered; fewer: pt_build_tree finishes the order per
+// leaf and gathers).
+pt_status sort_stage(pt_ctx* c, int bits) {
   pt_status st = check_ctx(c);
   if (st) return st;
   if ((st = need(c, S_KEYS, "pt_sort"))) return st;
   if ((st = ensure_scratch(c, pt_sort_scratch_bytes(c->n)))) return st;
   Timer t(c, PT_T_SORT);
-  PT_TRY_LAUNCH(c, pt_radix_sort(c->keys.p, c->skeys.p, c->perm.p, c->n, 0, 64, c->scratch.p,
-                                 c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->prec.p, c->xs.p, c->ys.p, c->zs.p,
-                                    c->ms.p, c->es.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_radix_sort(c->keys.p, c->skeys.p, c->perm.p, c->n, 64 - bits, 64,
+                                 c->scratch.p, c->stream));
+  c->sort_bits = bits;
+  if (bits >= 64)
+    PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->prec.p, c->xs.p, c->ys.p, c->zs.p,
+                                      c->ms.p, c->es.p, c->stream));
   c->stage = S_SORTED;
   return PT_OK;
 }
+
+// pt_build_tree's answer when a top-bits sort did not cover the tree's
+// splits (internal; the caller sorts the full keys and builds again).
+constexpr pt_status kRedoFullSort = (pt_status)1;
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_sort(pt_ctx* c) { return sort_stage(c, 64); }
 
 // Level-parallel build driven from the device: one split launch per level
 // (internal-node counts stay on the device), a host check every 4 levels, and
@@ -498,10 +522,26 @@ pt_status pt_build_tree(pt_ctx* c) {
     PT_TRY_LAUNCH(c, pt_launch_buckets(tr, node_cap, c->n, levels, c->level_cnt.p, c->derr.p,
                                        c->head.p, c->head_pos.p, c->scratch.p, c->u32_total.p,
                                        c->bucket_node.p, c->tstat.p, c->stream));
+    if (c->sort_bits < 64) {
+      // top-bits sort: each leaf's particles into full key order, then the
+      // sorted copies (idempotent if a retry below builds again)
+      PT_TRY_LAUNCH(c, pt_launch_leaf_fixup(tr, node_cap, c->tstat.p, c->skeys.p, c->perm.p,
+                                            c->stream));
+      PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->prec.p, c->xs.p, c->ys.p, c->zs.p,
+                                        c->ms.p, c->es.p, c->stream));
+    }
     int64_t stat[3 + PT_MAX_DEPTH + 2] = {0};
     PT_TRY_HIP(c, hipMemcpyAsync(stat, c->tstat.p, sizeof(stat), hipMemcpyDeviceToHost,
                                  c->stream));
     PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->sort_bits < 64) {
+      // valid when every split compared only sorted bits: a split creating
+      // depth d + 1 reads the top 3 (d + 1) + 1 bits
+      int dmax = 0;
+      for (int d = 0; d <= PT_MAX_DEPTH; d++)
+        if (stat[3 + d] > 0) dmax = d + 1;
+      if (stat[2] || stat[0] == 0 || 3 * dmax + 1 > c->sort_bits) return kRedoFullSort;
+    }
     if (stat[2]) {
       if (++capacity_retries > 3) return fail(c, PT_ERR_CAPACITY, "octree node table overflow");
       level_cap *= 4;
@@ -532,6 +572,32 @@ pt_status pt_build_tree(pt_ctx* c) {
   c->stage = S_TREE;
   return PT_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Sort + build of a force evaluation.  The sort orders only the key bits the
+// previous tree's splits used (the first evaluation, buckets above kFixupMax
+// and deeper trees use all 64, the last after a failed check); the result is
+// the full sort either way.
+pt_status sort_and_build(pt_ctx* c) {
+  pt_status st;
+  const int bits = c->cfg.max_particles_per_leaf <= kFixupMax ? c->sort_bits_next : 64;
+  if ((st = sort_stage(c, bits))) return st;
+  st = pt_build_tree(c);
+  if (st == kRedoFullSort) {
+    if ((st = sort_stage(c, 64))) return st;
+    st = pt_build_tree(c);
+  }
+  if (st) return st;
+  c->sort_bits_next = std::min(64, ((3 * c->max_depth + 1 + 7) / 8) * 8);
+  return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 pt_status pt_upward(pt_ctx* c) {
   pt_status st = check_ctx(c);
@@ -786,8 +852,7 @@ pt_status pt_compute_forces(pt_ctx* c, double theta, const double offset[3],
   pt_status st;
   if ((st = pt_remake_universe(c, nullptr))) return st;
   if ((st = pt_assign_keys(c))) return st;
-  if ((st = pt_sort(c))) return st;
-  if ((st = pt_build_tree(c))) return st;
+  if ((st = sort_and_build(c))) return st;
   if ((st = pt_upward(c))) return st;
   return gravity_fused(c, theta, offset, precision, counters);
 }
@@ -982,8 +1047,7 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
   }
   if ((st = pt_remake_universe(c, nullptr))) return st;
   if ((st = pt_assign_keys(c))) return st;
-  if ((st = pt_sort(c))) return st;
-  if ((st = pt_build_tree(c))) return st;
+  if ((st = sort_and_build(c))) return st;
   if ((st = pt_upward(c))) return st;
   if ((st = pt_traverse(c, theta, nullptr, nullptr))) return st;
   const int64_t nn = c->nn;

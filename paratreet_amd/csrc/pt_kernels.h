@@ -40,6 +40,11 @@ int pt_radix_sort(const uint64_t* src, uint64_t* keys, uint32_t* vals, int64_t n
 int pt_launch_iota(uint32_t* v, int64_t n, hipStream_t s);
 
 int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s);
+// Largest leaf the fixup after a top-bits sort handles (larger buckets use
+// the full sort).
+constexpr int kFixupMax = 64;
+int pt_launch_leaf_fixup(const PtTree& t, int64_t node_cap, const int64_t* status,
+                         uint64_t* keys, uint32_t* perm, hipStream_t s);
 int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
                           int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
                           int64_t level_cap, int32_t* overflow, hipStream_t s);

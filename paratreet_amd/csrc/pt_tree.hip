@@ -88,7 +88,11 @@ split_level_kernel(const uint64_t* __restrict__ keys, int32_t* __restrict__ leve
   const int64_t e = b + t.count[p];
   const uint64_t child_key = (t.key[p] << 3) | (uint64_t)c;
   const int64_t lo = (c == 0) ? b : first_ge(keys, b, e, left_align(child_key));
-  const int64_t hi = (c == 7) ? e : first_ge(keys, b, e, left_align(child_key + 1));
+  int64_t hi = (c == 7) ? e : first_ge(keys, b, e, left_align(child_key + 1));
+  // ordered keys give lo <= hi; keys ordered on fewer bits than this split
+  // reads (a top-bits sort the host then rejects) must still give in-range,
+  // non-negative counts
+  if (hi < lo) hi = lo;
   const int ty = node_type(hi - lo, bucket, d + 1);
   t.key[id] = child_key;
   t.begin[id] = (int32_t)lo;
@@ -149,12 +153,47 @@ bucket_ids_kernel(const PtTree t, const int64_t* __restrict__ status,
   if (i < status[0] && t.type[i] == PT_NODE_LEAF) bucket_node[head_pos[t.begin[i]]] = (int32_t)i;
 }
 
+// After a sort of the keys' top bits only (ties kept in input order), puts
+// every leaf's particles in full (key, input index) order: the sort the rest
+// of the path expects.  Exact when every split of the build used only sorted
+// bits, i.e. each leaf holds whole groups of equal sorted bits.  One thread
+// per leaf, insertion sort over its <= 64 particles; grids cover the node
+// capacity, the live count is status[0].
+__global__ void __launch_bounds__(256)
+leaf_fixup_kernel(const PtTree t, const int64_t* __restrict__ status,
+                  uint64_t* __restrict__ keys, uint32_t* __restrict__ perm) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= status[0] || t.type[i] != PT_NODE_LEAF) return;
+  const int64_t b = t.begin[i];
+  const int32_t n = t.count[i];
+  if (n < 2 || n > kFixupMax) return;
+  for (int j = 1; j < n; j++) {
+    const uint64_t k = keys[b + j];
+    const uint32_t v = perm[b + j];
+    int q = j - 1;
+    while (q >= 0 && (keys[b + q] > k || (keys[b + q] == k && perm[b + q] > v))) {
+      keys[b + q + 1] = keys[b + q];
+      perm[b + q + 1] = perm[b + q];
+      q--;
+    }
+    keys[b + q + 1] = k;
+    perm[b + q + 1] = v;
+  }
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
 int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s) {
   root_kernel<<<1, 64, 0, s>>>(n, bucket, t, level_cnt);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_leaf_fixup(const PtTree& t, int64_t node_cap, const int64_t* status,
+                         uint64_t* keys, uint32_t* perm, hipStream_t s) {
+  if (node_cap <= 0) return 0;
+  leaf_fixup_kernel<<<grid_for(node_cap), 256, 0, s>>>(t, status, keys, perm);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

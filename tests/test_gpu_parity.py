@@ -303,6 +303,43 @@ def test_run_matches_oracle_iterations():
     assert np.abs(np.asarray(cnt, dtype=np.int64) - ocnt).max() <= 1e-6 * ocnt.max()
 
 
+def test_top_bits_sort_and_fallback():
+    """pt_compute_forces sorts only the key bits the previous tree's splits
+    read, then orders each leaf by the full key: the sorted keys, the
+    permutation and the forces equal the oracle's on the second evaluation;
+    a much deeper tree on the next input fails the check and is rebuilt
+    from the full sort with the same result."""
+    from paratreet_amd.gravity import Configuration, Driver
+    pos, _, m = ics.plummer(20000, seed=41)
+    e = np.full(20000, 0.01)
+    d = Driver(Configuration(max_particles_per_leaf=12))
+
+    def check(pos, m, e):
+        P = O.Pipeline(pos, m, e, bucket=12, theta=0.7)
+        sk, perm = d.sorted()
+        assert np.array_equal(sk, P.sorted_keys)
+        assert np.array_equal(perm.astype(np.int64), P.perm.astype(np.int64))
+        acc, _ = d.accelerations()
+        oacc, _, ocnt = P.gravity_unsorted()
+        assert rel_err(acc, oacc).max() <= FP64_TOL
+        return ocnt
+
+    d.load(pos, m, e)
+    d.compute_forces(theta=0.7, precision=0)   # full sort
+    cnt = d.compute_forces(theta=0.7, precision=0)  # top bits + leaf order
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in check(pos, m, e))
+    assert d.tree_size()[2] <= 10
+    # a tight clump with distant outliers: a far deeper tree
+    rng = np.random.default_rng(3)
+    deep = rng.standard_normal((20000, 3)) * 1e-3
+    deep[:8] = rng.uniform(-10.0, 10.0, (8, 3))
+    d.load(deep, m, e)
+    cnt = d.compute_forces(theta=0.7, precision=0)
+    assert d.tree_size()[2] >= 12
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in check(deep, m, e))
+    d.close()
+
+
 def test_deeper_tree_in_same_context():
     """The build launches the previous tree's depth + 1 levels of splits; a
     particle set whose tree is deeper is detected and rebuilt with every
