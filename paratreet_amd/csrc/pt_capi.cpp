@@ -525,8 +525,8 @@ pt_status pt_build_tree(pt_ctx* c) {
     if (c->sort_bits < 64) {
       // top-bits sort: each leaf's particles into full key order, then the
       // sorted copies (idempotent if a retry below builds again)
-      PT_TRY_LAUNCH(c, pt_launch_leaf_fixup(tr, node_cap, c->tstat.p, c->skeys.p, c->perm.p,
-                                            c->stream));
+      PT_TRY_LAUNCH(c, pt_launch_leaf_fixup(tr, c->n, c->bucket_node.p, c->tstat.p, c->skeys.p,
+                                            c->perm.p, c->stream));
       PT_TRY_LAUNCH(c, pt_launch_gather(c->perm.p, c->n, c->prec.p, c->xs.p, c->ys.p, c->zs.p,
                                         c->ms.p, c->es.p, c->stream));
     }

@@ -43,8 +43,9 @@ int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, h
 // Largest leaf the fixup after a top-bits sort handles (larger buckets use
 // the full sort).
 constexpr int kFixupMax = 64;
-int pt_launch_leaf_fixup(const PtTree& t, int64_t node_cap, const int64_t* status,
-                         uint64_t* keys, uint32_t* perm, hipStream_t s);
+// status: the build read-back ({nn, nb, ...}); max_buckets bounds nb.
+int pt_launch_leaf_fixup(const PtTree& t, int64_t max_buckets, const int32_t* bucket_node,
+                         const int64_t* status, uint64_t* keys, uint32_t* perm, hipStream_t s);
 int pt_launch_split_level(const uint64_t* keys, int32_t* level_nodes, int32_t* level_cnt, int d,
                           int64_t max_frontier, int bucket, const PtTree& t, int64_t node_cap,
                           int64_t level_cap, int32_t* overflow, hipStream_t s);

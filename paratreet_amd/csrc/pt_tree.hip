@@ -157,27 +157,65 @@ bucket_ids_kernel(const PtTree t, const int64_t* __restrict__ status,
 // every leaf's particles in full (key, input index) order: the sort the rest
 // of the path expects.  Exact when every split of the build used only sorted
 // bits, i.e. each leaf holds whole groups of equal sorted bits.  One thread
-// per leaf, insertion sort over its <= 64 particles; grids cover the node
-// capacity, the live count is status[0].
+// per bucket (non-empty leaf; status[1] of them): up to 16 particles are
+// sorted in registers by an odd-even transposition network, up to kFixupMax
+// by insertion in place.
+__device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+  return ka < kb || (ka == kb && va < vb);
+}
+
 __global__ void __launch_bounds__(256)
-leaf_fixup_kernel(const PtTree t, const int64_t* __restrict__ status,
-                  uint64_t* __restrict__ keys, uint32_t* __restrict__ perm) {
+leaf_fixup_kernel(const PtTree t, const int32_t* __restrict__ bucket_node,
+                  const int64_t* __restrict__ status, uint64_t* __restrict__ keys,
+                  uint32_t* __restrict__ perm) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i >= status[0] || t.type[i] != PT_NODE_LEAF) return;
-  const int64_t b = t.begin[i];
-  const int32_t n = t.count[i];
+  if (i >= status[1]) return;
+  const int32_t leaf = bucket_node[i];
+  const int64_t b = t.begin[leaf];
+  const int32_t n = t.count[leaf];
   if (n < 2 || n > kFixupMax) return;
+  if (n <= 16) {
+    uint64_t k[16];
+    uint32_t v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      k[j] = j < n ? keys[b + j] : ~0ull;
+      v[j] = j < n ? perm[b + j] : ~0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+      for (int j = r & 1; j + 1 < 16; j += 2) {
+        if (pair_less(k[j + 1], v[j + 1], k[j], v[j])) {
+          const uint64_t tk = k[j];
+          k[j] = k[j + 1];
+          k[j + 1] = tk;
+          const uint32_t tv = v[j];
+          v[j] = v[j + 1];
+          v[j + 1] = tv;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      if (j < n) {
+        keys[b + j] = k[j];
+        perm[b + j] = v[j];
+      }
+    }
+    return;
+  }
   for (int j = 1; j < n; j++) {
-    const uint64_t k = keys[b + j];
-    const uint32_t v = perm[b + j];
+    const uint64_t kj = keys[b + j];
+    const uint32_t vj = perm[b + j];
     int q = j - 1;
-    while (q >= 0 && (keys[b + q] > k || (keys[b + q] == k && perm[b + q] > v))) {
+    while (q >= 0 && pair_less(kj, vj, keys[b + q], perm[b + q])) {
       keys[b + q + 1] = keys[b + q];
       perm[b + q + 1] = perm[b + q];
       q--;
     }
-    keys[b + q + 1] = k;
-    perm[b + q + 1] = v;
+    keys[b + q + 1] = kj;
+    perm[b + q + 1] = vj;
   }
 }
 
@@ -190,10 +228,10 @@ int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, h
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_leaf_fixup(const PtTree& t, int64_t node_cap, const int64_t* status,
-                         uint64_t* keys, uint32_t* perm, hipStream_t s) {
-  if (node_cap <= 0) return 0;
-  leaf_fixup_kernel<<<grid_for(node_cap), 256, 0, s>>>(t, status, keys, perm);
+int pt_launch_leaf_fixup(const PtTree& t, int64_t max_buckets, const int32_t* bucket_node,
+                         const int64_t* status, uint64_t* keys, uint32_t* perm, hipStream_t s) {
+  if (max_buckets <= 0) return 0;
+  leaf_fixup_kernel<<<grid_for(max_buckets), 256, 0, s>>>(t, bucket_node, status, keys, perm);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
