@@ -8,10 +8,11 @@ interactions (FP32 bucket-shifted Kahan form = the headline; the plain FP64
 form is timed the same way for the ratio).
 
 value = (n_part_ints + n_node_ints) / seconds per step (counter definitions of
-src/Traverser.h:15-37).  The roofline object covers the interaction kernel:
-algorithmic FLOPs per launch (20 per P-P pair, 67 per P-C particle-node
-interaction) over its average duration, from HIP events recorded on the
-stream the kernel runs on.  Errors are measured against an FP64 direct sum on
+src/Traverser.h:15-37), the steps timed with the stage timers off.  The
+roofline object covers the interaction kernel: algorithmic FLOPs per launch
+(20 per P-P pair, 67 per P-C particle-node interaction) over its average
+duration, from HIP events recorded on the stream the kernel runs on during a
+second pass of the same number of steps with the stage timers on.  Errors are measured against an FP64 direct sum on
 4096 seeded random targets.  cpu_baseline times the oracle (CPU restatement,
 OpenMP over buckets) on the host cores over the same 1M-particle tree.
 """
@@ -55,24 +56,31 @@ def make_input(name):
 
 
 def time_path(d, theta, precision, steps, warmup):
+    """Warm-up, then `steps` force evaluations with the stage timers off (the
+    step time: nothing but the pipeline in the timed region), then `steps`
+    more with them on (HIP events around each stage on the context's stream:
+    the per-stage and fused-kernel times).  Returns (ms per step, counters,
+    mean force-stage ms, last stage times)."""
     import torch
     for _ in range(warmup):
         d.compute_forces(theta=theta, precision=precision)
-    d.set_timing(True)
-    force_ms = []
-    stage = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     cnt = None
     for _ in range(steps):
         cnt = d.compute_forces(theta=theta, precision=precision)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / steps
+    d.set_timing(True)
+    force_ms = []
+    stage = None
+    for _ in range(steps):
+        d.compute_forces(theta=theta, precision=precision)
         tm = d.timings()
         force_ms.append(tm["force"])
         stage = tm
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     d.set_timing(False)
-    ms_step = (t1 - t0) * 1000.0 / steps
     return ms_step, cnt, float(np.mean(force_ms)), stage
 
 
