@@ -613,8 +613,9 @@ pt_status pt_upward(pt_ctx* c) {
   PT_TRY_HIP(c, ensure(c->gn32, nn));
   PT_TRY_HIP(c, ensure(c->bk, std::max<int64_t>(c->nb, 1)));
   PtTree tr = tree_of(c);
+  const PtPackOut po{c->tnode.p, c->gn.p, c->gn32.p, c->bk.p, c->head_pos.p};
   PT_TRY_LAUNCH(c, pt_launch_leaf_moments(tr, nn, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p,
-                                          c->nd.p, c->stream));
+                                          c->nd.p, po, c->stream));
   // internal nodes, deepest level first: a level of more than kRunNodes
   // nodes gets its own launch, consecutive smaller ones share one work-group
   constexpr int64_t kRunNodes = 64;
@@ -630,16 +631,13 @@ pt_status pt_upward(pt_ctx* c) {
       continue;
     }
     PT_TRY_LAUNCH(c, pt_launch_upward_run(tr, c->level_nodes.p, run, c->nd.p, nn, c->ord_h.p,
-                                          c->ord_b.p, c->stream));
+                                          c->ord_b.p, po, c->stream));
     run.nlev = 0;
     PT_TRY_LAUNCH(c, pt_launch_upward_level(tr, c->level_nodes.p + a, (int32_t)(b - a), c->nd.p,
-                                            nn, c->ord_h.p, c->ord_b.p, c->stream));
+                                            nn, c->ord_h.p, c->ord_b.p, po, c->stream));
   }
   PT_TRY_LAUNCH(c, pt_launch_upward_run(tr, c->level_nodes.p, run, c->nd.p, nn, c->ord_h.p,
-                                        c->ord_b.p, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_pack(tr, c->nd.p, nn, c->es.p, c->bucket_node.p, c->nb, c->tnode.p,
-                                  c->gn.p,
-                                  c->gn32.p, c->bk.p, c->stream));
+                                        c->ord_b.p, po, c->stream));
   if (c->cfg.hexadecapole) {
     // third and fourth moments once every centre of mass is final
     PT_TRY_HIP(c, ensure(c->hraw, (int64_t)PT_HEXA_RAW * nn));

@@ -65,9 +65,19 @@ int pt_launch_hexa_level(const PtTree& t, const int32_t* level_nodes, int32_t nl
 int pt_launch_hexa_pack(const PtTree& t, const double* nd, const double* raw, int64_t nn,
                         HexNode* hn, HexOpen* ho, hipStream_t s);
 // Leaf CentroidData of every leaf and the default record of every empty leaf.
+// Where the moment kernels write each node's walk and interaction records as
+// soon as its CentroidData is final, and each bucket's record (index
+// head_pos[first particle]).
+struct PtPackOut {
+  TreeNode* tnode;
+  GravNode* gn;
+  GravNode32* gn32;
+  Bucket* bk;
+  const uint32_t* head_pos;
+};
 int pt_launch_leaf_moments(const PtTree& t, int64_t nn, const double* x, const double* y,
                            const double* z, const double* m, const double* e, double* nd,
-                           hipStream_t s);
+                           const PtPackOut& o, hipStream_t s);
 // Consecutive small levels of the upward pass (deepest first) in one
 // work-group: off/cnt of each level within level_nodes.
 #define PT_MAX_RUN 24
@@ -77,13 +87,11 @@ struct PtLevelRun {
   int32_t cnt[PT_MAX_RUN];
 };
 int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLevelRun& run,
-                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
+                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b,
+                         const PtPackOut& o, hipStream_t s);
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
-                           int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s);
-// e: the sorted softenings (for the leaves' softening bounds).
-int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const double* e,
-                   const int32_t* bucket_node, int64_t nb, TreeNode* tnode, GravNode* gn,
-                   GravNode32* gn32, Bucket* bk, hipStream_t s);
+                           int64_t nn, int32_t* ord_h, uint64_t* ord_b, const PtPackOut& o,
+                           hipStream_t s);
 
 // List-producing traversal (count pass when fill is false).  partials holds
 // pt_traverse_blocks(nb) x 4 counters, reduced by pt_launch_reduce_counters.

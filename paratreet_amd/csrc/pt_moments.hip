@@ -164,24 +164,116 @@ __device__ __forceinline__ void cd_radius(CData& d) {
   }
 }
 
+// The smallest float >= v (v >= 0, finite).
+__device__ __forceinline__ float round_up_f32(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = __int_as_float(__float_as_int(f) + 1);
+  return f;
+}
+
+// Writes node i's walk and interaction records from its final CentroidData
+// (the same values a pass over the SoA node table would read back).  Leaves
+// also get their reach and softening bounds (leaf_soft: the largest particle
+// softening) and, when non-empty, their bucket record (bucket index
+// head_pos[begin]).
+__device__ __forceinline__ void pack_node(const PtTree& t, int64_t i, const CData& d,
+                                          double leaf_soft, const PtPackOut& o) {
+  TreeNode a;
+  a.cx = d.cm[0];
+  a.cy = d.cm[1];
+  a.cz = d.cm[2];
+  a.radius = d.radius;
+  a.count = t.count[i];
+  a.first_child = t.first_child[i];
+  a.type = t.type[i];
+  a.begin = t.begin[i];
+  a.leaf_reach = 0.f;
+  a.leaf_soft = 0.f;
+  if (a.type == PT_NODE_LEAF) {
+    double r2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const double d1 = d.cm[k] - d.lo[k];
+      const double d2 = d.hi[k] - d.cm[k];
+      const double e = d1 > d2 ? d1 : d2;
+      r2 += e * e;
+    }
+    a.leaf_reach = round_up_f32(sqrt(r2));
+    a.leaf_soft = round_up_f32(leaf_soft);
+  }
+  a.pad[0] = a.pad[1] = 0;
+  o.tnode[i] = a;
+  GravNode g;
+  g.cx = d.cm[0];
+  g.cy = d.cm[1];
+  g.cz = d.cm[2];
+  g.mass = d.mass;
+  g.soft = d.soft;
+  g.qxx = d.q[0];
+  g.qxy = d.q[1];
+  g.qxz = d.q[2];
+  g.qyy = d.q[3];
+  g.qyz = d.q[4];
+  g.qzz = d.q[5];
+  g.tr = 0.5 * (g.qxx + g.qyy + g.qzz);
+  o.gn[i] = g;
+  GravNode32 h;
+  h.cx = g.cx;
+  h.cy = g.cy;
+  h.cz = g.cz;
+  h.soft64 = g.soft;
+  h.mass = (float)g.mass;
+  h.soft = (float)g.soft;
+  h.qxx = (float)g.qxx;
+  h.qxy = (float)g.qxy;
+  h.qxz = (float)g.qxz;
+  h.qyy = (float)g.qyy;
+  h.qyz = (float)g.qyz;
+  h.qzz = (float)g.qzz;
+  o.gn32[i] = h;
+  if (a.type == PT_NODE_LEAF && a.count > 0) {
+    Bucket b;
+    b.lox = d.lo[0];
+    b.loy = d.lo[1];
+    b.loz = d.lo[2];
+    b.hix = d.hi[0];
+    b.hiy = d.hi[1];
+    b.hiz = d.hi[2];
+    b.cx = d.cm[0];
+    b.cy = d.cm[1];
+    b.cz = d.cm[2];
+    b.soft = d.soft;
+    b.pad0 = 0.0;
+    b.pad1 = 0.0;
+    b.node = (int32_t)i;
+    b.begin = a.begin;
+    b.count = a.count;
+    b.pad2 = 0;
+    o.bk[o.head_pos[a.begin]] = b;
+  }
+}
+
 // One thread per node: a leaf runs the sequential particle recurrence of the
 // CentroidData leaf constructor in sorted particle order; an empty leaf stores
-// the default CentroidData; internal nodes are left to the upward pass.
+// the default CentroidData; internal nodes are left to the upward pass.  The
+// node's walk/interaction records (and a leaf's bucket record) follow.
 __global__ void __launch_bounds__(256)
 leaf_kernel(const PtTree t, int64_t nn, const double* __restrict__ x,
             const double* __restrict__ y, const double* __restrict__ z,
-            const double* __restrict__ m, const double* __restrict__ e, double* __restrict__ nd) {
+            const double* __restrict__ m, const double* __restrict__ e, double* __restrict__ nd,
+            const PtPackOut o) {
   const int64_t id = blockIdx.x * 256ll + threadIdx.x;
   if (id >= nn) return;
   const int ty = t.type[id];
   if (ty == PT_NODE_INTERNAL) return;
   CData d;
   cd_default(d);
+  double smax = 0.0;
   if (ty == PT_NODE_LEAF) {
     const int64_t b = t.begin[id];
     const int64_t n = t.count[id];
     for (int64_t i = b; i < b + n; i++) {
       const double px = x[i], py = y[i], pz = z[i];
+      smax = e[i] > smax ? e[i] : smax;
       mom_add_particle(d, px, py, pz, m[i], e[i]);
       const double p[3] = {px, py, pz};
       for (int k = 0; k < 3; k++) {
@@ -197,6 +289,7 @@ leaf_kernel(const PtTree t, int64_t nn, const double* __restrict__ x,
     }
   }
   cd_store(d, nd, nn, id);
+  pack_node(t, id, d, smax, o);
 }
 
 __device__ __forceinline__ uint64_t left_align(uint64_t k) {
@@ -217,7 +310,8 @@ constexpr int kUpThreads = 64;  // threads (= parents) per work-group of the upw
 __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* __restrict__ nd,
                                             int64_t nn, int32_t* __restrict__ ord_h,
                                             uint64_t* __restrict__ ord_b,
-                                            double (*__restrict__ slab)[kUpFields]) {
+                                            double (*__restrict__ slab)[kUpFields],
+                                            const PtPackOut& o) {
   const int32_t fc = t.first_child[p];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -316,17 +410,18 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
   }
   cd_radius(d);
   cd_store(d, nd, nn, p);
+  pack_node(t, p, d, 0.0, o);
 }
 
 // One thread per internal node of one level (deepest level first).
 __global__ void __launch_bounds__(kUpThreads)
 upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t nl,
               double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
-              uint64_t* __restrict__ ord_b) {
+              uint64_t* __restrict__ ord_b, const PtPackOut o) {
   __shared__ double slab[kUpThreads][8][kUpFields];
   const int64_t li = (int64_t)blockIdx.x * kUpThreads + threadIdx.x;
   if (li >= nl) return;
-  upward_node(t, level_nodes[li], nd, nn, ord_h, ord_b, slab[threadIdx.x]);
+  upward_node(t, level_nodes[li], nd, nn, ord_h, ord_b, slab[threadIdx.x], o);
 }
 
 // A run of consecutive small levels in one work-group, deepest first, with a
@@ -335,115 +430,14 @@ upward_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, int32_t n
 __global__ void __launch_bounds__(kUpThreads)
 upward_run_kernel(const PtTree t, const int32_t* __restrict__ level_nodes, PtLevelRun run,
                   double* __restrict__ nd, int64_t nn, int32_t* __restrict__ ord_h,
-                  uint64_t* __restrict__ ord_b) {
+                  uint64_t* __restrict__ ord_b, const PtPackOut o) {
   __shared__ double slab[kUpThreads][8][kUpFields];
   for (int k = 0; k < run.nlev; k++) {
     const int32_t off = run.off[k], cnt = run.cnt[k];
     for (int32_t li = threadIdx.x; li < cnt; li += kUpThreads)
-      upward_node(t, level_nodes[off + li], nd, nn, ord_h, ord_b, slab[threadIdx.x]);
+      upward_node(t, level_nodes[off + li], nd, nn, ord_h, ord_b, slab[threadIdx.x], o);
     __syncthreads();
   }
-}
-
-// Packs the walk and interaction records from the SoA node table.
-// The smallest float >= v (v >= 0, finite).
-__device__ __forceinline__ float round_up_f32(double v) {
-  float f = (float)v;
-  if ((double)f < v) f = __int_as_float(__float_as_int(f) + 1);
-  return f;
-}
-
-__global__ void __launch_bounds__(256)
-pack_nodes_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
-                  const double* __restrict__ pe, TreeNode* __restrict__ tnode,
-                  GravNode* __restrict__ gn, GravNode32* __restrict__ gn32) {
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i >= nn) return;
-  TreeNode a;
-  a.cx = nd[(PT_CD_CM + 0) * nn + i];
-  a.cy = nd[(PT_CD_CM + 1) * nn + i];
-  a.cz = nd[(PT_CD_CM + 2) * nn + i];
-  a.radius = nd[PT_CD_RADIUS * nn + i];
-  a.count = t.count[i];
-  a.first_child = t.first_child[i];
-  a.type = t.type[i];
-  a.begin = t.begin[i];
-  a.leaf_reach = 0.f;
-  a.leaf_soft = 0.f;
-  if (a.type == PT_NODE_LEAF) {
-    const double cm[3] = {a.cx, a.cy, a.cz};
-    double r2 = 0.0;
-    for (int k = 0; k < 3; k++) {
-      const double d1 = cm[k] - nd[(PT_CD_LO + k) * nn + i];
-      const double d2 = nd[(PT_CD_HI + k) * nn + i] - cm[k];
-      const double d = d1 > d2 ? d1 : d2;
-      r2 += d * d;
-    }
-    double smax = 0.0;
-    for (int j = 0; j < a.count; j++) {
-      const double e = pe[a.begin + j];
-      smax = e > smax ? e : smax;
-    }
-    a.leaf_reach = round_up_f32(sqrt(r2));
-    a.leaf_soft = round_up_f32(smax);
-  }
-  a.pad[0] = a.pad[1] = 0;
-  tnode[i] = a;
-  GravNode g;
-  g.cx = a.cx;
-  g.cy = a.cy;
-  g.cz = a.cz;
-  g.mass = nd[PT_CD_MASS * nn + i];
-  g.soft = nd[PT_CD_SOFT * nn + i];
-  g.qxx = nd[(PT_CD_Q + 0) * nn + i];
-  g.qxy = nd[(PT_CD_Q + 1) * nn + i];
-  g.qxz = nd[(PT_CD_Q + 2) * nn + i];
-  g.qyy = nd[(PT_CD_Q + 3) * nn + i];
-  g.qyz = nd[(PT_CD_Q + 4) * nn + i];
-  g.qzz = nd[(PT_CD_Q + 5) * nn + i];
-  g.tr = 0.5 * (g.qxx + g.qyy + g.qzz);
-  gn[i] = g;
-  GravNode32 h;
-  h.cx = g.cx;
-  h.cy = g.cy;
-  h.cz = g.cz;
-  h.soft64 = g.soft;
-  h.mass = (float)g.mass;
-  h.soft = (float)g.soft;
-  h.qxx = (float)g.qxx;
-  h.qxy = (float)g.qxy;
-  h.qxz = (float)g.qxz;
-  h.qyy = (float)g.qyy;
-  h.qyz = (float)g.qyz;
-  h.qzz = (float)g.qzz;
-  gn32[i] = h;
-}
-
-__global__ void __launch_bounds__(256)
-pack_buckets_kernel(const PtTree t, const double* __restrict__ nd, int64_t nn,
-                    const int32_t* __restrict__ bucket_node, int64_t nb,
-                    Bucket* __restrict__ bk) {
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  if (i >= nb) return;
-  const int32_t id = bucket_node[i];
-  Bucket b;
-  b.lox = nd[(PT_CD_LO + 0) * nn + id];
-  b.loy = nd[(PT_CD_LO + 1) * nn + id];
-  b.loz = nd[(PT_CD_LO + 2) * nn + id];
-  b.hix = nd[(PT_CD_HI + 0) * nn + id];
-  b.hiy = nd[(PT_CD_HI + 1) * nn + id];
-  b.hiz = nd[(PT_CD_HI + 2) * nn + id];
-  b.cx = nd[(PT_CD_CM + 0) * nn + id];
-  b.cy = nd[(PT_CD_CM + 1) * nn + id];
-  b.cz = nd[(PT_CD_CM + 2) * nn + id];
-  b.soft = nd[PT_CD_SOFT * nn + id];
-  b.pad0 = 0.0;
-  b.pad1 = 0.0;
-  b.node = id;
-  b.begin = t.begin[id];
-  b.count = t.count[id];
-  b.pad2 = 0;
-  bk[i] = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -657,34 +651,29 @@ inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 int pt_launch_leaf_moments(const PtTree& t, int64_t nn, const double* x, const double* y,
                            const double* z, const double* m, const double* e, double* nd,
-                           hipStream_t s) {
+                           const PtPackOut& o, hipStream_t s) {
   if (nn <= 0) return 0;
-  leaf_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, x, y, z, m, e, nd);
+  leaf_kernel<<<grid_for(nn), 256, 0, s>>>(t, nn, x, y, z, m, e, nd, o);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int pt_launch_upward_run(const PtTree& t, const int32_t* level_nodes, const PtLevelRun& run,
-                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s) {
+                         double* nd, int64_t nn, int32_t* ord_h, uint64_t* ord_b,
+                         const PtPackOut& o, hipStream_t s) {
   if (run.nlev <= 0) return 0;
-  upward_run_kernel<<<1, kUpThreads, 0, s>>>(t, level_nodes, run, nd, nn, ord_h, ord_b);
+  upward_run_kernel<<<1, kUpThreads, 0, s>>>(t, level_nodes, run, nd, nn, ord_h, ord_b, o);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int pt_launch_upward_level(const PtTree& t, const int32_t* level_nodes, int32_t nl, double* nd,
-                           int64_t nn, int32_t* ord_h, uint64_t* ord_b, hipStream_t s) {
+                           int64_t nn, int32_t* ord_h, uint64_t* ord_b, const PtPackOut& o,
+                           hipStream_t s) {
   if (nl <= 0) return 0;
   upward_kernel<<<(unsigned)((nl + kUpThreads - 1) / kUpThreads), kUpThreads, 0, s>>>(
-      t, level_nodes, nl, nd, nn, ord_h, ord_b);
+      t, level_nodes, nl, nd, nn, ord_h, ord_b, o);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_pack(const PtTree& t, const double* nd, int64_t nn, const double* e,
-                   const int32_t* bucket_node, int64_t nb, TreeNode* tnode, GravNode* gn,
-                   GravNode32* gn32, Bucket* bk, hipStream_t s) {
-  if (nn > 0) pack_nodes_kernel<<<grid_for(nn), 256, 0, s>>>(t, nd, nn, e, tnode, gn, gn32);
-  if (nb > 0) pack_buckets_kernel<<<grid_for(nb), 256, 0, s>>>(t, nd, nn, bucket_node, nb, bk);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
 
 int pt_launch_hexa_leaf(const PtTree& t, const int32_t* bucket_node, int64_t nb, const double* x,
                         const double* y, const double* z, const double* m, const double* nd,
