@@ -181,8 +181,9 @@ pt_status pt_direct_sum(pt_ctx* ctx, const int64_t* targets, int64_t nt, double*
  *
  * Vector-Jacobian product of one force evaluation at the current positions:
  * for w (n x 3) it returns grad_x = (da/dx)^T w (n x 3) and
- * grad_m = (da/dm)^T w (n), on the same traversal lists as pt_traverse
- * (tree) or over all pairs (direct); accelerations are those of
+ * grad_m = (da/dm)^T w (n), over the interactions pt_traverse lists (tree:
+ * walked once per group of neighbouring buckets, each entry marked with the
+ * buckets it serves) or over all pairs (direct); accelerations are those of
  * GravityVisitor::node/leaf (examples/GravityVisitor.h:144-224), opening
  * decisions held fixed. */
 pt_status pt_force_vjp(pt_ctx* ctx, double theta, pt_force_mode mode, const double* w,
@@ -190,8 +191,8 @@ pt_status pt_force_vjp(pt_ctx* ctx, double theta, pt_force_mode mode, const doub
 
 /* Arithmetic of the tree-mode adjoint in pt_force_vjp and pt_leapfrog_adjoint
  * (default PT_FP64).  PT_FP32C evaluates the pair expressions in FP32 on
- * coordinates shifted to each bucket's box centre, with every sum over pairs
- * carried in FP64 (the form of the compensated forward).  The direct-sum
+ * coordinates shifted to the centre of each bucket group's box, with every
+ * sum over pairs carried in FP64 (the form of the compensated forward).  The direct-sum
  * adjoint is always FP64. */
 pt_status pt_set_adjoint_precision(pt_ctx* ctx, pt_precision precision);
 

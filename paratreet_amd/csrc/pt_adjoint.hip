@@ -5,8 +5,10 @@
 //   g_j  = sum_i (da_i/dx_j)^T w_i      (positions)
 //   gm_j = sum_i (da_i/dm_j)^T w_i      (masses)
 // for the tree force of GravityVisitor::node / leaf
-// (examples/GravityVisitor.h:37-160,184-224) on the same per-bucket
-// interaction lists the forward pass used, and for the FP64 direct sum.
+// (examples/GravityVisitor.h:37-160,184-224) on the interactions the forward
+// pass evaluates (the lists of pt_launch_group_traverse: per group of
+// consecutive buckets, the union of the buckets' lists with member masks), and
+// for the FP64 direct sum.
 //
 // leaf() pair (target t, source j, d = x_j + off - x_t):  a_t += m_j b(|d|) d:
 //   da_t/dd = m_j K,  K = b I + (b'/|d|) d d^T  (symmetric), so the target
@@ -39,7 +41,7 @@ namespace {
 
 constexpr int kAdjWaves = 4;
 constexpr int kAdjThreads = kAdjWaves * PT_WAVE;
-constexpr int kANT = 8;       // targets per pass over a bucket's lists
+constexpr int kANT = PT_GROUP_TARGETS;  // targets per pass over a group's lists
 constexpr int kNodeAdj = 11;  // node cotangent: cm(3), M, Q(6: xx xy xz yy yz zz), soft
 // Nodes of the top levels (ids < kHotNodes in level order) are in the lists
 // of most buckets; their cotangents are accumulated into kHotParts private
@@ -128,43 +130,69 @@ __device__ __forceinline__ SplineQ splineq_grad(double r, double r2, double twoh
 }
 
 struct TgtA {
-  double x, y, z, e, wx, wy, wz, pad;
+  double x, y, z, e, wx, wy, wz;
+  int32_t bt, pad;  // bt: the target's bucket within its group
 };
 
-// One wave per bucket.  Node sources: one list entry per lane, looping over
-// the bucket's targets (8 per pass) from LDS; the node cotangent is summed
-// over the targets of a pass before one atomic add per field.  Leaf sources:
-// flattened to one source particle per lane; the source's gradient is summed
-// over the targets of a pass before its atomic add.  Target gradients are
-// wave-reduced per pass.
+// One wave per bucket group (consecutive buckets, at most PT_GROUP_TARGETS
+// particles unless one bucket is larger; neighbouring buckets share most of
+// their sources, so a group's union list is far shorter than the sum of its
+// buckets' lists).  Node sources: one list entry per lane, looping over the
+// group's targets (PT_GROUP_TARGETS per pass) from LDS and evaluating the
+// pairs whose target bucket is in the entry's member mask; the node cotangent
+// is summed over the targets of a pass before one atomic add per field.  Leaf
+// sources: flattened to one source particle per lane (the leaf's mask
+// carried along); the source's gradient is summed over the targets of a pass
+// before its atomic adds.  Target gradients are wave-reduced per pass.
 __global__ void __launch_bounds__(kAdjThreads)
-adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
+adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
+                     const int32_t* __restrict__ gnb, int64_t ns,
+                     const TreeNode* __restrict__ nodes,
                     const GravNode* __restrict__ gn, const double* __restrict__ px,
                     const double* __restrict__ py, const double* __restrict__ pz,
                     const double* __restrict__ pm, const double* __restrict__ pe,
                     const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
-                    const int64_t* __restrict__ pc_off, const int32_t* __restrict__ pc,
+                    const uint8_t* __restrict__ ppm, const int64_t* __restrict__ pc_off,
+                    const int32_t* __restrict__ pc, const uint8_t* __restrict__ pcm,
                     double ox, double oy, double oz, const double* __restrict__ wx,
                     const double* __restrict__ wy, const double* __restrict__ wz,
                     double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
                     double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn,
                     double* __restrict__ hot, int64_t nhot) {
   __shared__ TgtA s_t[kAdjWaves][kANT];
+  __shared__ Bucket s_bk[kAdjWaves][PT_GROUP_BUCKETS];
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
+  __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
-  const int64_t bi = (int64_t)blockIdx.x * kAdjWaves + wave;
-  if (bi >= nb) return;
-  const Bucket b = bk[bi];
-  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
-  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
-  for (int g0 = 0; g0 < b.count; g0 += kANT) {
-    const int ng = (b.count - g0) < kANT ? (b.count - g0) : kANT;
+  const int64_t gi = (int64_t)blockIdx.x * kAdjWaves + wave;
+  if (gi >= ns) return;
+  const int nbk = gnb[gi];
+  if (nbk <= 0) return;
+  const int64_t b0 = gfirst[gi];
+  if (lane < nbk) s_bk[wave][lane] = bk[b0 + lane];
+  __builtin_amdgcn_wave_barrier();
+  // the group's targets are consecutive: its buckets' ranges in order
+  const int64_t first = s_bk[wave][0].begin;
+  const int nt = (int)((int64_t)s_bk[wave][nbk - 1].begin + s_bk[wave][nbk - 1].count - first);
+  const int64_t pc0 = pc_off[gi], pc1 = pc_off[gi + 1];
+  const int64_t pp0 = pp_off[gi], pp1 = pp_off[gi + 1];
+  for (int g0 = 0; g0 < nt; g0 += kANT) {
+    const int ng = (nt - g0) < kANT ? (nt - g0) : kANT;
+    // member buckets with a target in this pass
+    unsigned pass = 0;
+    for (int k = 0; k < nbk; k++) {
+      const int64_t lo = s_bk[wave][k].begin, hi = lo + s_bk[wave][k].count;
+      if (hi > first + g0 && lo < first + g0 + ng) pass |= 1u << k;
+    }
     __builtin_amdgcn_wave_barrier();
     if (lane < ng) {
-      const int64_t i = (int64_t)b.begin + g0 + lane;
-      s_t[wave][lane] = TgtA{px[i], py[i], pz[i], pe[i], wx[i], wy[i], wz[i], 0.0};
+      const int64_t i = first + g0 + lane;
+      int bt = 0;
+      for (int k = 1; k < nbk; k++)
+        if (i >= s_bk[wave][k].begin) bt = k;
+      s_t[wave][lane] = TgtA{px[i], py[i], pz[i], pe[i], wx[i], wy[i], wz[i], bt, 0};
     }
     __builtin_amdgcn_wave_barrier();
     double tg[kANT][3];
@@ -174,19 +202,24 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
     // ---- node() sources
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
       const int64_t e = e0 + lane;
-      if (e < pc1) {
+      const unsigned mask = e < pc1 ? (unsigned)pcm[e] & pass : 0u;
+      if (mask != 0u) {
         const int32_t sid = pc[e];
         const GravNode s = gn[sid];
         const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
-        const bool mono = pt_open_softening(cx, cy, cz, s.soft, b);
+        unsigned mono = 0;  // members for which openSoftening sends it to addGravity
+        for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
+          const int k = __builtin_ctz(mm);
+          if (pt_open_softening(cx, cy, cz, s.soft, s_bk[wave][k])) mono |= 1u << k;
+        }
         double lam[kNodeAdj];
 #pragma unroll
         for (int k = 0; k < kNodeAdj; k++) lam[k] = 0.0;
 #pragma unroll
         for (int t = 0; t < kANT; t++) {
-          if (t < ng) {
+          if (t < ng && ((mask >> s_t[wave][t].bt) & 1u)) {
             const TgtA tt = s_t[wave][t];
-            if (mono) {
+            if ((mono >> tt.bt) & 1u) {
               const double dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
               const double r2 = dx * dx + dy * dy + dz * dz;
               if (r2 != 0) {
@@ -262,7 +295,8 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       const int64_t l = l0 + lane;
       int32_t cnt = 0, beg = 0;
-      if (l < pp1) {
+      const unsigned msk = l < pp1 ? (unsigned)ppm[l] & pass : 0u;
+      if (msk != 0u) {
         const TreeNode nd = nodes[pp[l]];
         cnt = nd.count;
         beg = nd.begin;
@@ -272,6 +306,7 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
       __builtin_amdgcn_wave_barrier();
       s_beg[wave][lane] = beg;
       s_exc[wave][lane] = incl - cnt;
+      s_msk[wave][lane] = (int32_t)msk;
       __builtin_amdgcn_wave_barrier();
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
         const int slot = s0 + lane;
@@ -286,12 +321,13 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
             else hi = mid - 1;
           }
           const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
+          const unsigned jmask = (unsigned)s_msk[wave][lo];
           const double jx = px[j] + ox, jy = py[j] + oy, jz = pz[j] + oz;
           const double jm = pm[j], je = pe[j];
           double sgx = 0.0, sgy = 0.0, sgz = 0.0, sgm = 0.0;
 #pragma unroll
           for (int t = 0; t < kANT; t++) {
-            if (t < ng) {
+            if (t < ng && ((jmask >> s_t[wave][t].bt) & 1u)) {
               const TgtA tt = s_t[wave][t];
               const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
               const double r2 = dx * dx + dy * dy + dz * dz;
@@ -330,7 +366,7 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         const double ry = pt_wave_sum(tg[t][1]);
         const double rz = pt_wave_sum(tg[t][2]);
         if (lane == t) {
-          const int64_t i = (int64_t)b.begin + g0 + t;
+          const int64_t i = first + g0 + t;
           atomicAdd(&gx[i], rx);
           atomicAdd(&gy[i], ry);
           atomicAdd(&gz[i], rz);
@@ -342,7 +378,7 @@ adjoint_list_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
 
 // ---------------------------------------------------------------------------
 // FP32 form: the same pair expressions in FP32 on coordinates shifted to the
-// bucket's box centre (in FP64, then rounded), with the hardware reciprocal
+// centre of the group's box (in FP64, then rounded), with the hardware reciprocal
 // and reciprocal square root; every sum that spans pairs (target gradients,
 // node and source-particle cotangents) is carried in FP64.
 // ---------------------------------------------------------------------------
@@ -425,44 +461,75 @@ __device__ __forceinline__ SplineQ32 splineq_grad32(float r, float ir, float r2,
 }
 
 struct TgtA32 {
-  float x, y, z, e, wx, wy, wz, pad;
+  float x, y, z, e, wx, wy, wz;
+  int32_t bt;
 };
 
-// adjoint_list_kernel in the FP32 form (same work mapping; node moments from
+// adjoint_group_kernel in the FP32 form (same work mapping; node moments from
 // the FP32 copies, node centres and softening decisions from the FP64 fields
 // of the same records).
 __global__ void __launch_bounds__(kAdjThreads)
-adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
+adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
+                       const int32_t* __restrict__ gnb, int64_t ns,
                       const TreeNode* __restrict__ nodes, const GravNode32* __restrict__ gn32,
                       const double* __restrict__ px,
                       const double* __restrict__ py, const double* __restrict__ pz,
                       const double* __restrict__ pm, const double* __restrict__ pe,
                       const int64_t* __restrict__ pp_off, const int32_t* __restrict__ pp,
-                      const int64_t* __restrict__ pc_off, const int32_t* __restrict__ pc,
+                      const uint8_t* __restrict__ ppm, const int64_t* __restrict__ pc_off,
+                      const int32_t* __restrict__ pc, const uint8_t* __restrict__ pcm,
                       double ox, double oy, double oz, const double* __restrict__ wx,
                       const double* __restrict__ wy, const double* __restrict__ wz,
                       double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz,
                       double* __restrict__ gmass, double* __restrict__ lam_node, int64_t nn,
                       double* __restrict__ hot, int64_t nhot) {
   __shared__ TgtA32 s_t[kAdjWaves][kANT];
+  __shared__ Bucket s_bk[kAdjWaves][PT_GROUP_BUCKETS];
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
+  __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
-  const int64_t bi = (int64_t)blockIdx.x * kAdjWaves + wave;
-  if (bi >= nb) return;
-  const Bucket b = bk[bi];
-  const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
-               ccz = (b.loz + b.hiz) * 0.5;
-  const int64_t pc0 = pc_off[bi], pc1 = pc_off[bi + 1];
-  const int64_t pp0 = pp_off[bi], pp1 = pp_off[bi + 1];
-  for (int g0 = 0; g0 < b.count; g0 += kANT) {
-    const int ng = (b.count - g0) < kANT ? (b.count - g0) : kANT;
+  const int64_t gi = (int64_t)blockIdx.x * kAdjWaves + wave;
+  if (gi >= ns) return;
+  const int nbk = gnb[gi];
+  if (nbk <= 0) return;
+  const int64_t b0 = gfirst[gi];
+  if (lane < nbk) s_bk[wave][lane] = bk[b0 + lane];
+  __builtin_amdgcn_wave_barrier();
+  // the group's targets are consecutive: its buckets' ranges in order
+  const int64_t first = s_bk[wave][0].begin;
+  const int nt = (int)((int64_t)s_bk[wave][nbk - 1].begin + s_bk[wave][nbk - 1].count - first);
+  // FP32 frame: the centre of the box around the group's bucket boxes
+  double lx = s_bk[wave][0].lox, ly = s_bk[wave][0].loy, lz = s_bk[wave][0].loz;
+  double hx = s_bk[wave][0].hix, hy = s_bk[wave][0].hiy, hz = s_bk[wave][0].hiz;
+  for (int k = 1; k < nbk; k++) {
+    lx = fmin(lx, s_bk[wave][k].lox);
+    ly = fmin(ly, s_bk[wave][k].loy);
+    lz = fmin(lz, s_bk[wave][k].loz);
+    hx = fmax(hx, s_bk[wave][k].hix);
+    hy = fmax(hy, s_bk[wave][k].hiy);
+    hz = fmax(hz, s_bk[wave][k].hiz);
+  }
+  const double ccx = (lx + hx) * 0.5, ccy = (ly + hy) * 0.5, ccz = (lz + hz) * 0.5;
+  const int64_t pc0 = pc_off[gi], pc1 = pc_off[gi + 1];
+  const int64_t pp0 = pp_off[gi], pp1 = pp_off[gi + 1];
+  for (int g0 = 0; g0 < nt; g0 += kANT) {
+    const int ng = (nt - g0) < kANT ? (nt - g0) : kANT;
+    // member buckets with a target in this pass
+    unsigned pass = 0;
+    for (int k = 0; k < nbk; k++) {
+      const int64_t lo = s_bk[wave][k].begin, hi = lo + s_bk[wave][k].count;
+      if (hi > first + g0 && lo < first + g0 + ng) pass |= 1u << k;
+    }
     __builtin_amdgcn_wave_barrier();
     if (lane < ng) {
-      const int64_t i = (int64_t)b.begin + g0 + lane;
+      const int64_t i = first + g0 + lane;
+      int bt = 0;
+      for (int k = 1; k < nbk; k++)
+        if (i >= s_bk[wave][k].begin) bt = k;
       s_t[wave][lane] = TgtA32{(float)(px[i] - ccx), (float)(py[i] - ccy), (float)(pz[i] - ccz),
-                               (float)pe[i], (float)wx[i], (float)wy[i], (float)wz[i], 0.0f};
+                               (float)pe[i], (float)wx[i], (float)wy[i], (float)wz[i], bt};
     }
     __builtin_amdgcn_wave_barrier();
     double tg[kANT][3];
@@ -472,11 +539,16 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
     // ---- node() sources
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
       const int64_t e = e0 + lane;
-      if (e < pc1) {
+      const unsigned mask = e < pc1 ? (unsigned)pcm[e] & pass : 0u;
+      if (mask != 0u) {
         const int32_t sid = pc[e];
         const GravNode32 s = gn32[sid];
         const double cxd = s.cx + ox, cyd = s.cy + oy, czd = s.cz + oz;
-        const bool mono = pt_open_softening(cxd, cyd, czd, s.soft64, b);
+        unsigned mono = 0;  // members for which openSoftening sends it to addGravity
+        for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
+          const int k = __builtin_ctz(mm);
+          if (pt_open_softening(cxd, cyd, czd, s.soft64, s_bk[wave][k])) mono |= 1u << k;
+        }
         const float cx = (float)(cxd - ccx), cy = (float)(cyd - ccy), cz = (float)(czd - ccz);
         const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
         float lam[kNodeAdj];
@@ -484,9 +556,9 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
         for (int k = 0; k < kNodeAdj; k++) lam[k] = 0.0f;
 #pragma unroll
         for (int t = 0; t < kANT; t++) {
-          if (t < ng) {
+          if (t < ng && ((mask >> s_t[wave][t].bt) & 1u)) {
             const TgtA32 tt = s_t[wave][t];
-            if (mono) {
+            if ((mono >> tt.bt) & 1u) {
               const float dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
               const float r2 = dx * dx + dy * dy + dz * dz;
               if (r2 != 0.0f) {
@@ -561,7 +633,8 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       const int64_t l = l0 + lane;
       int32_t cnt = 0, beg = 0;
-      if (l < pp1) {
+      const unsigned msk = l < pp1 ? (unsigned)ppm[l] & pass : 0u;
+      if (msk != 0u) {
         const TreeNode nd = nodes[pp[l]];
         cnt = nd.count;
         beg = nd.begin;
@@ -571,6 +644,7 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
       __builtin_amdgcn_wave_barrier();
       s_beg[wave][lane] = beg;
       s_exc[wave][lane] = incl - cnt;
+      s_msk[wave][lane] = (int32_t)msk;
       __builtin_amdgcn_wave_barrier();
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
         const int slot = s0 + lane;
@@ -582,13 +656,14 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
             else hi = mid - 1;
           }
           const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
+          const unsigned jmask = (unsigned)s_msk[wave][lo];
           const float jx = (float)((px[j] + ox) - ccx), jy = (float)((py[j] + oy) - ccy),
                       jz = (float)((pz[j] + oz) - ccz);
           const float jm = (float)pm[j], je = (float)pe[j];
           float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sgm = 0.0f;
 #pragma unroll
           for (int t = 0; t < kANT; t++) {
-            if (t < ng) {
+            if (t < ng && ((jmask >> s_t[wave][t].bt) & 1u)) {
               const TgtA32 tt = s_t[wave][t];
               const float dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
               const float r2 = dx * dx + dy * dy + dz * dz;
@@ -628,7 +703,7 @@ adjoint_list32_kernel(const Bucket* __restrict__ bk, int64_t nb,
         const double ry = pt_wave_sum(tg[t][1]);
         const double rz = pt_wave_sum(tg[t][2]);
         if (lane == t) {
-          const int64_t i = (int64_t)b.begin + g0 + t;
+          const int64_t i = first + g0 + t;
           atomicAdd(&gx[i], rx);
           atomicAdd(&gy[i], ry);
           atomicAdd(&gz[i], rz);
@@ -867,27 +942,29 @@ int64_t pt_adjoint_node_scratch(int64_t nn) {
 
 int pt_node_adjoint_fields() { return kNodeAdj; }
 
-int pt_launch_adjoint_lists(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                            const GravNode* gn, const GravNode32* gn32, const double* x,
-                            const double* y, const double* z, const double* m, const double* e,
-                            const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
-                            const int32_t* pc, const double off[3], const double* wx,
-                            const double* wy, const double* wz, double* gx, double* gy,
-                            double* gz, double* gm, double* lam_node, int64_t nn, hipStream_t s) {
-  if (nb <= 0) return 0;
+int pt_launch_adjoint_groups(int precision, const Bucket* bk, const int32_t* gfirst,
+                             const int32_t* gnb, int64_t ns, const TreeNode* nodes,
+                             const GravNode* gn, const GravNode32* gn32, const double* x,
+                             const double* y, const double* z, const double* m, const double* e,
+                             const int64_t* pp_off, const int32_t* pp, const uint8_t* ppm,
+                             const int64_t* pc_off, const int32_t* pc, const uint8_t* pcm,
+                             const double off[3], const double* wx, const double* wy,
+                             const double* wz, double* gx, double* gy, double* gz, double* gm,
+                             double* lam_node, int64_t nn, hipStream_t s) {
+  if (ns <= 0) return 0;
   // lam_node holds pt_adjoint_node_scratch(nn) doubles, all zero on entry:
   // the node cotangents, then the private copies of the hot ones
   const int64_t nhot = std::min<int64_t>(nn, kHotNodes);
   double* hot = lam_node + (int64_t)kNodeAdj * nn;
-  const unsigned grid = (unsigned)((nb + kAdjWaves - 1) / kAdjWaves);
+  const unsigned grid = (unsigned)((ns + kAdjWaves - 1) / kAdjWaves);
   if (precision != 0)  // the compensated-FP32 precision
-    adjoint_list32_kernel<<<grid, kAdjThreads, 0, s>>>(
-        bk, nb, nodes, gn32, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2],
-        wx, wy, wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
+    adjoint_group32_kernel<<<grid, kAdjThreads, 0, s>>>(
+        bk, gfirst, gnb, ns, nodes, gn32, x, y, z, m, e, pp_off, pp, ppm, pc_off, pc, pcm, off[0],
+        off[1], off[2], wx, wy, wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
   else
-    adjoint_list_kernel<<<grid, kAdjThreads, 0, s>>>(
-        bk, nb, nodes, gn, x, y, z, m, e, pp_off, pp, pc_off, pc, off[0], off[1], off[2], wx, wy,
-        wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
+    adjoint_group_kernel<<<grid, kAdjThreads, 0, s>>>(
+        bk, gfirst, gnb, ns, nodes, gn, x, y, z, m, e, pp_off, pp, ppm, pc_off, pc, pcm, off[0],
+        off[1], off[2], wx, wy, wz, gx, gy, gz, gm, lam_node, nn, hot, nhot);
   hot_reduce_kernel<<<grid_for(nhot * kNodeAdj), 256, 0, s>>>(hot, nhot, lam_node, nn);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -106,6 +106,10 @@ struct pt_ctx {
   DBuf<double> seg;       // positions of the steps of one segment, 3n each
   DBuf<double> adj;       // adjoint work arrays (17n)
   DBuf<double> lam_node;  // node cotangents, field-major
+  // bucket groups and their member-masked lists (the adjoint's lists)
+  DBuf<int32_t> gfirst, gnb, gpp_cnt, gpc_cnt, gpp, gpc;
+  DBuf<uint8_t> gppm, gpcm;
+  DBuf<int64_t> gpp_off, gpc_off;
   bool traj_valid = false;
   int32_t traj_steps = 0, traj_every = 1;
   int traj_mode = 0, traj_prec = 0;
@@ -339,6 +343,9 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
   release(c->ckpt); release(c->seg); release(c->adj); release(c->lam_node);
+  release(c->gfirst); release(c->gnb); release(c->gpp_cnt); release(c->gpc_cnt);
+  release(c->gpp); release(c->gpc); release(c->gppm); release(c->gpcm);
+  release(c->gpp_off); release(c->gpc_off);
   for (int i = 0; i < PT_T_COUNT; i++) {
     if (c->ev[i][0]) (void)hipEventDestroy(c->ev[i][0]);
     if (c->ev[i][1]) (void)hipEventDestroy(c->ev[i][1]);
@@ -1030,6 +1037,52 @@ pt_status traj_force(pt_ctx* c, int mode, double theta, pt_precision prec, pt_co
   return PT_OK;
 }
 
+// Groups of consecutive buckets and, for each group, the union of its
+// buckets' interaction lists with per-entry member masks (count pass, scan,
+// fill pass), for the current tree at opening angle theta.
+pt_status group_lists(pt_ctx* c, double theta) {
+  pt_status st;
+  double gf;
+  if ((st = set_visitor(c, theta, nullptr, &gf))) return st;
+  const int64_t nb = c->nb;
+  const int64_t ns = pt_group_slots(nb);
+  if (ns <= 0) return fail(c, PT_ERR_STATE, "no buckets");
+  Timer t(c, PT_T_TRAVERSE);
+  PT_TRY_HIP(c, ensure(c->gfirst, ns));
+  PT_TRY_HIP(c, ensure(c->gnb, ns));
+  PT_TRY_HIP(c, ensure(c->gpp_cnt, ns));
+  PT_TRY_HIP(c, ensure(c->gpc_cnt, ns));
+  PT_TRY_HIP(c, ensure(c->gpp_off, ns + 1));
+  PT_TRY_HIP(c, ensure(c->gpc_off, ns + 1));
+  if ((st = ensure_scratch(c, pt_scan_scratch_bytes(ns + 1)))) return st;
+  PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_group_buckets(c->bk.p, nb, c->gfirst.p, c->gnb.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_group_traverse(false, c->tnode.p, c->bk.p, c->gfirst.p, c->gnb.p,
+                                            ns, gf, c->offset, c->gpp_cnt.p, c->gpc_cnt.p,
+                                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                            c->derr.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->gpp_cnt.p, c->gpp_off.p, ns, c->scratch.p,
+                                                c->gpp_off.p + ns, c->stream));
+  PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->gpc_cnt.p, c->gpc_off.p, ns, c->scratch.p,
+                                                c->gpc_off.p + ns, c->stream));
+  int64_t tot[2] = {0, 0};
+  int32_t herr = 0;
+  PT_TRY_HIP(c, hipMemcpyAsync(&tot[0], c->gpp_off.p + ns, 8, hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipMemcpyAsync(&tot[1], c->gpc_off.p + ns, 8, hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  if (herr) return fail(c, PT_ERR_CAPACITY, "group walk stack overflow");
+  PT_TRY_HIP(c, ensure(c->gpp, std::max<int64_t>(tot[0], 1)));
+  PT_TRY_HIP(c, ensure(c->gppm, std::max<int64_t>(tot[0], 1)));
+  PT_TRY_HIP(c, ensure(c->gpc, std::max<int64_t>(tot[1], 1)));
+  PT_TRY_HIP(c, ensure(c->gpcm, std::max<int64_t>(tot[1], 1)));
+  PT_TRY_LAUNCH(c, pt_launch_group_traverse(true, c->tnode.p, c->bk.p, c->gfirst.p, c->gnb.p,
+                                            ns, gf, c->offset, nullptr, nullptr, c->gpp_off.p,
+                                            c->gpc_off.p, c->gpp.p, c->gppm.p, c->gpc.p,
+                                            c->gpcm.p, c->derr.p, c->stream));
+  return PT_OK;
+}
+
 // g (caller order) += (da/dx)^T w, gm += (da/dm)^T w at the current
 // positions; w in caller order.
 pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, const double* wx,
@@ -1047,7 +1100,7 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
   if ((st = pt_assign_keys(c))) return st;
   if ((st = sort_and_build(c))) return st;
   if ((st = pt_upward(c))) return st;
-  if ((st = pt_traverse(c, theta, nullptr, nullptr))) return st;
+  if ((st = group_lists(c, theta))) return st;
   const int64_t nn = c->nn;
   const int64_t nscr = pt_adjoint_node_scratch(std::max<int64_t>(nn, 1));
   PT_TRY_HIP(c, ensure(c->lam_node, nscr));
@@ -1055,12 +1108,12 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
   if ((st = zero(c, c->lam_node.p, nscr))) return st;
   if ((st = zero(c, v.gsx, 4 * std::max<int64_t>(n, 1)))) return st;  // gsx gsy gsz gsm
   PT_TRY_LAUNCH(c, pt_launch_gather3(c->perm.p, n, wx, wy, wz, v.wsx, v.wsy, v.wsz, c->stream));
-  PT_TRY_LAUNCH(c, pt_launch_adjoint_lists(c->adj_prec, c->bk.p, c->nb, c->tnode.p, c->gn.p,
-                                           c->gn32.p, c->xs.p, c->ys.p,
-                                           c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
-                                           c->pc_off.p, c->pc.p, c->offset, v.wsx, v.wsy, v.wsz,
-                                           v.gsx, v.gsy, v.gsz, v.gsm, c->lam_node.p, nn,
-                                           c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_adjoint_groups(
+                       c->adj_prec, c->bk.p, c->gfirst.p, c->gnb.p, pt_group_slots(c->nb),
+                       c->tnode.p, c->gn.p, c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
+                       c->es.p, c->gpp_off.p, c->gpp.p, c->gppm.p, c->gpc_off.p, c->gpc.p,
+                       c->gpcm.p, c->offset, v.wsx, v.wsy, v.wsz, v.gsx, v.gsy, v.gsz, v.gsm,
+                       c->lam_node.p, nn, c->stream));
   PT_TRY_LAUNCH(c, pt_launch_adjoint_down(n, c->head.p, c->head_pos.p, c->bucket_node.p,
                                           c->nparent.p, c->gn.p, c->lam_node.p, nn, c->xs.p,
                                           c->ys.p, c->zs.p, c->ms.p, c->es.p, v.gsx, v.gsy,

@@ -165,20 +165,47 @@ int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const 
 int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 
-// Adjoint of the interactions (precision 0: FP64; otherwise FP32 pair
-// arithmetic on bucket-centre-shifted coordinates with FP64 sums).  lam_node
-// holds pt_node_adjoint_fields() x nn doubles (field-major) and must be zero
-// on entry; g* accumulate.
+// Bucket groups of the adjoint: runs of consecutive buckets (their particles
+// are consecutive) of at most PT_GROUP_BUCKETS buckets and, unless a single
+// bucket is larger, PT_GROUP_TARGETS particles, formed greedily inside aligned
+// windows of PT_GROUP_BUCKETS buckets.  Window w owns the slots
+// [w * PT_GROUP_BUCKETS, (w + 1) * PT_GROUP_BUCKETS); unused slots hold
+// gnb = 0.  pt_group_slots(nb) slots in all.
+#define PT_GROUP_BUCKETS 8
+#define PT_GROUP_TARGETS 16
+int64_t pt_group_slots(int64_t nb);
+int pt_launch_group_buckets(const Bucket* bk, int64_t nb, int32_t* gfirst, int32_t* gnb,
+                            hipStream_t s);
+// The walk of pt_launch_traverse for every bucket of a group at once: a node
+// is visited when some member bucket visits it, and each list entry carries
+// the member mask (bit k = the group's k-th bucket) for which the node is a
+// node() source (pc/pcm) or a leaf() source (pp/ppm).  For each member bucket
+// the entries with its bit set are exactly its pt_launch_traverse list.
+// Count pass (fill false: pp_cnt/pc_cnt per slot) and fill pass; err is set
+// when a walk stack overflows.
+int pt_launch_group_traverse(bool fill, const TreeNode* nodes, const Bucket* bk,
+                             const int32_t* gfirst, const int32_t* gnb, int64_t ns, double gf,
+                             const double off[3], int32_t* pp_cnt, int32_t* pc_cnt,
+                             const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                             uint8_t* ppm, int32_t* pc, uint8_t* pcm, int32_t* err,
+                             hipStream_t s);
+
+// Adjoint of the interactions over the group lists (precision 0: FP64;
+// otherwise FP32 pair arithmetic on group-centre-shifted coordinates with
+// FP64 sums).  lam_node holds pt_adjoint_node_scratch(nn) doubles, all zero
+// on entry (node cotangents field-major, then private copies of the hot
+// ones); g* accumulate.
 int pt_node_adjoint_fields();
-// Doubles of node-cotangent scratch the list adjoint needs for nn nodes.
 int64_t pt_adjoint_node_scratch(int64_t nn);
-int pt_launch_adjoint_lists(int precision, const Bucket* bk, int64_t nb, const TreeNode* nodes,
-                            const GravNode* gn, const GravNode32* gn32, const double* x,
-                            const double* y, const double* z, const double* m, const double* e,
-                            const int64_t* pp_off, const int32_t* pp, const int64_t* pc_off,
-                            const int32_t* pc, const double off[3], const double* wx,
-                            const double* wy, const double* wz, double* gx, double* gy,
-                            double* gz, double* gm, double* lam_node, int64_t nn, hipStream_t s);
+int pt_launch_adjoint_groups(int precision, const Bucket* bk, const int32_t* gfirst,
+                             const int32_t* gnb, int64_t ns, const TreeNode* nodes,
+                             const GravNode* gn, const GravNode32* gn32, const double* x,
+                             const double* y, const double* z, const double* m, const double* e,
+                             const int64_t* pp_off, const int32_t* pp, const uint8_t* ppm,
+                             const int64_t* pc_off, const int32_t* pc, const uint8_t* pcm,
+                             const double off[3], const double* wx, const double* wy,
+                             const double* wz, double* gx, double* gy, double* gz, double* gm,
+                             double* lam_node, int64_t nn, hipStream_t s);
 int pt_launch_adjoint_down(int64_t n, const uint32_t* head, const uint32_t* head_pos,
                            const int32_t* bucket_node, const int32_t* parent, const GravNode* gn,
                            const double* lam_node, int64_t nn, const double* x, const double* y,

@@ -121,6 +121,176 @@ reduce_counters_kernel(const unsigned long long* __restrict__ part, int64_t nblo
   if (threadIdx.x < 4) atomicAdd(&out[threadIdx.x], s[threadIdx.x][0]);
 }
 
+// Groups of consecutive buckets (see pt_group_slots): one thread per window
+// of PT_GROUP_BUCKETS buckets, greedy in bucket order.
+__global__ void __launch_bounds__(256)
+group_buckets_kernel(const Bucket* __restrict__ bk, int64_t nb, int32_t* __restrict__ gfirst,
+                     int32_t* __restrict__ gnb) {
+  const int64_t w = blockIdx.x * 256ll + threadIdx.x;
+  if (w * PT_GROUP_BUCKETS >= nb) return;
+  const int64_t b0 = w * PT_GROUP_BUCKETS;
+  const int64_t b1 = b0 + PT_GROUP_BUCKETS < nb ? b0 + PT_GROUP_BUCKETS : nb;
+  int32_t* f = gfirst + b0;
+  int32_t* c = gnb + b0;
+  int slot = 0;
+  int64_t start = b0;
+  int tot = 0;
+  for (int64_t b = b0; b < b1; b++) {
+    const int cnt = bk[b].count;
+    if (b > start && tot + cnt > PT_GROUP_TARGETS) {
+      f[slot] = (int32_t)start;
+      c[slot] = (int32_t)(b - start);
+      slot++;
+      start = b;
+      tot = 0;
+    }
+    tot += cnt;
+  }
+  f[slot] = (int32_t)start;
+  c[slot] = (int32_t)(b1 - start);
+  for (slot++; slot < PT_GROUP_BUCKETS; slot++) {
+    f[slot] = (int32_t)b0;
+    c[slot] = 0;
+  }
+}
+
+// One wavefront per bucket group.  The walk of pt_walk_step with a member
+// mask per stack entry: a popped entry's 8 children inherit the mask of the
+// buckets that opened their parent; each lane tests its child against every
+// bucket of that mask (GravityVisitor::open, the same FP64 test as the
+// per-bucket walk), so each bucket sees exactly its own decisions.
+constexpr int kGroupStack = 1024;
+
+template <bool FILL>
+__global__ void __launch_bounds__(kTravThreads)
+group_traverse_kernel(const TreeNode* __restrict__ nodes, const Bucket* __restrict__ bk,
+                      const int32_t* __restrict__ gfirst, const int32_t* __restrict__ gnb,
+                      int64_t ns, double gf, double ox, double oy, double oz,
+                      int32_t* __restrict__ pp_cnt, int32_t* __restrict__ pc_cnt,
+                      const int64_t* __restrict__ pp_off, const int64_t* __restrict__ pc_off,
+                      int32_t* __restrict__ pp, uint8_t* __restrict__ ppm,
+                      int32_t* __restrict__ pc, uint8_t* __restrict__ pcm,
+                      int32_t* __restrict__ err) {
+  __shared__ int32_t stack[kTravWaves][kGroupStack];
+  __shared__ uint8_t smask[kTravWaves][kGroupStack];
+  __shared__ Bucket s_bk[kTravWaves][PT_GROUP_BUCKETS];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t gi = pt_xcd_block(blockIdx.x, gridDim.x) * kTravWaves + wave;
+  if (gi >= ns) return;
+  const int nbk = gnb[gi];
+  if (nbk <= 0) {
+    if (!FILL && lane == 0) {
+      pp_cnt[gi] = 0;
+      pc_cnt[gi] = 0;
+    }
+    return;
+  }
+  const int64_t b0 = gfirst[gi];
+  if (lane < nbk) s_bk[wave][lane] = bk[b0 + lane];
+  __builtin_amdgcn_wave_barrier();
+  int32_t* st = stack[wave];
+  uint8_t* sm = smask[wave];
+  int32_t* ppw = FILL ? pp + pp_off[gi] : nullptr;
+  uint8_t* ppmw = FILL ? ppm + pp_off[gi] : nullptr;
+  int32_t* pcw = FILL ? pc + pc_off[gi] : nullptr;
+  uint8_t* pcmw = FILL ? pcm + pc_off[gi] : nullptr;
+  const unsigned all = (1u << nbk) - 1u;
+  int sp = 0;
+  bool overflow = false;
+  int64_t npp = 0, npc = 0;
+  {
+    const TreeNode root = nodes[0];
+    if (root.type == PT_NODE_LEAF) {
+      if (FILL && lane == 0) {
+        ppw[0] = 0;
+        ppmw[0] = (uint8_t)all;
+      }
+      npp = 1;
+    } else if (root.type == PT_NODE_INTERNAL) {
+      unsigned om = 0;
+      for (int k = 0; k < nbk; k++)
+        if (pt_gv_open(root, s_bk[wave][k], gf, ox, oy, oz)) om |= 1u << k;
+      const unsigned cm = all & ~om;
+      if (cm) {
+        if (FILL && lane == 0) {
+          pcw[0] = 0;
+          pcmw[0] = (uint8_t)cm;
+        }
+        npc = 1;
+      }
+      if (om) {
+        if (lane == 0) {
+          st[0] = root.first_child;
+          sm[0] = (uint8_t)om;
+        }
+        sp = 1;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  while (sp > 0) {
+    const int k = sp < 8 ? sp : 8;
+    sp -= k;
+    const int slot = lane >> 3;
+    int32_t node = -1, fc = -1;
+    unsigned vm = 0, om = 0, cm = 0;
+    bool leaf = false;
+    if (slot < k) {
+      node = st[sp + slot] + (lane & 7);
+      vm = sm[sp + slot];
+      const TreeNode nd = nodes[node];
+      fc = nd.first_child;
+      if (nd.type == PT_NODE_LEAF) {
+        leaf = true;
+      } else if (nd.type == PT_NODE_INTERNAL) {
+        for (unsigned m = vm; m != 0u; m &= m - 1u) {
+          const int b = __builtin_ctz(m);
+          if (pt_gv_open(nd, s_bk[wave][b], gf, ox, oy, oz)) om |= 1u << b;
+        }
+        cm = vm & ~om;
+      }
+    }
+    const unsigned long long leafm = __ballot(leaf);
+    const unsigned long long closem = __ballot(cm != 0u);
+    const unsigned long long openm = __ballot(om != 0u);
+    const int n_open = __popcll(openm);
+    if (sp + n_open > kGroupStack) {
+      overflow = true;
+      break;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (om != 0u) {
+      const int p = sp + pt_rank_in(openm);
+      st[p] = fc;
+      sm[p] = (uint8_t)om;
+    }
+    if (FILL) {
+      if (leaf) {
+        const int64_t p = npp + pt_rank_in(leafm);
+        ppw[p] = node;
+        ppmw[p] = (uint8_t)vm;
+      }
+      if (cm != 0u) {
+        const int64_t p = npc + pt_rank_in(closem);
+        pcw[p] = node;
+        pcmw[p] = (uint8_t)cm;
+      }
+    }
+    npp += __popcll(leafm);
+    npc += __popcll(closem);
+    sp += n_open;
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    if (overflow) atomicExch(err, 1);
+    if (!FILL) {
+      pp_cnt[gi] = (int32_t)npp;
+      pc_cnt[gi] = (int32_t)npc;
+    }
+  }
+}
+
 }  // namespace
 
 int64_t pt_traverse_blocks(int64_t nb) { return (nb + kTravWaves - 1) / kTravWaves; }
@@ -155,5 +325,36 @@ int pt_launch_traverse(bool fill, const TreeNode* nodes, const HexOpen* ho, cons
     else PT_TRAV(false, false);
   }
 #undef PT_TRAV
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int64_t pt_group_slots(int64_t nb) {
+  return (nb + PT_GROUP_BUCKETS - 1) / PT_GROUP_BUCKETS * PT_GROUP_BUCKETS;
+}
+
+int pt_launch_group_buckets(const Bucket* bk, int64_t nb, int32_t* gfirst, int32_t* gnb,
+                            hipStream_t s) {
+  if (nb <= 0) return 0;
+  const int64_t nw = (nb + PT_GROUP_BUCKETS - 1) / PT_GROUP_BUCKETS;
+  group_buckets_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(bk, nb, gfirst, gnb);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_group_traverse(bool fill, const TreeNode* nodes, const Bucket* bk,
+                             const int32_t* gfirst, const int32_t* gnb, int64_t ns, double gf,
+                             const double off[3], int32_t* pp_cnt, int32_t* pc_cnt,
+                             const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                             uint8_t* ppm, int32_t* pc, uint8_t* pcm, int32_t* err,
+                             hipStream_t s) {
+  if (ns <= 0) return 0;
+  const unsigned grid = (unsigned)((ns + kTravWaves - 1) / kTravWaves);
+  if (fill)
+    group_traverse_kernel<true><<<grid, kTravThreads, 0, s>>>(
+        nodes, bk, gfirst, gnb, ns, gf, off[0], off[1], off[2], pp_cnt, pc_cnt, pp_off, pc_off,
+        pp, ppm, pc, pcm, err);
+  else
+    group_traverse_kernel<false><<<grid, kTravThreads, 0, s>>>(
+        nodes, bk, gfirst, gnb, ns, gf, off[0], off[1], off[2], pp_cnt, pc_cnt, pp_off, pc_off,
+        pp, ppm, pc, pcm, err);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
