@@ -51,8 +51,9 @@ constexpr int64_t kHotNodes = 16384;
 constexpr int kHotParts = 32;
 
 // GravityVisitor::SPLINE force factor b and db/dr (examples/GravityVisitor.h:
-// 82-120); r is |d|, twoh the summed softening.
-__device__ __forceinline__ void spline_b_db(double r, double twoh, double& b, double& db) {
+// 82-120); r is |d|, dir = 1 / r, twoh the summed softening.
+__device__ __forceinline__ void spline_b_db(double r, double dir, double twoh, double& b,
+                                            double& db) {
   if (r < twoh) {
     const double dih = 2.0 / twoh;
     const double u = r * dih;
@@ -60,14 +61,13 @@ __device__ __forceinline__ void spline_b_db(double r, double twoh, double& b, do
       b = dih * dih * dih * (4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u);
       db = dih * dih * dih * dih * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
     } else {
-      const double dir = 1.0 / r;
       b = -1.0 / 15.0 * dir * dir * dir +
           dih * dih * dih * (8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u - 1.0 / 6.0 * u * u * u);
       db = 1.0 / 5.0 * dir * dir * dir * dir +
            dih * dih * dih * dih * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
     }
   } else {
-    const double a = 1.0 / r;
+    const double a = dir;
     b = a * a * a;
     db = -3.0 * b * a;
   }
@@ -80,9 +80,8 @@ struct SplineQ {
   double b, c, d, br, cr, dr_, bt, ct, dt_;
 };
 
-__device__ __forceinline__ SplineQ splineq_grad(double r, double r2, double twoh) {
+__device__ __forceinline__ SplineQ splineq_grad(double r, double dir, double r2, double twoh) {
   SplineQ s;
-  const double dir = 1.0 / r;
   if (r2 < twoh * twoh) {
     const double dih = 2.0 / twoh;
     const double u = dih * r;
@@ -223,7 +222,7 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
               const double dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
               const double r2 = dx * dx + dy * dy + dz * dz;
               if (r2 != 0) {
-                const double ir = 1.0 / sqrt(r2);
+                const double ir = pt_rsqrt64(r2);
                 const double ir3 = ir * ir * ir;
                 const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
                 const double f = -3.0 * s.mass * ir3 * ir * ir * dw;
@@ -242,8 +241,9 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
               const double rx = (tt.x - s.cx) - ox, ry = (tt.y - s.cy) - oy,
                            rz = (tt.z - s.cz) - oz;
               const double r2 = rx * rx + ry * ry + rz * rz;
-              const double r = sqrt(r2);
-              const SplineQ q = splineq_grad(r, r2, s.soft + tt.e);
+              const double ir = pt_rsqrt64(r2);
+              const double r = r2 * ir;
+              const SplineQ q = splineq_grad(r, ir, r2, s.soft + tt.e);
               const double qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
               const double qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
               const double qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
@@ -254,7 +254,6 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
               const double Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
               const double Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
               const double Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
-              const double ir = 1.0 / r;
               const double k1 = (s.mass * q.br + qir * q.dr_ - s.tr * q.cr) * ir;
               const double k2 = q.cr * ir;
               const double Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
@@ -332,11 +331,12 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
               const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
               const double r2 = dx * dx + dy * dy + dz * dz;
               if (r2 != 0) {
-                const double r = sqrt(r2);
+                const double ir = pt_rsqrt64(r2);
+                const double r = r2 * ir;
                 double bb, db;
-                spline_b_db(r, je + tt.e, bb, db);
+                spline_b_db(r, ir, je + tt.e, bb, db);
                 const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
-                const double f = db / r * dw;
+                const double f = db * ir * dw;
                 const double Kx = jm * (bb * tt.wx + f * dx);
                 const double Ky = jm * (bb * tt.wy + f * dy);
                 const double Kz = jm * (bb * tt.wz + f * dz);
@@ -770,7 +770,8 @@ direct_all_kernel(int64_t n, const double* __restrict__ x, const double* __restr
     const double r2 = dx * dx + dy * dy + dz * dz;
     if (r2 != 0) {
       double b, db;
-      spline_b_db(sqrt(r2), e[j] + te, b, db);
+      const double r = sqrt(r2);
+      spline_b_db(r, 1.0 / r, e[j] + te, b, db);
       const double f = b * m[j];
       sx += dx * f;
       sy += dy * f;
@@ -818,7 +819,7 @@ direct_adjoint_kernel(int64_t n, const double* __restrict__ x, const double* __r
     if (r2 != 0) {
       const double r = sqrt(r2);
       double b, db;
-      spline_b_db(r, e[j] + te, b, db);
+      spline_b_db(r, 1.0 / r, e[j] + te, b, db);
       const double vx = tm * wx[j] - m[j] * twx;
       const double vy = tm * wy[j] - m[j] * twy;
       const double vz = tm * wz[j] - m[j] * twz;

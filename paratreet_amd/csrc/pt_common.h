@@ -154,6 +154,19 @@ __device__ __forceinline__ double pt_wave_sum(double v) {
       (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
 }
 
+// 1/sqrt(x), x > 0, in FP64: the hardware estimate refined by two Newton
+// steps, within a few ulp of the correctly rounded 1.0 / sqrt(x) (which
+// costs a square-root and a divide expansion).
+__device__ __forceinline__ double pt_rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double t = x * y;
+  double e = fma(-t, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  t = x * y;
+  e = fma(-t, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 __device__ __forceinline__ unsigned long long pt_ballot(bool p) {
   return __ballot(p);
 }

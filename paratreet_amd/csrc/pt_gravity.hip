@@ -107,18 +107,6 @@ __device__ __forceinline__ void splineq(R dir, R r2, R twoh, R& a, R& b, R& c, R
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// 1/sqrt(x), x > 0, in FP64: the hardware estimate refined by two Newton
-// steps, within a few ulp of the correctly rounded 1.0 / sqrt(x) (which
-// costs a square-root and a divide expansion).
-__device__ __forceinline__ double rsqrt64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double t = x * y;
-  double e = fma(-t, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  t = x * y;
-  e = fma(-t, y, 1.0);
-  return fma(0.5 * y, e, y);
-}
 
 // Compensated (Kahan) FP32 accumulation.
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -539,7 +527,7 @@ __device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& 
     if (s.soft < 0.0) {
       const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double ir = rsqrt64(rsq);
+      const double ir = pt_rsqrt64(rsq);
       double f = s.mass * (ir * ir * ir);
       f = rsq != 0 ? f : 0.0;
       px = fma(dx, f, px);
@@ -548,7 +536,7 @@ __device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& 
     } else {
       const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
       const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-      const double dir = rsqrt64(rsq);
+      const double dir = pt_rsqrt64(rsq);
       double a, bb, c, d;
       splineq<double>(dir, rsq, s.soft + L.te, a, bb, c, d);
       const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
@@ -598,7 +586,7 @@ __device__ __forceinline__ void node_pair_far(
   } else {
     const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
     const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
-    const double a = rsqrt64(rsq);
+    const double a = pt_rsqrt64(rsq);
     const double bb = a * a * a;
     const double c = 3.0 * bb * a * a;
     const double d = 5.0 * c * a * a;
@@ -699,7 +687,7 @@ __device__ __forceinline__ void eval_particles(const W_t& W, const Group& g, Lan
       const SrcP64 s = src[g.s + k * g.S];
       const double dx = (s.x + ox) - L.tx, dy = (s.y + oy) - L.ty, dz = (s.z + oz) - L.tz;
       const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double a = rsqrt64(rsq);
+      const double a = pt_rsqrt64(rsq);
       const double f = (a * a * a) * s.m;
       L.ax = fma(dx, f, L.ax);
       L.ay = fma(dy, f, L.ay);
@@ -952,7 +940,7 @@ __device__ __forceinline__ void hexa_eval(const R* __restrict__ h, R m, R rx, R 
   const R rsq = fma_r(rx, rx, fma_r(ry, ry, rz * rz));
   R ir;
   if constexpr (std::is_same<R, float>::value) ir = rsqrt32(rsq);
-  else ir = rsqrt64(rsq);
+  else ir = pt_rsqrt64(rsq);
   const R ir2 = ir * ir;
   const R x2 = rx * rx, y2 = ry * ry, z2 = rz * rz;
   const R xyp = rx * ry, xzp = rx * rz, yzp = ry * rz;
@@ -1000,7 +988,7 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
     // addGravity: unsoftened monopole, no potential
     const double dx = (s.cx + ox) - L.tx, dy = (s.cy + oy) - L.ty, dz = (s.cz + oz) - L.tz;
     const double rsq = dx * dx + dy * dy + dz * dz;
-    const double ir = rsqrt64(rsq);
+    const double ir = pt_rsqrt64(rsq);
     double f = s.mass * (ir * ir * ir);
     f = rsq != 0 ? f : 0.0;
     L.ax += dx * f;
