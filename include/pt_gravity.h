@@ -236,6 +236,16 @@ pt_status pt_get_buckets(pt_ctx* ctx, int32_t* bucket_nodes);       /* DFS order
 pt_status pt_get_lists_size(pt_ctx* ctx, int64_t* n_pp, int64_t* n_pc);
 /* Lists per bucket in canonical DFS order (ascending left-aligned key). */
 pt_status pt_get_lists(pt_ctx* ctx, int64_t* pp_off, int32_t* pp, int64_t* pc_off, int32_t* pc);
+/* The bucket groups and member-masked lists the tree adjoint walks (no
+ * counterpart in the reference; read back for checking), built for the
+ * current tree at opening angle theta.  sizes = {group slots, pp entries,
+ * pc entries}; with gfirst non-NULL every array is also filled (offsets:
+ * slots + 1 entries).  Slot s holds buckets [gfirst[s], gfirst[s] + gnb[s])
+ * (gnb 0: unused); bit k of an entry's mask marks the slot's k-th bucket,
+ * entries in walk order. */
+pt_status pt_get_group_lists(pt_ctx* ctx, double theta, int64_t sizes[3], int32_t* gfirst,
+                             int32_t* gnb, int64_t* pp_off, int32_t* pp, uint8_t* ppm,
+                             int64_t* pc_off, int32_t* pc, uint8_t* pcm);
 /* Accelerations / potentials in original particle order (n x 3 and n). */
 pt_status pt_get_accelerations(pt_ctx* ctx, double* acc, double* pot);
 pt_status pt_get_state(pt_ctx* ctx, double* pos, double* vel);       /* n x 3 each */

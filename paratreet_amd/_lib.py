@@ -28,7 +28,7 @@ EXPORTS = (
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
     "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint", "pt_get_node_hexadecapole",
-    "pt_set_adjoint_precision",
+    "pt_set_adjoint_precision", "pt_get_group_lists",
 )
 
 
@@ -131,6 +131,7 @@ def load():
         "pt_get_buckets": (I32, [P, P]),
         "pt_get_lists_size": (I32, [P, P, P]),
         "pt_get_lists": (I32, [P, P, P, P, P]),
+        "pt_get_group_lists": (I32, [P, D, P, P, P, P, P, P, P, P, P]),
         "pt_get_accelerations": (I32, [P, P, P]),
         "pt_get_state": (I32, [P, P, P]),
         "pt_get_counters": (I32, [P, P]),

@@ -110,6 +110,7 @@ struct pt_ctx {
   DBuf<int32_t> gfirst, gnb, gpp_cnt, gpc_cnt, gpp, gpc;
   DBuf<uint8_t> gppm, gpcm;
   DBuf<int64_t> gpp_off, gpc_off;
+  int64_t n_gpp = 0, n_gpc = 0;
   bool traj_valid = false;
   int32_t traj_steps = 0, traj_every = 1;
   int traj_mode = 0, traj_prec = 0;
@@ -1072,6 +1073,8 @@ pt_status group_lists(pt_ctx* c, double theta) {
   PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   if (herr) return fail(c, PT_ERR_CAPACITY, "group walk stack overflow");
+  c->n_gpp = tot[0];
+  c->n_gpc = tot[1];
   PT_TRY_HIP(c, ensure(c->gpp, std::max<int64_t>(tot[0], 1)));
   PT_TRY_HIP(c, ensure(c->gppm, std::max<int64_t>(tot[0], 1)));
   PT_TRY_HIP(c, ensure(c->gpc, std::max<int64_t>(tot[1], 1)));
@@ -1388,6 +1391,42 @@ pt_status pt_get_lists(pt_ctx* c, int64_t* pp_off, int32_t* pp, int64_t* pc_off,
       std::sort(l.ids + l.off[b], l.ids + l.off[b + 1],
                 [&](int32_t a, int32_t q) { return left_aligned(a) < left_aligned(q); });
     }
+  }
+  return PT_OK;
+}
+
+pt_status pt_get_group_lists(pt_ctx* c, double theta, int64_t sizes[3], int32_t* gfirst,
+                             int32_t* gnb, int64_t* pp_off, int32_t* pp, uint8_t* ppm,
+                             int64_t* pc_off, int32_t* pc, uint8_t* pcm) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if ((st = need(c, S_MOMENTS, "pt_get_group_lists"))) return st;
+  if (c->cfg.hexadecapole)
+    return fail(c, PT_ERR_ARG, "group lists follow the quadrupole open() (no adjoint in the "
+                               "HEXADECAPOLE build)");
+  if (!sizes) return fail(c, PT_ERR_ARG, "null sizes");
+  if ((st = group_lists(c, theta))) return st;
+  const int64_t ns = pt_group_slots(c->nb);
+  sizes[0] = ns;
+  sizes[1] = c->n_gpp;
+  sizes[2] = c->n_gpc;
+  if (gfirst) {
+    if (!gnb || !pp_off || !pp || !ppm || !pc_off || !pc || !pcm)
+      return fail(c, PT_ERR_ARG, "null output array");
+    const hipMemcpyKind d2h = hipMemcpyDeviceToHost;
+    PT_TRY_HIP(c, hipMemcpyAsync(gfirst, c->gfirst.p, ns * 4, d2h, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(gnb, c->gnb.p, ns * 4, d2h, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(pp_off, c->gpp_off.p, (ns + 1) * 8, d2h, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(pc_off, c->gpc_off.p, (ns + 1) * 8, d2h, c->stream));
+    if (c->n_gpp > 0) {
+      PT_TRY_HIP(c, hipMemcpyAsync(pp, c->gpp.p, c->n_gpp * 4, d2h, c->stream));
+      PT_TRY_HIP(c, hipMemcpyAsync(ppm, c->gppm.p, c->n_gpp, d2h, c->stream));
+    }
+    if (c->n_gpc > 0) {
+      PT_TRY_HIP(c, hipMemcpyAsync(pc, c->gpc.p, c->n_gpc * 4, d2h, c->stream));
+      PT_TRY_HIP(c, hipMemcpyAsync(pcm, c->gpcm.p, c->n_gpc, d2h, c->stream));
+    }
+    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   }
   return PT_OK;
 }

@@ -381,6 +381,25 @@ class Driver:
         self._check(self.L.pt_get_lists(self.ctx, _ptr(pp_off), _ptr(pp), _ptr(pc_off), _ptr(pc)))
         return pp_off, pp[:npp.value], pc_off, pc[:npc.value]
 
+    def group_lists(self, theta=0.7):
+        """Bucket groups and member-masked lists of the tree adjoint for the
+        current tree: (gfirst, gnb, pp_off, pp, ppm, pc_off, pc, pcm)."""
+        sizes = np.zeros(3, np.int64)
+        self._check(self.L.pt_get_group_lists(self.ctx, float(theta), _ptr(sizes), *([None] * 8)))
+        ns, npp, npc = (int(v) for v in sizes)
+        gfirst = np.zeros(ns, np.int32)
+        gnb = np.zeros(ns, np.int32)
+        pp_off = np.zeros(ns + 1, np.int64)
+        pc_off = np.zeros(ns + 1, np.int64)
+        pp = np.zeros(max(1, npp), np.int32)
+        pc = np.zeros(max(1, npc), np.int32)
+        ppm = np.zeros(max(1, npp), np.uint8)
+        pcm = np.zeros(max(1, npc), np.uint8)
+        self._check(self.L.pt_get_group_lists(self.ctx, float(theta), _ptr(sizes), _ptr(gfirst),
+                                              _ptr(gnb), _ptr(pp_off), _ptr(pp), _ptr(ppm),
+                                              _ptr(pc_off), _ptr(pc), _ptr(pcm)))
+        return gfirst, gnb, pp_off, pp[:npp], ppm[:npp], pc_off, pc[:npc], pcm[:npc]
+
     def accelerations(self):
         acc = np.zeros((self.n, 3))
         pot = np.zeros(self.n)

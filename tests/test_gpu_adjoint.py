@@ -1,5 +1,7 @@
 """GPU tests of the hand-written adjoint (through the C ABI).
 
+* the bucket groups and member-masked lists the tree adjoint walks against
+  the per-bucket lists;
 * direct-sum VJP against the oracle's (itself pinned by finite differences);
 * tree VJP against central finite differences of the GPU tree force, the
   translation-invariance and mass-homogeneity identities, and the direct-sum
@@ -37,6 +39,59 @@ def _tree_acc(d, pos, m, e, theta):
     d.load(pos, m, e)
     d.compute_forces(theta=theta)
     return d.accelerations()[0]
+
+
+@pytest.mark.parametrize("case", ["plummer20k_b16", "halo12k_b32", "plummer300_b12"])
+def test_group_lists_match_bucket_lists(case):
+    """The adjoint walks groups of consecutive buckets: the groups partition
+    the buckets in order (at most 8 buckets and 16 targets unless one bucket
+    is larger), no node appears twice in a group's list, and for every member
+    bucket the entries carrying its bit are exactly its pt_traverse lists."""
+    from paratreet_amd.gravity import GravityVisitor
+    if case == "plummer20k_b16":
+        pos, _, m, e = _plummer(20000, 11, 0.005)
+        bucket, theta = 16, 0.7
+    elif case == "halo12k_b32":
+        pos, _, m = ics.halo_mixture(12000, n_halos=6, seed=8)
+        e = np.full(len(pos), 0.05)
+        bucket, theta = 32, 0.9
+    else:
+        pos, _, m, e = _plummer(300, 3, 0.01)
+        bucket, theta = 12, 0.5
+    d = _driver(bucket)
+    d.load(pos, m, e)
+    d.remake_universe()
+    d.assign_keys()
+    d.sort()
+    d._check(d.L.pt_build_tree(d.ctx))
+    d._check(d.L.pt_upward(d.ctx))
+    d.traverse(GravityVisitor((0.0, 0.0, 0.0), theta))
+    pp_off, pp, pc_off, pc = d.lists()
+    gfirst, gnb, gpp_off, gpp, gppm, gpc_off, gpc, gpcm = d.group_lists(theta)
+    counts = np.asarray(d.nodes()["count"])[d.buckets()]
+    nb = len(counts)
+    assert len(gnb) == -(-nb // 8) * 8
+    b_next = 0
+    for s in range(len(gnb)):
+        k = int(gnb[s])
+        if k == 0:
+            assert gpp_off[s + 1] == gpp_off[s] and gpc_off[s + 1] == gpc_off[s]
+            continue
+        assert gfirst[s] == b_next and 1 <= k <= 8
+        assert k == 1 or counts[b_next:b_next + k].sum() <= 16
+        for off, ids, msk, boff, bids in ((gpc_off, gpc, gpcm, pc_off, pc),
+                                          (gpp_off, gpp, gppm, pp_off, pp)):
+            a, z = off[s], off[s + 1]
+            gi, gm = ids[a:z], msk[a:z].astype(np.int64)
+            assert len(np.unique(gi)) == len(gi)
+            assert np.all(gm > 0) and np.all(gm < (1 << k))
+            for j in range(k):
+                b = b_next + j
+                mine = np.sort(gi[(gm >> j) & 1 == 1])
+                assert np.array_equal(mine, np.sort(bids[boff[b]:boff[b + 1]])), (s, j)
+        b_next += k
+    assert b_next == nb
+    d.close()
 
 
 def test_direct_vjp_matches_oracle():
