@@ -65,7 +65,14 @@ def _declare(L):
     L.or_buckets.restype = _I64
     L.or_gravity.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _D, _P, _P, _I64,
                              _P, _P, _P, _P, _P, ctypes.c_int]
+    L.or_gravity_soft_pairs.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P,
+                                        _P, _P, _P]
     L.or_lists.argtypes = [_P, _I64, _P, _D, _P, _P, _P, _P, _P, _P, _P]
+    L.or_traversal_counters.argtypes = [_P, _I64, _P, _D, _P, _P, ctypes.c_int]
+    L.or_dual.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _D, _P, ctypes.c_int32,
+                          ctypes.c_int32, _P, _P, _P, _P, _I64, _P]
+    L.or_dual_lists.argtypes = [_P, _I64, _P, _D, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
+                                _P, _P, _P, _P]
     L.or_direct.argtypes = [_P, _P, _P, _P, _P, _I64, _P, _I64, _P, _P, _P, ctypes.c_int]
     L.or_spline.argtypes = [_D, _D, _P, _P]
     L.or_splineq.argtypes = [_D, _D, _D, _P, _P, _P, _P]
@@ -207,6 +214,29 @@ def gravity(nodes, data, x, y, z, m, eps, theta, offset=(0.0, 0.0, 0.0),
     return np.stack([ax, ay, az], axis=1), pot, cnt
 
 
+def gravity_soft_pairs(nodes, data, x, y, z, m, eps, theta, offset=(0.0, 0.0, 0.0)):
+    """gravity() over all buckets (quadrupole build) plus the number of P-C
+    target pairs that reach SPLINEQ's softened branch: (acc, pot, counters,
+    (pairs with u < 1, pairs with u >= 1))."""
+    n = len(x)
+    x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
+    ax, ay, az, pot = (np.zeros(n) for _ in range(4))
+    cnt = np.zeros(4, dtype=np.uint64)
+    sp = np.zeros(2, dtype=np.uint64)
+    lib().or_gravity_soft_pairs(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m),
+                                _p(eps), float(theta), _p(_f64(offset)), _p(ax), _p(ay), _p(az),
+                                _p(pot), _p(cnt), _p(sp))
+    return np.stack([ax, ay, az], axis=1), pot, cnt, (int(sp[0]), int(sp[1]))
+
+
+def traversal_counters(nodes, data, theta, offset=(0.0, 0.0, 0.0), nthreads=0):
+    """gravity()'s counters (quadrupole build) from the walk alone."""
+    cnt = np.zeros(4, dtype=np.uint64)
+    lib().or_traversal_counters(_p(nodes), len(nodes), _p(data), float(theta), _p(_f64(offset)),
+                                _p(cnt), int(nthreads))
+    return cnt
+
+
 def lists(nodes, data, theta, offset=(0.0, 0.0, 0.0), hexa=None):
     """Per-bucket interaction lists in DFS order: (pp_off, pp, pc_off, pc) with
     node ids into `nodes` (HEXADECAPOLE open() when hexa is given)."""
@@ -231,6 +261,42 @@ def lists(nodes, data, theta, offset=(0.0, 0.0, 0.0), hexa=None):
     pp = np.zeros(max(1, pp_off[-1]), dtype=np.int32)
     pc = np.zeros(max(1, pc_off[-1]), dtype=np.int32)
     call(_p(ppc), _p(pcc), _p(pp_off), _p(pc_off), _p(pp), _p(pc))
+    return pp_off, pp[:pp_off[-1]], pc_off, pc[:pc_off[-1]]
+
+
+def dual(nodes, data, x, y, z, m, eps, theta, source, target, offset=(0.0, 0.0, 0.0)):
+    """DualTraverser<GravityVisitor> from the node pair (source, target) (ids
+    into `nodes`): accelerations and potentials in sorted order and the
+    counters {n_node_ints, n_part_ints, n_opens, n_closes} (uint64, wrapping
+    as the reference's do)."""
+    n = len(x)
+    x, y, z, m, eps = (_f64(a) for a in (x, y, z, m, eps))
+    ax, ay, az, pot = (np.zeros(n) for _ in range(4))
+    cnt = np.zeros(4, dtype=np.uint64)
+    lib().or_dual(_p(nodes), len(nodes), _p(data), _p(x), _p(y), _p(z), _p(m), _p(eps),
+                  float(theta), _p(_f64(offset)), int(source), int(target), _p(ax), _p(ay),
+                  _p(az), _p(pot), n, _p(cnt))
+    return np.stack([ax, ay, az], axis=1), pot, cnt
+
+
+def dual_lists(nodes, data, theta, source, target, offset=(0.0, 0.0, 0.0)):
+    """The leaf() and node() calls of dual() as per-bucket lists (pp_off, pp,
+    pc_off, pc), each bucket's entries in the traversal's order."""
+    nb = len(buckets(nodes))
+    off = _f64(offset)
+    ppc = np.zeros(nb, dtype=np.int64)
+    pcc = np.zeros(nb, dtype=np.int64)
+    L = lib()
+    L.or_dual_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), int(source),
+                    int(target), _p(ppc), _p(pcc), None, None, None, None)
+    pp_off = np.zeros(nb + 1, dtype=np.int64)
+    pc_off = np.zeros(nb + 1, dtype=np.int64)
+    np.cumsum(ppc, out=pp_off[1:])
+    np.cumsum(pcc, out=pc_off[1:])
+    pp = np.zeros(max(1, pp_off[-1]), dtype=np.int32)
+    pc = np.zeros(max(1, pc_off[-1]), dtype=np.int32)
+    L.or_dual_lists(_p(nodes), len(nodes), _p(data), float(theta), _p(off), int(source),
+                    int(target), _p(ppc), _p(pcc), _p(pp_off), _p(pc_off), _p(pp), _p(pc))
     return pp_off, pp[:pp_off[-1]], pc_off, pc[:pc_off[-1]]
 
 
@@ -378,3 +444,26 @@ class Pipeline:
 
     def lists(self):
         return lists(self.nodes, self.data, self.theta, self.offset, hexa=self.hexa)
+
+    def node_id(self, key):
+        """Index into self.nodes of the node with this key."""
+        i = np.nonzero(self.nodes["key"] == np.uint64(key))[0]
+        if len(i) != 1:
+            raise KeyError(key)
+        return int(i[0])
+
+    def dual(self, source_key=1, target_key=1):
+        """Subtree::startDual's traversal from the pair of nodes with these
+        keys (root, root by default), in the caller's original order."""
+        acc, pot, cnt = dual(self.nodes, self.data, self.x, self.y, self.z, self.m, self.eps,
+                             self.theta, self.node_id(source_key), self.node_id(target_key),
+                             self.offset)
+        out_a = np.empty_like(acc)
+        out_p = np.empty_like(pot)
+        out_a[self.perm] = acc
+        out_p[self.perm] = pot
+        return out_a, out_p, cnt
+
+    def dual_lists(self, source_key=1, target_key=1):
+        return dual_lists(self.nodes, self.data, self.theta, self.node_id(source_key),
+                          self.node_id(target_key), self.offset)

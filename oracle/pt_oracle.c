@@ -750,6 +750,9 @@ typedef struct {
   int64_t npp, npc;
   int32_t *pp, *pc;
   uint64_t cnt[4];
+  /* P-C target pairs that reach SPLINEQ's softened branch (r^2 < twoh^2
+   * with openSoftening false), u < 1 and u >= 1: a coverage count for tests */
+  uint64_t soft_pc[2];
 } trav_ctx;
 
 /* GravityVisitor::open, non-HEXADECAPOLE branch (examples/GravityVisitor.h:
@@ -835,6 +838,7 @@ static void gv_node(trav_ctx* t, const or_cdata* s, const or_node* tgt, const or
     const double dir = 1.0 / sqrt(rsq);
     const double twoh = s->soft + t->e[i];
     double a, b, c, d;
+    if (rsq < twoh * twoh) t->soft_pc[(2.0 / twoh) / dir < 1.0 ? 0 : 1]++;
     or_splineq(dir, rsq, twoh, &a, &b, &c, &d);
     const double qx = s->q[0] * rx + s->q[1] * ry + s->q[2] * rz;
     const double qy = s->q[1] * rx + s->q[3] * ry + s->q[4] * rz;
@@ -940,17 +944,18 @@ static void gravity_impl(const or_node* nodes, int64_t nn, const or_cdata* d,
                          const double* m, const double* eps, double theta,
                          const double offset[3], const int32_t* bucket_sel, int64_t n_sel,
                          double* ax, double* ay, double* az, double* pot, uint64_t counters[4],
-                         int nthreads) {
+                         int nthreads, uint64_t* soft_pairs) {
   const int64_t nb = or_buckets(nodes, nn, NULL);
   int32_t* bn = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
   or_buckets(nodes, nn, bn);
   const int64_t ns = bucket_sel ? n_sel : nb;
-  uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, s0 = 0, s1 = 0;
   const double gf = gravity_factor(theta);
   const double gm = mono_factor(theta);
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) reduction(+ : c0, c1, c2, c3)
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) \
+    reduction(+ : c0, c1, c2, c3, s0, s1)
 #endif
   for (int64_t s = 0; s < ns; s++) {
     const int32_t b = bn[bucket_sel ? bucket_sel[s] : s];
@@ -972,11 +977,17 @@ static void gravity_impl(const or_node* nodes, int64_t nn, const or_cdata* d,
     c1 += t.cnt[1];
     c2 += t.cnt[2];
     c3 += t.cnt[3];
+    s0 += t.soft_pc[0];
+    s1 += t.soft_pc[1];
   }
   counters[0] = c0;
   counters[1] = c1;
   counters[2] = c2;
   counters[3] = c3;
+  if (soft_pairs) {
+    soft_pairs[0] = s0;
+    soft_pairs[1] = s1;
+  }
   free(bn);
   (void)nthreads;
 }
@@ -989,7 +1000,16 @@ void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
                 double* ax, double* ay, double* az, double* pot,
                 uint64_t counters[4], int nthreads) {
   gravity_impl(nodes, nn, d, NULL, x, y, z, m, eps, theta, offset, bucket_sel, n_sel, ax, ay, az,
-               pot, counters, nthreads);
+               pot, counters, nthreads, NULL);
+}
+
+void or_gravity_soft_pairs(const or_node* nodes, int64_t nn, const or_cdata* d,
+                           const double* x, const double* y, const double* z,
+                           const double* m, const double* eps, double theta,
+                           const double offset[3], double* ax, double* ay, double* az,
+                           double* pot, uint64_t counters[4], uint64_t soft_pairs[2]) {
+  gravity_impl(nodes, nn, d, NULL, x, y, z, m, eps, theta, offset, NULL, 0, ax, ay, az, pot,
+               counters, 0, soft_pairs);
 }
 
 void or_gravity_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
@@ -998,7 +1018,7 @@ void or_gravity_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const 
                      const int32_t* bucket_sel, int64_t n_sel, double* ax, double* ay,
                      double* az, double* pot, uint64_t counters[4], int nthreads) {
   gravity_impl(nodes, nn, d, hexa, x, y, z, m, eps, theta, offset, bucket_sel, n_sel, ax, ay, az,
-               pot, counters, nthreads);
+               pot, counters, nthreads, NULL);
 }
 
 static void lists_impl(const or_node* nodes, int64_t nn, const or_cdata* d, const double* hexa,
@@ -1030,6 +1050,38 @@ static void lists_impl(const or_node* nodes, int64_t nn, const or_cdata* d, cons
     pc_count[s] = t.npc;
   }
   free(bn);
+}
+
+void or_traversal_counters(const or_node* nodes, int64_t nn, const or_cdata* d, double theta,
+                           const double offset[3], uint64_t counters[4], int nthreads) {
+  const int64_t nb = or_buckets(nodes, nn, NULL);
+  int32_t* bn = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb > 0 ? nb : 1));
+  or_buckets(nodes, nn, bn);
+  const double gf = gravity_factor(theta);
+  uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : c0, c1, c2, c3)
+#endif
+  for (int64_t s = 0; s < nb; s++) {
+    trav_ctx t;
+    memset(&t, 0, sizeof(t));
+    t.nodes = nodes;
+    t.d = d;
+    t.gravity_factor = gf;
+    for (int k = 0; k < 3; k++) t.off[k] = offset[k];
+    visit(&t, 0, bn[s], 1); /* lists mode with no arrays: the walk and counts only */
+    c0 += t.cnt[0];
+    c1 += t.cnt[1];
+    c2 += t.cnt[2];
+    c3 += t.cnt[3];
+  }
+  counters[0] = c0;
+  counters[1] = c1;
+  counters[2] = c2;
+  counters[3] = c3;
+  free(bn);
+  (void)nthreads;
 }
 
 void or_lists(const or_node* nodes, int64_t nn, const or_cdata* d,
@@ -1264,4 +1316,208 @@ void or_kick_drift(double* x, double* y, double* z,
     y[i] += (vy[i] * dt);
     z[i] += (vz[i] * dt);
   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dual-tree traversal                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* Space::enclose(A, B) as defined in the header comment: A's box contains
+ * B's box, per-axis lesser_A <= lesser_B and greater_B <= greater_A. */
+static int enclose_box(const or_cdata* a, const or_cdata* b) {
+  for (int k = 0; k < 3; k++)
+    if (!(a->lo[k] <= b->lo[k]) || !(b->hi[k] <= a->hi[k])) return 0;
+  return 1;
+}
+
+/* SpatialNode::n_particles as the reference's nodes hold it: -1 for an
+ * internal node (src/Node.h:97-107), 0 for an empty leaf, the count for a
+ * leaf. */
+static int64_t n_particles(const or_node* nd) {
+  return nd->type == OR_INTERNAL ? -1 : (nd->type == OR_LEAF ? nd->count : 0);
+}
+
+typedef struct {
+  trav_ctx t;
+  const int32_t* bucket_of; /* node -> bucket ordinal, -1 if not a bucket */
+  int64_t *pp_count, *pc_count;
+  const int64_t *pp_off, *pc_off;
+  int32_t *pp, *pc;
+} dual_ctx;
+
+typedef struct {
+  int32_t s, t;
+} dual_pair;
+
+typedef struct {
+  dual_pair* v;
+  int64_t n, cap;
+} dual_stack;
+
+static void ds_push(dual_stack* st, int32_t s, int32_t t) {
+  if (st->n == st->cap) {
+    st->cap = st->cap ? 2 * st->cap : 256;
+    st->v = (dual_pair*)realloc(st->v, sizeof(dual_pair) * (size_t)st->cap);
+  }
+  st->v[st->n].s = s;
+  st->v[st->n].t = t;
+  st->n++;
+}
+
+static void add_count(uint64_t* c, int64_t v) { *c += (uint64_t)v; }
+
+/* doLeaf (src/Traverser.h:23-29): GravityVisitor::leaf plus the count
+ * n_source * n_target with the reference's n_particles values. */
+static void dual_leaf(dual_ctx* c, int32_t s, int32_t t) {
+  const or_node* sn = &c->t.nodes[s];
+  const or_node* tn = &c->t.nodes[t];
+  add_count(&c->t.cnt[1], n_particles(sn) * n_particles(tn));
+  /* leaf() loops over the target's and the source's n_particles: only two
+   * leaves with particles interact */
+  if (sn->type != OR_LEAF || tn->type != OR_LEAF) return;
+  const int32_t b = c->bucket_of[t];
+  if (c->pp_count) {
+    if (c->pp) c->pp[c->pp_off[b] + c->pp_count[b]] = s;
+    c->pp_count[b]++;
+  } else {
+    gv_leaf(&c->t, sn, tn);
+  }
+}
+
+/* doNode (src/Traverser.h:31-37): GravityVisitor::node plus the count
+ * n_target. */
+static void dual_node(dual_ctx* c, int32_t s, int32_t t) {
+  const or_node* tn = &c->t.nodes[t];
+  add_count(&c->t.cnt[0], n_particles(tn));
+  /* node() loops over the target's n_particles (-1 for an internal node) and
+   * returns at once for a source without particles */
+  if (tn->type != OR_LEAF || c->t.d[s].count == 0) return;
+  const int32_t b = c->bucket_of[t];
+  if (c->pc_count) {
+    if (c->pc) c->pc[c->pc_off[b] + c->pc_count[b]] = s;
+    c->pc_count[b]++;
+  } else {
+    gv_node(&c->t, &c->t.d[s], tn, &c->t.d[t]);
+  }
+}
+
+/* doOpen / doCell (src/Traverser.h:15-21,39-45) with the counts. */
+static int dual_open(dual_ctx* c, int32_t s, int32_t t) {
+  const int o = gv_open(&c->t, &c->t.d[s], &c->t.d[t]);
+  c->t.cnt[o ? 2 : 3]++;
+  return o;
+}
+
+static int dual_cell(dual_ctx* c, int32_t s, int32_t t) {
+  /* GravityVisitor::cell (examples/GravityVisitor.h:226-230) */
+  const int o = !enclose_box(&c->t.d[s], &c->t.d[t]);
+  c->t.cnt[o ? 2 : 3]++;
+  return o;
+}
+
+/* DualTraverser::runInvertedTraversal (src/Traverser.h:509-529). */
+static void dual_inverted(dual_ctx* c, int32_t source, int32_t target) {
+  dual_stack st = {0, 0, 0};
+  ds_push(&st, source, target);
+  while (st.n > 0) {
+    const int32_t node = st.v[--st.n].t;
+    const or_node* nd = &c->t.nodes[node];
+    if (nd->type != OR_INTERNAL) {          /* isLeaf(): Leaf or EmptyLeaf */
+      dual_leaf(c, source, node);
+    } else if (dual_open(c, source, node)) {
+      for (int j = 0; j < 8; j++) ds_push(&st, source, nd->child[j]);
+    } else {
+      dual_node(c, source, node);
+    }
+  }
+  free(st.v);
+}
+
+/* DualTraverser::doTrav (src/Traverser.h:539-608) from one (source, target)
+ * pair, GravityVisitor's flags TargetMustBeLeaf and ForceEvenDepth
+ * (examples/GravityVisitor.h:12-13).  The pairs are held on a LIFO stack as
+ * the reference's std::stack. */
+static void dual_trav(dual_ctx* c, int32_t source, int32_t target) {
+  dual_stack st = {0, 0, 0};
+  ds_push(&st, source, target);
+  while (st.n > 0) {
+    st.n--;
+    const int32_t s = st.v[st.n].s, t = st.v[st.n].t;
+    const or_node* sn = &c->t.nodes[s];
+    const or_node* tn = &c->t.nodes[t];
+    if (tn->type == OR_EMPTY) continue;
+    if (sn->type == OR_LEAF) {
+      if (tn->type != OR_INTERNAL) dual_leaf(c, s, t);
+      else dual_inverted(c, s, t);
+    } else if (sn->type == OR_INTERNAL) {
+      if (tn->type != OR_INTERNAL) {
+        if (dual_open(c, s, t)) {
+          for (int i = 0; i < 8; i++) ds_push(&st, sn->child[i], t);
+        } else {
+          dual_node(c, s, t);
+        }
+      } else if (!dual_cell(c, s, t)) {
+        dual_inverted(c, s, t);
+      } else {
+        for (int i = 0; i < 8; i++) {
+          if (tn->depth <= sn->depth) {
+            for (int j = 0; j < 8; j++) ds_push(&st, sn->child[i], tn->child[j]);
+          } else {
+            ds_push(&st, sn->child[i], t);
+          }
+        }
+      }
+    }
+    /* an EmptyLeaf source: the switch's default, nothing */
+  }
+  free(st.v);
+}
+
+static void dual_setup(dual_ctx* c, const or_node* nodes, int64_t nn, const or_cdata* d,
+                       double theta, const double offset[3], int32_t* bucket_of) {
+  memset(c, 0, sizeof(*c));
+  c->t.nodes = nodes;
+  c->t.d = d;
+  c->t.gravity_factor = gravity_factor(theta);
+  c->t.mono_factor = mono_factor(theta);
+  for (int k = 0; k < 3; k++) c->t.off[k] = offset[k];
+  for (int64_t i = 0; i < nn; i++) bucket_of[i] = -1;
+  int64_t b = 0;
+  for (int64_t i = 0; i < nn; i++)
+    if (nodes[i].type == OR_LEAF) bucket_of[i] = (int32_t)b++;
+  c->bucket_of = bucket_of;
+}
+
+void or_dual(const or_node* nodes, int64_t nn, const or_cdata* d, const double* x,
+             const double* y, const double* z, const double* m, const double* eps, double theta,
+             const double offset[3], int32_t source, int32_t target, double* ax, double* ay,
+             double* az, double* pot, int64_t n, uint64_t counters[4]) {
+  int32_t* bo = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn > 0 ? nn : 1));
+  dual_ctx c;
+  dual_setup(&c, nodes, nn, d, theta, offset, bo);
+  c.t.x = x; c.t.y = y; c.t.z = z; c.t.m = m; c.t.e = eps;
+  c.t.ax = ax; c.t.ay = ay; c.t.az = az; c.t.pot = pot;
+  for (int64_t i = 0; i < n; i++) ax[i] = ay[i] = az[i] = pot[i] = 0.0;
+  dual_trav(&c, source, target);
+  for (int k = 0; k < 4; k++) counters[k] = c.t.cnt[k];
+  free(bo);
+}
+
+void or_dual_lists(const or_node* nodes, int64_t nn, const or_cdata* d, double theta,
+                   const double offset[3], int32_t source, int32_t target, int64_t* pp_count,
+                   int64_t* pc_count, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                   int32_t* pc) {
+  int32_t* bo = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn > 0 ? nn : 1));
+  dual_ctx c;
+  dual_setup(&c, nodes, nn, d, theta, offset, bo);
+  const int64_t nb = or_buckets(nodes, nn, NULL);
+  for (int64_t b = 0; b < nb; b++) pp_count[b] = pc_count[b] = 0;
+  c.pp_count = pp_count;
+  c.pc_count = pc_count;
+  c.pp_off = pp_off;
+  c.pc_off = pc_off;
+  c.pp = pp;
+  c.pc = pc;
+  dual_trav(&c, source, target);
+  free(bo);
 }

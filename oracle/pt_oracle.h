@@ -96,6 +96,20 @@ void or_gravity(const or_node* nodes, int64_t nn, const or_cdata* d,
                 double* ax, double* ay, double* az, double* pot,
                 uint64_t counters[4], int nthreads);
 
+/* or_gravity over all buckets that also counts the P-C target pairs reaching
+ * SPLINEQ's softened branch (examples/GravityVisitor.h:40-73: r^2 < twoh^2
+ * after openSoftening was false): soft_pairs = {u < 1, u >= 1}. */
+void or_gravity_soft_pairs(const or_node* nodes, int64_t nn, const or_cdata* d,
+                           const double* x, const double* y, const double* z,
+                           const double* m, const double* eps, double theta,
+                           const double offset[3], double* ax, double* ay, double* az,
+                           double* pot, uint64_t counters[4], uint64_t soft_pairs[2]);
+
+/* The counters of or_gravity over all buckets (quadrupole build) from the
+ * walk alone, without evaluating the forces. */
+void or_traversal_counters(const or_node* nodes, int64_t nn, const or_cdata* d, double theta,
+                           const double offset[3], uint64_t counters[4], int nthreads);
+
 /* Interaction lists in DFS order, per bucket.  First call with pp/pc NULL
  * fills pp_count/pc_count (per bucket); second call with offsets (exclusive
  * prefix sums) fills node ids.  kind: leaf() sources go to pp, node() sources
@@ -161,6 +175,27 @@ void or_ewald_general(const double* x, const double* y, const double* z, int64_t
                       double mass, const double cm[3], const double q[6], double L,
                       int n_replicas, double a_fac, double ew_cut, double h_cut, double* ax,
                       double* ay, double* az, double* pot);
+
+/* DualTraverser<CentroidData, GravityVisitor> (src/Traverser.h:485-610):
+ * doTrav from the pair (source, target) of node ids -- Subtree::startDual
+ * (src/Subtree.h:233-240) is the pair (root, root) -- with
+ * runInvertedTraversal, GravityVisitor::open/node/leaf/cell and its flags
+ * TargetMustBeLeaf and ForceEvenDepth (examples/GravityVisitor.h:11-13,
+ * 226-230).  leaf() and node() act only where the reference's n_particles
+ * make them act (two leaves with particles; a node onto a leaf), while the
+ * counters add n_particles as the reference holds it (-1 for an internal
+ * node), modulo 2^64 as its unsigned long long counters do.  or_dual zeroes
+ * ax/ay/az/pot (n particles, sorted order) and applies the visitor calls in
+ * the reference's stack order; or_dual_lists returns the same calls as
+ * per-bucket lists (two calls, as or_lists). */
+void or_dual(const or_node* nodes, int64_t nn, const or_cdata* d, const double* x,
+             const double* y, const double* z, const double* m, const double* eps, double theta,
+             const double offset[3], int32_t source, int32_t target, double* ax, double* ay,
+             double* az, double* pot, int64_t n, uint64_t counters[4]);
+void or_dual_lists(const or_node* nodes, int64_t nn, const or_cdata* d, double theta,
+                   const double offset[3], int32_t source, int32_t target, int64_t* pp_count,
+                   int64_t* pc_count, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
+                   int32_t* pc);
 
 /* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
 /* (da/dx)^T w and (da/dm)^T w of or_direct over all targets (test-only

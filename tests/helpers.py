@@ -46,3 +46,36 @@ def two_cluster(n, seed=7):
 
 def oracle_pipeline(pos, mass, eps, bucket, theta):
     return O.Pipeline(pos, mass, eps, bucket=bucket, theta=theta)
+
+
+# FP32-compensated forces against the FP64 path on the same interactions:
+# RMS and per-particle maximum of the relative error.  The maximum catches a
+# near/far misclassification on a few targets, which the RMS alone would not.
+FP32C_RMS_TOL = 1e-5
+FP32C_MAX_TOL = 1e-4
+
+
+def check_fp32c(a32, a64, name=""):
+    e = rel_err(a32, a64)
+    rms = float(np.sqrt(np.mean(e * e)))
+    mx = float(e.max())
+    print("%s fp32c vs fp64: rms %.3g p99 %.3g max %.3g" % (
+        name, rms, float(np.percentile(e, 99)), mx))
+    assert rms <= FP32C_RMS_TOL, "%s: fp32c rms %.3g" % (name, rms)
+    assert mx <= FP32C_MAX_TOL, "%s: fp32c max %.3g at particle %d" % (name, mx, int(e.argmax()))
+    return rms, mx
+
+
+def gas_dark_mixture(n, seed=12, gas_fraction=0.2, eps_gas=0.5, eps_dark=0.002,
+                     gas_mass_factor=0.05):
+    """A Plummer sphere whose particles are two species, as a Tipsy file of
+    gas (large hsmooth softening, small mass) and dark matter (small eps):
+    unequal softenings let a P-C pair reach SPLINEQ's softened branch, which
+    one softening for all particles never does."""
+    from paratreet_amd import ics
+    pos, vel, m0 = ics.plummer(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    gas = rng.random(n) < gas_fraction
+    eps = np.where(gas, eps_gas, eps_dark)
+    m = np.where(gas, gas_mass_factor, 1.0) * m0
+    return pos, vel, m, eps

@@ -6,7 +6,7 @@ eps 0.005), against the oracle's C restatement on the same input:
 * CentroidData of every node bit-exact;
 * interaction counters of the fused walk exact;
 * FP64 accelerations and potentials within 1e-12 relative per particle, and
-  the FP32-compensated form within 1e-5 RMS of FP64;
+  the FP32-compensated form within 1e-5 RMS and 1e-4 maximum of FP64;
 * size-independent invariants: sorted keys ascending, the permutation a
   permutation, buckets partition the particles, the root holds the total mass.
 """
@@ -15,7 +15,7 @@ import pytest
 
 from oracle import oracle as O
 from paratreet_amd import ics
-from tests.helpers import bits, oracle_node_data, rel_err
+from tests.helpers import bits, check_fp32c, oracle_node_data, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -95,6 +95,4 @@ def test_forces_and_counters(full):
     cnt32 = d.compute_forces(theta=THETA, precision=1)
     a32, _ = d.accelerations()
     assert tuple(int(v) for v in cnt32) == tuple(int(v) for v in ocnt)
-    rms = np.sqrt(np.mean(rel_err(a32, acc) ** 2))
-    print("fullsize fp32c vs fp64: rms %.3g" % rms)
-    assert rms <= 1e-5
+    check_fp32c(a32, acc, "fullsize")

@@ -11,12 +11,12 @@ import pytest
 
 from oracle import oracle as O
 from paratreet_amd import ics
-from tests.helpers import bits, left_aligned, oracle_node_data, rel_err, two_cluster
+from tests.helpers import (bits, check_fp32c, left_aligned, oracle_node_data, rel_err,
+                           two_cluster)
 
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
-FP32C_RMS_TOL = 1e-5
 
 
 def _lambs():
@@ -158,11 +158,7 @@ def test_forces_fp32c_precision(pair):
     a64, p64 = d.accelerations()
     d.interact(1)
     a32, p32 = d.accelerations()
-    e = rel_err(a32, a64)
-    rms = float(np.sqrt(np.mean(e * e)))
-    print("%s fp32c vs fp64 (same lists): rms %.3g p99 %.3g max %.3g" % (
-        name, rms, np.percentile(e, 99), e.max()))
-    assert rms <= FP32C_RMS_TOL, name
+    check_fp32c(a32, a64, name + " (same lists)")
 
 
 def test_fused_path_matches_oracle(pair):
@@ -180,10 +176,7 @@ def test_fused_path_matches_oracle(pair):
     cnt32 = d.compute_forces(theta=P.theta, precision=1)
     a32, _ = d.accelerations()
     assert tuple(cnt32) == tuple(cnt), name
-    e32 = rel_err(a32, acc)
-    rms = float(np.sqrt(np.mean(e32 * e32)))
-    print("%s fused fp32c vs fp64: rms %.3g max %.3g" % (name, rms, e32.max()))
-    assert rms <= FP32C_RMS_TOL, name
+    check_fp32c(a32, acc, name + " fused")
 
 
 def test_walk_stack_overflow_rerun():
@@ -204,8 +197,7 @@ def test_walk_stack_overflow_rerun():
     a32, _ = d.accelerations()
     d.close()
     assert tuple(cnt32) == tuple(cnt)
-    e32 = rel_err(a32, acc)
-    assert float(np.sqrt(np.mean(e32 * e32))) <= FP32C_RMS_TOL
+    check_fp32c(a32, acc, "deep walk")
 
 
 def test_lambs_tree_error_vs_reference_direct():
