@@ -36,26 +36,34 @@ unsort_kernel(const uint32_t* __restrict__ perm, int64_t n, const double* __rest
   o3[j] = a3[i];
 }
 
-// Particle::kick then Particle::perturb (src/Particle.C:7-21): v += a dt/2
-// twice, then x += v dt.
+// Particle::kick and Particle::perturb (src/Particle.C:7-21): the kick adds
+// a dt/2 to the velocity, the perturb adds a dt/2 again and moves the
+// position by v dt.  mode: PT_STEP_KICK_DRIFT both in sequence, PT_STEP_KICK
+// the kick alone, PT_STEP_PERTURB the perturb alone (the first two in turn
+// give the same bits as the combined step).
 __global__ void __launch_bounds__(256)
-kick_drift_kernel(int64_t n, double dt, double* __restrict__ x, double* __restrict__ y,
-                  double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
-                  double* __restrict__ vz, const double* __restrict__ ax,
-                  const double* __restrict__ ay, const double* __restrict__ az) {
+kick_drift_kernel(int64_t n, double dt, int mode, double* __restrict__ x,
+                  double* __restrict__ y, double* __restrict__ z, double* __restrict__ vx,
+                  double* __restrict__ vy, double* __restrict__ vz,
+                  const double* __restrict__ ax, const double* __restrict__ ay,
+                  const double* __restrict__ az) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= n) return;
   double v0 = vx[i], v1 = vy[i], v2 = vz[i];
   const double a0 = ax[i], a1 = ay[i], a2 = az[i];
-  v0 += a0 * dt / 2;
-  v1 += a1 * dt / 2;
-  v2 += a2 * dt / 2;
-  v0 += (a0 * dt / 2);
-  v1 += (a1 * dt / 2);
-  v2 += (a2 * dt / 2);
-  x[i] += (v0 * dt);
-  y[i] += (v1 * dt);
-  z[i] += (v2 * dt);
+  if (mode != PT_STEP_PERTURB) {
+    v0 += a0 * dt / 2;
+    v1 += a1 * dt / 2;
+    v2 += a2 * dt / 2;
+  }
+  if (mode != PT_STEP_KICK) {
+    v0 += (a0 * dt / 2);
+    v1 += (a1 * dt / 2);
+    v2 += (a2 * dt / 2);
+    x[i] += (v0 * dt);
+    y[i] += (v1 * dt);
+    z[i] += (v2 * dt);
+  }
   vx[i] = v0;
   vy[i] = v1;
   vz[i] = v2;
@@ -276,11 +284,11 @@ int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const do
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int pt_launch_kick_drift(int64_t n, double dt, double* x, double* y, double* z, double* vx,
-                         double* vy, double* vz, const double* ax, const double* ay,
+int pt_launch_kick_drift(int64_t n, double dt, int mode, double* x, double* y, double* z,
+                         double* vx, double* vy, double* vz, const double* ax, const double* ay,
                          const double* az, hipStream_t s) {
   if (n <= 0) return 0;
-  kick_drift_kernel<<<grid_for(n), 256, 0, s>>>(n, dt, x, y, z, vx, vy, vz, ax, ay, az);
+  kick_drift_kernel<<<grid_for(n), 256, 0, s>>>(n, dt, mode, x, y, z, vx, vy, vz, ax, ay, az);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

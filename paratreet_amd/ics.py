@@ -15,6 +15,8 @@
 All generators draw from ``numpy.random.default_rng(seed)`` so a seed fixes the
 particle set bit for bit.
 """
+import os
+
 import numpy as np
 
 
@@ -88,29 +90,105 @@ def halo_mixture(n, n_halos=64, seed=4242, box=100.0, background=0.2, sigma=1.0,
     return pos[order].copy(), vel[order].copy(), np.full(n, 1.0 / n)
 
 
+# Tipsy record layouts (big-endian, as the utility's TipsyFile reads and
+# writes them): mass, pos[3], vel[3] and then the per-type float fields.
+# Positions and velocities are float, or double in a file written with
+# double positions and velocities (TipsyWriter's use_double, src/Writer.C:110-116).
+_TIPSY_HEADER = np.dtype(">f8,>i4,>i4,>i4,>i4,>i4,>i4")
+_TIPSY_TAIL = {"gas": ("rho", "temp", "hsmooth", "metals", "phi"),
+               "dark": ("eps", "phi"),
+               "star": ("metals", "tform", "eps", "phi")}
+# Particle::Type values (src/Particle.h:25-30)
+TIPSY_GAS, TIPSY_DARK, TIPSY_STAR = 2, 3, 1
+
+
+def _tipsy_dtype(kind, double):
+    v = ">f8" if double else ">f4"
+    fields = [("mass", ">f4"), ("pos", v, (3,)), ("vel", v, (3,))]
+    fields += [(name, ">f4") for name in _TIPSY_TAIL[kind]]
+    return np.dtype(fields)
+
+
 def read_tipsy(path):
     """Tipsy standard binary (big-endian) reader, as src/Reader.C:11-108 loads
     it: header {double time; int nbodies, ndim, nsph, ndark, nstar; pad}, then
-    gas (48 B), dark (36 B) and star (44 B) float records.  Returns a dict with
-    time, pos, vel, mass, soft (gas hsmooth / dark eps / star eps) and type
-    (2 gas, 3 dark, 1 star, as Particle::Type)."""
+    the gas, dark and star records.  Float records (48 / 36 / 44 bytes) and
+    records with double positions and velocities (72 / 60 / 68 bytes) are
+    told apart by the file size.  Returns a dict with time, pos, vel, mass,
+    soft (gas hsmooth, dark eps, 0 for stars as Reader.C sets them), type
+    (Particle::Type: 2 gas, 3 dark, 1 star), order (the file index,
+    Particle::order) and double (the record layout found)."""
+    size = os.path.getsize(path)
     with open(path, "rb") as f:
-        hdr = np.frombuffer(f.read(32), dtype=np.dtype(">f8,>i4,>i4,>i4,>i4,>i4,>i4"))[0]
+        hdr = np.frombuffer(f.read(32), dtype=_TIPSY_HEADER)[0]
         t, nbodies, _ndim, nsph, ndark, nstar = (hdr[i] for i in range(6))
-        gas = np.frombuffer(f.read(48 * nsph), dtype=np.dtype(">f4")).reshape(nsph, 12)
-        dark = np.frombuffer(f.read(36 * ndark), dtype=np.dtype(">f4")).reshape(ndark, 9)
-        star = np.frombuffer(f.read(44 * nstar), dtype=np.dtype(">f4")).reshape(nstar, 11)
-    if nsph + ndark + nstar != nbodies:
-        raise ValueError("inconsistent tipsy header")
-    mass = np.concatenate([gas[:, 0], dark[:, 0], star[:, 0]]).astype(np.float64)
-    pos = np.concatenate([gas[:, 1:4], dark[:, 1:4], star[:, 1:4]]).astype(np.float64)
-    vel = np.concatenate([gas[:, 4:7], dark[:, 4:7], star[:, 4:7]]).astype(np.float64)
-    # gas: mass pos vel rho temp hsmooth metals phi; dark: mass pos vel eps phi;
-    # star: mass pos vel metals tform eps phi (Reader.C sets soft = hsmooth,
-    # eps, and 0 for stars).
-    soft = np.concatenate([gas[:, 9], dark[:, 7], np.zeros(nstar, np.float32)]).astype(np.float64)
-    ptype = np.concatenate([np.full(nsph, 2), np.full(ndark, 3), np.full(nstar, 1)]).astype(np.int8)
-    return {"time": float(t), "pos": pos, "vel": vel, "mass": mass, "soft": soft, "type": ptype}
+        if nsph + ndark + nstar != nbodies or min(nsph, ndark, nstar) < 0:
+            raise ValueError("inconsistent tipsy header")
+        double = None
+        for d in (False, True):
+            body = sum(c * _tipsy_dtype(k, d).itemsize
+                       for c, k in ((nsph, "gas"), (ndark, "dark"), (nstar, "star")))
+            if size == 32 + body:
+                double = d
+                break
+        if double is None:
+            raise ValueError("tipsy file size %d matches neither float nor double records" % size)
+        gas = np.frombuffer(f.read(nsph * _tipsy_dtype("gas", double).itemsize),
+                            dtype=_tipsy_dtype("gas", double))
+        dark = np.frombuffer(f.read(ndark * _tipsy_dtype("dark", double).itemsize),
+                             dtype=_tipsy_dtype("dark", double))
+        star = np.frombuffer(f.read(nstar * _tipsy_dtype("star", double).itemsize),
+                             dtype=_tipsy_dtype("star", double))
+    recs = (gas, dark, star)
+    mass = np.concatenate([r["mass"] for r in recs]).astype(np.float64)
+    pos = np.concatenate([r["pos"].reshape(-1, 3) for r in recs]).astype(np.float64)
+    vel = np.concatenate([r["vel"].reshape(-1, 3) for r in recs]).astype(np.float64)
+    soft = np.concatenate([gas["hsmooth"], dark["eps"],
+                           np.zeros(nstar, np.float32)]).astype(np.float64)
+    ptype = np.concatenate([np.full(nsph, TIPSY_GAS), np.full(ndark, TIPSY_DARK),
+                            np.full(nstar, TIPSY_STAR)]).astype(np.int8)
+    return {"time": float(t), "pos": pos, "vel": vel, "mass": mass, "soft": soft,
+            "type": ptype, "order": np.arange(nbodies, dtype=np.int64), "double": double}
+
+
+def tipsy_snapshot_name(prefix, iteration):
+    """TipsyWriter::do_write's file name (src/Writer.C:112)."""
+    return "%s.%d.tipsy" % (prefix, iteration)
+
+
+def write_tipsy(path, pos, vel, mass, ptype, time=0.0, order=None, use_double=True):
+    """TipsyWriter::write / do_write (src/Writer.C:87-159): the particles
+    sorted by Particle::order, one record per particle by its type with mass,
+    position and velocity set and the other fields zero (the reference fills
+    only those three), positions and velocities in double when use_double
+    (the reference's choice when Real is double).  The header counts gas,
+    dark and star particles; a Tipsy file holds the gas records first, then
+    dark, then star, so the types must come in that order once sorted."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    vel = np.asarray(vel, dtype=np.float64).reshape(-1, 3)
+    mass = np.asarray(mass, dtype=np.float64)
+    ptype = np.asarray(ptype)
+    n = len(mass)
+    idx = np.arange(n) if order is None else np.argsort(np.asarray(order), kind="stable")
+    ptype = ptype[idx]
+    rank = np.select([ptype == TIPSY_GAS, ptype == TIPSY_DARK, ptype == TIPSY_STAR], [0, 1, 2], -1)
+    if np.any(rank < 0):
+        raise ValueError("particle type outside gas/dark/star")
+    if np.any(np.diff(rank) < 0):
+        raise ValueError("tipsy records must be gas, then dark, then star in particle order")
+    counts = [int(np.count_nonzero(rank == r)) for r in range(3)]
+    hdr = np.array([(float(time), n, 3, counts[0], counts[1], counts[2], 0)], dtype=_TIPSY_HEADER)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        start = 0
+        for kind, c in zip(("gas", "dark", "star"), counts):
+            sel = idx[start:start + c]
+            start += c
+            rec = np.zeros(c, dtype=_tipsy_dtype(kind, use_double))
+            rec["mass"] = mass[sel]
+            rec["pos"] = pos[sel]
+            rec["vel"] = vel[sel]
+            f.write(rec.tobytes())
 
 
 def write_acc(path, acc):
