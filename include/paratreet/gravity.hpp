@@ -75,6 +75,7 @@ struct Configuration {
   double fPeriod[3] = {0, 0, 0};   // dxPeriod dyPeriod dzPeriod
   int nReplicas = 0;               // nReplicas
   bool hexadecapole = false;       // the -DHEXADECAPOLE build (examples/Makefile:5)
+  bool dual_tree = false;          // the application's dual_tree ('-e', examples/Main.C:69-71)
 
   int branchFactor() const {
     switch (tree_type) {
@@ -144,6 +145,7 @@ struct Configuration {
     c.hexadecapole = hexadecapole ? 1 : 0;
     for (int k = 0; k < 3; k++) c.period[k] = fPeriod[k];
     c.n_replicas = nReplicas;
+    c.dual_tree = dual_tree ? 1 : 0;
     return c;
   }
 
@@ -220,6 +222,169 @@ struct CentroidData {
   int32_t count;
 };
 
+// Tipsy standard binary files (big-endian), as Reader::load consumes them
+// (src/Reader.C:11-108) and TipsyWriter writes them (src/Writer.C:87-159):
+// a 32-byte header {double time; int nbodies, ndim, nsph, ndark, nstar, pad},
+// then gas, dark and star records of mass, pos[3], vel[3] and the per-type
+// floats (gas: rho temp hsmooth metals phi; dark: eps phi; star: metals tform
+// eps phi).  Positions and velocities are float (48 / 36 / 44-byte records)
+// or double (72 / 60 / 68 bytes, TipsyWriter's use_double); the reader tells
+// them apart by the file size.  soft = gas hsmooth, dark eps, 0 for stars as
+// Reader.C sets them (Reader::setSoft overrides it when dSoft is configured).
+enum TipsyType : int8_t { kTipsyStar = 1, kTipsyGas = 2, kTipsyDark = 3 };  // Particle::Type
+
+struct TipsyData {
+  double time = 0;
+  std::vector<double> pos, vel, mass, soft;  // pos, vel: n x 3
+  std::vector<int8_t> type;                  // TipsyType per particle, file order
+  bool double_records = false;
+};
+
+namespace tipsy_detail {
+inline uint64_t be_load(const unsigned char* b, int bytes) {
+  uint64_t v = 0;
+  for (int i = 0; i < bytes; i++) v = (v << 8) | b[i];
+  return v;
+}
+inline void be_store(unsigned char* b, uint64_t v, int bytes) {
+  for (int i = bytes - 1; i >= 0; i--, v >>= 8) b[i] = (unsigned char)(v & 0xff);
+}
+inline double be_f32(const unsigned char* b) {
+  const uint32_t u = (uint32_t)be_load(b, 4);
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+inline double be_f64(const unsigned char* b) {
+  const uint64_t u = be_load(b, 8);
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+inline void put_f32(unsigned char* b, double v) {
+  const float f = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  be_store(b, u, 4);
+}
+inline void put_f64(unsigned char* b, double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, 8);
+  be_store(b, u, 8);
+}
+// floats after vel: gas 5, dark 2, star 4
+inline int tail_floats(int kind) { return kind == 0 ? 5 : kind == 1 ? 2 : 4; }
+inline size_t record_bytes(int kind, bool dbl) {
+  return 4 + (dbl ? 48 : 24) + 4 * (size_t)tail_floats(kind);
+}
+}  // namespace tipsy_detail
+
+inline TipsyData readTipsy(const std::string& path) {
+  using namespace tipsy_detail;
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw Error(PT_ERR_ARG, "cannot open tipsy file " + path);
+  const uint64_t size = (uint64_t)f.tellg();
+  f.seekg(0);
+  auto rd = [&](void* p, size_t n) {
+    f.read((char*)p, (std::streamsize)n);
+    if (!f) throw Error(PT_ERR_ARG, "short tipsy file " + path);
+  };
+  unsigned char h[32];
+  rd(h, 32);
+  TipsyData t;
+  t.time = be_f64(h);
+  const int64_t nbodies = (int32_t)be_load(h + 8, 4), nsph = (int32_t)be_load(h + 16, 4),
+                ndark = (int32_t)be_load(h + 20, 4), nstar = (int32_t)be_load(h + 24, 4);
+  if (nsph < 0 || ndark < 0 || nstar < 0 || nsph + ndark + nstar != nbodies)
+    throw Error(PT_ERR_ARG, "inconsistent tipsy header in " + path);
+  const int64_t cnt[3] = {nsph, ndark, nstar};
+  int layout = -1;
+  for (int dbl = 0; dbl < 2 && layout < 0; dbl++) {
+    uint64_t body = 0;
+    for (int k = 0; k < 3; k++) body += (uint64_t)cnt[k] * record_bytes(k, dbl != 0);
+    if (size == 32 + body) layout = dbl;
+  }
+  if (layout < 0) throw Error(PT_ERR_ARG, "tipsy file size matches neither float nor double records: " + path);
+  t.double_records = layout == 1;
+  const int vb = t.double_records ? 8 : 4;
+  static const int8_t types[3] = {kTipsyGas, kTipsyDark, kTipsyStar};
+  // position of the softening among the trailing floats (-1: none, stars)
+  static const int soft_at[3] = {2, 0, -1};
+  t.pos.reserve(3 * nbodies);
+  t.vel.reserve(3 * nbodies);
+  for (int k = 0; k < 3; k++) {
+    const size_t rb = record_bytes(k, t.double_records);
+    std::vector<unsigned char> raw(rb * (size_t)cnt[k]);
+    if (!raw.empty()) rd(raw.data(), raw.size());
+    for (int64_t i = 0; i < cnt[k]; i++) {
+      const unsigned char* r = &raw[rb * (size_t)i];
+      t.mass.push_back(be_f32(r));
+      for (int d = 0; d < 3; d++) {
+        t.pos.push_back(vb == 8 ? be_f64(r + 4 + 8 * d) : be_f32(r + 4 + 4 * d));
+        t.vel.push_back(vb == 8 ? be_f64(r + 4 + 3 * vb + 8 * d) : be_f32(r + 4 + 3 * vb + 4 * d));
+      }
+      const unsigned char* tail = r + 4 + 6 * vb;
+      t.soft.push_back(soft_at[k] >= 0 ? be_f32(tail + 4 * soft_at[k]) : 0.0);
+      t.type.push_back(types[k]);
+    }
+  }
+  return t;
+}
+
+// TipsyWriter::do_write (src/Writer.C:100-159): particles in Particle::order
+// (here: their index, the order the reader assigned), each as the record of
+// its type with mass, position and velocity set and the other fields zero,
+// positions and velocities in double when use_double (the reference's choice
+// when Real is double).  The types must run gas, then dark, then star.
+inline void writeTipsy(const std::string& path, double time, const std::vector<double>& pos,
+                       const std::vector<double>& vel, const std::vector<double>& mass,
+                       const std::vector<int8_t>& type, bool use_double = true) {
+  using namespace tipsy_detail;
+  const int64_t n = (int64_t)mass.size();
+  if ((int64_t)type.size() != n || (int64_t)pos.size() != 3 * n || (int64_t)vel.size() != 3 * n)
+    throw Error(PT_ERR_ARG, "writeTipsy: array sizes differ");
+  int64_t cnt[3] = {0, 0, 0};
+  int last = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int k = type[i] == kTipsyGas ? 0 : type[i] == kTipsyDark ? 1 : type[i] == kTipsyStar ? 2 : -1;
+    if (k < 0) throw Error(PT_ERR_ARG, "writeTipsy: particle type outside gas/dark/star");
+    if (k < last) throw Error(PT_ERR_ARG, "writeTipsy: records must run gas, dark, star");
+    last = k;
+    cnt[k]++;
+  }
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw Error(PT_ERR_ARG, "cannot write " + path);
+  unsigned char h[32] = {0};
+  put_f64(h, time);
+  be_store(h + 8, (uint64_t)n, 4);
+  be_store(h + 12, 3, 4);
+  for (int k = 0; k < 3; k++) be_store(h + 16 + 4 * k, (uint64_t)cnt[k], 4);
+  bool ok = std::fwrite(h, 1, 32, f) == 32;
+  std::vector<unsigned char> r;
+  for (int64_t i = 0; i < n && ok; i++) {
+    const int k = type[i] == kTipsyGas ? 0 : type[i] == kTipsyDark ? 1 : 2;
+    r.assign(record_bytes(k, use_double), 0);
+    put_f32(r.data(), mass[i]);
+    for (int d = 0; d < 3; d++) {
+      if (use_double) {
+        put_f64(r.data() + 4 + 8 * d, pos[3 * i + d]);
+        put_f64(r.data() + 28 + 8 * d, vel[3 * i + d]);
+      } else {
+        put_f32(r.data() + 4 + 4 * d, pos[3 * i + d]);
+        put_f32(r.data() + 16 + 4 * d, vel[3 * i + d]);
+      }
+    }
+    ok = std::fwrite(r.data(), 1, r.size(), f) == r.size();
+  }
+  if (std::fclose(f) != 0) ok = false;
+  if (!ok) throw Error(PT_ERR_ARG, "short write to " + path);
+}
+
+// TipsyWriter's file name (src/Writer.C:112).
+inline std::string tipsySnapshotName(const std::string& prefix, int iteration) {
+  return prefix + "." + std::to_string(iteration) + ".tipsy";
+}
+
 // The stages of one Driver::run iteration for the gravity visitor on one GPU.
 class Driver {
  public:
@@ -245,6 +410,7 @@ class Driver {
     check(pt_set_particles(ctx_, n_, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
                            mass.data(), soft.empty() ? nullptr : soft.data(), PT_HOST),
           "pt_set_particles");
+    mass_ = mass;
   }
 
   // remakeUniverse + assignKeys + Subtree::buildTree (sort, recursiveBuild,
@@ -266,6 +432,16 @@ class Driver {
     return c;
   }
 
+  // Subtree::startDual<GravityVisitor>: the dual-tree traversal from the node
+  // pair with these keys (1, 1: the root pair), evaluated in `prec`.
+  pt_counters startDual(const GravityVisitor& v, uint64_t source_key = 1, uint64_t target_key = 1,
+                        pt_precision prec = PT_FP64) {
+    pt_counters c{};
+    check(pt_start_dual(ctx_, v.theta, v.offset, source_key, target_key, prec, &c),
+          "pt_start_dual");
+    return c;
+  }
+
   // Everything above in one fused evaluation (no lists kept).
   pt_counters computeForces(const GravityVisitor& v, pt_precision prec = PT_FP64) {
     pt_counters c{};
@@ -275,6 +451,10 @@ class Driver {
 
   // Partition::kick + Partition::perturb.
   void kickDrift(double dt) { check(pt_kick_drift(ctx_, dt), "pt_kick_drift"); }
+  // Partition::kick (v += a dt / 2) and Partition::perturb (v += a dt / 2,
+  // x += v dt) apart, with the iteration's output between them.
+  void kick(double dt) { check(pt_kick(ctx_, dt), "pt_kick"); }
+  void perturb(double dt) { check(pt_perturb(ctx_, dt), "pt_perturb"); }
 
   std::vector<double> accelerations() {
     std::vector<double> acc(3 * n_), pot(n_);
@@ -315,6 +495,20 @@ class Driver {
     std::fclose(f);
   }
 
+  // paratreet::outputTipsy (src/Paratreet.h:252-259) -> TipsyWriter: the
+  // current positions and velocities with the loaded masses, in particle
+  // order, to <prefix>.<iteration>.tipsy; returns the file name.  `type`
+  // gives each particle's Tipsy type (empty: all dark).
+  std::string outputTipsy(const std::string& prefix, int iteration, double time,
+                          const std::vector<int8_t>& type = {}, bool use_double = true) {
+    std::vector<double> pos(3 * n_), vel(3 * n_);
+    check(pt_get_state(ctx_, pos.data(), vel.data()), "pt_get_state");
+    const std::string name = tipsySnapshotName(prefix, iteration);
+    writeTipsy(name, time, pos, vel, mass_,
+               type.empty() ? std::vector<int8_t>((size_t)n_, (int8_t)kTipsyDark) : type, use_double);
+    return name;
+  }
+
   int64_t size() const { return n_; }
   pt_ctx* context() { return ctx_; }
   const Configuration& config() const { return config_; }
@@ -327,60 +521,7 @@ class Driver {
   TreeSpec spec_;
   pt_ctx* ctx_ = nullptr;
   int64_t n_ = 0;
+  std::vector<double> mass_;
 };
-
-// Tipsy standard binary reader (big-endian), as Reader::load consumes it
-// (src/Reader.C:11-108): gas, dark and star records; soft = hsmooth / eps /
-// star eps, then Reader::setSoft when dSoft is configured.
-struct TipsyData {
-  double time = 0;
-  std::vector<double> pos, vel, mass, soft;
-};
-
-inline TipsyData readTipsy(const std::string& path) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) throw Error(PT_ERR_ARG, "cannot open tipsy file " + path);
-  auto rd = [&](void* p, size_t n) {
-    f.read((char*)p, (std::streamsize)n);
-    if (!f) throw Error(PT_ERR_ARG, "short tipsy file " + path);
-  };
-  auto be32 = [](const unsigned char* b) {
-    return (uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | (uint32_t)b[3];
-  };
-  unsigned char h[32];
-  rd(h, 32);
-  uint64_t tb = 0;
-  for (int i = 0; i < 8; i++) tb = (tb << 8) | h[i];
-  TipsyData t;
-  std::memcpy(&t.time, &tb, 8);
-  const int nbodies = (int)be32(h + 8), nsph = (int)be32(h + 16), ndark = (int)be32(h + 20),
-            nstar = (int)be32(h + 24);
-  if (nsph + ndark + nstar != nbodies) throw Error(PT_ERR_ARG, "inconsistent tipsy header");
-  auto readf = [&](int cnt, std::vector<float>& out) {
-    std::vector<unsigned char> raw(4 * (size_t)cnt);
-    rd(raw.data(), raw.size());
-    out.resize(cnt);
-    for (int i = 0; i < cnt; i++) {
-      const uint32_t u = be32(&raw[4 * (size_t)i]);
-      std::memcpy(&out[i], &u, 4);
-    }
-  };
-  std::vector<float> g, d, s;
-  readf(12 * nsph, g);
-  readf(9 * ndark, d);
-  readf(11 * nstar, s);
-  auto add = [&](const std::vector<float>& r, int w, int cnt, int soft_idx) {
-    for (int i = 0; i < cnt; i++) {
-      const float* q = &r[(size_t)w * i];
-      t.mass.push_back(q[0]);
-      for (int k = 0; k < 3; k++) { t.pos.push_back(q[1 + k]); t.vel.push_back(q[4 + k]); }
-      t.soft.push_back(soft_idx >= 0 ? q[soft_idx] : 0.0);
-    }
-  };
-  add(g, 12, nsph, 9);
-  add(d, 9, ndark, 7);
-  add(s, 11, nstar, -1);
-  return t;
-}
 
 }  // namespace paratreet

@@ -78,6 +78,13 @@ typedef struct pt_config {
                                      0 = the quadrupole build */
   double period[3];               /* dxPeriod dyPeriod dzPeriod (used when periodic) */
   int32_t n_replicas;             /* nReplicas, 0..16 (used when periodic) */
+  int32_t dual_tree;              /* the application's dual_tree flag ('-e' in
+                                     examples/Main.C:69-71): pt_compute_forces runs
+                                     Subtree::startDual's traversal (pt_start_dual from
+                                     the root pair) before the down traversal, both adding
+                                     to the accelerations and counters
+                                     (examples/Gravity.C:30-33); not with periodic or
+                                     hexadecapole */
 } pt_config;
 
 /* Interaction counters, src/ThreadStateHolder.h:10-13 (COUNT_INTERACTIONS). */
@@ -155,6 +162,21 @@ pt_status pt_traverse(pt_ctx* ctx, double theta, const double offset[3], pt_coun
  * potential per particle (the reference accumulates onto zeroed particles). */
 pt_status pt_interact(pt_ctx* ctx, pt_precision precision);
 
+/* Subtree::startDual<GravityVisitor> (src/Subtree.h:233-240) ->
+ * DualTraverser::doTrav / runInvertedTraversal (src/Traverser.h:485-610) for
+ * GravityVisitor(offset, theta) with its cell() (examples/GravityVisitor.h:
+ * 226-230), from the node pair with keys (source_key, target_key): (1, 1) is
+ * startDual's pair (the root against the root); another pair is a doTrav
+ * resumed at that pair (DualTraverser::resumeTrav, :531-537).  Leaves the
+ * leaf() and node() calls that move particles as the context's lists
+ * (pt_get_lists), evaluates them in `precision` (accelerations and
+ * potentials of this traversal alone, pt_get_accelerations) and returns the
+ * counters, which add n_particles as the reference's nodes hold it (-1 for an
+ * internal node) modulo 2^64.  Quadrupole, non-periodic contexts only (the
+ * reference aborts on dual_tree with periodic, examples/Gravity.C:30). */
+pt_status pt_start_dual(pt_ctx* ctx, double theta, const double offset[3], uint64_t source_key,
+                        uint64_t target_key, pt_precision precision, pt_counters* counters);
+
 /* One force evaluation: universe, keys, sort, build, upward, traverse,
  * interact -- paratreet::traversalFn with everything before it in one
  * Driver::run iteration (src/Driver.h:190-221, examples/Gravity.C:29-33). */
@@ -164,6 +186,16 @@ pt_status pt_compute_forces(pt_ctx* ctx, double theta, const double offset[3],
 /* Partition::kick + Partition::perturb (src/Partition.h:329-379 ->
  * src/Particle.C:7-21) with the last computed accelerations. */
 pt_status pt_kick_drift(pt_ctx* ctx, double dt);
+
+/* The two halves of pt_kick_drift, as Driver::run calls them with the
+ * iteration's output between them (src/Driver.h:222-247):
+ * Partition::kick -> Particle::kick (src/Partition.h:329-336,
+ * src/Particle.C:7-9), v += a dt / 2; and Partition::perturb ->
+ * Particle::perturb (src/Partition.h:338-379, src/Particle.C:11-21),
+ * v += a dt / 2 then x += v dt.  pt_kick then pt_perturb gives the same bits
+ * as pt_kick_drift. */
+pt_status pt_kick(pt_ctx* ctx, double dt);
+pt_status pt_perturb(pt_ctx* ctx, double dt);
 
 /* Driver::run (src/Driver.h:183-288) restricted to this path: `steps`
  * iterations of compute_forces + kick_drift with a fixed dt. */

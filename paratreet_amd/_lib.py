@@ -22,8 +22,8 @@ TIMERS = ("universe", "keys", "sort", "build", "moments", "traverse", "force", "
 EXPORTS = (
     "pt_config_default", "pt_ctx_create", "pt_ctx_destroy", "pt_last_error",
     "pt_get_device_view", "pt_set_particles", "pt_remake_universe", "pt_assign_keys",
-    "pt_sort", "pt_build_tree", "pt_upward", "pt_traverse", "pt_interact",
-    "pt_compute_forces", "pt_kick_drift", "pt_run", "pt_direct_sum", "pt_get_universe",
+    "pt_sort", "pt_build_tree", "pt_upward", "pt_traverse", "pt_interact", "pt_start_dual",
+    "pt_compute_forces", "pt_kick_drift", "pt_kick", "pt_perturb", "pt_run", "pt_direct_sum", "pt_get_universe",
     "pt_get_keys", "pt_get_sorted", "pt_get_tree_size", "pt_get_nodes", "pt_get_node_data",
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
@@ -46,6 +46,7 @@ class PtConfig(ctypes.Structure):
         ("hexadecapole", ctypes.c_int32),
         ("period", ctypes.c_double * 3),
         ("n_replicas", ctypes.c_int32),
+        ("dual_tree", ctypes.c_int32),
     ]
 
 
@@ -117,8 +118,11 @@ def load():
         "pt_upward": (I32, [P]),
         "pt_traverse": (I32, [P, D, P, P]),
         "pt_interact": (I32, [P, ctypes.c_int]),
+        "pt_start_dual": (I32, [P, D, P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]),
         "pt_compute_forces": (I32, [P, D, P, ctypes.c_int, P]),
         "pt_kick_drift": (I32, [P, D]),
+        "pt_kick": (I32, [P, D]),
+        "pt_perturb": (I32, [P, D]),
         "pt_run": (I32, [P, ctypes.c_int, D, D, ctypes.c_int, P]),
         "pt_direct_sum": (I32, [P, P, I64, P]),
         "pt_get_universe": (I32, [P, P]),

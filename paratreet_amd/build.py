@@ -28,6 +28,7 @@ SOURCES = {
     "pt_tree.hip": EXACT,
     "pt_moments.hip": EXACT,
     "pt_traverse.hip": EXACT,
+    "pt_dual.hip": EXACT,
     "pt_gravity.hip": EXACT,
     "pt_misc.hip": EXACT,
     # hardware FP64 atomic adds (device memory from hipMalloc is coarse-grained)

@@ -117,6 +117,12 @@ struct pt_ctx {
   int adj_prec = PT_FP64;  // arithmetic of the tree adjoint (pt_set_adjoint_precision)
   double traj_dt = 0.0, traj_theta = 0.7;
   int64_t traj_n = 0;
+  // dual-tree traversal: two frontiers of node pairs, leaf() / node() records,
+  // their sorted copy, node -> bucket map, start ids, per-step state
+  DBuf<uint64_t> dual_a, dual_b, dual_pp, dual_pc, dual_sorted;
+  DBuf<uint32_t> dual_perm;
+  DBuf<int32_t> dual_bucket_of, dual_ids;
+  DBuf<unsigned long long> dual_state;
   // scratch
   DBuf<char> scratch;
   // timing
@@ -272,6 +278,7 @@ void pt_config_default(pt_config* cfg) {
   cfg->hexadecapole = 0;
   cfg->period[0] = cfg->period[1] = cfg->period[2] = 0.0;
   cfg->n_replicas = 0;
+  cfg->dual_tree = 0;
 }
 
 pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx** out) {
@@ -285,6 +292,7 @@ pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx**
     for (int k = 0; k < 3; k++)
       if (!(cfg->period[k] > 0.0) || !std::isfinite(cfg->period[k])) return PT_ERR_ARG;
   }
+  if (cfg->dual_tree && (cfg->periodic || cfg->hexadecapole)) return PT_ERR_UNSUPPORTED;
   pt_ctx* c = new pt_ctx();
   c->cfg = *cfg;
   c->device = device;
@@ -347,6 +355,9 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->gfirst); release(c->gnb); release(c->gpp_cnt); release(c->gpc_cnt);
   release(c->gpp); release(c->gpc); release(c->gppm); release(c->gpcm);
   release(c->gpp_off); release(c->gpc_off);
+  release(c->dual_a); release(c->dual_b); release(c->dual_pp); release(c->dual_pc);
+  release(c->dual_sorted); release(c->dual_perm); release(c->dual_bucket_of);
+  release(c->dual_ids); release(c->dual_state);
   for (int i = 0; i < PT_T_COUNT; i++) {
     if (c->ev[i][0]) (void)hipEventDestroy(c->ev[i][0]);
     if (c->ev[i][1]) (void)hipEventDestroy(c->ev[i][1]);
@@ -745,6 +756,24 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
   return PT_OK;
 }
 
+// node() / leaf() over the context's lists into the sorted result arrays
+// (written, not added).
+static pt_status launch_list_interact(pt_ctx* c, pt_precision precision) {
+  Timer t(c, PT_T_FORCE);
+  if (c->cfg.hexadecapole) {
+    PT_TRY_LAUNCH(c, pt_launch_interact_hexa(c->bk.p, c->nb, c->tnode.p, c->hn.p, c->xs.p, c->ys.p,
+                                             c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
+                                             c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
+                                             c->azs.p, c->pots.p, c->stream));
+  } else {
+    PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p,
+                                        c->gn32.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p, c->es.p,
+                                        c->pp_off.p, c->pp.p, c->pc_off.p, c->pc.p, c->offset,
+                                        c->axs.p, c->ays.p, c->azs.p, c->pots.p, c->stream));
+  }
+  return PT_OK;
+}
+
 pt_status pt_interact(pt_ctx* c, pt_precision precision) {
   pt_status st = check_ctx(c);
   if (st) return st;
@@ -754,24 +783,129 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
     return fail(c, PT_ERR_UNSUPPORTED,
                 "the hexadecapole list path evaluates node() in FP64 only (pt_compute_forces "
                 "has the compensated-FP32 form)");
-  if (c->cfg.hexadecapole) {
-    Timer t(c, PT_T_FORCE);
-    PT_TRY_LAUNCH(c, pt_launch_interact_hexa(c->bk.p, c->nb, c->tnode.p, c->hn.p, c->xs.p, c->ys.p,
-                                             c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
-                                             c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
-                                             c->azs.p, c->pots.p, c->stream));
-  } else {
-    Timer t(c, PT_T_FORCE);
-    PT_TRY_LAUNCH(c, pt_launch_interact((int)precision, c->bk.p, c->nb, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
-                                        c->ys.p, c->zs.p, c->ms.p, c->es.p, c->pp_off.p, c->pp.p,
-                                        c->pc_off.p, c->pc.p, c->offset, c->axs.p, c->ays.p,
-                                        c->azs.p, c->pots.p, c->stream));
-  }
+  if ((st = launch_list_interact(c, precision))) return st;
   PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
                                     c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   c->stage = S_FORCES;
   c->have_forces = true;
+  return PT_OK;
+}
+
+// Frontier items and records of one dual-tree traversal at most (beyond it
+// the pair walk, which never prunes below two internal nodes whose boxes do
+// not enclose one another, is reported as a capacity error).
+constexpr int64_t kDualMaxItems = int64_t(1) << 28;
+
+// DualTraverser::doTrav from the pair of nodes with keys (skey, tkey), one
+// launch per depth of the pair walk: leaves the leaf() / node() calls as the
+// context's per-bucket lists and the counters in c->h_counters.
+static pt_status dual_lists(pt_ctx* c, double theta, const double offset[3], uint64_t skey,
+                            uint64_t tkey) {
+  pt_status st;
+  if ((st = need(c, S_MOMENTS, "pt_start_dual"))) return st;
+  if (c->cfg.periodic || c->cfg.hexadecapole)
+    return fail(c, PT_ERR_UNSUPPORTED,
+                "the dual-tree traversal runs on non-periodic quadrupole contexts "
+                "(examples/Gravity.C:30 aborts on dual_tree with periodic)");
+  double gf;
+  if ((st = set_visitor(c, theta, offset, &gf))) return st;
+  const int64_t nn = c->nn, nb = c->nb;
+  Timer tm(c, PT_T_TRAVERSE);
+  PT_TRY_HIP(c, ensure(c->dual_ids, 2));
+  PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->dual_ids.p, 2, 0xffffffffu, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_dual_find(c->nkey.p, nn, skey, tkey, c->dual_ids.p, c->stream));
+  int32_t ids[2] = {-1, -1};
+  PT_TRY_HIP(c, hipMemcpyAsync(ids, c->dual_ids.p, sizeof(ids), hipMemcpyDeviceToHost, c->stream));
+  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  if (ids[0] < 0 || ids[1] < 0) return fail(c, PT_ERR_ARG, "pt_start_dual: no node with that key");
+  PT_TRY_HIP(c, ensure(c->dual_bucket_of, std::max<int64_t>(nn, 1)));
+  PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->dual_bucket_of.p, nn, 0xffffffffu, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_dual_bucket_of(c->bk.p, nb, c->dual_bucket_of.p, c->stream));
+  PT_TRY_HIP(c, ensure(c->dual_a, std::max<int64_t>(4 * nn, 1024)));
+  PT_TRY_HIP(c, ensure(c->dual_b, std::max<int64_t>(4 * nn, 1024)));
+  PT_TRY_HIP(c, ensure(c->dual_pp, std::max<int64_t>(8 * nb, 1024)));
+  PT_TRY_HIP(c, ensure(c->dual_pc, std::max<int64_t>(8 * nb, 1024)));
+  PT_TRY_HIP(c, ensure(c->dual_state, 8));
+  const unsigned long long seed =
+      ((unsigned long long)(uint32_t)ids[0] << 32) | (unsigned long long)(uint32_t)ids[1];
+  PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->dual_a.p, 1, seed, c->stream));
+  int64_t n_cur = 1, n_pp = 0, n_pc = 0;
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  for (int level = 0; n_cur > 0; level++) {
+    if (level > 4 * PT_MAX_DEPTH + 8)
+      return fail(c, PT_ERR_CAPACITY, "dual-tree traversal did not finish");
+    unsigned long long h[7];
+    for (int retry = 0;; retry++) {
+      unsigned long long init[7] = {0, (unsigned long long)n_pp, (unsigned long long)n_pc, 0, 0,
+                                    0, 0};
+      PT_TRY_HIP(c, hipMemcpyAsync(c->dual_state.p, init, sizeof(init), hipMemcpyHostToDevice,
+                                   c->stream));
+      PT_TRY_LAUNCH(c, pt_launch_dual_step(c->dual_a.p, n_cur, c->tnode.p, c->nd.p, c->ndepth.p,
+                                           c->dual_bucket_of.p, nn, gf, c->offset, c->dual_b.p,
+                                           c->dual_b.cap, c->dual_pp.p, c->dual_pp.cap,
+                                           c->dual_pc.p, c->dual_pc.cap, c->dual_state.p,
+                                           c->stream));
+      PT_TRY_HIP(c, hipMemcpyAsync(h, c->dual_state.p, sizeof(h), hipMemcpyDeviceToHost,
+                                   c->stream));
+      PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+      const int64_t want_next = (int64_t)h[0], want_pp = (int64_t)h[1], want_pc = (int64_t)h[2];
+      if (want_next <= c->dual_b.cap && want_pp <= c->dual_pp.cap && want_pc <= c->dual_pc.cap)
+        break;
+      if (retry > 0 || want_next > kDualMaxItems || want_pp > kDualMaxItems ||
+          want_pc > kDualMaxItems)
+        return fail(c, PT_ERR_CAPACITY, "dual-tree traversal exceeds its pair capacity");
+      // grow what overflowed (records keep what earlier depths wrote) and
+      // repeat this depth
+      PT_TRY_HIP(c, ensure(c->dual_b, want_next));
+      PT_TRY_HIP(c, grow_keep(c->dual_pp, want_pp, n_pp, c->stream));
+      PT_TRY_HIP(c, grow_keep(c->dual_pc, want_pc, n_pc, c->stream));
+    }
+    for (int k = 0; k < 4; k++) tot[k] += h[3 + k];
+    n_pp = (int64_t)h[1];
+    n_pc = (int64_t)h[2];
+    n_cur = (int64_t)h[0];
+    std::swap(c->dual_a, c->dual_b);
+  }
+  // records -> per-bucket lists in (bucket, source) order
+  int end_bit = 32;
+  while (end_bit < 64 && (int64_t(1) << (end_bit - 32)) <= nb) end_bit++;
+  const int64_t nrec = std::max<int64_t>(std::max(n_pp, n_pc), 1);
+  PT_TRY_HIP(c, ensure(c->dual_sorted, nrec));
+  PT_TRY_HIP(c, ensure(c->dual_perm, nrec));
+  if ((st = ensure_scratch(c, pt_sort_scratch_bytes(nrec)))) return st;
+  PT_TRY_HIP(c, ensure(c->pp, std::max<int64_t>(n_pp, 1)));
+  PT_TRY_HIP(c, ensure(c->pc, std::max<int64_t>(n_pc, 1)));
+  PT_TRY_HIP(c, ensure(c->pp_off, nb + 1));
+  PT_TRY_HIP(c, ensure(c->pc_off, nb + 1));
+  PT_TRY_LAUNCH(c, pt_radix_sort(c->dual_pp.p, c->dual_sorted.p, c->dual_perm.p, n_pp, 0, end_bit,
+                                 c->scratch.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_records_to_lists(c->dual_sorted.p, n_pp, nb, c->pp.p, c->pp_off.p,
+                                              c->stream));
+  PT_TRY_LAUNCH(c, pt_radix_sort(c->dual_pc.p, c->dual_sorted.p, c->dual_perm.p, n_pc, 0, end_bit,
+                                 c->scratch.p, c->stream));
+  PT_TRY_LAUNCH(c, pt_launch_records_to_lists(c->dual_sorted.p, n_pc, nb, c->pc.p, c->pc_off.p,
+                                              c->stream));
+  c->n_pp = n_pp;
+  c->n_pc = n_pc;
+  c->lists_valid = true;
+  c->h_counters.n_node_ints = tot[0];
+  c->h_counters.n_part_ints = tot[1];
+  c->h_counters.n_opens = tot[2];
+  c->h_counters.n_closes = tot[3];
+  return PT_OK;
+}
+
+pt_status pt_start_dual(pt_ctx* c, double theta, const double offset[3], uint64_t source_key,
+                        uint64_t target_key, pt_precision precision, pt_counters* counters) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
+  if ((st = dual_lists(c, theta, offset, source_key, target_key))) return st;
+  const pt_counters k = c->h_counters;
+  if ((st = pt_interact(c, precision))) return st;
+  c->h_counters = k;
+  if (counters) *counters = k;
   return PT_OK;
 }
 
@@ -801,10 +935,20 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
       for (int Z = -nr; Z <= nr; Z++)
         offs.push_back({c->offset[0] + X * c->cfg.period[0], c->offset[1] + Y * c->cfg.period[1],
                         c->offset[2] + Z * c->cfg.period[2]});
+  // examples/Gravity.C:30-33: with dual_tree, Subtree::startDual's traversal
+  // first, then the down traversal adds to the same accelerations
+  const bool dual = c->cfg.dual_tree != 0;
+  pt_counters dual_cnt{};
+  if (dual) {
+    if ((st = dual_lists(c, theta, offset, 1, 1))) return st;
+    dual_cnt = c->h_counters;
+    if ((st = set_visitor(c, theta, offset, &gf))) return st;
+  }
   for (int attempt = 0; attempt < 2; attempt++) {
     PT_TRY_LAUNCH(c, pt_launch_fill_u32((uint32_t*)c->derr.p, 1, 0u, c->stream));
+    if (dual && (st = launch_list_interact(c, precision))) return st;
     for (size_t r = 0; r < offs.size(); r++) {
-      const bool acc = r > 0;
+      const bool acc = r > 0 || dual;
       const double* off = offs[r].data();
       PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
       {
@@ -846,6 +990,12 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
     if (!herr) break;
     if (attempt == 1) return fail(c, PT_ERR_CAPACITY, "walk stack overflow at 1536 entries");
   }
+  if (dual) {
+    c->h_counters.n_node_ints += dual_cnt.n_node_ints;
+    c->h_counters.n_part_ints += dual_cnt.n_part_ints;
+    c->h_counters.n_opens += dual_cnt.n_opens;
+    c->h_counters.n_closes += dual_cnt.n_closes;
+  }
   if (counters) *counters = c->h_counters;
   c->lists_valid = false;
   c->stage = S_FORCES;
@@ -863,20 +1013,33 @@ pt_status pt_compute_forces(pt_ctx* c, double theta, const double offset[3],
   return gravity_fused(c, theta, offset, precision, counters);
 }
 
-pt_status pt_kick_drift(pt_ctx* c, double dt) {
+static pt_status step_particles(pt_ctx* c, double dt, int mode, const char* what) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  if (!c->have_forces) return fail(c, PT_ERR_STATE, "pt_kick_drift before any force evaluation");
+  if (!c->have_forces) return fail(c, PT_ERR_STATE, std::string(what) + " before any force evaluation");
   {
     Timer t(c, PT_T_INTEGRATE);
-    PT_TRY_LAUNCH(c, pt_launch_kick_drift(c->n, dt, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
-                                          c->vz.p, c->ax.p, c->ay.p, c->az.p, c->stream));
+    PT_TRY_LAUNCH(c, pt_launch_kick_drift(c->n, dt, mode, c->x.p, c->y.p, c->z.p, c->vx.p,
+                                          c->vy.p, c->vz.p, c->ax.p, c->ay.p, c->az.p,
+                                          c->stream));
   }
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
-  // positions moved: everything downstream must be rebuilt (Partition::rebuild)
-  c->stage = S_PARTICLES;
-  c->lists_valid = false;
+  if (mode != PT_STEP_KICK) {
+    // positions moved: everything downstream must be rebuilt (Partition::rebuild)
+    c->stage = S_PARTICLES;
+    c->lists_valid = false;
+  }
   return PT_OK;
+}
+
+pt_status pt_kick_drift(pt_ctx* c, double dt) {
+  return step_particles(c, dt, PT_STEP_KICK_DRIFT, "pt_kick_drift");
+}
+
+pt_status pt_kick(pt_ctx* c, double dt) { return step_particles(c, dt, PT_STEP_KICK, "pt_kick"); }
+
+pt_status pt_perturb(pt_ctx* c, double dt) {
+  return step_particles(c, dt, PT_STEP_PERTURB, "pt_perturb");
 }
 
 pt_status pt_run(pt_ctx* c, int steps, double dt, double theta, pt_precision precision,

@@ -156,8 +156,24 @@ int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, do
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,
                      const double* a2, const double* a3, double* o0, double* o1, double* o2,
                      double* o3, hipStream_t s);
-int pt_launch_kick_drift(int64_t n, double dt, double* x, double* y, double* z, double* vx,
-                         double* vy, double* vz, const double* ax, const double* ay,
+// Dual-tree traversal (pt_dual.hip): node -> bucket map, the node ids of two
+// keys, one breadth-first step over node pairs, and per-bucket lists from
+// records sorted by (bucket << 32 | source).
+int pt_launch_dual_bucket_of(const Bucket* bk, int64_t nb, int32_t* bucket_of, hipStream_t s);
+int pt_launch_dual_find(const uint64_t* key, int64_t nn, uint64_t k0, uint64_t k1, int32_t* ids,
+                        hipStream_t s);
+int pt_launch_dual_step(const uint64_t* items, int64_t n_items, const TreeNode* tn,
+                        const double* nd, const int32_t* depth, const int32_t* bucket_of,
+                        int64_t nn, double gf, const double off[3], uint64_t* next,
+                        int64_t next_cap, uint64_t* pp_rec, int64_t pp_cap, uint64_t* pc_rec,
+                        int64_t pc_cap, unsigned long long* state, hipStream_t s);
+int pt_launch_records_to_lists(const uint64_t* rec, int64_t n, int64_t nb, int32_t* ids,
+                               int64_t* off, hipStream_t s);
+
+// Step modes of pt_launch_kick_drift (Particle::kick, Particle::perturb).
+enum { PT_STEP_KICK_DRIFT = 0, PT_STEP_KICK = 1, PT_STEP_PERTURB = 2 };
+int pt_launch_kick_drift(int64_t n, double dt, int mode, double* x, double* y, double* z,
+                         double* vx, double* vy, double* vz, const double* ax, const double* ay,
                          const double* az, hipStream_t s);
 int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const double* y,
                      const double* z, const double* m, const double* e, int64_t n, double* out,

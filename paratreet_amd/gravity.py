@@ -56,6 +56,10 @@ class Configuration:
     # the reference's -DHEXADECAPOLE build (examples/Makefile:5): a build flag
     # there, a context option here
     hexadecapole: bool = False
+    # the application's dual_tree flag (examples/Main.C:69-71, '-e' there;
+    # 'e' is also dSoft's letter, so it is a field here): startDual before
+    # startDown in every force evaluation (examples/Gravity.C:30-33)
+    dual_tree: bool = False
     origin: dict = field(default_factory=dict)
 
     LETTERS = {"n": ("min_n_subtrees", int), "p": ("min_n_partitions", int),
@@ -126,6 +130,7 @@ class Configuration:
         c.has_dsoft = 1 if "dSoft" in self.origin else 0
         c.periodic = self.periodic
         c.hexadecapole = 1 if self.hexadecapole else 0
+        c.dual_tree = 1 if self.dual_tree else 0
         for k in range(3):
             c.period[k] = float(self.fPeriod[k])
         c.n_replicas = int(self.nReplicas)
@@ -251,6 +256,17 @@ class Driver:
     def interact(self, precision=PT_FP64):
         self._check(self.L.pt_interact(self.ctx, int(precision)))
 
+    def start_dual(self, visitor, source_key=1, target_key=1, precision=PT_FP64):
+        """Subtree::startDual<GravityVisitor>: the dual-tree traversal from the
+        node pair with these keys (the root pair by default), its lists
+        (lists()) and accelerations (accelerations()).  Returns the counters
+        (n_node_ints, n_part_ints, n_opens, n_closes), modulo 2^64."""
+        cnt = _lib.PtCounters()
+        off = _f64(visitor.offset)
+        self._check(self.L.pt_start_dual(self.ctx, visitor.theta, _ptr(off), int(source_key),
+                                         int(target_key), int(precision), ctypes.byref(cnt)))
+        return cnt.as_tuple()
+
     def start_down(self, visitor, precision=PT_FP64):
         """Partition::startDown<GravityVisitor>: traversal + visitor calls.
         Returns counters (n_node_ints, n_part_ints, n_opens, n_closes)."""
@@ -268,6 +284,14 @@ class Driver:
     def kick_drift(self, dt):
         """Partition::kick then Partition::perturb with timestep dt."""
         self._check(self.L.pt_kick_drift(self.ctx, float(dt)))
+
+    def kick(self, dt):
+        """Partition::kick (src/Particle.C:7-9): v += a dt / 2."""
+        self._check(self.L.pt_kick(self.ctx, float(dt)))
+
+    def perturb(self, dt):
+        """Partition::perturb (src/Particle.C:11-21): v += a dt / 2, x += v dt."""
+        self._check(self.L.pt_perturb(self.ctx, float(dt)))
 
     def run(self, steps, dt, theta=0.7, precision=PT_FP64):
         cnt = _lib.PtCounters()
