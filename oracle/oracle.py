@@ -88,6 +88,9 @@ def _declare(L):
                                   _P, _P, _P, _P, _P, ctypes.c_int]
     L.or_lists_hexa.argtypes = [_P, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P]
     L.or_ewald.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _P, _P, _P, _P]
+    L.or_ewald_hexa.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _P, _P, _P, _P]
+    L.or_ewald_hexa_general.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _D, _D,
+                                        _D, _P, _P, _P, _P]
     L.or_ewald_general.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _D, _D, _D,
                                    _P, _P, _P, _P]
 
@@ -343,6 +346,26 @@ def ewald(pos, mass, cm, q, L, n_replicas, a_fac=None, ew_cut=2.6, h_cut=2.8):
 _ewald_fn = ewald
 
 
+def ewald_hexa(pos, mass, cm, h, L, n_replicas, a_fac=None, ew_cut=2.6, h_cut=2.8):
+    """LeafEwaldFn's HEXADECAPOLE-form lattice correction from a root (mass,
+    cm, the 21 reduced moments h in FMOMR order, as hexa_moments()[:, 1:]):
+    (acc n x 3, pot n).  a_fac=None uses the reference's constants."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (_f64(pos[:, k]) for k in range(3))
+    n = len(x)
+    ax, ay, az, pot = (np.zeros(n) for _ in range(4))
+    h = _f64(h)
+    assert h.shape == (21,)
+    if a_fac is None:
+        lib().or_ewald_hexa(_p(x), _p(y), _p(z), n, float(mass), _p(_f64(cm)), _p(h), float(L),
+                            int(n_replicas), _p(ax), _p(ay), _p(az), _p(pot))
+    else:
+        lib().or_ewald_hexa_general(_p(x), _p(y), _p(z), n, float(mass), _p(_f64(cm)), _p(h),
+                                    float(L), int(n_replicas), float(a_fac), float(ew_cut),
+                                    float(h_cut), _p(ax), _p(ay), _p(az), _p(pot))
+    return np.stack([ax, ay, az], axis=1), pot
+
+
 def spline(r2, twoh):
     a, b = ctypes.c_double(), ctypes.c_double()
     lib().or_spline(float(r2), float(twoh), ctypes.byref(a), ctypes.byref(b))
@@ -415,7 +438,9 @@ class Pipeline:
         of the traversals with offsets (X, Y, Z) * period, X, Y, Z in
         [-N, N] (X outermost), in the caller's original order, with the
         counters summed over the replicas; with ewald, plus LeafEwaldFn's
-        lattice correction from the root (quadrupole form, L = period[0])."""
+        lattice correction from the root (L = period[0]): the quadrupole form,
+        or in the HEXADECAPOLE build the form with the root's moments to
+        fourth order."""
         n = len(self.x)
         acc = np.zeros((n, 3))
         pot = np.zeros(n)
@@ -432,8 +457,14 @@ class Pipeline:
                     cnt += c
         if ewald:
             root = self.data[0]
-            ea, ep = _ewald_fn(np.stack([self.x, self.y, self.z], axis=1), root["mass"],
-                               root["cm"], root["q"], period[0], n_replicas)
+            xyz = np.stack([self.x, self.y, self.z], axis=1)
+            if self.hexa is None:
+                ea, ep = _ewald_fn(xyz, root["mass"], root["cm"], root["q"], period[0],
+                                   n_replicas)
+            else:
+                # the HEXADECAPOLE branch: the root's moments to fourth order
+                ea, ep = ewald_hexa(xyz, root["mass"], root["cm"], self.hexa[0, 1:], period[0],
+                                    n_replicas)
             acc += ea
             pot += ep
         out_a = np.empty_like(acc)

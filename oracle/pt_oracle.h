@@ -197,6 +197,18 @@ void or_dual_lists(const or_node* nodes, int64_t nn, const or_cdata* d, double t
                    int64_t* pc_count, const int64_t* pp_off, const int64_t* pc_off, int32_t* pp,
                    int32_t* pc);
 
+/* LeafEwaldFn + EwaldData::EwaldInit, HEXADECAPOLE form (examples/Ewald.h,
+ * examples/Ewald.C): the lattice of the root's moments to fourth order (mass,
+ * cm, 21 reduced moments in FMOMR order) beyond the explicit replicas, added
+ * to ax/ay/az/pot.  Constants as or_ewald / or_ewald_general. */
+void or_ewald_hexa(const double* x, const double* y, const double* z, int64_t n, double mass,
+                   const double cm[3], const double h[21], double L, int n_replicas, double* ax,
+                   double* ay, double* az, double* pot);
+void or_ewald_hexa_general(const double* x, const double* y, const double* z, int64_t n,
+                           double mass, const double cm[3], const double h[21], double L,
+                           int n_replicas, double a_fac, double ew_cut, double h_cut, double* ax,
+                           double* ay, double* az, double* pot);
+
 /* Particle::kick then Particle::perturb (src/Particle.C:7-21). */
 /* (da/dx)^T w and (da/dm)^T w of or_direct over all targets (test-only
  * reference for the adjoint). */

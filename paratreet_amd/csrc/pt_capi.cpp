@@ -976,12 +976,18 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
     if (c->cfg.periodic) {
       // LeafEwaldFn after the replicas (examples/Gravity.C:47-50)
       Timer t(c, PT_T_FORCE);
-      PT_TRY_HIP(c, ensure(c->ewt, 4 * pt_ewald_table_entries()));
+      PT_TRY_HIP(c, ensure(c->ewt, 5 * pt_ewald_table_entries()));
       PT_TRY_HIP(c, ensure(c->ewcount, 1));
-      PT_TRY_LAUNCH(c, pt_launch_ewald(c->n, c->xs.p, c->ys.p, c->zs.p, c->nd.p, c->nn,
-                                       c->cfg.period[0], c->cfg.n_replicas, c->ewt.p,
-                                       c->ewcount.p, c->axs.p, c->ays.p, c->azs.p,
-                                       c->pots.p, c->stream));
+      if (hexa)
+        PT_TRY_LAUNCH(c, pt_launch_ewald_hexa(c->n, c->xs.p, c->ys.p, c->zs.p, c->nd.p, c->nn,
+                                              c->hn.p, c->cfg.period[0], c->cfg.n_replicas,
+                                              c->ewt.p, c->ewcount.p, c->axs.p, c->ays.p,
+                                              c->azs.p, c->pots.p, c->stream));
+      else
+        PT_TRY_LAUNCH(c, pt_launch_ewald(c->n, c->xs.p, c->ys.p, c->zs.p, c->nd.p, c->nn,
+                                         c->cfg.period[0], c->cfg.n_replicas, c->ewt.p,
+                                         c->ewcount.p, c->axs.p, c->ays.p, c->azs.p,
+                                         c->pots.p, c->stream));
     }
     // the last replica's counters, the unsort to caller order and the one
     // read-back; a rerun (stack overflow, rare) repeats every replica

@@ -151,6 +151,14 @@ int pt_launch_ewald(int64_t n, const double* x, const double* y, const double* z
                     int32_t* count, double* ax, double* ay, double* az, double* pot,
                     hipStream_t s);
 
+// LeafEwaldFn of the HEXADECAPOLE build: the root's moments to fourth order
+// (nd for mass and centre, hn[0] for the reduced moments); table holds
+// 5 * pt_ewald_table_entries() doubles.
+int pt_launch_ewald_hexa(int64_t n, const double* x, const double* y, const double* z,
+                         const double* nd, int64_t nn, const HexNode* hn, double L,
+                         int n_replicas, double* table, int32_t* count, double* ax, double* ay,
+                         double* az, double* pot, hipStream_t s);
+
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
                      double* ys, double* zs, double* ms, double* es, hipStream_t s);
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,

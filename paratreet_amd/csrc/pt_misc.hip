@@ -267,6 +267,208 @@ ewald_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__
   pot[i] += p;
 }
 
+// ---------------------------------------------------------------------------
+// Ewald sum of the periodic HEXADECAPOLE build: the HEXADECAPOLE branches of
+// LeafEwaldFn and EwaldData::EwaldInit (examples/Ewald.h:6-13,31-42,57-67,
+// 127-166; examples/Ewald.C:11-51,89-158).  The root's reduced moments to
+// fourth order (FMOMR order, completed with their z components by
+// tracelessness as momFmomr2Momc does, examples/moments.C:1563-1597) replace
+// the quadrupole: g0..g5 in real space, and in Fourier space a cosine
+// coefficient -g (m - w^2 M2[h^2]/2 + w^4 M4[h^4]/24) and a sine coefficient
+// g w^3 M3[h^3]/6 per vector (w = 2 pi / L, g = e^{-k4 h^2}/(pi h^2 L)).
+// Same operations, in the same order, as the oracle's restatement.
+// ---------------------------------------------------------------------------
+
+struct Momc {
+  double m;
+  double xx, yy, xy, xz, yz, zz;
+  double xxx, xyy, xxy, yyy, xxz, yyz, xyz, xzz, yzz, zzz;
+  double xxxx, xyyy, xxxy, yyyy, xxxz, yyyz, xxyy, xxyz, xyyz, xxzz, xyzz, xzzz, yyzz, yzzz, zzzz;
+};
+
+__device__ __forceinline__ void momc_from_reduced(double mass, const double* __restrict__ h,
+                                                  Momc& c) {
+  c.m = mass;
+  c.xx = h[0]; c.yy = h[1]; c.xy = h[2]; c.xz = h[3]; c.yz = h[4];
+  c.xxx = h[5]; c.xyy = h[6]; c.xxy = h[7]; c.yyy = h[8]; c.xxz = h[9]; c.yyz = h[10];
+  c.xyz = h[11];
+  c.xxxx = h[12]; c.xyyy = h[13]; c.xxxy = h[14]; c.yyyy = h[15]; c.xxxz = h[16];
+  c.yyyz = h[17]; c.xxyy = h[18]; c.xxyz = h[19]; c.xyyz = h[20];
+  c.zz = -(c.xx + c.yy);
+  c.xzz = -(c.xxx + c.xyy);
+  c.yzz = -(c.xxy + c.yyy);
+  c.zzz = -(c.xxz + c.yyz);
+  c.xxzz = -(c.xxxx + c.xxyy);
+  c.xyzz = -(c.xxxy + c.xyyy);
+  c.xzzz = -(c.xxxz + c.xyyz);
+  c.yyzz = -(c.xxyy + c.yyyy);
+  c.yzzz = -(c.xxyz + c.yyyz);
+  c.zzzz = -(c.xxzz + c.yyzz);
+}
+
+// M2.v, M3[., v, v] / 2 and M4[., v, v, v] / 6.
+__device__ __forceinline__ void momc_contract(const Momc& M, double x, double y, double z,
+                                              double q2[3], double q3[3], double q4[3]) {
+  q2[0] = M.xx * x + M.xy * y + M.xz * z;
+  q2[1] = M.xy * x + M.yy * y + M.yz * z;
+  q2[2] = M.xz * x + M.yz * y + M.zz * z;
+  const double xx = 0.5 * x * x, yy = 0.5 * y * y, zz = 0.5 * z * z;
+  const double xy = x * y, xz = x * z, yz = y * z;
+  q3[0] = M.xxx * xx + M.xxy * xy + M.xxz * xz + M.xyy * yy + M.xyz * yz + M.xzz * zz;
+  q3[1] = M.xxy * xx + M.xyy * xy + M.xyz * xz + M.yyy * yy + M.yyz * yz + M.yzz * zz;
+  q3[2] = M.xxz * xx + M.xyz * xy + M.xzz * xz + M.yyz * yy + M.yzz * yz + M.zzz * zz;
+  const double onethird = 1.0 / 3.0;
+  const double xxx = onethird * xx * x, xxy = xx * y, xxz = xx * z, xyy = yy * x,
+               yyy = onethird * yy * y;
+  const double yyz = yy * z, xzz = zz * x, yzz = zz * y, zzz = onethird * zz * z, xyz = xy * z;
+  q4[0] = M.xxxx * xxx + M.xxxy * xxy + M.xxxz * xxz + M.xxyy * xyy + M.xxyz * xyz +
+          M.xxzz * xzz + M.xyyy * yyy + M.xyyz * yyz + M.xyzz * yzz + M.xzzz * zzz;
+  q4[1] = M.xxxy * xxx + M.xxyy * xxy + M.xxyz * xxz + M.xyyy * xyy + M.xyyz * xyz +
+          M.xyzz * xzz + M.yyyy * yyy + M.yyyz * yyz + M.yyzz * yzz + M.yzzz * zzz;
+  q4[2] = M.xxxz * xxx + M.xxyz * xxy + M.xxzz * xxz + M.xyyz * xyy + M.xyzz * xyz +
+          M.xzzz * xzz + M.yyyz * yyy + M.yyzz * yyz + M.yzzz * yzz + M.zzzz * zzz;
+}
+
+// One thread: the Fourier table {2 pi h / L, C(h), S(h)} in the lattice order
+// of the loops (h_x outermost).
+__global__ void ewald_hexa_table_kernel(const double* __restrict__ nd, int64_t nn,
+                                        const HexNode* __restrict__ hn, double L,
+                                        double* __restrict__ table, int32_t* __restrict__ count) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double mass = nd[PT_CD_MASS * nn];
+  Momc M;
+  momc_from_reduced(mass, hn[0].h, M);
+  const double alpha = kEwAlphaFac / L, alpha2 = alpha * alpha;
+  const double k4 = kPi * kPi / (alpha2 * L * L);
+  const double w = 2.0 * kPi / L;
+  const int hr = (int)ceil(kEwHCut);
+  int nh = 0;
+  for (int hx = -hr; hx <= hr; hx++)
+    for (int hy = -hr; hy <= hr; hy++)
+      for (int hz = -hr; hz <= hr; hz++) {
+        const int h2 = hx * hx + hy * hy + hz * hz;
+        if (h2 == 0 || h2 > kEwHCut * kEwHCut) continue;
+        const double g = exp(-k4 * h2) / (kPi * h2 * L);
+        double q2[3], q3[3], q4[3];
+        momc_contract(M, hx, hy, hz, q2, q3, q4);
+        const double m2 = 0.5 * (q2[0] * hx + q2[1] * hy + q2[2] * hz);
+        const double m3 = (1.0 / 3.0) * (q3[0] * hx + q3[1] * hy + q3[2] * hz);
+        const double m4 = 0.25 * (q4[0] * hx + q4[1] * hy + q4[2] * hz);
+        table[5 * nh + 0] = w * hx;
+        table[5 * nh + 1] = w * hy;
+        table[5 * nh + 2] = w * hz;
+        table[5 * nh + 3] = -g * (mass - w * w * m2 + w * w * w * w * m4);
+        table[5 * nh + 4] = g * w * w * w * m3;
+        nh++;
+      }
+  *count = nh;
+}
+
+// One thread per particle (sorted order); adds to the accelerations and
+// potentials the replica traversals left.
+__global__ void __launch_bounds__(256)
+ewald_hexa_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                  const double* __restrict__ z, const double* __restrict__ nd, int64_t nn,
+                  const HexNode* __restrict__ hn, double L, int n_replicas,
+                  const double* __restrict__ table, const int32_t* __restrict__ count,
+                  double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+                  double* __restrict__ pot) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const double mass = nd[PT_CD_MASS * nn];
+  const double cm[3] = {nd[PT_CD_CM * nn], nd[(PT_CD_CM + 1) * nn], nd[(PT_CD_CM + 2) * nn]};
+  Momc M;
+  momc_from_reduced(mass, hn[0].h, M);
+  const double Q4xx = 0.5 * (M.xxxx + M.xxyy + M.xxzz), Q4xy = 0.5 * (M.xxxy + M.xyyy + M.xyzz);
+  const double Q4xz = 0.5 * (M.xxxz + M.xyyz + M.xzzz), Q4yy = 0.5 * (M.xxyy + M.yyyy + M.yyzz);
+  const double Q4yz = 0.5 * (M.xxyz + M.yyyz + M.yzzz), Q4zz = 0.5 * (M.xxzz + M.yyzz + M.zzzz);
+  const double Q4 = 0.25 * (Q4xx + Q4yy + Q4zz);
+  const double Q3x = 0.5 * (M.xxx + M.xyy + M.xzz), Q3y = 0.5 * (M.xxy + M.yyy + M.yzz);
+  const double Q3z = 0.5 * (M.xxz + M.yyz + M.zzz);
+  const double Q2 = 0.5 * (M.xx + M.yy + M.zz);
+  const double alpha = kEwAlphaFac / L, alpha2 = alpha * alpha;
+  const double k1 = kPi / (alpha2 * L * L * L);
+  const double ka = 2.0 * alpha / sqrt(kPi);
+  int nrep = (int)ceil(kEwCut);
+  if (n_replicas > nrep) nrep = n_replicas;
+  const double cut2 = kEwCut * kEwCut * L * L, inner2 = 1.2e-3 * L * L;
+  double p = mass * k1, fx = 0.0, fy = 0.0, fz = 0.0;
+  const double dx = x[i] - cm[0], dy = y[i] - cm[1], dz = z[i] - cm[2];
+  for (int ix = -nrep; ix <= nrep; ix++) {
+    const bool hole_x = ix >= -n_replicas && ix <= n_replicas;
+    const double rx = dx + ix * L;
+    for (int iy = -nrep; iy <= nrep; iy++) {
+      const bool hole_xy = hole_x && iy >= -n_replicas && iy <= n_replicas;
+      const double ry = dy + iy * L;
+      for (int iz = -nrep; iz <= nrep; iz++) {
+        const bool hole = hole_xy && iz >= -n_replicas && iz <= n_replicas;
+        const double rz = dz + iz * L;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > cut2 && !hole) continue;
+        double g0, g1, g2, g3, g4, g5;
+        if (r2 < inner2) {
+          const double s2 = r2 * alpha2;
+          double an = ka;
+          g0 = an * ((1.0 / 3.0) * s2 - 1.0);
+          an *= 2.0 * alpha2;
+          g1 = an * ((1.0 / 5.0) * s2 - (1.0 / 3.0));
+          an *= 2.0 * alpha2;
+          g2 = an * ((1.0 / 7.0) * s2 - (1.0 / 5.0));
+          an *= 2.0 * alpha2;
+          g3 = an * ((1.0 / 9.0) * s2 - (1.0 / 7.0));
+          an *= 2.0 * alpha2;
+          g4 = an * ((1.0 / 11.0) * s2 - (1.0 / 9.0));
+          an *= 2.0 * alpha2;
+          g5 = an * ((1.0 / 13.0) * s2 - (1.0 / 11.0));
+        } else {
+          const double r = sqrt(r2);
+          const double ir2 = 1.0 / r2;
+          const double a = exp(-r2 * alpha2) * ka * ir2;
+          g0 = (hole ? -erf(alpha * r) : erfc(alpha * r)) / r;
+          g1 = g0 * ir2 + a;
+          double an = 2.0 * alpha2;
+          g2 = 3.0 * g1 * ir2 + an * a;
+          an *= 2.0 * alpha2;
+          g3 = 5.0 * g2 * ir2 + an * a;
+          an *= 2.0 * alpha2;
+          g4 = 7.0 * g3 * ir2 + an * a;
+          an *= 2.0 * alpha2;
+          g5 = 9.0 * g4 * ir2 + an * a;
+        }
+        double q2[3], q3[3], q4[3];
+        momc_contract(M, rx, ry, rz, q2, q3, q4);
+        const double Q4vx = Q4xx * rx + Q4xy * ry + Q4xz * rz;
+        const double Q4vy = Q4xy * rx + Q4yy * ry + Q4yz * rz;
+        const double Q4vz = Q4xz * rx + Q4yz * ry + Q4zz * rz;
+        const double Q2mir = 0.5 * (q2[0] * rx + q2[1] * ry + q2[2] * rz) -
+                             (Q3x * rx + Q3y * ry + Q3z * rz) + Q4;
+        const double Q3mir = (1.0 / 3.0) * (q3[0] * rx + q3[1] * ry + q3[2] * rz) -
+                             0.5 * (Q4vx * rx + Q4vy * ry + Q4vz * rz);
+        const double Q4mir = 0.25 * (q4[0] * rx + q4[1] * ry + q4[2] * rz);
+        const double Qta = g1 * mass - g2 * Q2 + g3 * Q2mir + g4 * Q3mir + g5 * Q4mir;
+        p -= g0 * mass - g1 * Q2 + g2 * Q2mir + g3 * Q3mir + g4 * Q4mir;
+        fx += g2 * (q2[0] - Q3x) + g3 * (q3[0] - Q4vx) + g4 * q4[0] - rx * Qta;
+        fy += g2 * (q2[1] - Q3y) + g3 * (q3[1] - Q4vy) + g4 * q4[1] - ry * Qta;
+        fz += g2 * (q2[2] - Q3z) + g3 * (q3[2] - Q4vz) + g4 * q4[2] - rz * Qta;
+      }
+    }
+  }
+  const int nh = *count;
+  for (int k = 0; k < nh; k++) {
+    const double hd = table[5 * k] * dx + table[5 * k + 1] * dy + table[5 * k + 2] * dz;
+    const double c = cos(hd), sn = sin(hd), cc = table[5 * k + 3], cs = table[5 * k + 4];
+    p += cc * c + cs * sn;
+    const double f = cc * sn - cs * c;
+    fx += table[5 * k] * f;
+    fy += table[5 * k + 1] * f;
+    fz += table[5 * k + 2] * f;
+  }
+  ax[i] += fx;
+  ay[i] += fy;
+  az[i] += fz;
+  pot[i] += p;
+}
+
 }  // namespace
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
@@ -312,8 +514,9 @@ int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, h
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Fourier table entries the Ewald kernel can hold: the integer vectors of a
-// 7^3 cube (the cut 2.8 keeps 80 of them).
+// Fourier table entries the Ewald kernels can hold: the integer vectors of a
+// 7^3 cube (the cut 2.8 keeps 80 of them); doubles per entry: 4 in the
+// quadrupole form, 5 in the HEXADECAPOLE form.
 int64_t pt_ewald_table_entries() { return 343; }
 
 int pt_launch_ewald(int64_t n, const double* x, const double* y, const double* z,
@@ -324,5 +527,17 @@ int pt_launch_ewald(int64_t n, const double* x, const double* y, const double* z
   ewald_table_kernel<<<1, 64, 0, s>>>(nd, nn, L, table, count);
   ewald_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, nd, nn, L, n_replicas,
                                                            table, count, ax, ay, az, pot);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_ewald_hexa(int64_t n, const double* x, const double* y, const double* z,
+                         const double* nd, int64_t nn, const HexNode* hn, double L,
+                         int n_replicas, double* table, int32_t* count, double* ax, double* ay,
+                         double* az, double* pot, hipStream_t s) {
+  if (n <= 0) return 0;
+  ewald_hexa_table_kernel<<<1, 64, 0, s>>>(nd, nn, hn, L, table, count);
+  ewald_hexa_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, nd, nn, hn, L,
+                                                                n_replicas, table, count, ax, ay,
+                                                                az, pot);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -1,8 +1,9 @@
 """GPU parity of the periodic build (ExMain::traversalFn with bPeriodic,
 examples/Gravity.C:35-50): pt_compute_forces sums the (2N+1)^3 replica
-traversals and adds LeafEwaldFn's lattice correction (quadrupole form);
-counters exact and FP64 forces within 1e-12 of the oracle, FP32C within 1e-5
-RMS of FP64, for the quadrupole and hexadecapole builds."""
+traversals and adds LeafEwaldFn's lattice correction (the quadrupole form,
+and in the hexadecapole build the form with the root's moments to fourth
+order); counters exact and FP64 forces within 1e-12 of the oracle, FP32C
+within the RMS and maximum bounds of FP64, for both builds."""
 import numpy as np
 import pytest
 

@@ -676,3 +676,95 @@ def test_periodic_build_approaches_the_exact_periodic_force():
     assert np.sqrt(np.mean(rel0 ** 2)) > 0.5
     # the exact periodic forces conserve momentum
     assert np.abs((m[:, None] * exact).sum(0)).max() < 1e-12
+
+
+def _ewald_hexa_setup(seed=3, n=300):
+    """A lopsided cluster whose root has sizeable octupole and hexadecapole
+    moments: the reduced moments from the oracle's own per-node moments."""
+    pos, M, cm, q = _ewald_setup(seed, n)
+    m = np.random.default_rng(seed).uniform(0.5, 1.5, n)
+    m *= M / m.sum()
+    raw = O.hexa_raw_moments(pos, m, cm)
+    h = O.hexa_reduce(M, q, raw)[1:]
+    return pos, M, cm, q, h
+
+
+def test_ewald_hexa_reduces_to_the_quadrupole_form():
+    """With the octupole and hexadecapole set to zero the fourth-order form is
+    the quadrupole form fed the trace-free second moments (whose traces, the
+    Q2 terms, then vanish): the two restatements agree to rounding."""
+    pos, M, cm, q, h = _ewald_hexa_setup()
+    h2 = np.zeros(21)
+    h2[:5] = h[:5]
+    # the trace-free second moments as the quadrupole form's six components
+    xx, yy, xy, xz, yz = h[:5]
+    qtf = np.array([xx, xy, xz, yy, yz, -(xx + yy)])
+    t = pos[::25]
+    a1, p1 = O.ewald_hexa(t, M, cm, h2, 1.0, 1)
+    a2, p2 = O.ewald(t, M, cm, qtf, 1.0, 1)
+    assert np.abs(a1 - a2).max() < 1e-13 * np.abs(a2).max()
+    assert np.abs(p1 - p2).max() < 1e-13 * np.abs(p2).max()
+
+
+@pytest.mark.parametrize("n_rep", [0, 1])
+def test_ewald_hexa_is_independent_of_the_split(n_rep):
+    """The third- and fourth-order terms of the real-space sum (g4, g5) and of
+    the Fourier sum (the sine and h^4 coefficients) must balance for every
+    alpha: alpha = 2/L with the reference's cuts, with wider cuts and at
+    1.5/L agree (a wrong coefficient in either sum breaks this), and the
+    higher moments change the correction (they are not dead terms)."""
+    pos, M, cm, q, h = _ewald_hexa_setup()
+    t = pos[::30]
+    a1, p1 = O.ewald_hexa(t, M, cm, h, 1.0, n_rep)
+    a2, p2 = O.ewald_hexa(t, M, cm, h, 1.0, n_rep, a_fac=2.0, ew_cut=4.0, h_cut=5.0)
+    a3, p3 = O.ewald_hexa(t, M, cm, h, 1.0, n_rep, a_fac=1.5, ew_cut=4.0, h_cut=3.0)
+    scale = np.abs(a1).max()
+    assert np.abs(a1 - a2).max() < 1e-6 * scale and np.abs(a1 - a3).max() < 1e-6 * scale
+    assert np.abs(p1 - p2).max() < 1e-6 * np.abs(p1).max()
+    assert np.abs(p1 - p3).max() < 1e-6 * np.abs(p1).max()
+    hq = h.copy()
+    hq[5:] = 0.0
+    aq, _ = O.ewald_hexa(t, M, cm, hq, 1.0, n_rep)
+    assert np.abs(a1 - aq).max() > 1e-4 * scale
+
+
+def test_ewald_hexa_small_r_series_is_continuous():
+    _, M, cm, q, h = _ewald_hexa_setup()
+    r0 = np.sqrt(1.2e-3)
+    u = np.array([0.6, 0.48, 0.64])
+    pts = np.array([cm + u * r0 * (1 - 1e-7), cm + u * r0 * (1 + 1e-7)])
+    a, p = O.ewald_hexa(pts, M, cm, h, 1.0, 1)
+    assert np.abs(a[0] - a[1]).max() < 1e-5 * np.abs(a).max()
+    assert abs(p[0] - p[1]) < 1e-6 * abs(p[0])
+
+
+def test_periodic_hexadecapole_build_approaches_the_exact_periodic_force():
+    """The HEXADECAPOLE build's replicas + its fourth-order lattice correction
+    against the exact periodic force (as for the quadrupole build): closer
+    than the quadrupole-form correction of the same root."""
+    rng = np.random.default_rng(4)
+    n, L = 64, 1.0
+    pos = rng.uniform(0, 1, (n, 3))
+    pos[:24] = 0.3 + 0.05 * rng.standard_normal((24, 3))
+    m = rng.uniform(0.5, 1.5, n) / n
+    e = np.full(n, 1e-7)
+    P = O.Pipeline(pos, m, e, bucket=4, theta=0.1, hexadecapole=True)
+    acc, _, _ = P.gravity_replicas(1, (L, L, L), ewald=True)
+    acc0, _, _ = P.gravity_replicas(1, (L, L, L), ewald=False)
+    root = P.data[0]
+    xyz = np.stack([P.x, P.y, P.z], axis=1)
+    eq, _ = O.ewald(xyz, root["mass"], root["cm"], root["q"], L, 1)
+    accq = acc0.copy()
+    accq[P.perm] += eq
+    reps = [np.array([X, Y, Z], float) for X in (-1, 0, 1) for Y in (-1, 0, 1) for Z in (-1, 0, 1)]
+    allpos = np.concatenate([pos + r for r in reps])
+    exact = O.direct(allpos[:, 0], allpos[:, 1], allpos[:, 2], np.tile(m, 27), np.tile(e, 27),
+                     np.arange(13 * n, 14 * n))
+    for j in range(n):
+        a, _ = O.ewald(pos, m[j], pos[j], np.zeros(6), L, 1)
+        exact += a
+    rms = lambda a: float(np.sqrt(np.mean(  # noqa: E731
+        (np.linalg.norm(a - exact, axis=1) / np.linalg.norm(exact, axis=1)) ** 2)))
+    print("periodic hexadecapole vs exact: rms %.3g (quadrupole-form correction %.3g, "
+          "replicas alone %.3g)" % (rms(acc), rms(accq), rms(acc0)))
+    assert rms(acc) < rms(accq) and rms(acc) < 1e-2
