@@ -12,9 +12,12 @@ src/Traverser.h:15-37), the steps timed with the stage timers off.  The
 roofline object covers the interaction kernel: algorithmic FLOPs per launch
 (20 per P-P pair, 67 per P-C particle-node interaction) over its average
 duration, from HIP events recorded on the stream the kernel runs on during a
-second pass of the same number of steps with the stage timers on.  Errors are measured against an FP64 direct sum on
-4096 seeded random targets.  cpu_baseline times the oracle (CPU restatement,
-OpenMP over buckets) on the host cores over the same 1M-particle tree.
+second pass of the same number of steps with the stage timers on; its traffic
+is the counter pass recorded in profiles/plummer1m_kernels.md, next to the
+bytes the kernel must move at least once.  Errors are measured against an
+FP64 direct sum on 4096 seeded random targets.  cpu_baseline times the oracle
+(CPU restatement) on the host cores through the same phases on the same
+input: all threads over the whole step, one thread over a sample.
 """
 import argparse
 import json
@@ -33,16 +36,57 @@ CONFIGS = {
     "plummer100k": (100_000, 12, 0.7, 0.01, "plummer"),
     "halo16m": (16_000_000, 32, 0.7, 0.01, "halo"),
 }
-# HBM bytes per dispatch of gravity_kernel<FP32C> on this workload: FETCH_SIZE
-# + WRITE_SIZE from two rocprofv3 counter passes (write counter calibrated
-# against an 8 MB device copy: factor 1.0).  Most of the write traffic is the
-# kernel's register spill area (44 B per lane under the 6-wave register
-# budget); the algorithmic output is 32 B per particle.
-TRAFFIC_BYTES_FP32C = (129238.5 + 500326.4) * 1024
+# HBM bytes per dispatch of gravity_kernel<FP32C> on the 1M workload: FETCH_SIZE
+# + WRITE_SIZE from the two rocprofv3 counter passes recorded in this profile
+# (KiB per dispatch; the counters are read with factor 1.0, calibrated against
+# a device copy of known size).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plummer1m_kernels.md")
+TRAFFIC_KERNEL = "gravity_kernel<true, 192>"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
 FLOPS_PC = 67.0
+
+
+def profile_traffic(path=TRAFFIC_PROFILE, kernel=TRAFFIC_KERNEL):
+    """FETCH_SIZE + WRITE_SIZE (bytes per dispatch) of `kernel` from the
+    counter table of a committed profile, or None when it has no such row."""
+    try:
+        with open(path) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    head = None
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if "FETCH_SIZE (KiB)" in cells and "WRITE_SIZE (KiB)" in cells:
+            head = cells
+            continue
+        if head and len(cells) == len(head) and cells[0] == kernel:
+            fetch = float(cells[head.index("FETCH_SIZE (KiB)")])
+            write = float(cells[head.index("WRITE_SIZE (KiB)")])
+            return {"fetch_bytes": fetch * 1024, "write_bytes": write * 1024,
+                    "bytes": (fetch + write) * 1024, "source": os.path.relpath(path, ROOT)}
+    return None
+
+
+def algorithmic_bytes_fp32c(n, nn, nb):
+    """Bytes the fused FP32C kernel must move at least once per dispatch:
+    the sorted particles' x, y, z, m, eps (40 B), each node's walk record
+    (TreeNode, 64 B) and source record (GravNode32, 64 B), each bucket's
+    record (128 B), and the results written (a, pot: 32 B per particle)."""
+    return 40 * n + 128 * nn + 128 * nb + 32 * n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def make_input(name):
@@ -99,18 +143,54 @@ def time_list_path(d, theta, precision):
     return {"traverse_2pass": tm["traverse"], "interact": tm["force"]}
 
 
-def cpu_baseline(pos, m, eps, bucket, theta):
-    """Oracle (port) on the host cores: full force evaluation of the same tree."""
+def cpu_baseline(pos, m, eps, bucket, theta, sample_frac=0.2):
+    """The oracle (the CPU restatement, kind "port") on the host cores, the
+    same phases as a GPU step: universe + keys, sort (with the gather of the
+    sorted copies), tree build, moments (single-threaded C), and the force
+    evaluation (OpenMP over buckets) with all threads over every bucket and
+    with one thread over a seeded sample of the buckets.  value = whole-step
+    interactions/s with all threads."""
     from oracle import oracle as O
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    P = O.Pipeline(pos, m, eps, bucket=bucket, theta=theta)
+    t = {}
     t0 = time.perf_counter()
-    _, _, cnt = P.gravity(nthreads=nthreads)
-    dt = time.perf_counter() - t0
+    box = O.universe(pos[:, 0], pos[:, 1], pos[:, 2])
+    keys = O.keys(pos[:, 0], pos[:, 1], pos[:, 2], box)
+    t["keys"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sk, perm = O.sort(keys)
+    x, y, z = (np.ascontiguousarray(pos[perm, k]) for k in range(3))
+    ms, es = m[perm].copy(), eps[perm].copy()
+    t["sort"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nodes = O.build_tree(sk, bucket)
+    t["build"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    data = O.moments(nodes, x, y, z, ms, es)
+    t["moments"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, cnt = O.gravity(nodes, data, x, y, z, ms, es, theta, nthreads=nthreads)
+    t["force"] = time.perf_counter() - t0
     ints = float(cnt[0] + cnt[1])
-    return {"value": ints / dt, "unit": "interactions/s", "cores": nthreads, "kind": "port",
-            "sample": "full force evaluation (all %d buckets, %.3g interactions) of the same "
-                      "tree, OpenMP over buckets, %.1f s" % (len(P.bucket_nodes), ints, dt)}
+    nb = len(O.buckets(nodes))
+    rng = np.random.default_rng(5)
+    sel = np.sort(rng.choice(nb, max(1, int(sample_frac * nb)), replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    _, _, scnt = O.gravity(nodes, data, x, y, z, ms, es, theta, bucket_sel=sel, nthreads=1)
+    t1 = time.perf_counter() - t0
+    ips1 = float(scnt[0] + scnt[1]) / t1
+    serial = t["keys"] + t["sort"] + t["build"] + t["moments"]
+    step = serial + t["force"]
+    return {"value": ints / step, "unit": "interactions/s", "cores": nthreads, "kind": "port",
+            "sample": "whole step on the same 1M input (keys, sort, build and moments "
+                      "single-threaded, %.3g s; force over all %d buckets with %d OpenMP "
+                      "threads, %.3g s); one-thread force on %d seeded buckets (%.1f s)" % (
+                          serial, nb, nthreads, t["force"], len(sel), t1),
+            "cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
+            "phase_ms": {k: v * 1e3 for k, v in t.items()},
+            "force_ips_all_threads": ints / t["force"],
+            "force_ips_1_thread": ips1,
+            "step_ips_1_thread_est": ints / (serial + ints / ips1)}
 
 
 ADJOINT_CONFIGS = {
@@ -315,6 +395,9 @@ def main():
     achieved32 = flops / (f32_ms / 1000.0) / 1e12
     achieved64 = flops / (f64_ms / 1000.0) / 1e12
 
+    nn, nb, _ = d.tree_size()
+    algo_bytes = algorithmic_bytes_fp32c(n, nn, nb)
+    traffic = profile_traffic() if args.config == "plummer1m" else None
     list_path = {"fp32c": time_list_path(d, theta, PT_FP32C),
                  "fp64": time_list_path(d, theta, PT_FP64)}
 
@@ -368,7 +451,9 @@ def main():
         "list_path_ms": list_path,
         "roofline": {"bound": "valu", "achieved": achieved32, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved32 / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES_FP32C if args.config == "plummer1m" else None,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic,
+                     "algorithmic_bytes": algo_bytes,
                      "kernel": "gravity_kernel<FP32C>",
                      "fp64_kernel": {"achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
                                      "frac": achieved64 / PEAK_FP64_TFLOPS}},
