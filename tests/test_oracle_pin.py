@@ -768,3 +768,14 @@ def test_periodic_hexadecapole_build_approaches_the_exact_periodic_force():
     print("periodic hexadecapole vs exact: rms %.3g (quadrupole-form correction %.3g, "
           "replicas alone %.3g)" % (rms(acc), rms(accq), rms(acc0)))
     assert rms(acc) < rms(accq) and rms(acc) < 1e-2
+
+
+def test_walk_only_counters_equal_the_gravity_counters():
+    """or_traversal_counters runs or_gravity's walk without the forces: the
+    same four counters on a Plummer and a clumped input."""
+    for pos, bucket, theta in ((ics.plummer(20000, seed=3)[0], 12, 0.7),
+                               (ics.halo_mixture(20000, n_halos=8, seed=5)[0], 32, 0.5)):
+        n = len(pos)
+        P = O.Pipeline(pos, np.full(n, 1.0 / n), np.full(n, 0.01), bucket=bucket, theta=theta)
+        _, _, c = P.gravity()
+        assert np.array_equal(O.traversal_counters(P.nodes, P.data, theta), c)
