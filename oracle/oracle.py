@@ -87,6 +87,7 @@ def _declare(L):
     L.or_gravity_hexa.argtypes = [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _I64,
                                   _P, _P, _P, _P, _P, ctypes.c_int]
     L.or_lists_hexa.argtypes = [_P, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P]
+    L.or_set_ewald_libm_perturbation.argtypes = [_D, _D]
     L.or_ewald.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _P, _P, _P, _P]
     L.or_ewald_hexa.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _P, _P, _P, _P]
     L.or_ewald_hexa_general.argtypes = [_P, _P, _P, _I64, _D, _P, _P, _D, ctypes.c_int, _D, _D,
@@ -364,6 +365,14 @@ def ewald_hexa(pos, mass, cm, h, L, n_replicas, a_fac=None, ew_cut=2.6, h_cut=2.
                                     float(L), int(n_replicas), float(a_fac), float(ew_cut),
                                     float(h_cut), _p(ax), _p(ay), _p(az), _p(pot))
     return np.stack([ax, ay, az], axis=1), pot
+
+
+def set_ewald_libm_perturbation(erf_ulps=0.0, exp_ulps=0.0):
+    """Scales the real-space erf / erfc and exp values of the Ewald forms by
+    1 + ulps * 2^-52 (about that many ulps) until reset with (0, 0): a probe
+    of how far a libm that differs by that much moves the restated
+    algorithm's result."""
+    lib().or_set_ewald_libm_perturbation(float(erf_ulps), float(exp_ulps))
 
 
 def spline(r2, twoh):

@@ -1183,6 +1183,18 @@ void or_direct_vjp(const double* x, const double* y, const double* z,
 /* ------------------------------------------------------------------------- */
 /* Ewald sum of the periodic build (quadrupole form)                           */
 /* ------------------------------------------------------------------------- */
+/* Relative perturbation of the erf / erfc and exp values in the real-space
+ * Ewald terms (both forms), 1.0 = none.  The hole terms g1..g5 cancel
+ * catastrophically just outside the series radius, so a one-ulp change of
+ * these values moves the result by far more than one ulp; the parity tests
+ * set this to measure that sensitivity of the restated algorithm. */
+static double or_ew_erf_scale = 1.0, or_ew_exp_scale = 1.0;
+
+void or_set_ewald_libm_perturbation(double erf_ulps, double exp_ulps) {
+  or_ew_erf_scale = 1.0 + erf_ulps * 0x1p-52;
+  or_ew_exp_scale = 1.0 + exp_ulps * 0x1p-52;
+}
+
 /*
  * LeafEwaldFn with EwaldData::EwaldInit (examples/Ewald.h, examples/Ewald.C,
  * non-HEXADECAPOLE branches): after the explicit replica traversals, every
@@ -1259,8 +1271,8 @@ void or_ewald_general(const double* x, const double* y, const double* z, int64_t
           } else {
             const double r = sqrt(r2);
             const double ir2 = 1.0 / r2;
-            const double a = exp(-r2 * alpha2) * ka * ir2;
-            g0 = (hole ? -erf(alpha * r) : erfc(alpha * r)) / r;
+            const double a = (exp(-r2 * alpha2) * or_ew_exp_scale) * ka * ir2;
+            g0 = ((hole ? -erf(alpha * r) : erfc(alpha * r)) * or_ew_erf_scale) / r;
             g1 = g0 * ir2 + a;
             g2 = 3.0 * g1 * ir2 + 2.0 * alpha2 * a;
             g3 = 5.0 * g2 * ir2 + 4.0 * alpha2 * alpha2 * a;
@@ -1667,8 +1679,8 @@ void or_ewald_hexa_general(const double* x, const double* y, const double* z, in
           } else {
             const double r = sqrt(r2);
             const double ir2 = 1.0 / r2;
-            const double a = exp(-r2 * alpha2) * ka * ir2;
-            g0 = (hole ? -erf(alpha * r) : erfc(alpha * r)) / r;
+            const double a = (exp(-r2 * alpha2) * or_ew_exp_scale) * ka * ir2;
+            g0 = ((hole ? -erf(alpha * r) : erfc(alpha * r)) * or_ew_erf_scale) / r;
             g1 = g0 * ir2 + a;
             double an = 2.0 * alpha2;
             g2 = 3.0 * g1 * ir2 + an * a;

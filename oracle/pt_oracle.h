@@ -163,6 +163,10 @@ void or_lists_hexa(const or_node* nodes, int64_t nn, const or_cdata* d, const do
                    double theta, const double offset[3], int64_t* pp_count, int64_t* pc_count,
                    const int64_t* pp_off, const int64_t* pc_off, int32_t* pp, int32_t* pc);
 
+/* Test knob: scales the real-space erf / erfc and exp values of both Ewald
+ * forms by 1 + ulps * 2^-52 (0, 0 = unperturbed, the default). */
+void or_set_ewald_libm_perturbation(double erf_ulps, double exp_ulps);
+
 /* LeafEwaldFn + EwaldData::EwaldInit, quadrupole form (examples/Ewald.h,
  * examples/Ewald.C): the lattice of the root's monopole + quadrupole beyond
  * the explicit replicas, added to ax/ay/az/pot.  or_ewald uses the

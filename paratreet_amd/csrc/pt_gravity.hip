@@ -128,7 +128,10 @@ struct __align__(16) SrcC64 {  // closed node, FP64
 struct __align__(16) SrcP64 {  // source particle, FP64
   double x, y, z, m, e, pad;
 };
-struct SrcC32 {  // closed node, FP32, centre in the bucket frame
+// Closed node, FP32, centre in the bucket frame, moments scaled by 3:
+// Q' = 3Q and tr = 3 * (0.5 * trace(Q)), the factors node_pair_far's form
+// of the quadrupole terms needs.
+struct SrcC32 {
   float cx, cy, cz, mass, soft, qxx, qxy, qxz, qyy, qyz, qzz, tr;
 };
 struct SrcP32 {  // source particle, FP32, bucket frame
@@ -275,7 +278,7 @@ __device__ __forceinline__ float frame32(double v, double o, double cc) {
 struct Group {
   int ng;      // targets in this pass
   int S;       // source slots = 64 / ng
-  float rS;    // 1 / S
+  int magic;   // ceil(2^16 / S): ceil_div_s in integer arithmetic
   int s, t;    // this lane's slot and target (inactive lanes read slot 0)
   bool act;    // lane participates
   int64_t first;  // sorted index of the group's first target
@@ -286,7 +289,7 @@ __device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
   const int rem = b.count - g0;
   g.ng = rem < kGroupMax ? rem : kGroupMax;
   g.S = PT_WAVE / g.ng;
-  g.rS = 1.0f / (float)g.S;
+  g.magic = (65536 + g.S - 1) / g.S;
   const int lane = __lane_id();
   const int s = lane / g.ng;
   g.act = s < g.S;
@@ -305,12 +308,13 @@ __device__ __forceinline__ int uniform_if(int v) {
   else return v;
 }
 
-// ceil(n / S) for 0 <= n <= 128 without an integer division: with
-// y = n + S - 1, floor((y + 0.5) / S) = floor(y / S) because y / S is either
-// an integer or at least 1/S above one, and the FP32 error of (y + 0.5) * rS
-// is far below 0.5 / S.
+// ceil(n / S) for 0 <= n <= 128 without a division: with y = n + S - 1 <= 191
+// and magic = ceil(2^16 / S), (y * magic) >> 16 = floor(y / S) because
+// y * magic / 2^16 exceeds y / S by less than 191 / 2^16 < 1 / 64 <= 1 / S,
+// and y / S is either an integer or at least 1 / S below the next one.  All
+// operands are wave-uniform, so this is scalar arithmetic.
 __device__ __forceinline__ int ceil_div_s(int n, const Group& g) {
-  return (int)(((float)(n + g.S - 1) + 0.5f) * g.rS);
+  return ((n + g.S - 1) * g.magic) >> 16;
 }
 
 template <bool F32>
@@ -360,13 +364,13 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
                           : pt_open_softening(cx, cy, cz, soft, b);
     r.mass = h.mass;
     r.soft = mono ? -1.f : h.soft;
-    r.qxx = h.qxx;
-    r.qxy = h.qxy;
-    r.qxz = h.qxz;
-    r.qyy = h.qyy;
-    r.qyz = h.qyz;
-    r.qzz = h.qzz;
-    r.tr = 0.5f * (h.qxx + h.qyy + h.qzz);
+    r.qxx = 3.0f * h.qxx;
+    r.qxy = 3.0f * h.qxy;
+    r.qxz = 3.0f * h.qxz;
+    r.qyy = 3.0f * h.qyy;
+    r.qyz = 3.0f * h.qyz;
+    r.qzz = 3.0f * h.qzz;
+    r.tr = 1.5f * (h.qxx + h.qyy + h.qzz);
     near = mono || near32(r.cx, r.cy, r.cz, h.soft, bf, emax);
     return r;
   } else {
@@ -508,20 +512,22 @@ __device__ __forceinline__ void node_pair(const SrcC<F32>& s, const Lanes<F32>& 
         c = 3.0f * bb * dir2;
         d = 5.0f * c * dir2;
       }
+      // the staged moments are 3Q and 3 tr: qx = 3 (Qr)_x, qir = r.Q.r / 2
       const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
       const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
       const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
-      const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
-      const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
+      const float qir = (1.0f / 6.0f) * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+      const float c3 = (1.0f / 3.0f) * c, bb3 = (1.0f / 3.0f) * bb;
+      const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c3 * s.tr));
       pp = fmaf(-s.mass, a, pp);
       pp = fmaf(-c, qir, pp);
-      pp = fmaf(bb, s.tr, pp);
+      pp = fmaf(bb3, s.tr, pp);
       px = fmaf(-qir3, rx, px);
-      px = fmaf(c, qx, px);
+      px = fmaf(c3, qx, px);
       py = fmaf(-qir3, ry, py);
-      py = fmaf(c, qy, py);
+      py = fmaf(c3, qy, py);
       pz = fmaf(-qir3, rz, pz);
-      pz = fmaf(c, qz, pz);
+      pz = fmaf(c3, qz, pz);
     }
   } else {
     if (s.soft < 0.0) {
@@ -559,30 +565,31 @@ __device__ __forceinline__ void node_pair_far(
     typename std::conditional<F32, float, double>::type& px,
     typename std::conditional<F32, float, double>::type& py,
     typename std::conditional<F32, float, double>::type& pz,
-    typename std::conditional<F32, float, double>::type& pp, double ox, double oy, double oz) {
+    typename std::conditional<F32, float, double>::type& pp,
+    typename std::conditional<F32, float, double>::type& pq, double ox, double oy, double oz) {
   if constexpr (F32) {
+    // The unsoftened branch of SPLINEQ (a = 1/r, b = 1/r^3, c = 3/r^5,
+    // d = 15/r^7) regrouped over the staged 3Q and 3 tr: with u = 1/r^2,
+    // q = 3Qr, k = u (r.q) = 3 r.Q.r / r^2 and C = b u,
+    //   acc = -qir3 r + c Qr      = C (q - (m r^2 + 2.5 k - 3 tr) r),
+    //   pot = -m a - c qir + b tr = -m a + (b / 3) (3 tr - 1.5 k),
+    // the (b / 3) term summed in pq and scaled once per chunk.
     const float rx = L.tx - s.cx, ry = L.ty - s.cy, rz = L.tz - s.cz;
     const float rsq = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-    const float dir = rsqrt32(rsq);
-    const float dir2 = dir * dir;
-    const float a = dir;
-    const float bb = dir * dir2;
-    const float c = 3.0f * bb * dir2;
-    const float d = 5.0f * c * dir2;
+    const float v = rsqrt32(rsq);
+    const float u = v * v;
+    const float b = v * u;
+    const float cu = b * u;
     const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
     const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
     const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
-    const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
-    const float qir3 = fmaf(bb, s.mass, fmaf(d, qir, -c * s.tr));
-    pp = fmaf(-s.mass, a, pp);
-    pp = fmaf(-c, qir, pp);
-    pp = fmaf(bb, s.tr, pp);
-    px = fmaf(-qir3, rx, px);
-    px = fmaf(c, qx, px);
-    py = fmaf(-qir3, ry, py);
-    py = fmaf(c, qy, py);
-    pz = fmaf(-qir3, rz, pz);
-    pz = fmaf(c, qz, pz);
+    const float k = u * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+    const float gq = fmaf(s.mass, rsq, fmaf(2.5f, k, -s.tr));
+    px = fmaf(cu, fmaf(-gq, rx, qx), px);
+    py = fmaf(cu, fmaf(-gq, ry, qy), py);
+    pz = fmaf(cu, fmaf(-gq, rz, qz), pz);
+    pp = fmaf(-s.mass, v, pp);
+    pq = fmaf(b, fmaf(-1.5f, k, s.tr), pq);
   } else {
     const double rx = (L.tx - s.cx) - ox, ry = (L.ty - s.cy) - oy, rz = (L.tz - s.cz) - oz;
     const double rsq = fma(rx, rx, fma(ry, ry, rz * rz));
@@ -607,11 +614,12 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
                                            int kend, double ox, double oy, double oz) {
   const SrcC<F32>* src = reinterpret_cast<const SrcC<F32>*>(W.stage);
   if constexpr (F32) {
-    float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f, pq = 0.f;
     for (int k = 0; k < kfar; k++) node_pair<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
 #pragma unroll 2
     for (int k = kfar; k < kend; k++)
-      node_pair_far<true>(src[g.s + k * g.S], L, px, py, pz, pp, ox, oy, oz);
+      node_pair_far<true>(src[g.s + k * g.S], L, px, py, pz, pp, pq, ox, oy, oz);
+    pp = fmaf(1.0f / 3.0f, pq, pp);
     kahan_add(L.ax, L.cx, px);
     kahan_add(L.ay, L.cy, py);
     kahan_add(L.az, L.cz, pz);
@@ -620,7 +628,7 @@ __device__ __forceinline__ void eval_nodes(const W_t& W, const Group& g, Lanes<F
     for (int k = 0; k < kfar; k++)
       node_pair<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
     for (int k = kfar; k < kend; k++)
-      node_pair_far<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, ox, oy, oz);
+      node_pair_far<false>(src[g.s + k * g.S], L, L.ax, L.ay, L.az, L.ap, L.ap, ox, oy, oz);
   }
 }
 
@@ -1319,6 +1327,10 @@ __device__ __forceinline__ void gravity_bucket(
     load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
     int sp = 0;
     int npc = 0, npp = 0;
+    // this walk's open() outcomes, and the source-leaf particles this lane
+    // visited (summed over the wave once the walk is done)
+    int w_open = 0, w_close = 0;
+    int lane_leaf_parts = 0;
     WalkCounts wc;
     {
       const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, W.stack, sp, wc);
@@ -1332,24 +1344,18 @@ __device__ __forceinline__ void gravity_bucket(
         }
         npp = 1;
       }
-      if (count) {
-        c_open += wc.n_open;
-        c_close += wc.n_close;
-        c_node += (unsigned long long)wc.n_close * nt;
-        c_part += (unsigned long long)wc.leaf_parts * nt;
-      }
+      w_open = wc.n_open;
+      w_close = wc.n_close;
+      lane_leaf_parts = lane == 0 ? wc.leaf_parts : 0;
     }
     __builtin_amdgcn_wave_barrier();
     while (sp > 0) {
       unsigned long long leafm, closem;
-      const WalkLane r = pt_walk_step<STACK>(nodes, b, gf, ox, oy, oz, W.stack, sp, overflow,
-                                             wc, leafm, closem);
-      if (count) {
-        c_open += wc.n_open;
-        c_close += wc.n_close;
-        c_node += (unsigned long long)wc.n_close * nt;
-        c_part += (unsigned long long)wc.leaf_parts * nt;
-      }
+      const WalkLane r = pt_walk_step<STACK, false, false>(nodes, b, gf, ox, oy, oz, W.stack, sp,
+                                                           overflow, wc, leafm, closem);
+      w_open += wc.n_open;
+      w_close += wc.n_close;
+      lane_leaf_parts += r.is_leaf ? r.count : 0;
       if (r.is_closed) W.pcbuf[npc + pt_rank_in(closem)] = r.node;
       if (r.is_leaf) {
         const int k = npp + pt_rank_in(leafm);
@@ -1394,6 +1400,12 @@ __device__ __forceinline__ void gravity_bucket(
       const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
       do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
     }
+    if (count) {
+      c_open += (unsigned)w_open;
+      c_close += (unsigned)w_close;
+      c_node += (unsigned long long)w_close * nt;
+      c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
+    }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
   }
 }
@@ -1408,20 +1420,23 @@ __device__ __forceinline__ void gravity_bucket(
 // fewer resident waves; the FP64 kernel gains from the dynamic balance.)
 // The FP32C kernel is compiled for six waves per SIMD (80 VGPRs, a 40-byte
 // spill): 3.79 ms against 3.95 ms at five on the 1M Plummer.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool OFF>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 6 : 1, 8)))
 gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __restrict__ nodes,
                const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
                const double* __restrict__ px,
                const double* __restrict__ py, const double* __restrict__ pz,
                const double* __restrict__ pm, const double* __restrict__ pe, double gf,
-               double ox, double oy, double oz, double* __restrict__ ax,
+               double ox_in, double oy_in, double oz_in, double* __restrict__ ax,
                double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
                unsigned long long* __restrict__ partials, int32_t* __restrict__ err,
                unsigned long long* __restrict__ queue, int acc) {
   using W_t = WaveLds<F32, STACK>;
   __shared__ W_t lds[kWaves];
   __shared__ unsigned long long s_cnt[kWaves][4];
+  // without a replica offset the offset is -0.0, which every v + offset
+  // folds away exactly (v + -0.0 == v for every v, -0.0 included)
+  const double ox = OFF ? ox_in : -0.0, oy = OFF ? oy_in : -0.0, oz = OFF ? oz_in : -0.0;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   W_t& W = lds[wave];
@@ -1556,17 +1571,22 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
   if (nb <= 0) return 0;
   if (pt_launch_fill_u64(queue, 8, 0ull, s) != 0) return -1;
   const unsigned grid = (unsigned)pt_gravity_blocks(nb, precision);
-#define PT_GRAVITY_LAUNCH(F32, STACK)                                                          \
-  gravity_kernel<F32, STACK><<<grid, kThreads, 0, s>>>(                                        \
+  const bool has_off = off[0] != 0.0 || off[1] != 0.0 || off[2] != 0.0;
+#define PT_GRAVITY_LAUNCH(F32, STACK, OFF)                                                     \
+  gravity_kernel<F32, STACK, OFF><<<grid, kThreads, 0, s>>>(                                   \
       bk, nb, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, ay, az, pot,      \
       partials, err, queue, accumulate ? 1 : 0)
+#define PT_GRAVITY_LAUNCH_OFF(F32, STACK)   \
+  if (has_off) PT_GRAVITY_LAUNCH(F32, STACK, true); \
+  else PT_GRAVITY_LAUNCH(F32, STACK, false)
   if (precision == 0) {
-    if (big_stack) PT_GRAVITY_LAUNCH(false, kBigStack);
-    else PT_GRAVITY_LAUNCH(false, kSmallStack);
+    if (big_stack) PT_GRAVITY_LAUNCH_OFF(false, kBigStack);
+    else PT_GRAVITY_LAUNCH_OFF(false, kSmallStack);
   } else {
-    if (big_stack) PT_GRAVITY_LAUNCH(true, kBigStack);
-    else PT_GRAVITY_LAUNCH(true, kSmallStack);
+    if (big_stack) PT_GRAVITY_LAUNCH_OFF(true, kBigStack);
+    else PT_GRAVITY_LAUNCH_OFF(true, kSmallStack);
   }
+#undef PT_GRAVITY_LAUNCH_OFF
 #undef PT_GRAVITY_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -180,8 +180,9 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
 // One step: pops up to 8 opened nodes and tests their 64 children.  Returns
 // this lane's child and its classification; pushes opened children.  A push
 // beyond STACK entries sets overflow and ends the walk (the caller reports
-// it; nothing is silently dropped).
-template <int STACK, bool HEXA = false>
+// it; nothing is silently dropped).  With SUM_LEAVES false wc.leaf_parts is
+// left 0 and the caller sums the leaf lanes' counts itself.
+template <int STACK, bool HEXA = false, bool SUM_LEAVES = true>
 __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ nodes,
                                                  const Bucket& b, double gf, double ox,
                                                  double oy, double oz,
@@ -235,7 +236,8 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   sp += n_open;
   wc.n_open = n_open;
   wc.n_close = __popcll(closem);
-  wc.leaf_parts = pt_wave_sum_int(r.is_leaf ? r.count : 0);
+  if constexpr (SUM_LEAVES) wc.leaf_parts = pt_wave_sum_int(r.is_leaf ? r.count : 0);
+  else wc.leaf_parts = 0;
   return r;
 }
 

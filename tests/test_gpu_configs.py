@@ -183,9 +183,11 @@ def test_c4_plummer256k_adjoint_fd_and_direct():
     gx, gv, gm = _gradients(d, pos, vel, m, e, steps, dt, theta, PT_FP64, FORCE_TREE, 0)
 
     def loss(p, v, mm):
+        # per-particle terms: the differences are summed, not the two losses
+        # (each ~1e-5, whose difference would lose the gradient to rounding)
         d.load(p, mm, e, vel=v)
         d.leapfrog_forward(steps, dt, theta=theta, precision=PT_FP64, mode=FORCE_TREE)
-        return np.sum(mm * np.sum((d.state()[0] - p) ** 2, axis=1))
+        return mm * np.sum((d.state()[0] - p) ** 2, axis=1)
 
     rng = np.random.default_rng(7)
     errs = {"x": [], "v": [], "m": []}
@@ -197,25 +199,26 @@ def test_c4_plummer256k_adjoint_fd_and_direct():
             p1, p2 = pos.copy(), pos.copy()
             p1[i, k] += H
             p2[i, k] -= H
-            fd = (loss(p1, vel, m) - loss(p2, vel, m)) / (2 * H)
+            fd = np.sum(loss(p1, vel, m) - loss(p2, vel, m)) / (2 * H)
             errs["x"].append(abs(fd - gx[i, k]) / np.abs(gx).max())
         else:
             v1, v2 = vel.copy(), vel.copy()
             v1[i, k] += H
             v2[i, k] -= H
-            fd = (loss(pos, v1, m) - loss(pos, v2, m)) / (2 * H)
+            fd = np.sum(loss(pos, v1, m) - loss(pos, v2, m)) / (2 * H)
             errs["v"].append(abs(fd - gv[i, k]) / np.abs(gv).max())
     for j in rng.choice(n, 8, replace=False):
         hm = 1e-6 * m[j]
         m1, m2 = m.copy(), m.copy()
         m1[j] += hm
         m2[j] -= hm
-        fd = (loss(pos, vel, m1) - loss(pos, vel, m2)) / (2 * hm)
+        fd = np.sum(loss(pos, vel, m1) - loss(pos, vel, m2)) / (2 * hm)
         errs["m"].append(abs(fd - gm[j]) / np.abs(gm).max())
+    errs = {k: np.array(v) for k, v in errs.items()}
     for k, v in errs.items():
-        v = np.array(v)
         _say("C4 FD %s (%d): median %.2e max %.2e of the gradient scale" % (
             k, len(v), np.median(v), v.max()))
+    for k, v in errs.items():
         assert np.median(v) < 1e-5, k
         assert (v < 1e-4).mean() >= 0.9, k
     _say("C4: FP64 direct-sum trajectory gradient")

@@ -41,10 +41,38 @@ def test_periodic_replicas_match_oracle(hexa):
     # relative to the particle's force, or to the median force where the
     # replicas nearly cancel
     on = np.linalg.norm(oacc, axis=1)
-    err = np.linalg.norm(acc - oacc, axis=1) / np.maximum(on, np.median(on))
-    print("periodic %s fp64 vs oracle: max rel %.3g" % ("hexa" if hexa else "quad", err.max()))
-    assert err.max() <= FP64_TOL
-    assert np.abs(pot - opot).max() <= FP64_TOL * np.abs(opot).max()
+    scale = np.maximum(on, np.median(on))
+    err = np.linalg.norm(acc - oacc, axis=1) / scale
+    # The real-space hole terms g1..g5 of the Ewald recursion cancel
+    # catastrophically just outside the series radius (r^2 = 1.2e-3 L^2), so
+    # the erf / exp values of two correct libms (the device's and the host's,
+    # each within a few ulps) move the result by far more than their ulps:
+    # the oracle measures how far one ulp of each moves it, per particle, and
+    # the bound allows two.  In the quadrupole form (g up to g3) that reach
+    # stays below 1e-13; in the fourth-order form it reaches 6e-11.
+    sens_a, sens_p = np.zeros(n), np.zeros(n)
+    for eu, xu in ((1, 1), (-1, -1), (1, -1), (-1, 1)):
+        O.set_ewald_libm_perturbation(eu, xu)
+        try:
+            pa, pp, _ = P.gravity_replicas(1, period, ewald=True)
+        finally:
+            O.set_ewald_libm_perturbation(0, 0)
+        sens_a = np.maximum(sens_a, np.linalg.norm(pa - oacc, axis=1))
+        sens_p = np.maximum(sens_p, np.abs(pp - opot))
+    tol = FP64_TOL + 2.0 * sens_a / scale
+    print("periodic %s fp64 vs oracle: max rel %.3g (largest libm reach %.3g)" % (
+        "hexa" if hexa else "quad", err.max(), (sens_a / scale).max()))
+    worst = np.argsort(err / tol)[-5:]
+    xs = np.stack([P.x, P.y, P.z], axis=1)[np.argsort(P.perm)]
+    cm = P.data[0]["cm"]
+    for i in worst:
+        r2 = float(np.sum((xs[i] - cm) ** 2))
+        print("  particle %d: err %.3g tol %.3g libm reach %.3g |a| %.3g r^2/L^2 %.4g" % (
+            i, err[i], tol[i], sens_a[i] / scale[i], on[i], r2 / period[0] ** 2))
+    if not hexa:
+        assert (sens_a / scale).max() <= 1e-13
+    assert np.all(err <= tol)
+    assert np.all(np.abs(pot - opot) <= FP64_TOL * np.abs(opot).max() + 2.0 * sens_p)
     cnt32 = d.compute_forces(theta=0.7, precision=1)
     a32, _ = d.accelerations()
     assert tuple(cnt32) == tuple(cnt)

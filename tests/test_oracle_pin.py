@@ -728,6 +728,30 @@ def test_ewald_hexa_is_independent_of_the_split(n_rep):
     assert np.abs(a1 - aq).max() > 1e-4 * scale
 
 
+def test_ewald_libm_perturbation_knob():
+    """The knob the GPU parity test uses to measure the libm sensitivity: at
+    (0, 0) the results are bit for bit the unperturbed ones (also after a
+    perturbed call), and a one-ulp change moves the fourth-order form's hole
+    terms just outside the series radius by far more than an ulp, and far
+    more than it moves the quadrupole form's."""
+    pos, M, cm, q, h = _ewald_hexa_setup()
+    r0 = np.sqrt(1.2e-3)
+    pts = np.array([cm + np.array([0.6, 0.48, 0.64]) * r0 * 1.5, pos[0]])
+    a0, p0 = O.ewald_hexa(pts, M, cm, h, 1.0, 1)
+    q0, _ = O.ewald(pts, M, cm, q, 1.0, 1)
+    O.set_ewald_libm_perturbation(1, 1)
+    try:
+        a1, _ = O.ewald_hexa(pts, M, cm, h, 1.0, 1)
+        q1, _ = O.ewald(pts, M, cm, q, 1.0, 1)
+    finally:
+        O.set_ewald_libm_perturbation(0, 0)
+    a2, p2 = O.ewald_hexa(pts, M, cm, h, 1.0, 1)
+    assert np.array_equal(a0, a2) and np.array_equal(p0, p2)
+    d_hexa = np.abs(a1[0] - a0[0]).max() / np.abs(a0[0]).max()
+    d_quad = np.abs(q1[0] - q0[0]).max() / np.abs(q0[0]).max()
+    assert d_hexa > 1e-11 and d_hexa > 30.0 * d_quad
+
+
 def test_ewald_hexa_small_r_series_is_continuous():
     _, M, cm, q, h = _ewald_hexa_setup()
     r0 = np.sqrt(1.2e-3)
