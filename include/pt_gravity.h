@@ -144,6 +144,20 @@ pt_status pt_assign_keys(pt_ctx* ctx);
  * (src/Reader.C:237-241): ascending key, ties by original index. */
 pt_status pt_sort(pt_ctx* ctx);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The sort of pt_sort over caller-owned device arrays:
+ * keys (n, read only) -> sorted_keys (n, ascending, ties by input index) and
+ * perm (n, sorted slot -> input index), enqueued on `stream` (a hipStream_t
+ * of the current device, or NULL) and asynchronous.  Its temporaries live in
+ * a fixed per-GPU device array held until the process ends
+ * (26 MiB, n <= PT_SORT_NOALLOC_MAX_N; larger n returns PT_ERR_ARG); it calls
+ * no allocator.  Calls that overlap on one GPU share that array and race.
+ * Errors: pt_last_error(NULL) returns the calling thread's last message. */
+#define PT_SORT_NOALLOC_MAX_N (1 << 21)
+pt_status pt_sort_noalloc(const uint64_t* keys, int64_t n, uint64_t* sorted_keys, uint32_t* perm,
+                          void* stream);
+
 /* Subtree::buildTree / recursiveBuild (src/Subtree.h:275-349). */
 pt_status pt_build_tree(pt_ctx* ctx);
 

@@ -28,7 +28,7 @@ EXPORTS = (
     "pt_get_buckets", "pt_get_lists_size", "pt_get_lists", "pt_get_accelerations",
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
     "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint", "pt_get_node_hexadecapole",
-    "pt_set_adjoint_precision", "pt_get_group_lists",
+    "pt_set_adjoint_precision", "pt_get_group_lists", "pt_sort_noalloc",
 )
 
 
@@ -114,6 +114,7 @@ def load():
         "pt_remake_universe": (I32, [P, P]),
         "pt_assign_keys": (I32, [P]),
         "pt_sort": (I32, [P]),
+        "pt_sort_noalloc": (I32, [P, I64, P, P, P]),
         "pt_build_tree": (I32, [P]),
         "pt_upward": (I32, [P]),
         "pt_traverse": (I32, [P, D, P, P]),

@@ -31,6 +31,7 @@ SOURCES = {
     "pt_dual.hip": EXACT,
     "pt_gravity.hip": EXACT,
     "pt_misc.hip": EXACT,
+    "pt_noalloc.hip": [],
     # hardware FP64 atomic adds (device memory from hipMalloc is coarse-grained)
     "pt_adjoint.hip": EXACT + ["-munsafe-fp-atomics"],
     "pt_capi.cpp": [],

@@ -366,7 +366,11 @@ void pt_ctx_destroy(pt_ctx* c) {
   delete c;
 }
 
-const char* pt_last_error(const pt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* pt_last_error(const pt_ctx* c) {
+  if (c) return c->err.c_str();
+  const char* m = pt_noalloc_last_error();
+  return m ? m : "null context";
+}
 
 pt_status pt_get_device_view(pt_ctx* c, pt_device_view* out) {
   if (!c || !out) return PT_ERR_ARG;

@@ -38,6 +38,8 @@ int64_t pt_sort_scratch_bytes(int64_t n);
 int pt_radix_sort(const uint64_t* src, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
                   int end_bit, void* scratch, hipStream_t s);
 int pt_launch_iota(uint32_t* v, int64_t n, hipStream_t s);
+// Message of the calling thread's last failed pt_sort_noalloc, or NULL.
+const char* pt_noalloc_last_error();
 
 int pt_launch_root(int64_t n, int bucket, const PtTree& t, int32_t* level_cnt, hipStream_t s);
 // Largest leaf the fixup after a top-bits sort handles (larger buckets use
