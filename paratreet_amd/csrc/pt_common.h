@@ -39,7 +39,10 @@ struct __align__(64) TreeNode {
   // box) and of the particles' softening, rounded up to FP32; 0 otherwise
   float leaf_reach;
   float leaf_soft;
-  int32_t pad[2];
+  // internal nodes: the node's softening (CentroidData soft) rounded up to
+  // FP32; 0 for leaves
+  float node_soft;
+  int32_t pad;
 };
 // Packed source record for particle-cell interactions (FP64 path).
 // tr = 0.5 * (qxx + qyy + qzz) as GravityVisitor::node computes it.

@@ -270,6 +270,31 @@ __device__ __forceinline__ int32_t leaf_entry(int32_t count, double cx, double c
   return far ? (count | kFarLeaf) : count;
 }
 
+// The same classifications from the walk's exact squared box distance dsq
+// of the source's centre (+ offset), with margins (1e-4 relative on the
+// radii, 2e-4 on the reaches, 4e-6 of the bucket extent) that cover the FP32
+// rounding of the bucket-frame coordinates the pair loops see.  A leaf is far
+// when every particle (within leaf_reach of its centre) is beyond softening
+// reach of every target; a closed node is far when, in addition, the spheres
+// openSoftening tests cannot meet (dsq above (2 soft + 2 bucket soft)^2 rules
+// out both of its tests), so node() takes SPLINEQ's unsoftened branch.
+__device__ __forceinline__ bool leaf_far_dsq(double dsq, float lreach, float lsoft,
+                                             const BucketF& f, float emax) {
+  const float reach = fmaf(lreach + lsoft + emax, 1.0002f, fmaf(3e-6f, lreach, 4e-6f * f.ext));
+  return (float)dsq > 1.0001f * (reach * reach);
+}
+
+__device__ __forceinline__ bool node_far_dsq(double dsq, float nsoft, const BucketF& f,
+                                             float emax) {
+  const float mono = 2.0f * (nsoft + f.soft);
+  const float reach = fmaf(nsoft + emax, 1.0002f, 4e-6f * f.ext);
+  const float m = fmaxf(mono, reach);
+  return (float)dsq > 1.0001f * (m * m);
+}
+
+// A closed node's id with the far flag of node_far_dsq in bit 30.
+constexpr int32_t kFarNode = 1 << 30;
+
 __device__ __forceinline__ float frame32(double v, double o, double cc) {
   return (float)((v + o) - cc);
 }
@@ -279,6 +304,10 @@ struct Group {
   int ng;      // targets in this pass
   int S;       // source slots = 64 / ng
   int magic;   // ceil(2^16 / S): ceil_div_s in integer arithmetic
+  int cw;      // sources per full chunk of the FP64 fused walk: the
+               // largest multiple of S up to 64, so a full chunk fills every
+               // slot (measured faster in FP64, slower in FP32C, which keeps
+               // chunks of 64)
   int s, t;    // this lane's slot and target (inactive lanes read slot 0)
   bool act;    // lane participates
   int64_t first;  // sorted index of the group's first target
@@ -290,6 +319,9 @@ __device__ __forceinline__ Group make_group(const Bucket& b, int g0) {
   g.ng = rem < kGroupMax ? rem : kGroupMax;
   g.S = PT_WAVE / g.ng;
   g.magic = (65536 + g.S - 1) / g.S;
+  // floor(64 / S) = (64 * magic) >> 16: the excess of magic / 2^16 over 1 / S
+  // moves 64 / S by less than 64 / 2^16 < 1 / S
+  g.cw = g.S * ((PT_WAVE * g.magic) >> 16);
   const int lane = __lane_id();
   const int s = lane / g.ng;
   g.act = s < g.S;
@@ -346,7 +378,8 @@ __device__ __forceinline__ void load_targets(Lanes<F32>& L, const Group& g,
 // Closed-node source record; `near` marks a node that takes the monopole
 // (openSoftening) branch or may be softened for some target of the group.
 template <bool F32>
-__device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __restrict__ gn,
+__device__ __forceinline__ SrcC<F32> node_record(int32_t id, bool far,
+                                                 const GravNode* __restrict__ gn,
                                                  const GravNode32* __restrict__ gn32,
                                                  const Bucket& b, const BucketF& bf, float emax,
                                                  double ox, double oy, double oz, double ccx,
@@ -359,9 +392,13 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
     r.cx = (float)(cx - ccx);
     r.cy = (float)(cy - ccy);
     r.cz = (float)(cz - ccz);
-    const bool mono = softening_far32(r.cx, r.cy, r.cz, h.soft, bf)
-                          ? false
-                          : pt_open_softening(cx, cy, cz, soft, b);
+    bool mono = false;
+    near = false;
+    if (!far) {
+      mono = softening_far32(r.cx, r.cy, r.cz, h.soft, bf) ? false
+                                                            : pt_open_softening(cx, cy, cz, soft, b);
+      near = mono || near32(r.cx, r.cy, r.cz, h.soft, bf, emax);
+    }
     r.mass = h.mass;
     r.soft = mono ? -1.f : h.soft;
     r.qxx = 3.0f * h.qxx;
@@ -371,7 +408,6 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
     r.qyz = 3.0f * h.qyz;
     r.qzz = 3.0f * h.qzz;
     r.tr = 1.5f * (h.qxx + h.qyy + h.qzz);
-    near = mono || near32(r.cx, r.cy, r.cz, h.soft, bf, emax);
     return r;
   } else {
     const GravNode s = gn[id];
@@ -382,7 +418,13 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
     r.cy = s.cy;
     r.cz = s.cz;
     r.mass = s.mass;
-    const bool mono = pt_open_softening(cx, cy, cz, s.soft, b);
+    bool mono = false;
+    near = false;
+    if (!far) {
+      mono = pt_open_softening(cx, cy, cz, s.soft, b);
+      near = mono || near32((float)(cx - ccx), (float)(cy - ccy), (float)(cz - ccz),
+                            (float)soft, bf, emax);
+    }
     r.soft = mono ? -1.0 : s.soft;
     r.qxx = s.qxx;
     r.qxy = s.qxy;
@@ -391,8 +433,6 @@ __device__ __forceinline__ SrcC<F32> node_record(int32_t id, const GravNode* __r
     r.qyz = s.qyz;
     r.qzz = s.qzz;
     r.tr = s.tr;
-    near = mono || near32((float)(cx - ccx), (float)(cy - ccy), (float)(cz - ccz), (float)soft,
-                          bf, emax);
     return r;
   }
 }
@@ -730,25 +770,28 @@ __device__ __forceinline__ void do_leaves(W_t& W, int32_t beg, int32_t cntf, con
   const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
   const int exc = far ? near_total + incl_f - cf : incl_n - cn;
   int carry = 0;
-  for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+  // windows of cw particles (FP64: a multiple of S, full windows need no
+  // padding)
+  const int cw = F32 ? PT_WAVE : g.cw;
+  for (int s0 = 0; s0 < total; s0 += cw) {
     W.fl_mark[lane] = -1;
     __builtin_amdgcn_wave_barrier();
-    if (cnt > 0 && exc >= s0 && exc < s0 + PT_WAVE) W.fl_mark[exc - s0] = lane;
+    if (cnt > 0 && exc >= s0 && exc < s0 + cw) W.fl_mark[exc - s0] = lane;
     __builtin_amdgcn_wave_barrier();
     const unsigned long long starts = __ballot(W.fl_mark[lane] >= 0);
     const unsigned long long upto = starts & (~0ull >> (PT_WAVE - 1 - lane));
     const int leaf = upto ? W.fl_mark[PT_WAVE - 1 - __clzll(upto)] : carry;
-    carry = __builtin_amdgcn_readlane(leaf, PT_WAVE - 1);
+    carry = __builtin_amdgcn_readlane(leaf, cw - 1);
     const int lbeg = __shfl(beg, leaf, PT_WAVE);
     const int lexc = __shfl(exc, leaf, PT_WAVE);
     const int slot = s0 + lane;
-    if (slot < total) {
+    if (lane < cw && slot < total) {
       const int64_t j = (int64_t)lbeg + (slot - lexc);
       bool nr;
       reinterpret_cast<SrcP<F32>*>(W.stage)[lane] = particle_record<F32>(
           j, px, py, pz, pm, pe, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
     }
-    const int nsrc = (total - s0) < PT_WAVE ? (total - s0) : PT_WAVE;
+    const int nsrc = (total - s0) < cw ? (total - s0) : cw;
     const int nn = near_total - s0;
     const int nnear = nn < 0 ? 0 : (nn < nsrc ? nn : nsrc);
     pad_chunk<F32, false, W_t>(W, nsrc, g);
@@ -768,7 +811,9 @@ __device__ __forceinline__ void do_nodes(W_t& W, int32_t id, int nsrc, const Gro
   const bool act = (int)__lane_id() < nsrc;
   bool nr = false;
   SrcC<F32> r{};
-  if (act) r = node_record<F32>(id, gn, gn32, b, bf, L.emax, ox, oy, oz, ccx, ccy, ccz, nr);
+  if (act)
+    r = node_record<F32>(id & ~kFarNode, (id & kFarNode) != 0, gn, gn32, b, bf, L.emax, ox, oy,
+                         oz, ccx, ccy, ccz, nr);
   // near (monopole or possibly softened) nodes first, then far ones
   const unsigned long long mn = pt_ballot(act && nr), mf = pt_ballot(act && !nr);
   const int nnear = __popcll(mn);
@@ -1236,8 +1281,9 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
         if (r.is_leaf) {
           const int k = npp + pt_rank_in(leafm);
           W.ppbeg[k] = r.begin;
-          W.ppcnt[k] = leaf_entry(r.count, r.cx, r.cy, r.cz, r.leaf_reach, r.leaf_soft, bf,
-                                  L.emax, ox, oy, oz, ccx, ccy, ccz);
+          W.ppcnt[k] = leaf_far_dsq(r.dsq, r.leaf_reach, r.leaf_soft, bf, L.emax)
+                           ? (r.count | kFarLeaf)
+                           : r.count;
         }
         npc += __popcll(closem);
         npp += __popcll(leafm);
@@ -1356,21 +1402,28 @@ __device__ __forceinline__ void gravity_bucket(
       w_open += wc.n_open;
       w_close += wc.n_close;
       lane_leaf_parts += r.is_leaf ? r.count : 0;
-      if (r.is_closed) W.pcbuf[npc + pt_rank_in(closem)] = r.node;
+      if (r.is_closed)
+        W.pcbuf[npc + pt_rank_in(closem)] =
+            node_far_dsq(r.dsq, r.node_soft, bf, L.emax) ? (r.node | kFarNode) : r.node;
       if (r.is_leaf) {
         const int k = npp + pt_rank_in(leafm);
         W.ppbeg[k] = r.begin;
-        W.ppcnt[k] = leaf_entry(r.count, r.cx, r.cy, r.cz, r.leaf_reach, r.leaf_soft, bf, L.emax,
-                                ox, oy, oz, ccx, ccy, ccz);
+        W.ppcnt[k] = leaf_far_dsq(r.dsq, r.leaf_reach, r.leaf_soft, bf, L.emax)
+                         ? (r.count | kFarLeaf)
+                         : r.count;
       }
       npc += __popcll(closem);
       npp += __popcll(leafm);
       __builtin_amdgcn_wave_barrier();
-      if (npc >= PT_WAVE) {
-        do_nodes<F32, W_t>(W, W.pcbuf[lane], PT_WAVE, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy,
-                           ccz);
-        const int rest = npc - PT_WAVE;
-        const int32_t v = lane < rest ? W.pcbuf[PT_WAVE + lane] : 0;
+      // full chunks of cw until fewer than cw are left: a step then starts
+      // with npc < cw and adds at most 64, so npc < cw + 64 <= 128 (the
+      // buffer) and each rest (< 64) fits one move
+      const int cw = F32 ? PT_WAVE : g.cw;
+      while (npc >= cw) {
+        do_nodes<F32, W_t>(W, lane < cw ? W.pcbuf[lane] : 0, cw, g, L, gn, gn32, b, bf, ox, oy,
+                           oz, ccx, ccy, ccz);
+        const int rest = npc - cw;
+        const int32_t v = lane < rest ? W.pcbuf[cw + lane] : 0;
         __builtin_amdgcn_wave_barrier();
         if (lane < rest) W.pcbuf[lane] = v;
         npc = rest;

@@ -189,6 +189,7 @@ __device__ __forceinline__ void pack_node(const PtTree& t, int64_t i, const CDat
   a.begin = t.begin[i];
   a.leaf_reach = 0.f;
   a.leaf_soft = 0.f;
+  a.node_soft = a.type == PT_NODE_INTERNAL ? round_up_f32(d.soft) : 0.f;
   if (a.type == PT_NODE_LEAF) {
     double r2 = 0.0;
     for (int k = 0; k < 3; k++) {
@@ -200,7 +201,7 @@ __device__ __forceinline__ void pack_node(const PtTree& t, int64_t i, const CDat
     a.leaf_reach = round_up_f32(sqrt(r2));
     a.leaf_soft = round_up_f32(leaf_soft);
   }
-  a.pad[0] = a.pad[1] = 0;
+  a.pad = 0;
   o.tnode[i] = a;
   GravNode g;
   g.cx = d.cm[0];

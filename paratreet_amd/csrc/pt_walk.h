@@ -20,9 +20,10 @@
 // Walk stack entries per wavefront in the list-producing traversal.
 #define PT_WALK_STACK 1024
 
-// Space::intersect(box, point, rsq): per-axis clamp distance, dsq <= rsq.
-__device__ __forceinline__ bool pt_intersect_box_point(const Bucket& b, double px, double py,
-                                                       double pz, double rsq) {
+// The squared distance from the box to the point as Space::intersect
+// (box, point, rsq) sums it: per-axis clamp distances, x then y then z.
+__device__ __forceinline__ double pt_box_point_dsq(const Bucket& b, double px, double py,
+                                                   double pz) {
   double dsq = 0.0;
   if (px < b.lox) {
     const double t = b.lox - px;
@@ -45,7 +46,21 @@ __device__ __forceinline__ bool pt_intersect_box_point(const Bucket& b, double p
     const double t = pz - b.hiz;
     dsq += t * t;
   }
-  return dsq <= rsq;
+  return dsq;
+}
+
+// Space::intersect(box, point, rsq): dsq <= rsq.
+__device__ __forceinline__ bool pt_intersect_box_point(const Bucket& b, double px, double py,
+                                                       double pz, double rsq) {
+  return pt_box_point_dsq(b, px, py, pz) <= rsq;
+}
+
+// GravityVisitor::open (examples/GravityVisitor.h:162-164,180) from the
+// squared box distance dsq of the node's centre (+ offset).
+__device__ __forceinline__ bool pt_gv_open_dsq(const TreeNode& s, double gf, double dsq) {
+  if (s.count <= PT_MIN_PARTICLE_NODE) return true;
+  const double dataRsq = s.radius * s.radius * gf;
+  return dsq <= dataRsq;
 }
 
 // GravityVisitor::open (examples/GravityVisitor.h:162-164,180).
@@ -132,6 +147,8 @@ struct WalkLane {
   bool is_closed;    // -> node() (P-C)
   double cx, cy, cz;        // its centre of mass
   float leaf_reach, leaf_soft;  // leaf bounds (TreeNode)
+  float node_soft;              // internal nodes' softening bound (TreeNode)
+  double dsq;                   // squared distance of cx + offset from the bucket box
 };
 
 // Wave-uniform per-step counts for the interaction counters.
@@ -151,7 +168,7 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
                                                  WalkCounts& wc,
                                                  const HexOpen* __restrict__ ho = nullptr,
                                                  double gm = 0.0) {
-  WalkLane r{0, 0, 0, false, false, 0.0, 0.0, 0.0, 0.f, 0.f};
+  WalkLane r{0, 0, 0, false, false, 0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.0};
   wc = WalkCounts{0, 0, 0};
   const TreeNode root = nodes[0];
   r.count = root.count;
@@ -161,6 +178,8 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
   r.cz = root.cz;
   r.leaf_reach = root.leaf_reach;
   r.leaf_soft = root.leaf_soft;
+  r.node_soft = root.node_soft;
+  r.dsq = pt_box_point_dsq(b, root.cx + ox, root.cy + oy, root.cz + oz);
   if (root.type == PT_NODE_LEAF) {
     r.is_leaf = true;
     wc.leaf_parts = root.count;
@@ -197,27 +216,33 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   sp -= k;
   const int slot = lane >> 3;
   const bool active = slot < k;
-  WalkLane r{-1, 0, 0, false, false, 0.0, 0.0, 0.0, 0.f, 0.f};
+  // lanes beyond the k popped entries read the first one's children again
+  // (k >= 1 while the walk runs) and are masked out of every outcome: the
+  // loads need no branch and the lane state no defaults
+  WalkLane r;
+  r.node = stack[sp + (active ? slot : 0)] + (lane & 7);
+  const TreeNode nd = nodes[r.node];
+  r.count = nd.count;
+  r.begin = nd.begin;
+  r.cx = nd.cx;
+  r.cy = nd.cy;
+  r.cz = nd.cz;
+  r.leaf_reach = nd.leaf_reach;
+  r.leaf_soft = nd.leaf_soft;
+  r.node_soft = nd.node_soft;
+  const int32_t fc = nd.first_child;
+  r.is_leaf = active && nd.type == PT_NODE_LEAF;
+  const bool internal = active && nd.type == PT_NODE_INTERNAL;
+  // every lane's box distance: the opening test's for internal nodes, the
+  // near / far classification's for leaves and closed nodes
+  r.dsq = pt_box_point_dsq(b, nd.cx + ox, nd.cy + oy, nd.cz + oz);
   bool op = false;
-  int32_t fc = -1;
-  if (active) {
-    r.node = stack[sp + slot] + (lane & 7);
-    const TreeNode nd = nodes[r.node];
-    r.count = nd.count;
-    r.begin = nd.begin;
-    r.cx = nd.cx;
-    r.cy = nd.cy;
-    r.cz = nd.cz;
-    r.leaf_reach = nd.leaf_reach;
-    r.leaf_soft = nd.leaf_soft;
-    fc = nd.first_child;
-    if (nd.type == PT_NODE_LEAF) {
-      r.is_leaf = true;
-    } else if (nd.type == PT_NODE_INTERNAL) {
-      op = pt_open_any<HEXA>(nd, r.node, ho, b, gf, gm, ox, oy, oz);
-      r.is_closed = !op;
-    }
+  if constexpr (HEXA) {
+    if (internal) op = pt_open_any<HEXA>(nd, r.node, ho, b, gf, gm, ox, oy, oz);
+  } else {
+    op = internal && pt_gv_open_dsq(nd, gf, r.dsq);
   }
+  r.is_closed = internal && !op;
   leafm = __ballot(r.is_leaf);
   closem = __ballot(r.is_closed);
   const unsigned long long openm = __ballot(op);
