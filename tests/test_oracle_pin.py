@@ -81,6 +81,52 @@ def test_splineq_far_field_closed_form():
         assert np.allclose([a, b, c, d], [1 / r, r ** -3, 3 * r ** -5, 15 * r ** -7], rtol=1e-14)
 
 
+def _spline_softened_closed_form(r):
+    """The cubic-spline softened potential and its derivative chain with
+    softening length h = 1 (Monaghan & Lattanzio 1985; Hernquist & Katz 1989,
+    eq. A2), u = r: a = -phi, b = -a'/r, c = -b'/r, d = -c'/r."""
+    u = r
+    if u < 1.0:
+        a = 7 / 5 - 2 / 3 * u ** 2 + 3 / 10 * u ** 4 - 1 / 10 * u ** 5
+        b = 4 / 3 - 6 / 5 * u ** 2 + 1 / 2 * u ** 3
+        c = 12 / 5 - 3 / 2 * u
+        d = 3 / 2 / u
+    else:
+        a = -1 / (15 * u) + 8 / 5 - 4 / 3 * u ** 2 + u ** 3 - 3 / 10 * u ** 4 + 1 / 30 * u ** 5
+        b = -1 / (15 * u ** 3) + 8 / 3 - 3 * u + 6 / 5 * u ** 2 - 1 / 6 * u ** 3
+        c = -1 / (5 * u ** 5) + 3 / u - 12 / 5 + 1 / 2 * u
+        d = -1 / u ** 7 + 3 / u ** 3 - 1 / (2 * u)
+    return a, b, c, d
+
+
+def test_splineq_softened_branches_at_unit_softening_length():
+    """SPLINEQ's softened branches (examples/GravityVisitor.h:40-73), restated
+    with the reference's grouping, in which the terms in powers of u do not
+    carry the powers of dih = 2 / twoh.  At twoh = 2 (dih = 1)
+    that grouping coincides with the spline kernel, so there every
+    coefficient of both branches is checked against the closed forms, the
+    derivative chain b = -a'/r, c = -b'/r, d = -c'/r (central differences),
+    and continuity at u = 1 and at r = twoh with the unsoftened forms."""
+    twoh = 2.0
+    for r in (0.05, 0.3, 0.7, 0.95, 1.05, 1.3, 1.7, 1.95):
+        got = O.splineq(1.0 / r, r * r, twoh)
+        want = _spline_softened_closed_form(r)
+        assert np.allclose(got, want, rtol=1e-13, atol=0), (r, got, want)
+        h = 1e-5 * r
+        lo = O.splineq(1.0 / (r - h), (r - h) ** 2, twoh)
+        hi = O.splineq(1.0 / (r + h), (r + h) ** 2, twoh)
+        for k in range(3):
+            deriv = (hi[k] - lo[k]) / (2 * h)
+            assert abs(-deriv / r - got[k + 1]) <= 1e-7 * abs(got[k + 1]), (r, k)
+    for k in range(4):
+        below = O.splineq(1.0 / (1 - 1e-9), (1 - 1e-9) ** 2, twoh)[k]
+        above = O.splineq(1.0 / (1 + 1e-9), (1 + 1e-9) ** 2, twoh)[k]
+        assert abs(below - above) <= 1e-7 * abs(above)
+        inside = O.splineq(1.0 / (2 - 1e-9), (2 - 1e-9) ** 2, twoh)[k]
+        outside = O.splineq(1.0 / (2 + 1e-9), (2 + 1e-9) ** 2, twoh)[k]
+        assert abs(inside - outside) <= 1e-7 * abs(outside)
+
+
 # ---------------------------------------------------------------- keys
 
 def _keys_numpy(pos, box):
