@@ -182,10 +182,10 @@ def cpu_baseline(pos, m, eps, bucket, theta, sample_frac=0.2):
     serial = t["keys"] + t["sort"] + t["build"] + t["moments"]
     step = serial + t["force"]
     return {"value": ints / step, "unit": "interactions/s", "cores": nthreads, "kind": "port",
-            "sample": "whole step on the same 1M input (keys, sort, build and moments "
+            "sample": "whole step on the same %d-particle input (keys, sort, build and moments "
                       "single-threaded, %.3g s; force over all %d buckets with %d OpenMP "
                       "threads, %.3g s); one-thread force on %d seeded buckets (%.1f s)" % (
-                          serial, nb, nthreads, t["force"], len(sel), t1),
+                          len(x), serial, nb, nthreads, t["force"], len(sel), t1),
             "cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
             "phase_ms": {k: v * 1e3 for k, v in t.items()},
             "force_ips_all_threads": ints / t["force"],
