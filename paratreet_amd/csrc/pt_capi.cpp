@@ -59,6 +59,7 @@ struct pt_ctx {
   // universe
   DBuf<double> box, bbox, box_parts;
   double h_box[6] = {0, 0, 0, 0, 0, 0};
+  double soft_min = INFINITY;  // smallest particle softening (read back by pt_set_particles)
   // tree
   int64_t nn = 0, nb = 0;
   int32_t max_depth = 0;
@@ -434,6 +435,12 @@ pt_status pt_set_particles(pt_ctx* c, int64_t n, const double* x, const double* 
     PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->ay.p, n, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->az.p, n, 0ull, c->stream));
     PT_TRY_LAUNCH(c, pt_launch_fill_u64((unsigned long long*)c->pot.p, n, 0ull, c->stream));
+    double* smin = c->box_parts.p + 6 * kMaxBoxParts - 1;
+    PT_TRY_LAUNCH(c, pt_launch_min(c->e.p, n, c->box_parts.p, kMaxBoxParts, smin, c->stream));
+    PT_TRY_HIP(c, hipMemcpyAsync(&c->soft_min, smin, sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+  } else {
+    c->soft_min = INFINITY;
   }
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   c->stage = S_PARTICLES;
@@ -760,9 +767,32 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
   return PT_OK;
 }
 
+// The compensated-FP32 pair terms grow as powers of 1/softening (the
+// softened SPLINEQ terms to (2/twoh)^7 in the forward, their r-derivatives to
+// (2/twoh)^8 in the adjoint) and reach the FP32 range (3.4e38) when a
+// softening is below about 2e-5 length units: the FP32C forms refuse such
+// inputs instead of returning infinities.  FP64 has no such limit.
+constexpr double kFp32cMinSoft = 2.5e-5;
+
+static pt_status check_fp32c_range(pt_ctx* c) {
+  if (!(c->soft_min >= kFp32cMinSoft)) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "PT_FP32C needs every softening >= %g (the smallest is %g): its FP32 pair terms "
+                  "grow as softening^-8; use PT_FP64, or lengths in larger units",
+                  kFp32cMinSoft, c->soft_min);
+    return fail(c, PT_ERR_UNSUPPORTED, msg);
+  }
+  return PT_OK;
+}
+
 // node() / leaf() over the context's lists into the sorted result arrays
 // (written, not added).
 static pt_status launch_list_interact(pt_ctx* c, pt_precision precision) {
+  if (precision != PT_FP64) {
+    pt_status st = check_fp32c_range(c);
+    if (st) return st;
+  }
   Timer t(c, PT_T_FORCE);
   if (c->cfg.hexadecapole) {
     PT_TRY_LAUNCH(c, pt_launch_interact_hexa(c->bk.p, c->nb, c->tnode.p, c->hn.p, c->xs.p, c->ys.p,
@@ -923,6 +953,7 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (precision != PT_FP64 && precision != PT_FP32C) return fail(c, PT_ERR_ARG, "bad precision");
   double gf;
   if ((st = set_visitor(c, theta, offset, &gf))) return st;
+  if (precision != PT_FP64 && (st = check_fp32c_range(c))) return st;
   const int64_t nb = c->nb;
   const bool hexa = c->cfg.hexadecapole != 0;
   const int64_t nblk = std::max<int64_t>(
@@ -1272,6 +1303,7 @@ pt_status vjp_accumulate(pt_ctx* c, int mode, double theta, const AdjViews& v, c
                                               wz, gx, gy, gz, gm, c->stream));
     return PT_OK;
   }
+  if (c->adj_prec != PT_FP64 && (st = check_fp32c_range(c))) return st;
   if ((st = pt_remake_universe(c, nullptr))) return st;
   if ((st = pt_assign_keys(c))) return st;
   if ((st = sort_and_build(c))) return st;

@@ -190,6 +190,10 @@ int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const 
                      hipStream_t s);
 int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int pt_launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+// *out = min(v[0..n)) (+inf for n = 0); parts: max_parts doubles of scratch,
+// out outside them.
+int pt_launch_min(const double* v, int64_t n, double* parts, int max_parts, double* out,
+                  hipStream_t s);
 
 // Bucket groups of the adjoint: runs of consecutive buckets (their particles
 // are consecutive) of at most PT_GROUP_BUCKETS buckets and, unless a single

@@ -130,6 +130,38 @@ fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Smallest value of v[0..n): per-block minima, then one block over them.
+__global__ void __launch_bounds__(256)
+min_partial_kernel(const double* __restrict__ v, int64_t n, double* __restrict__ part) {
+  __shared__ double s[256];
+  double lo = INFINITY;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    lo = v[i] < lo ? v[i] : lo;
+  s[threadIdx.x] = lo;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s[threadIdx.x + w] < s[threadIdx.x])
+      s[threadIdx.x] = s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(256)
+min_final_kernel(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+  __shared__ double s[256];
+  double lo = INFINITY;
+  for (int b = threadIdx.x; b < nparts; b += 256) lo = part[b] < lo ? part[b] : lo;
+  s[threadIdx.x] = lo;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s[threadIdx.x + w] < s[threadIdx.x])
+      s[threadIdx.x] = s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
 __global__ void __launch_bounds__(256)
 fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
@@ -505,6 +537,16 @@ int pt_launch_direct(const int64_t* targets, int64_t nt, const double* x, const 
 int pt_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
   if (n <= 0) return 0;
   fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(p, n, v);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_min(const double* v, int64_t n, double* parts, int max_parts, double* out,
+                  hipStream_t s) {
+  int nb = (int)((n + 255) / 256);
+  if (nb > max_parts) nb = max_parts;
+  if (nb < 1) nb = 1;
+  min_partial_kernel<<<nb, 256, 0, s>>>(v, n, parts);
+  min_final_kernel<<<1, 256, 0, s>>>(parts, nb, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -103,3 +103,41 @@ def test_fused_fp64_and_fp32c(case):
     a32, _ = d.accelerations()
     assert tuple(int(v) for v in cnt32) == tuple(int(v) for v in cnt), name
     check_fp32c(a32, acc, name + " fused")
+
+
+def test_fp32c_refuses_softenings_below_its_range():
+    """The compensated-FP32 pair terms grow as powers of 1/softening (to the
+    eighth in the adjoint) and leave the FP32 range below ~2e-5 length units:
+    the FP32C forms return PT_ERR_UNSUPPORTED there (forward and adjoint)
+    while FP64 evaluates the same input; at the threshold FP32C runs."""
+    from paratreet_amd import _lib
+    from paratreet_amd.gravity import Configuration, Driver
+    rng = np.random.default_rng(11)
+    n = 3000
+    pos = rng.normal(size=(n, 3)) * 1e-3
+    m = np.full(n, 1.0 / n)
+    e = np.full(n, 2.5e-5)
+    e[17] = 1e-6
+    d = Driver(Configuration(max_particles_per_leaf=12))
+    d.load(pos, m, e)
+    with pytest.raises(_lib.PtError) as err:
+        d.compute_forces(theta=0.7, precision=1)
+    assert err.value.status == _lib.PT_ERR_UNSUPPORTED and "softening" in str(err.value)
+    d.compute_forces(theta=0.7, precision=0)
+    a64, _ = d.accelerations()
+    assert np.all(np.isfinite(a64))
+    d.set_adjoint_precision(1)
+    with pytest.raises(_lib.PtError) as err:
+        d.force_vjp(np.ones((n, 3)), theta=0.7)
+    assert err.value.status == _lib.PT_ERR_UNSUPPORTED
+    d.set_adjoint_precision(0)
+    d.force_vjp(np.ones((n, 3)), theta=0.7)
+    e[17] = 2.5e-5
+    d.load(pos, m, e)
+    d.compute_forces(theta=0.7, precision=1)
+    a32, _ = d.accelerations()
+    d.compute_forces(theta=0.7, precision=0)
+    a64, _ = d.accelerations()
+    assert np.all(np.isfinite(a32))
+    check_fp32c(a32, a64, "softening 2.5e-5")
+    d.close()
