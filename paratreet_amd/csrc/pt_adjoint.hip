@@ -133,6 +133,163 @@ struct TgtA {
   int32_t bt, pad;  // bt: the target's bucket within its group
 };
 
+// Squared distance from the box [lo, hi] to the point p.
+__device__ __forceinline__ double box_dsq(const double lo[3], const double hi[3], double px,
+                                          double py, double pz) {
+  const double p[3] = {px, py, pz};
+  double d = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double t = p[k] < lo[k] ? lo[k] - p[k] : (p[k] > hi[k] ? p[k] - hi[k] : 0.0);
+    d += t * t;
+  }
+  return d;
+}
+
+// The far classification of a pass: a closed node whose centre lies farther
+// from the box around the group's buckets than twice its softening plus the
+// largest bucket softening fails both openSoftening tests for every member
+// (each member's box and centre lie inside that box), and one farther than
+// its softening plus the largest target softening is beyond the softened
+// range of every pair; a source particle likewise beyond its softening plus
+// the largest target softening, and a leaf when its centre is farther than
+// its reach plus those softenings.  Margins of 2e-4 on the reaches and 1e-4
+// on the squares also cover the FP32 form's shifted coordinates.
+// Per wave, in LDS: lo[3], hi[3] of the box around the group's buckets, the
+// largest bucket softening, the largest target softening of the pass.
+constexpr int kPassFar = 8;
+
+__device__ __forceinline__ bool node_far_pass(const double* __restrict__ f, double cx, double cy,
+                                              double cz, double soft) {
+  const double dsq = box_dsq(f, f + 3, cx, cy, cz);
+  const double mono = 2.0 * (soft + f[6]);
+  const double reach = (soft + f[7]) * 1.0002;
+  const double m = mono > reach ? mono : reach;
+  return dsq > 1.0001 * m * m;
+}
+
+__device__ __forceinline__ bool leaf_far_pass(const double* __restrict__ f, double cx, double cy,
+                                              double cz, double lreach, double lsoft) {
+  const double dsq = box_dsq(f, f + 3, cx, cy, cz);
+  const double reach = (lreach + lsoft + f[7]) * 1.0002;
+  return dsq > 1.0001 * reach * reach;
+}
+
+// One node() list entry against the pass's targets (FP64): the target
+// gradients into tg, the node cotangent into lam; mono: the members for which
+// openSoftening sends the node to addGravity.
+__device__ __forceinline__ void node_entry_pairs(const GravNode& s, unsigned mask, unsigned mono,
+                                                 const TgtA* __restrict__ st, int ng, double ox,
+                                                 double oy, double oz, double (&tg)[kANT][3],
+                                                 double (&lam)[kNodeAdj]) {
+  const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
+#pragma unroll
+  for (int t = 0; t < kANT; t++) {
+    if (t < ng && ((mask >> st[t].bt) & 1u)) {
+      const TgtA tt = st[t];
+      if ((mono >> tt.bt) & 1u) {
+        const double dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 != 0) {
+          const double ir = pt_rsqrt64(r2);
+          const double ir3 = ir * ir * ir;
+          const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+          const double f = -3.0 * s.mass * ir3 * ir * ir * dw;
+          const double Tx = s.mass * ir3 * tt.wx + f * dx;
+          const double Ty = s.mass * ir3 * tt.wy + f * dy;
+          const double Tz = s.mass * ir3 * tt.wz + f * dz;
+          tg[t][0] -= Tx;
+          tg[t][1] -= Ty;
+          tg[t][2] -= Tz;
+          lam[0] += Tx;
+          lam[1] += Ty;
+          lam[2] += Tz;
+          lam[3] += dw * ir3;
+        }
+      } else {
+        const double rx = (tt.x - s.cx) - ox, ry = (tt.y - s.cy) - oy, rz = (tt.z - s.cz) - oz;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double ir = pt_rsqrt64(r2);
+        const double r = r2 * ir;
+        const SplineQ q = splineq_grad(r, ir, r2, s.soft + tt.e);
+        const double qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
+        const double qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
+        const double qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
+        const double qir = 0.5 * (qx * rx + qy * ry + qz * rz);
+        const double qir3 = q.b * s.mass + q.d * qir - q.c * s.tr;
+        const double rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
+        const double qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
+        const double Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
+        const double Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
+        const double Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
+        const double k1 = (s.mass * q.br + qir * q.dr_ - s.tr * q.cr) * ir;
+        const double k2 = q.cr * ir;
+        const double Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
+                          (k2 * qw) * rx;
+        const double Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
+                          (k2 * qw) * ry;
+        const double Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
+                          (k2 * qw) * rz;
+        tg[t][0] += Jx;
+        tg[t][1] += Jy;
+        tg[t][2] += Jz;
+        lam[0] -= Jx;
+        lam[1] -= Jy;
+        lam[2] -= Jz;
+        lam[3] -= q.b * rw;
+        const double hd = -0.5 * q.d * rw, hc = 0.5 * q.c * rw;
+        lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
+        lam[5] += 2.0 * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
+        lam[6] += 2.0 * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
+        lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
+        lam[8] += 2.0 * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
+        lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
+        lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * s.tr) * rw + q.ct * qw;
+      }
+    }
+  }
+}
+
+// One source particle against the pass's targets (FP64).  FAR: every pair is
+// beyond its softening (and so not the target itself).
+template <bool FAR>
+__device__ __forceinline__ void particle_pairs(double jx, double jy, double jz, double jm,
+                                               double je, unsigned jmask,
+                                               const TgtA* __restrict__ st, int ng,
+                                               double (&tg)[kANT][3], double& sgx, double& sgy,
+                                               double& sgz, double& sgm) {
+#pragma unroll
+  for (int t = 0; t < kANT; t++) {
+    if (t < ng && ((jmask >> st[t].bt) & 1u)) {
+      const TgtA tt = st[t];
+      const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (FAR || r2 != 0) {
+        const double ir = pt_rsqrt64(r2);
+        double bb, db;
+        if (FAR) {
+          bb = ir * ir * ir;
+          db = -3.0 * bb * ir;
+        } else {
+          spline_b_db(r2 * ir, ir, je + tt.e, bb, db);
+        }
+        const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+        const double f = db * ir * dw;
+        const double Kx = jm * (bb * tt.wx + f * dx);
+        const double Ky = jm * (bb * tt.wy + f * dy);
+        const double Kz = jm * (bb * tt.wz + f * dz);
+        tg[t][0] -= Kx;
+        tg[t][1] -= Ky;
+        tg[t][2] -= Kz;
+        sgx += Kx;
+        sgy += Ky;
+        sgz += Kz;
+        sgm += bb * dw;
+      }
+    }
+  }
+}
+
 // One wave per bucket group (consecutive buckets, at most PT_GROUP_TARGETS
 // particles unless one bucket is larger; neighbouring buckets share most of
 // their sources, so a group's union list is far shorter than the sum of its
@@ -163,6 +320,7 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
+  __shared__ double s_far[kAdjWaves][kPassFar];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t gi = (int64_t)blockIdx.x * kAdjWaves + wave;
@@ -177,6 +335,16 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
   const int nt = (int)((int64_t)s_bk[wave][nbk - 1].begin + s_bk[wave][nbk - 1].count - first);
   const int64_t pc0 = pc_off[gi], pc1 = pc_off[gi + 1];
   const int64_t pp0 = pp_off[gi], pp1 = pp_off[gi + 1];
+  if (lane < 7) {
+    // the group's box and largest bucket softening
+    double v = lane < 3 ? INFINITY : (lane < 6 ? -INFINITY : 0.0);
+    for (int k = 0; k < nbk; k++) {
+      const Bucket& bb = s_bk[wave][k];
+      const double c[7] = {bb.lox, bb.loy, bb.loz, bb.hix, bb.hiy, bb.hiz, bb.soft};
+      v = lane < 3 ? fmin(v, c[lane]) : fmax(v, c[lane]);
+    }
+    s_far[wave][lane] = v;
+  }
   for (int g0 = 0; g0 < nt; g0 += kANT) {
     const int ng = (nt - g0) < kANT ? (nt - g0) : kANT;
     // member buckets with a target in this pass
@@ -197,89 +365,39 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
     double tg[kANT][3];
 #pragma unroll
     for (int t = 0; t < kANT; t++) tg[t][0] = tg[t][1] = tg[t][2] = 0.0;
+    if (lane == 0) {
+      double emax = 0.0;
+      for (int t = 0; t < ng; t++) emax = s_t[wave][t].e > emax ? s_t[wave][t].e : emax;
+      s_far[wave][7] = emax;
+    }
+    __builtin_amdgcn_wave_barrier();
 
-    // ---- node() sources
+    // ---- node() sources: in chunks in which every entry is far no member's
+    // openSoftening can hold, so its tests are skipped
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
       const int64_t e = e0 + lane;
       const unsigned mask = e < pc1 ? (unsigned)pcm[e] & pass : 0u;
+      int32_t sid = 0;
       if (mask != 0u) {
-        const int32_t sid = pc[e];
+        sid = pc[e];
         const GravNode s = gn[sid];
-        const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
-        unsigned mono = 0;  // members for which openSoftening sends it to addGravity
-        for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
-          const int k = __builtin_ctz(mm);
-          if (pt_open_softening(cx, cy, cz, s.soft, s_bk[wave][k])) mono |= 1u << k;
-        }
+        const bool far = node_far_pass(s_far[wave], s.cx + ox, s.cy + oy, s.cz + oz, s.soft);
+        // over the lanes holding an entry (the others are inactive here)
+        const bool all_far = __ballot(!far) == 0ull;
         double lam[kNodeAdj];
 #pragma unroll
         for (int k = 0; k < kNodeAdj; k++) lam[k] = 0.0;
-#pragma unroll
-        for (int t = 0; t < kANT; t++) {
-          if (t < ng && ((mask >> s_t[wave][t].bt) & 1u)) {
-            const TgtA tt = s_t[wave][t];
-            if ((mono >> tt.bt) & 1u) {
-              const double dx = cx - tt.x, dy = cy - tt.y, dz = cz - tt.z;
-              const double r2 = dx * dx + dy * dy + dz * dz;
-              if (r2 != 0) {
-                const double ir = pt_rsqrt64(r2);
-                const double ir3 = ir * ir * ir;
-                const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
-                const double f = -3.0 * s.mass * ir3 * ir * ir * dw;
-                const double Tx = s.mass * ir3 * tt.wx + f * dx;
-                const double Ty = s.mass * ir3 * tt.wy + f * dy;
-                const double Tz = s.mass * ir3 * tt.wz + f * dz;
-                tg[t][0] -= Tx;
-                tg[t][1] -= Ty;
-                tg[t][2] -= Tz;
-                lam[0] += Tx;
-                lam[1] += Ty;
-                lam[2] += Tz;
-                lam[3] += dw * ir3;
-              }
-            } else {
-              const double rx = (tt.x - s.cx) - ox, ry = (tt.y - s.cy) - oy,
-                           rz = (tt.z - s.cz) - oz;
-              const double r2 = rx * rx + ry * ry + rz * rz;
-              const double ir = pt_rsqrt64(r2);
-              const double r = r2 * ir;
-              const SplineQ q = splineq_grad(r, ir, r2, s.soft + tt.e);
-              const double qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
-              const double qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
-              const double qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
-              const double qir = 0.5 * (qx * rx + qy * ry + qz * rz);
-              const double qir3 = q.b * s.mass + q.d * qir - q.c * s.tr;
-              const double rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
-              const double qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
-              const double Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
-              const double Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
-              const double Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
-              const double k1 = (s.mass * q.br + qir * q.dr_ - s.tr * q.cr) * ir;
-              const double k2 = q.cr * ir;
-              const double Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
-                                (k2 * qw) * rx;
-              const double Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
-                                (k2 * qw) * ry;
-              const double Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
-                                (k2 * qw) * rz;
-              tg[t][0] += Jx;
-              tg[t][1] += Jy;
-              tg[t][2] += Jz;
-              lam[0] -= Jx;
-              lam[1] -= Jy;
-              lam[2] -= Jz;
-              lam[3] -= q.b * rw;
-              const double hd = -0.5 * q.d * rw, hc = 0.5 * q.c * rw;
-              lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
-              lam[5] += 2.0 * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
-              lam[6] += 2.0 * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
-              lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
-              lam[8] += 2.0 * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
-              lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
-              lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * s.tr) * rw + q.ct * qw;
-            }
+        // members for which openSoftening sends it to addGravity (none when
+        // every entry of the chunk is far)
+        unsigned mono = 0;
+        if (!all_far) {
+          for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
+            const int k = __builtin_ctz(mm);
+            if (pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, s_bk[wave][k]))
+              mono |= 1u << k;
           }
         }
+        node_entry_pairs(s, mask, mono, s_t[wave], ng, ox, oy, oz, tg, lam);
         double* dst = sid < nhot
                           ? hot + (int64_t)(blockIdx.x % kHotParts) * kNodeAdj * nhot + sid
                           : lam_node + sid;
@@ -290,24 +408,35 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
       }
     }
 
-    // ---- leaf() sources, flattened to particles
+    // ---- leaf() sources, flattened to particles, the leaves that may hold a
+    // softened pair first: windows past them take the branch-free form
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       const int64_t l = l0 + lane;
       int32_t cnt = 0, beg = 0;
+      bool isnear = false;
       const unsigned msk = l < pp1 ? (unsigned)ppm[l] & pass : 0u;
       if (msk != 0u) {
         const TreeNode nd = nodes[pp[l]];
         cnt = nd.count;
         beg = nd.begin;
+        isnear = !leaf_far_pass(s_far[wave], nd.cx + ox, nd.cy + oy, nd.cz + oz, nd.leaf_reach,
+                                nd.leaf_soft);
       }
-      const int incl = pt_wave_incl_scan(cnt);
-      const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
+      const int cn = isnear ? cnt : 0, cf = isnear ? 0 : cnt;
+      const int incl_n = pt_wave_incl_scan(cn);
+      const int incl_f = pt_wave_incl_scan(cf);
+      const int near_total = __builtin_amdgcn_readlane(incl_n, PT_WAVE - 1);
+      const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
+      // entries in slot order (near ones first), so the starts ascend
+      const unsigned long long nearm = __ballot(isnear);
+      const int pos = isnear ? pt_rank_in(nearm) : __popcll(nearm) + pt_rank_in(~nearm);
       __builtin_amdgcn_wave_barrier();
-      s_beg[wave][lane] = beg;
-      s_exc[wave][lane] = incl - cnt;
-      s_msk[wave][lane] = (int32_t)msk;
+      s_beg[wave][pos] = beg;
+      s_exc[wave][pos] = isnear ? incl_n - cn : near_total + incl_f - cf;
+      s_msk[wave][pos] = (int32_t)msk;
       __builtin_amdgcn_wave_barrier();
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+        const bool wfar = s0 >= near_total;
         const int slot = s0 + lane;
         if (slot < total) {
           // the last entry whose exclusive start is <= slot holds the slot
@@ -324,32 +453,11 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
           const double jx = px[j] + ox, jy = py[j] + oy, jz = pz[j] + oz;
           const double jm = pm[j], je = pe[j];
           double sgx = 0.0, sgy = 0.0, sgz = 0.0, sgm = 0.0;
-#pragma unroll
-          for (int t = 0; t < kANT; t++) {
-            if (t < ng && ((jmask >> s_t[wave][t].bt) & 1u)) {
-              const TgtA tt = s_t[wave][t];
-              const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
-              const double r2 = dx * dx + dy * dy + dz * dz;
-              if (r2 != 0) {
-                const double ir = pt_rsqrt64(r2);
-                const double r = r2 * ir;
-                double bb, db;
-                spline_b_db(r, ir, je + tt.e, bb, db);
-                const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
-                const double f = db * ir * dw;
-                const double Kx = jm * (bb * tt.wx + f * dx);
-                const double Ky = jm * (bb * tt.wy + f * dy);
-                const double Kz = jm * (bb * tt.wz + f * dz);
-                tg[t][0] -= Kx;
-                tg[t][1] -= Ky;
-                tg[t][2] -= Kz;
-                sgx += Kx;
-                sgy += Ky;
-                sgz += Kz;
-                sgm += bb * dw;
-              }
-            }
-          }
+          if (wfar)
+            particle_pairs<true>(jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz, sgm);
+          else
+            particle_pairs<false>(jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz,
+                                  sgm);
           atomicAdd(&gx[j], sgx);
           atomicAdd(&gy[j], sgy);
           atomicAdd(&gz[j], sgz);
