@@ -573,6 +573,46 @@ struct TgtA32 {
   int32_t bt;
 };
 
+// One source particle against the pass's targets (FP32 pairs, target
+// gradients summed in FP64).  far (wave-uniform): every pair is beyond its
+// softening (and so not the target itself).
+__device__ __forceinline__ void particle_pairs32(bool far, float jx, float jy, float jz, float jm,
+                                                 float je, unsigned jmask,
+                                                 const TgtA32* __restrict__ st, int ng,
+                                                 double (&tg)[kANT][3], float& sgx, float& sgy,
+                                                 float& sgz, float& sgm) {
+#pragma unroll
+  for (int t = 0; t < kANT; t++) {
+    if (t < ng && ((jmask >> st[t].bt) & 1u)) {
+      const TgtA32 tt = st[t];
+      const float dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (far || r2 != 0.0f) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        float bb, db;
+        if (far) {
+          bb = ir * ir * ir;
+          db = -3.0f * bb * ir;
+        } else {
+          spline_b_db32(r2 * ir, ir, je + tt.e, bb, db);
+        }
+        const float dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+        const float f = db * ir * dw;
+        const float Kx = jm * (bb * tt.wx + f * dx);
+        const float Ky = jm * (bb * tt.wy + f * dy);
+        const float Kz = jm * (bb * tt.wz + f * dz);
+        tg[t][0] -= (double)Kx;
+        tg[t][1] -= (double)Ky;
+        tg[t][2] -= (double)Kz;
+        sgx += Kx;
+        sgy += Ky;
+        sgz += Kz;
+        sgm += bb * dw;
+      }
+    }
+  }
+}
+
 // adjoint_group_kernel in the FP32 form (same work mapping; node moments from
 // the FP32 copies, node centres and softening decisions from the FP64 fields
 // of the same records).
@@ -596,6 +636,7 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
   __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
+  __shared__ double s_far[kAdjWaves][kPassFar];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t gi = (int64_t)blockIdx.x * kAdjWaves + wave;
@@ -620,6 +661,16 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
     hz = fmax(hz, s_bk[wave][k].hiz);
   }
   const double ccx = (lx + hx) * 0.5, ccy = (ly + hy) * 0.5, ccz = (lz + hz) * 0.5;
+  if (lane < 7) {
+    // the group's box and largest bucket softening
+    double v = lane < 3 ? INFINITY : (lane < 6 ? -INFINITY : 0.0);
+    for (int k = 0; k < nbk; k++) {
+      const Bucket& bb = s_bk[wave][k];
+      const double c[7] = {bb.lox, bb.loy, bb.loz, bb.hix, bb.hiy, bb.hiz, bb.soft};
+      v = lane < 3 ? fmin(v, c[lane]) : fmax(v, c[lane]);
+    }
+    s_far[wave][lane] = v;
+  }
   const int64_t pc0 = pc_off[gi], pc1 = pc_off[gi + 1];
   const int64_t pp0 = pp_off[gi], pp1 = pp_off[gi + 1];
   for (int g0 = 0; g0 < nt; g0 += kANT) {
@@ -643,6 +694,12 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
     double tg[kANT][3];
 #pragma unroll
     for (int t = 0; t < kANT; t++) tg[t][0] = tg[t][1] = tg[t][2] = 0.0;
+    if (lane == 0) {
+      double emax = 0.0;
+      for (int t = 0; t < ng; t++) emax = fmax(emax, (double)s_t[wave][t].e);
+      s_far[wave][7] = emax;
+    }
+    __builtin_amdgcn_wave_barrier();
 
     // ---- node() sources
     for (int64_t e0 = pc0; e0 < pc1; e0 += PT_WAVE) {
@@ -652,10 +709,16 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
         const int32_t sid = pc[e];
         const GravNode32 s = gn32[sid];
         const double cxd = s.cx + ox, cyd = s.cy + oy, czd = s.cz + oz;
-        unsigned mono = 0;  // members for which openSoftening sends it to addGravity
-        for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
-          const int k = __builtin_ctz(mm);
-          if (pt_open_softening(cxd, cyd, czd, s.soft64, s_bk[wave][k])) mono |= 1u << k;
+        // over the lanes holding an entry (the others are inactive here)
+        const bool all_far = __ballot(!node_far_pass(s_far[wave], cxd, cyd, czd, s.soft64)) == 0ull;
+        // members for which openSoftening sends it to addGravity (none when
+        // every entry of the chunk is far)
+        unsigned mono = 0;
+        if (!all_far) {
+          for (unsigned mm = mask; mm != 0u; mm &= mm - 1u) {
+            const int k = __builtin_ctz(mm);
+            if (pt_open_softening(cxd, cyd, czd, s.soft64, s_bk[wave][k])) mono |= 1u << k;
+          }
         }
         const float cx = (float)(cxd - ccx), cy = (float)(cyd - ccy), cz = (float)(czd - ccz);
         const float tr = 0.5f * (s.qxx + s.qyy + s.qzz);
@@ -737,26 +800,38 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
       }
     }
 
-    // ---- leaf() sources, flattened to particles
+    // ---- leaf() sources, flattened to particles, the leaves that may hold a
+    // softened pair first: windows past them take the branch-free form
     for (int64_t l0 = pp0; l0 < pp1; l0 += PT_WAVE) {
       const int64_t l = l0 + lane;
       int32_t cnt = 0, beg = 0;
+      bool isnear = false;
       const unsigned msk = l < pp1 ? (unsigned)ppm[l] & pass : 0u;
       if (msk != 0u) {
         const TreeNode nd = nodes[pp[l]];
         cnt = nd.count;
         beg = nd.begin;
+        isnear = !leaf_far_pass(s_far[wave], nd.cx + ox, nd.cy + oy, nd.cz + oz, nd.leaf_reach,
+                                nd.leaf_soft);
       }
-      const int incl = pt_wave_incl_scan(cnt);
-      const int total = __builtin_amdgcn_readlane(incl, PT_WAVE - 1);
+      const int cn = isnear ? cnt : 0, cf = isnear ? 0 : cnt;
+      const int incl_n = pt_wave_incl_scan(cn);
+      const int incl_f = pt_wave_incl_scan(cf);
+      const int near_total = __builtin_amdgcn_readlane(incl_n, PT_WAVE - 1);
+      const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
+      // entries in slot order (near ones first), so the starts ascend
+      const unsigned long long nearm = __ballot(isnear);
+      const int pos = isnear ? pt_rank_in(nearm) : __popcll(nearm) + pt_rank_in(~nearm);
       __builtin_amdgcn_wave_barrier();
-      s_beg[wave][lane] = beg;
-      s_exc[wave][lane] = incl - cnt;
-      s_msk[wave][lane] = (int32_t)msk;
+      s_beg[wave][pos] = beg;
+      s_exc[wave][pos] = isnear ? incl_n - cn : near_total + incl_f - cf;
+      s_msk[wave][pos] = (int32_t)msk;
       __builtin_amdgcn_wave_barrier();
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
+        const bool wfar = s0 >= near_total;
         const int slot = s0 + lane;
         if (slot < total) {
+          // the last entry whose exclusive start is <= slot holds the slot
           int lo = 0, hi = PT_WAVE - 1;
           while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -769,32 +844,7 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
                       jz = (float)((pz[j] + oz) - ccz);
           const float jm = (float)pm[j], je = (float)pe[j];
           float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sgm = 0.0f;
-#pragma unroll
-          for (int t = 0; t < kANT; t++) {
-            if (t < ng && ((jmask >> s_t[wave][t].bt) & 1u)) {
-              const TgtA32 tt = s_t[wave][t];
-              const float dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
-              const float r2 = dx * dx + dy * dy + dz * dz;
-              if (r2 != 0.0f) {
-                const float ir = __builtin_amdgcn_rsqf(r2);
-                const float r = r2 * ir;
-                float bb, db;
-                spline_b_db32(r, ir, je + tt.e, bb, db);
-                const float dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
-                const float f = db * ir * dw;
-                const float Kx = jm * (bb * tt.wx + f * dx);
-                const float Ky = jm * (bb * tt.wy + f * dy);
-                const float Kz = jm * (bb * tt.wz + f * dz);
-                tg[t][0] -= (double)Kx;
-                tg[t][1] -= (double)Ky;
-                tg[t][2] -= (double)Kz;
-                sgx += Kx;
-                sgy += Ky;
-                sgz += Kz;
-                sgm += bb * dw;
-              }
-            }
-          }
+          particle_pairs32(wfar, jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz, sgm);
           atomicAdd(&gx[j], (double)sgx);
           atomicAdd(&gy[j], (double)sgy);
           atomicAdd(&gz[j], (double)sgz);
