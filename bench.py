@@ -36,12 +36,12 @@ CONFIGS = {
     "plummer100k": (100_000, 12, 0.7, 0.01, "plummer"),
     "halo16m": (16_000_000, 32, 0.7, 0.01, "halo"),
 }
-# HBM bytes per dispatch of gravity_kernel<FP32C> on the 1M workload: FETCH_SIZE
+# HBM bytes per dispatch of grouped_gravity_kernel<FP32C> on the 1M workload: FETCH_SIZE
 # + WRITE_SIZE from the two rocprofv3 counter passes recorded in this profile
 # (KiB per dispatch; the counters are read with factor 1.0, calibrated against
 # a device copy of known size).
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plummer1m_kernels.md")
-TRAFFIC_KERNEL = "gravity_kernel<true, 192, false>"
+TRAFFIC_KERNEL = "grouped_gravity_kernel<true, 192, false>"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
@@ -454,7 +454,7 @@ def main():
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic,
                      "algorithmic_bytes": algo_bytes,
-                     "kernel": "gravity_kernel<FP32C>",
+                     "kernel": "grouped_gravity_kernel<FP32C>",
                      "fp64_kernel": {"achieved": achieved64, "peak": PEAK_FP64_TFLOPS,
                                      "frac": achieved64 / PEAK_FP64_TFLOPS}},
         "errors": err,

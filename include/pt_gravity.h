@@ -296,6 +296,14 @@ pt_status pt_get_group_lists(pt_ctx* ctx, double theta, int64_t sizes[3], int32_
 pt_status pt_get_accelerations(pt_ctx* ctx, double* acc, double* pot);
 pt_status pt_get_state(pt_ctx* ctx, double* pos, double* vel);       /* n x 3 each */
 pt_status pt_get_counters(pt_ctx* ctx, pt_counters* out);
+/* How pt_compute_forces walks the quadrupole build (no reference counterpart;
+ * both give every bucket exactly the open()/node()/leaf() calls of
+ * TransposedDownTraverser, src/Traverser.h:154-217).  Enabled (the default):
+ * runs of consecutive buckets of at most 64 particles walk once with their
+ * union box and share the sources every member closes or reaches as leaves,
+ * each member then walking on its own below the nodes the members decide
+ * differently.  Disabled: one walk per bucket. */
+pt_status pt_set_fused_grouping(pt_ctx* ctx, int enabled);
 pt_status pt_set_timing(pt_ctx* ctx, int enabled);
 pt_status pt_get_timings(pt_ctx* ctx, float ms[PT_T_COUNT]);
 

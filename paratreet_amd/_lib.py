@@ -29,6 +29,7 @@ EXPORTS = (
     "pt_get_state", "pt_get_counters", "pt_set_timing", "pt_get_timings",
     "pt_force_vjp", "pt_leapfrog_forward", "pt_leapfrog_adjoint", "pt_get_node_hexadecapole",
     "pt_set_adjoint_precision", "pt_get_group_lists", "pt_sort_noalloc",
+    "pt_set_fused_grouping",
 )
 
 
@@ -141,6 +142,7 @@ def load():
         "pt_get_state": (I32, [P, P, P]),
         "pt_get_counters": (I32, [P, P]),
         "pt_set_timing": (I32, [P, ctypes.c_int]),
+        "pt_set_fused_grouping": (I32, [P, ctypes.c_int]),
         "pt_get_timings": (I32, [P, P]),
         "pt_force_vjp": (I32, [P, D, ctypes.c_int, P, P, P, ctypes.c_int]),
         "pt_set_adjoint_precision": (I32, [P, ctypes.c_int]),

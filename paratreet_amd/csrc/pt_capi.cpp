@@ -92,6 +92,10 @@ struct pt_ctx {
   DBuf<int32_t> ewcount;
   DBuf<unsigned long long> cpart;  // per-block counter partials
   DBuf<unsigned long long> wqueue;  // per-XCD work counters of the fused kernel
+  // bucket groups of the grouped fused kernel (pt_set_fused_grouping)
+  bool fused_grouping = true;
+  DBuf<int32_t> fg_tfirst, fg_tnb, fg_first, fg_nb;
+  DBuf<uint32_t> fg_wcount, fg_woff, fg_ng;
   bool lists_valid = false;
   // lists
   DBuf<int32_t> pp_cnt, pc_cnt, pp, pc;
@@ -349,6 +353,8 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->bucket_node); release(c->nd); release(c->ord_h); release(c->ord_b);
   release(c->tnode); release(c->gn); release(c->gn32); release(c->bk); release(c->cpart);
   release(c->wqueue);
+  release(c->fg_tfirst); release(c->fg_tnb); release(c->fg_first); release(c->fg_nb);
+  release(c->fg_wcount); release(c->fg_woff); release(c->fg_ng);
   release(c->pp_cnt); release(c->pc_cnt); release(c->pp); release(c->pc);
   release(c->pp_off); release(c->pc_off); release(c->i64_total);
   release(c->counters); release(c->derr); release(c->scratch);
@@ -956,10 +962,29 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (precision != PT_FP64 && (st = check_fp32c_range(c))) return st;
   const int64_t nb = c->nb;
   const bool hexa = c->cfg.hexadecapole != 0;
+  const bool grouped = !hexa && c->fused_grouping;
   const int64_t nblk = std::max<int64_t>(
-      hexa ? pt_hexa_gravity_blocks(nb) : pt_gravity_blocks(nb, (int)precision), 1);
+      hexa ? pt_hexa_gravity_blocks(nb)
+           : (grouped ? pt_grouped_gravity_blocks(nb, c->n) : pt_gravity_blocks(nb, (int)precision)),
+      1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
   PT_TRY_HIP(c, ensure(c->wqueue, 8));
+  if (grouped) {
+    // the bucket groups of this tree (every replica walks with them)
+    const int64_t nw = pt_fgroup_windows(nb);
+    PT_TRY_HIP(c, ensure(c->fg_tfirst, nb));
+    PT_TRY_HIP(c, ensure(c->fg_tnb, nb));
+    PT_TRY_HIP(c, ensure(c->fg_first, nb));
+    PT_TRY_HIP(c, ensure(c->fg_nb, nb));
+    PT_TRY_HIP(c, ensure(c->fg_wcount, nw));
+    PT_TRY_HIP(c, ensure(c->fg_woff, nw));
+    PT_TRY_HIP(c, ensure(c->fg_ng, 1));
+    if ((st = ensure_scratch(c, pt_scan_scratch_bytes(nw)))) return st;
+    Timer t(c, PT_T_TRAVERSE);
+    PT_TRY_LAUNCH(c, pt_launch_fgroups(c->bk.p, nb, c->fg_tfirst.p, c->fg_tnb.p, c->fg_wcount.p,
+                                       c->fg_woff.p, c->scratch.p, c->fg_ng.p, c->fg_first.p,
+                                       c->fg_nb.p, c->stream));
+  }
   // the visitor offsets: one, or the (2N+1)^3 period replicas in the order of
   // ExMain::traversalFn (examples/Gravity.C:35-46), each added to `offset`;
   // every replica's traversal adds to the accelerations and the counters
@@ -995,6 +1020,12 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
                                                   c->es.p, gf, c->gm, off, c->axs.p, c->ays.p,
                                                   c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc,
                                                   c->stream));
+        else if (grouped)
+          PT_TRY_LAUNCH(c, pt_launch_grouped_gravity(
+                               (int)precision, attempt == 1, c->bk.p, c->fg_first.p, c->fg_nb.p,
+                               c->fg_ng.p, nb, c->n, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
+                               c->ys.p, c->zs.p, c->ms.p, c->es.p, gf, off, c->axs.p, c->ays.p,
+                               c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc, c->stream));
         else
           PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb,
                                              c->tnode.p, c->gn.p, c->gn32.p, c->xs.p, c->ys.p,
@@ -1676,6 +1707,13 @@ pt_status pt_get_state(pt_ctx* c, double* pos, double* vel) {
 pt_status pt_get_counters(pt_ctx* c, pt_counters* out) {
   if (!c || !out) return PT_ERR_ARG;
   *out = c->h_counters;
+  return PT_OK;
+}
+
+pt_status pt_set_fused_grouping(pt_ctx* c, int enabled) {
+  pt_status st = check_ctx(c);
+  if (st) return st;
+  c->fused_grouping = enabled != 0;
   return PT_OK;
 }
 

@@ -30,7 +30,12 @@
 //   gravity_kernel   -- walks the tree itself (pt_walk.h, same decisions) and
 //                       evaluates each 64-entry chunk of closed nodes / leaf
 //                       sources as soon as the walk has produced it, so no
-//                       list ever reaches HBM.
+//                       list ever reaches HBM;
+//   grouped_gravity_kernel -- the same per bucket run of at most 64
+//                       particles: one walk with the run's union box shares
+//                       every source all its buckets agree on, each bucket
+//                       walks on its own only below the nodes they decide
+//                       differently (the default of pt_compute_forces).
 //
 // This file is compiled with -ffp-contract=off because it contains the walk's
 // opening test; the interaction arithmetic uses explicit fma() where the
@@ -1347,6 +1352,112 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
   }
 }
 
+// Evaluates every full chunk of the pending closed nodes (cw at a time) and
+// leaf sources (64 at a time).  A walk step starts with npc < cw and
+// npp < 64 and adds at most 64 to each, so the buffers (128 entries) never
+// overflow and each rest (< 64) fits one move.
+template <bool F32, typename W_t>
+__device__ __forceinline__ void drain_full(W_t& W, int& npc, int& npp, const Group& g,
+                                           Lanes<F32>& L, const GravNode* __restrict__ gn,
+                                           const GravNode32* __restrict__ gn32, const Bucket& b,
+                                           const BucketF& bf, const double* __restrict__ px,
+                                           const double* __restrict__ py,
+                                           const double* __restrict__ pz,
+                                           const double* __restrict__ pm,
+                                           const double* __restrict__ pe, double ox, double oy,
+                                           double oz, double ccx, double ccy, double ccz) {
+  const int lane = __lane_id();
+  const int cw = F32 ? PT_WAVE : g.cw;
+  while (npc >= cw) {
+    do_nodes<F32, W_t>(W, lane < cw ? W.pcbuf[lane] : 0, cw, g, L, gn, gn32, b, bf, ox, oy, oz,
+                       ccx, ccy, ccz);
+    const int rest = npc - cw;
+    const int32_t v = lane < rest ? W.pcbuf[cw + lane] : 0;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < rest) W.pcbuf[lane] = v;
+    npc = rest;
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (npp >= PT_WAVE) {
+    do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox, oy, oz,
+                        ccx, ccy, ccz);
+    const int rest = npp - PT_WAVE;
+    const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
+    const int32_t vc = lane < rest ? W.ppcnt[PT_WAVE + lane] : 0;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < rest) {
+      W.ppbeg[lane] = vb;
+      W.ppcnt[lane] = vc;
+    }
+    npp = rest;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Evaluates what is left in the buffers.
+template <bool F32, typename W_t>
+__device__ __forceinline__ void drain_rest(W_t& W, int npc, int npp, const Group& g,
+                                           Lanes<F32>& L, const GravNode* __restrict__ gn,
+                                           const GravNode32* __restrict__ gn32, const Bucket& b,
+                                           const BucketF& bf, const double* __restrict__ px,
+                                           const double* __restrict__ py,
+                                           const double* __restrict__ pz,
+                                           const double* __restrict__ pm,
+                                           const double* __restrict__ pe, double ox, double oy,
+                                           double oz, double ccx, double ccy, double ccz) {
+  const int lane = __lane_id();
+  if (npc > 0) {
+    const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
+    do_nodes<F32, W_t>(W, id, npc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
+  }
+  if (npp > 0) {
+    const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
+    const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
+    do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
+  }
+}
+
+// The walk of one bucket below the opened nodes on the stack ([0, sp); the
+// entries from `limit` up are kept), its chunks evaluated as they fill.
+// w_open / w_close: this walk's open() outcomes; lane_leaf_parts: the
+// source-leaf particles this lane visited.
+template <bool F32, int STACK>
+__device__ __forceinline__ void walk_below(
+    WaveLds<F32, STACK>& W, int& sp, int& npc, int& npp, int limit, const Bucket& b,
+    const BucketF& bf, const Group& g, Lanes<F32>& L, const TreeNode* __restrict__ nodes,
+    const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ pm, const double* __restrict__ pe, double gf, double ox,
+    double oy, double oz, double ccx, double ccy, double ccz, int& w_open, int& w_close,
+    int& lane_leaf_parts, bool& overflow) {
+  using W_t = WaveLds<F32, STACK>;
+  WalkCounts wc;
+  while (sp > 0) {
+    unsigned long long leafm, closem;
+    const WalkLane r = pt_walk_step<STACK, false, false>(nodes, b, gf, ox, oy, oz, W.stack, sp,
+                                                         overflow, wc, leafm, closem, nullptr, 0.0,
+                                                         limit);
+    w_open += wc.n_open;
+    w_close += wc.n_close;
+    lane_leaf_parts += r.is_leaf ? r.count : 0;
+    if (r.is_closed)
+      W.pcbuf[npc + pt_rank_in(closem)] =
+          node_far_dsq(r.dsq, r.node_soft, bf, L.emax) ? (r.node | kFarNode) : r.node;
+    if (r.is_leaf) {
+      const int k = npp + pt_rank_in(leafm);
+      W.ppbeg[k] = r.begin;
+      W.ppcnt[k] = leaf_far_dsq(r.dsq, r.leaf_reach, r.leaf_soft, bf, L.emax)
+                       ? (r.count | kFarLeaf)
+                       : r.count;
+    }
+    npc += __popcll(closem);
+    npp += __popcll(leafm);
+    __builtin_amdgcn_wave_barrier();
+    drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
+                         ccy, ccz);
+  }
+}
+
 // One target bucket of the fused kernel: the walk for the bucket box with
 // the bucket's chunks evaluated as they fill, counters added to the wave's.
 template <bool F32, int STACK>
@@ -1373,12 +1484,10 @@ __device__ __forceinline__ void gravity_bucket(
     load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
     int sp = 0;
     int npc = 0, npp = 0;
-    // this walk's open() outcomes, and the source-leaf particles this lane
-    // visited (summed over the wave once the walk is done)
     int w_open = 0, w_close = 0;
     int lane_leaf_parts = 0;
-    WalkCounts wc;
     {
+      WalkCounts wc;
       const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, W.stack, sp, wc);
       if (r.is_closed) {
         if (lane == 0) W.pcbuf[0] = 0;
@@ -1395,64 +1504,11 @@ __device__ __forceinline__ void gravity_bucket(
       lane_leaf_parts = lane == 0 ? wc.leaf_parts : 0;
     }
     __builtin_amdgcn_wave_barrier();
-    while (sp > 0) {
-      unsigned long long leafm, closem;
-      const WalkLane r = pt_walk_step<STACK, false, false>(nodes, b, gf, ox, oy, oz, W.stack, sp,
-                                                           overflow, wc, leafm, closem);
-      w_open += wc.n_open;
-      w_close += wc.n_close;
-      lane_leaf_parts += r.is_leaf ? r.count : 0;
-      if (r.is_closed)
-        W.pcbuf[npc + pt_rank_in(closem)] =
-            node_far_dsq(r.dsq, r.node_soft, bf, L.emax) ? (r.node | kFarNode) : r.node;
-      if (r.is_leaf) {
-        const int k = npp + pt_rank_in(leafm);
-        W.ppbeg[k] = r.begin;
-        W.ppcnt[k] = leaf_far_dsq(r.dsq, r.leaf_reach, r.leaf_soft, bf, L.emax)
-                         ? (r.count | kFarLeaf)
-                         : r.count;
-      }
-      npc += __popcll(closem);
-      npp += __popcll(leafm);
-      __builtin_amdgcn_wave_barrier();
-      // full chunks of cw until fewer than cw are left: a step then starts
-      // with npc < cw and adds at most 64, so npc < cw + 64 <= 128 (the
-      // buffer) and each rest (< 64) fits one move
-      const int cw = F32 ? PT_WAVE : g.cw;
-      while (npc >= cw) {
-        do_nodes<F32, W_t>(W, lane < cw ? W.pcbuf[lane] : 0, cw, g, L, gn, gn32, b, bf, ox, oy,
-                           oz, ccx, ccy, ccz);
-        const int rest = npc - cw;
-        const int32_t v = lane < rest ? W.pcbuf[cw + lane] : 0;
-        __builtin_amdgcn_wave_barrier();
-        if (lane < rest) W.pcbuf[lane] = v;
-        npc = rest;
-        __builtin_amdgcn_wave_barrier();
-      }
-      if (npp >= PT_WAVE) {
-        do_leaves<F32, W_t>(W, W.ppbeg[lane], W.ppcnt[lane], g, L, px, py, pz, pm, pe, bf, ox, oy, oz,
-                            ccx, ccy, ccz);
-        const int rest = npp - PT_WAVE;
-        const int32_t vb = lane < rest ? W.ppbeg[PT_WAVE + lane] : 0;
-        const int32_t vc = lane < rest ? W.ppcnt[PT_WAVE + lane] : 0;
-        __builtin_amdgcn_wave_barrier();
-        if (lane < rest) {
-          W.ppbeg[lane] = vb;
-          W.ppcnt[lane] = vc;
-        }
-        npp = rest;
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    if (npc > 0) {
-      const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
-      do_nodes<F32, W_t>(W, id, npc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
-    }
-    if (npp > 0) {
-      const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
-      const int32_t cnt = lane < npp ? W.ppcnt[lane] : 0;
-      do_leaves<F32, W_t>(W, beg, cnt, g, L, px, py, pz, pm, pe, bf, ox, oy, oz, ccx, ccy, ccz);
-    }
+    walk_below<F32, STACK>(W, sp, npc, npp, STACK, b, bf, g, L, nodes, gn, gn32, px, py, pz, pm,
+                           pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close, lane_leaf_parts,
+                           overflow);
+    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
+                         ccy, ccz);
     if (count) {
       c_open += (unsigned)w_open;
       c_close += (unsigned)w_close;
@@ -1460,6 +1516,265 @@ __device__ __forceinline__ void gravity_bucket(
       c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
     }
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped fused evaluation: one wave per run of consecutive buckets (at most
+// 64 targets; formed by fgroup_window_kernel).  The group first walks once
+// with its union box U (the members' boxes' hull):
+//   - a node whose opening sphere misses U is closed for every member, since
+//     the rounded box distance to a member box is never below the one to U
+//     (pt_box_point_far_dsq states the argument); one that also holds U's
+//     farthest corner is opened by every member, as is a node of at most
+//     PT_MIN_PARTICLE_NODE particles; otherwise each member's own test
+//     decides, and a node opened by all or closed by all is still shared;
+//   - a shared closed node that is far (node_far_dsq on U: no openSoftening
+//     and no softened pair for any member) is a node() source of every
+//     member; a leaf reached by the shared walk is a leaf() source of every
+//     member;
+//   - every other node is a residual node, kept at the top of the walk
+//     stack.
+// The shared sources are evaluated once for all the group's targets (frame:
+// U's centre).  Then each member bucket applies its own open() to every
+// residual node (opened -> its children are walked below it as in the
+// per-bucket walk; closed -> node() with the member's own openSoftening) and
+// adds the result to its targets' shared sums.  Each member thereby makes
+// exactly the calls of its own walk from the root.
+// ---------------------------------------------------------------------------
+
+// Target group over particles [first, first + count), count <= 64.
+__device__ __forceinline__ Group make_group_range(int64_t first, int count) {
+  Group g;
+  g.ng = count;
+  g.S = PT_WAVE / g.ng;
+  g.magic = (65536 + g.S - 1) / g.S;
+  g.cw = g.S * ((PT_WAVE * g.magic) >> 16);
+  const int lane = __lane_id();
+  const int s = lane / g.ng;
+  g.act = s < g.S;
+  g.s = g.act ? s : 0;
+  g.t = g.act ? lane - s * g.ng : 0;
+  g.first = first;
+  return g;
+}
+
+// finish_group for a member whose targets start at offset `off` of a group
+// target range whose sums its lanes off + t wrote: the member's sums are moved
+// to those lanes, each of which adds to what it wrote itself.
+template <bool F32, typename W_t>
+__device__ __forceinline__ void finish_member(W_t& W, const Group& g, const Lanes<F32>& L, int off,
+                                              double* __restrict__ ax, double* __restrict__ ay,
+                                              double* __restrict__ az,
+                                              double* __restrict__ pot) {
+  double* red = reinterpret_cast<double*>(W.stage);
+  const int lane = __lane_id();
+  double v0, v1, v2, v3;
+  if constexpr (F32) {
+    v0 = (double)L.ax - (double)L.cx;
+    v1 = (double)L.ay - (double)L.cy;
+    v2 = (double)L.az - (double)L.cz;
+    v3 = (double)L.ap - (double)L.cp;
+  } else {
+    v0 = L.ax;
+    v1 = L.ay;
+    v2 = L.az;
+    v3 = L.ap;
+  }
+  red[lane * 4 + 0] = g.act ? v0 : 0.0;
+  red[lane * 4 + 1] = g.act ? v1 : 0.0;
+  red[lane * 4 + 2] = g.act ? v2 : 0.0;
+  red[lane * 4 + 3] = g.act ? v3 : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  const int t = lane - off;
+  if (t >= 0 && t < g.ng) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int s = 0; s < g.S; s++) {
+      const int l = s * g.ng + t;
+      s0 += red[l * 4 + 0];
+      s1 += red[l * 4 + 1];
+      s2 += red[l * 4 + 2];
+      s3 += red[l * 4 + 3];
+    }
+    const int64_t i = g.first + t;
+    ax[i] += s0;
+    ay[i] += s1;
+    az[i] += s2;
+    pot[i] += s3;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One group of G >= 2 buckets starting at bucket b0 (every member holds at
+// most 63 particles, all of them at most 64).
+template <bool F32, int STACK>
+__device__ __forceinline__ void gravity_group(
+    WaveLds<F32, STACK>& W, int64_t b0, int G, const Bucket* __restrict__ bk,
+    const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
+    const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pm,
+    const double* __restrict__ pe, double gf, double ox, double oy, double oz,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ pot, unsigned long long& c_node, unsigned long long& c_part,
+    unsigned long long& c_open, unsigned long long& c_close, bool& overflow, bool acc) {
+  using W_t = WaveLds<F32, STACK>;
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  // union box, largest member softening, target range
+  Bucket ub = bk[b0];
+  for (int k = 1; k < G; k++) {
+    const Bucket& m = bk[b0 + k];
+    ub.lox = fmin(ub.lox, m.lox);
+    ub.loy = fmin(ub.loy, m.loy);
+    ub.loz = fmin(ub.loz, m.loz);
+    ub.hix = fmax(ub.hix, m.hix);
+    ub.hiy = fmax(ub.hiy, m.hiy);
+    ub.hiz = fmax(ub.hiz, m.hiz);
+    ub.soft = fmax(ub.soft, m.soft);
+  }
+  const int64_t first = ub.begin;
+  const int ntot = (int)((int64_t)bk[b0 + G - 1].begin + bk[b0 + G - 1].count - first);
+  const double gcx = (ub.lox + ub.hix) * 0.5, gcy = (ub.loy + ub.hiy) * 0.5,
+               gcz = (ub.loz + ub.hiz) * 0.5;
+  const BucketF ubf = bucket_frame(ub, gcx, gcy, gcz);
+  int nres = 0;
+  {
+    // ---- shared walk
+    const Group g = make_group_range(first, ntot);
+    Lanes<F32> L;
+    load_targets<F32>(L, g, px, py, pz, pe, gcx, gcy, gcz);
+    int sp = 0, npc = 0, npp = 0;
+    int sh_open = 0, sh_close = 0, lane_leaf_parts = 0;
+    for (bool root = true; root || sp > 0; root = false) {
+      int32_t node;
+      bool active;
+      if (root) {
+        node = 0;
+        active = lane == 0;
+      } else {
+        const int k = sp < 8 ? sp : 8;
+        sp -= k;
+        const int slot = lane >> 3;
+        active = slot < k;
+        node = W.stack[sp + (active ? slot : 0)] + (lane & 7);
+      }
+      const TreeNode nd = nodes[node];
+      const bool leaf = active && nd.type == PT_NODE_LEAF;
+      const bool internal = active && nd.type == PT_NODE_INTERNAL;
+      const double cx = nd.cx + ox, cy = nd.cy + oy, cz = nd.cz + oz;
+      const double dsq = pt_box_point_dsq(ub, cx, cy, cz);
+      bool op = false, cl = false;
+      if (internal) {
+        if (nd.count <= PT_MIN_PARTICLE_NODE) {
+          op = true;
+        } else {
+          const double rsq = nd.radius * nd.radius * gf;
+          if (dsq > rsq) {
+            cl = true;
+          } else if (pt_box_point_far_dsq(ub, cx, cy, cz) <= rsq) {
+            op = true;
+          } else {
+            bool any_open = false, any_closed = false;
+            for (int k = 0; k < G; k++) {
+              const bool o = pt_box_point_dsq(bk[b0 + k], cx, cy, cz) <= rsq;
+              any_open |= o;
+              any_closed |= !o;
+            }
+            op = !any_closed;
+            cl = !any_open;
+          }
+        }
+      }
+      const bool shared_close = cl && node_far_dsq(dsq, nd.node_soft, ubf, L.emax);
+      const bool res = internal && !op && !shared_close;
+      const unsigned long long leafm = __ballot(leaf), closem = __ballot(shared_close),
+                               openm = __ballot(op), resm = __ballot(res);
+      const int n_open = __popcll(openm), n_res = __popcll(resm);
+      if (sp + n_open + nres + n_res > STACK) {
+        overflow = true;
+        return;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
+      if (res) W.stack[STACK - 1 - (nres + pt_rank_in(resm))] = node;
+      if (shared_close) W.pcbuf[npc + pt_rank_in(closem)] = node | kFarNode;
+      if (leaf) {
+        const int k = npp + pt_rank_in(leafm);
+        W.ppbeg[k] = nd.begin;
+        W.ppcnt[k] = leaf_far_dsq(dsq, nd.leaf_reach, nd.leaf_soft, ubf, L.emax)
+                         ? (nd.count | kFarLeaf)
+                         : nd.count;
+      }
+      lane_leaf_parts += leaf ? nd.count : 0;
+      sp += n_open;
+      nres += n_res;
+      npc += __popcll(closem);
+      npp += __popcll(leafm);
+      sh_open += n_open;
+      sh_close += __popcll(closem);
+      __builtin_amdgcn_wave_barrier();
+      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+                           gcx, gcy, gcz);
+    }
+    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+                         gcx, gcy, gcz);
+    // every member made each shared call
+    c_open += (unsigned long long)sh_open * G;
+    c_close += (unsigned long long)sh_close * G;
+    c_node += (unsigned long long)sh_close * ntot;
+    c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * ntot;
+    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
+  }
+  if (nres == 0) return;
+  // ---- each member from the residual nodes
+  for (int k = 0; k < G; k++) {
+    const Bucket b = bk[b0 + k];
+    const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+                 ccz = (b.loz + b.hiz) * 0.5;
+    const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
+    const Group g = make_group(b, 0);
+    Lanes<F32> L;
+    load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+    const int limit = STACK - nres;
+    int sp = 0, npc = 0, npp = 0;
+    int w_open = 0, w_close = 0, lane_leaf_parts = 0;
+    for (int r0 = 0; r0 < nres; r0 += PT_WAVE) {
+      const bool active = r0 + lane < nres;
+      const int32_t node = W.stack[STACK - 1 - (active ? r0 + lane : r0)];
+      const TreeNode nd = nodes[node];
+      const double dsq = pt_box_point_dsq(b, nd.cx + ox, nd.cy + oy, nd.cz + oz);
+      const bool op = active && pt_gv_open_dsq(nd, gf, dsq);
+      const bool cl = active && !op;
+      const unsigned long long openm = __ballot(op), closem = __ballot(cl);
+      const int n_open = __popcll(openm);
+      if (sp + n_open > limit) {
+        overflow = true;
+        return;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
+      if (cl)
+        W.pcbuf[npc + pt_rank_in(closem)] =
+            node_far_dsq(dsq, nd.node_soft, bf, L.emax) ? (node | kFarNode) : node;
+      sp += n_open;
+      npc += __popcll(closem);
+      w_open += n_open;
+      w_close += __popcll(closem);
+      __builtin_amdgcn_wave_barrier();
+      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
+                           ccx, ccy, ccz);
+    }
+    walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz, pm,
+                           pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close, lane_leaf_parts,
+                           overflow);
+    if (overflow) return;
+    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
+                         ccy, ccz);
+    const int64_t nt = b.count;
+    c_open += (unsigned)w_open;
+    c_close += (unsigned)w_close;
+    c_node += (unsigned long long)w_close * nt;
+    c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
+    finish_member<F32, W_t>(W, g, L, (int)(b.begin - first), ax, ay, az, pot);
   }
 }
 
@@ -1528,6 +1843,122 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
     unsigned long long v = 0;
     for (int w = 0; w < kWaves; w++) v += s_cnt[w][threadIdx.x];
     partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
+  }
+}
+
+// The grouped form of gravity_kernel: one wave per bucket group (gfirst /
+// gnb, *ngroups of them), gravity_group for a group of two or more buckets,
+// gravity_bucket for a group of one.  Work-group i runs on XCD i % 8, which
+// takes the contiguous eighth [ng x / 8, ng (x + 1) / 8) of the groups in
+// order (neighbouring groups share most sources in its L2); the grid is sized
+// for an upper bound of the group count and the surplus waves find no group.
+// Waves per work-group of the grouped kernel.
+constexpr int kGWaves = 1;
+
+template <bool F32, int STACK, bool OFF>
+__global__ void __launch_bounds__(kGWaves * PT_WAVE) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 6 : 1, 8)))
+grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
+                       const int32_t* __restrict__ gnb, const uint32_t* __restrict__ ngroups,
+                       const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
+                       const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+                       const double* __restrict__ py, const double* __restrict__ pz,
+                       const double* __restrict__ pm, const double* __restrict__ pe, double gf,
+                       double ox_in, double oy_in, double oz_in, double* __restrict__ ax,
+                       double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
+                       unsigned long long* __restrict__ partials, int32_t* __restrict__ err,
+                       int acc) {
+  using W_t = WaveLds<F32, STACK>;
+  __shared__ W_t lds[kGWaves];
+  __shared__ unsigned long long s_cnt[kGWaves][4];
+  const double ox = OFF ? ox_in : -0.0, oy = OFF ? oy_in : -0.0, oz = OFF ? oz_in : -0.0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  W_t& W = lds[wave];
+  unsigned long long c_node = 0, c_part = 0, c_open = 0, c_close = 0;
+  bool overflow = false;
+  const int64_t ng = (int64_t)ngroups[0];
+  const int xcd = blockIdx.x % 8;
+  const int64_t g_lo = ng * xcd / 8, g_hi = ng * (xcd + 1) / 8;
+  const int64_t gi = g_lo + (int64_t)(blockIdx.x / 8) * kGWaves + wave;
+  if (gi < g_hi) {
+    const int G = gnb[gi];
+    const int64_t b0 = gfirst[gi];
+    if (G == 1)
+      gravity_bucket<F32, STACK>(W, b0, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
+                                 acc != 0);
+    else
+      gravity_group<F32, STACK>(W, b0, G, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+                                ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
+                                acc != 0);
+  }
+  if (lane == 0) {
+    if (overflow) atomicExch(err, 1);
+    s_cnt[wave][0] = c_node;
+    s_cnt[wave][1] = c_part;
+    s_cnt[wave][2] = c_open;
+    s_cnt[wave][3] = c_close;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long v = 0;
+    for (int w = 0; w < kGWaves; w++) v += s_cnt[w][threadIdx.x];
+    partials[(int64_t)blockIdx.x * 4 + threadIdx.x] = v;
+  }
+}
+
+// Bucket groups of the grouped kernel, formed greedily inside windows of
+// kFgWindow consecutive buckets: a group closes when the next bucket would
+// take it past 64 particles, so a bucket of more than 64 is a group of its
+// own.  One wave per window: the lanes load the window's counts, the greedy
+// pass runs on wave-uniform values.  Window w writes its groups to slots
+// [w kFgWindow, w kFgWindow + wcount[w]) of tfirst / tnb.
+constexpr int kFgWindow = PT_WAVE;
+
+__global__ void __launch_bounds__(256)
+fgroup_window_kernel(const Bucket* __restrict__ bk, int64_t nb, int32_t* __restrict__ tfirst,
+                     int32_t* __restrict__ tnb, uint32_t* __restrict__ wcount) {
+  const int64_t w = (int64_t)blockIdx.x * 4 + threadIdx.x / PT_WAVE;
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const int64_t nw = (nb + kFgWindow - 1) / kFgWindow;
+  if (w >= nw) return;
+  const int64_t b0 = w * kFgWindow;
+  const int n = (int)((nb - b0) < kFgWindow ? (nb - b0) : kFgWindow);
+  const int cnt = lane < n ? bk[b0 + lane].count : 0;
+  int slot = 0, start = 0, tot = 0;
+  for (int k = 0; k < n; k++) {
+    const int c = __builtin_amdgcn_readlane(cnt, k);
+    if (k > start && tot + c > PT_WAVE) {
+      if (lane == 0) {
+        tfirst[b0 + slot] = (int32_t)(b0 + start);
+        tnb[b0 + slot] = k - start;
+      }
+      slot++;
+      start = k;
+      tot = 0;
+    }
+    tot += c;
+  }
+  if (lane == 0) {
+    tfirst[b0 + slot] = (int32_t)(b0 + start);
+    tnb[b0 + slot] = n - start;
+    wcount[w] = (uint32_t)(slot + 1);
+  }
+}
+
+// Compacts the windows' groups in order (woff: exclusive scan of wcount).
+__global__ void __launch_bounds__(256)
+fgroup_compact_kernel(int64_t nb, const int32_t* __restrict__ tfirst,
+                      const int32_t* __restrict__ tnb, const uint32_t* __restrict__ wcount,
+                      const uint32_t* __restrict__ woff, int32_t* __restrict__ gfirst,
+                      int32_t* __restrict__ gnb) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t w = i / kFgWindow;
+  const uint32_t j = (uint32_t)(i - w * kFgWindow);
+  if (j < wcount[w]) {
+    gfirst[woff[w] + j] = tfirst[i];
+    gnb[woff[w] + j] = tnb[i];
   }
 }
 
@@ -1641,5 +2072,61 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
   }
 #undef PT_GRAVITY_LAUNCH_OFF
 #undef PT_GRAVITY_LAUNCH
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int64_t pt_fgroup_windows(int64_t nb) { return (nb + kFgWindow - 1) / kFgWindow; }
+
+// Two consecutive groups of a window hold more than 64 particles together,
+// so a window of T particles has at most 2 T / 65 + 1 groups.
+int64_t pt_fgroup_max_groups(int64_t nb, int64_t n) {
+  return pt_fgroup_windows(nb) + 2 * n / 65 + 1;
+}
+
+int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n) {
+  const int64_t per_xcd = (pt_fgroup_max_groups(nb, n) + 7) / 8 + 1;
+  return 8 * ((per_xcd + kGWaves - 1) / kGWaves);
+}
+
+int pt_launch_fgroups(const Bucket* bk, int64_t nb, int32_t* tfirst, int32_t* tnb,
+                      uint32_t* wcount, uint32_t* woff, void* scratch, uint32_t* ngroups,
+                      int32_t* gfirst, int32_t* gnb, hipStream_t s) {
+  if (nb <= 0) return 0;
+  const int64_t nw = pt_fgroup_windows(nb);
+  fgroup_window_kernel<<<(unsigned)((nw + 3) / 4), 256, 0, s>>>(bk, nb, tfirst, tnb, wcount);
+  if (hipGetLastError() != hipSuccess) return -1;
+  if (pt_exclusive_scan_u32(wcount, woff, nw, scratch, ngroups, s) != 0) return -1;
+  fgroup_compact_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(nb, tfirst, tnb, wcount,
+                                                                      woff, gfirst, gnb);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
+                              const int32_t* gfirst, const int32_t* gnb, const uint32_t* ngroups,
+                              int64_t nb, int64_t n, const TreeNode* nodes, const GravNode* gn,
+                              const GravNode32* gn32, const double* x, const double* y,
+                              const double* z, const double* m, const double* e, double gf,
+                              const double off[3], double* ax, double* ay, double* az,
+                              double* pot, unsigned long long* partials, int32_t* err,
+                              bool accumulate, hipStream_t s) {
+  if (nb <= 0) return 0;
+  const unsigned grid = (unsigned)pt_grouped_gravity_blocks(nb, n);
+  const bool has_off = off[0] != 0.0 || off[1] != 0.0 || off[2] != 0.0;
+#define PT_GROUPED_LAUNCH(F32, STACK, OFF)                                                     \
+  grouped_gravity_kernel<F32, STACK, OFF><<<grid, kGWaves * PT_WAVE, 0, s>>>(                           \
+      bk, gfirst, gnb, ngroups, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, \
+      ay, az, pot, partials, err, accumulate ? 1 : 0)
+#define PT_GROUPED_LAUNCH_OFF(F32, STACK)               \
+  if (has_off) PT_GROUPED_LAUNCH(F32, STACK, true);     \
+  else PT_GROUPED_LAUNCH(F32, STACK, false)
+  if (precision == 0) {
+    if (big_stack) PT_GROUPED_LAUNCH_OFF(false, kBigStack);
+    else PT_GROUPED_LAUNCH_OFF(false, kSmallStack);
+  } else {
+    if (big_stack) PT_GROUPED_LAUNCH_OFF(true, kBigStack);
+    else PT_GROUPED_LAUNCH_OFF(true, kSmallStack);
+  }
+#undef PT_GROUPED_LAUNCH_OFF
+#undef PT_GROUPED_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

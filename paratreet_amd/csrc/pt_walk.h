@@ -49,6 +49,23 @@ __device__ __forceinline__ double pt_box_point_dsq(const Bucket& b, double px, d
   return dsq;
 }
 
+// The squared distance from the point to the box's farthest corner, per-axis
+// terms summed in the order of pt_box_point_dsq.  For any box B inside b the
+// rounded pt_box_point_dsq(B, p) is at most this value: each of its per-axis
+// terms is a rounded difference of p and a face of B that is never larger
+// than the one summed here (rounding is monotone), and so are their squares
+// and sums.  Likewise pt_box_point_dsq(b, p) <= pt_box_point_dsq(B, p).
+__device__ __forceinline__ double pt_box_point_far_dsq(const Bucket& b, double px, double py,
+                                                       double pz) {
+  const double tx = fmax(px - b.lox, b.hix - px);
+  const double ty = fmax(py - b.loy, b.hiy - py);
+  const double tz = fmax(pz - b.loz, b.hiz - pz);
+  double dsq = tx * tx;
+  dsq += ty * ty;
+  dsq += tz * tz;
+  return dsq;
+}
+
 // Space::intersect(box, point, rsq): dsq <= rsq.
 __device__ __forceinline__ bool pt_intersect_box_point(const Bucket& b, double px, double py,
                                                        double pz, double rsq) {
@@ -198,8 +215,8 @@ __device__ __forceinline__ WalkLane pt_walk_root(const TreeNode* __restrict__ no
 
 // One step: pops up to 8 opened nodes and tests their 64 children.  Returns
 // this lane's child and its classification; pushes opened children.  A push
-// beyond STACK entries sets overflow and ends the walk (the caller reports
-// it; nothing is silently dropped).  With SUM_LEAVES false wc.leaf_parts is
+// beyond `limit` (<= STACK) entries sets overflow and ends the walk (the
+// caller reports it; nothing is silently dropped).  With SUM_LEAVES false wc.leaf_parts is
 // left 0 and the caller sums the leaf lanes' counts itself.
 template <int STACK, bool HEXA = false, bool SUM_LEAVES = true>
 __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ nodes,
@@ -210,7 +227,7 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
                                                  unsigned long long& leafm,
                                                  unsigned long long& closem,
                                                  const HexOpen* __restrict__ ho = nullptr,
-                                                 double gm = 0.0) {
+                                                 double gm = 0.0, int limit = STACK) {
   const int lane = __lane_id();
   const int k = sp < 8 ? sp : 8;
   sp -= k;
@@ -247,7 +264,7 @@ __device__ __forceinline__ WalkLane pt_walk_step(const TreeNode* __restrict__ no
   closem = __ballot(r.is_closed);
   const unsigned long long openm = __ballot(op);
   const int n_open = __popcll(openm);
-  if (sp + n_open > STACK) {
+  if (sp + n_open > limit) {
     overflow = true;
     sp = 0;
     leafm = closem = 0;

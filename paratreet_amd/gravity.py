@@ -436,6 +436,11 @@ class Driver:
         self._check(self.L.pt_get_state(self.ctx, _ptr(pos), _ptr(vel)))
         return pos, vel
 
+    def set_fused_grouping(self, on=True):
+        """Grouped (default) or per-bucket walk of compute_forces; the same
+        calls and counters either way."""
+        self._check(self.L.pt_set_fused_grouping(self.ctx, 1 if on else 0))
+
     def set_timing(self, on=True):
         self._check(self.L.pt_set_timing(self.ctx, 1 if on else 0))
 

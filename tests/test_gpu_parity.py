@@ -161,20 +161,28 @@ def test_forces_fp32c_precision(pair):
     check_fp32c(a32, a64, name + " (same lists)")
 
 
-def test_fused_path_matches_oracle(pair):
+@pytest.mark.parametrize("grouping", [True, False], ids=["grouped", "per_bucket"])
+def test_fused_path_matches_oracle(pair, grouping):
     """pt_compute_forces (fused walk + interactions, no lists) against the
-    oracle: counters exact, FP64 within 1e-12, FP32C within 1e-5 RMS."""
+    oracle, with the grouped walk (bucket runs sharing a walk) and with one
+    walk per bucket: counters exact, FP64 within 1e-12, FP32C within its
+    bounds."""
     name, P, d, _, _ = pair
-    cnt = d.compute_forces(theta=P.theta, precision=0)
-    acc, pot = d.accelerations()
+    d.set_fused_grouping(grouping)
+    try:
+        cnt = d.compute_forces(theta=P.theta, precision=0)
+        acc, pot = d.accelerations()
+        cnt32 = d.compute_forces(theta=P.theta, precision=1)
+        a32, _ = d.accelerations()
+    finally:
+        d.set_fused_grouping(True)
     oacc, opot, ocnt = P.gravity_unsorted()
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt), name
     e = rel_err(acc, oacc)
     ep = np.abs(pot - opot) / np.abs(opot)
-    print("%s fused fp64 vs oracle: max rel acc %.3g pot %.3g" % (name, e.max(), ep.max()))
+    print("%s fused fp64 vs oracle (%s): max rel acc %.3g pot %.3g" % (
+        name, "grouped" if grouping else "per bucket", e.max(), ep.max()))
     assert e.max() <= FP64_TOL and ep.max() <= FP64_TOL, name
-    cnt32 = d.compute_forces(theta=P.theta, precision=1)
-    a32, _ = d.accelerations()
     assert tuple(cnt32) == tuple(cnt), name
     check_fp32c(a32, acc, name + " fused")
 
