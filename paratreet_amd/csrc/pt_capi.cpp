@@ -56,6 +56,7 @@ struct pt_ctx {
   DBuf<uint32_t> perm;
   DBuf<ParticleRec> prec;  // x y z m e per particle, original order (sort gather)
   DBuf<double> xs, ys, zs, ms, es, axs, ays, azs, pots;
+  DBuf<double> ustage;  // unsort staging: one 4-double record per particle
   // universe
   DBuf<double> box, bbox, box_parts;
   double h_box[6] = {0, 0, 0, 0, 0, 0};
@@ -339,7 +340,7 @@ void pt_ctx_destroy(pt_ctx* c) {
   release(c->m); release(c->e); release(c->ax); release(c->ay); release(c->az); release(c->pot);
   release(c->keys); release(c->skeys); release(c->perm);
   release(c->xs); release(c->ys); release(c->zs); release(c->ms); release(c->es);
-  release(c->axs); release(c->ays); release(c->azs); release(c->pots);
+  release(c->axs); release(c->ays); release(c->azs); release(c->pots); release(c->ustage);
   release(c->box); release(c->bbox); release(c->box_parts);
   release(c->nkey); release(c->nbegin); release(c->ncount); release(c->ndepth); release(c->ntype);
   release(c->nparent); release(c->nfirst); release(c->level_nodes);
@@ -714,9 +715,12 @@ static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr, bool
                                  bool accumulate = false) {
   PT_TRY_LAUNCH(c, pt_launch_reduce_counters(c->cpart.p, nblocks, c->counters.p, c->derr.p,
                                              c->stream, accumulate));
-  if (unsort)
+  if (unsort) {
+    PT_TRY_HIP(c, ensure(c->ustage, 4 * c->n));
     PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                      c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+                                      c->ax.p, c->ay.p, c->az.p, c->pot.p, c->ustage.p,
+                                      c->stream));
+  }
   unsigned long long h[5];
   PT_TRY_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
@@ -824,8 +828,9 @@ pt_status pt_interact(pt_ctx* c, pt_precision precision) {
                 "the hexadecapole list path evaluates node() in FP64 only (pt_compute_forces "
                 "has the compensated-FP32 form)");
   if ((st = launch_list_interact(c, precision))) return st;
+  PT_TRY_HIP(c, ensure(c->ustage, 4 * c->n));
   PT_TRY_LAUNCH(c, pt_launch_unsort(c->perm.p, c->n, c->axs.p, c->ays.p, c->azs.p, c->pots.p,
-                                    c->ax.p, c->ay.p, c->az.p, c->pot.p, c->stream));
+                                    c->ax.p, c->ay.p, c->az.p, c->pot.p, c->ustage.p, c->stream));
   PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
   c->stage = S_FORCES;
   c->have_forces = true;

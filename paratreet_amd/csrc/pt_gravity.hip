@@ -1605,10 +1605,17 @@ __device__ __forceinline__ void finish_member(W_t& W, const Group& g, const Lane
   __builtin_amdgcn_wave_barrier();
 }
 
+// Walk stack entries the residual list leaves to the member walks.  A group
+// whose residual list would take more is evaluated by per-bucket walks.
+constexpr int kMemberReserve = 96;
+
 // One group of G >= 2 buckets starting at bucket b0 (every member holds at
-// most 63 particles, all of them at most 64).
+// most 63 particles, all of them at most 64).  Returns false, having written
+// nothing and counted nothing, when the shared walk's pending and residual
+// entries do not fit the stack with kMemberReserve entries to spare; the
+// caller then walks each member on its own.
 template <bool F32, int STACK>
-__device__ __forceinline__ void gravity_group(
+__device__ __forceinline__ bool gravity_group(
     WaveLds<F32, STACK>& W, int64_t b0, int G, const Bucket* __restrict__ bk,
     const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
     const GravNode32* __restrict__ gn32, const double* __restrict__ px,
@@ -1689,10 +1696,8 @@ __device__ __forceinline__ void gravity_group(
       const unsigned long long leafm = __ballot(leaf), closem = __ballot(shared_close),
                                openm = __ballot(op), resm = __ballot(res);
       const int n_open = __popcll(openm), n_res = __popcll(resm);
-      if (sp + n_open + nres + n_res > STACK) {
-        overflow = true;
-        return;
-      }
+      if (sp + n_open + nres + n_res > STACK || nres + n_res > STACK - kMemberReserve)
+        return false;
       __builtin_amdgcn_wave_barrier();
       if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
       if (res) W.stack[STACK - 1 - (nres + pt_rank_in(resm))] = node;
@@ -1724,7 +1729,7 @@ __device__ __forceinline__ void gravity_group(
     c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * ntot;
     finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
   }
-  if (nres == 0) return;
+  if (nres == 0) return true;
   // ---- each member from the residual nodes
   for (int k = 0; k < G; k++) {
     const Bucket b = bk[b0 + k];
@@ -1734,9 +1739,11 @@ __device__ __forceinline__ void gravity_group(
     const Group g = make_group(b, 0);
     Lanes<F32> L;
     load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
-    const int limit = STACK - nres;
+    const int limit = STACK - nres;  // >= kMemberReserve
     int sp = 0, npc = 0, npp = 0;
     int w_open = 0, w_close = 0, lane_leaf_parts = 0;
+    // 64 residual nodes at a time, each batch walked to the end, so the
+    // member's stack holds at most one batch plus what its walk pushes
     for (int r0 = 0; r0 < nres; r0 += PT_WAVE) {
       const bool active = r0 + lane < nres;
       const int32_t node = W.stack[STACK - 1 - (active ? r0 + lane : r0)];
@@ -1746,27 +1753,23 @@ __device__ __forceinline__ void gravity_group(
       const bool cl = active && !op;
       const unsigned long long openm = __ballot(op), closem = __ballot(cl);
       const int n_open = __popcll(openm);
-      if (sp + n_open > limit) {
-        overflow = true;
-        return;
-      }
       __builtin_amdgcn_wave_barrier();
-      if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
+      if (op) W.stack[pt_rank_in(openm)] = nd.first_child;
       if (cl)
         W.pcbuf[npc + pt_rank_in(closem)] =
             node_far_dsq(dsq, nd.node_soft, bf, L.emax) ? (node | kFarNode) : node;
-      sp += n_open;
+      sp = n_open;
       npc += __popcll(closem);
       w_open += n_open;
       w_close += __popcll(closem);
       __builtin_amdgcn_wave_barrier();
       drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
                            ccx, ccy, ccz);
+      walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz,
+                             pm, pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close,
+                             lane_leaf_parts, overflow);
+      if (overflow) return true;
     }
-    walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz, pm,
-                           pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close, lane_leaf_parts,
-                           overflow);
-    if (overflow) return;
     drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
                          ccy, ccz);
     const int64_t nt = b.count;
@@ -1776,6 +1779,7 @@ __device__ __forceinline__ void gravity_group(
     c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
     finish_member<F32, W_t>(W, g, L, (int)(b.begin - first), ax, ay, az, pot);
   }
+  return true;
 }
 
 // Fused walk + interactions, one wave per target bucket.  FP32C: one
@@ -1852,11 +1856,17 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
 // takes the contiguous eighth [ng x / 8, ng (x + 1) / 8) of the groups in
 // order (neighbouring groups share most sources in its L2); the grid is sized
 // for an upper bound of the group count and the surplus waves find no group.
-// Waves per work-group of the grouped kernel.
+// Waves per work-group of the grouped kernel, and its walk stack (pending
+// entries and, from the top, the residual list) per precision.  FP32C: 512
+// entries (five one-wave work-groups per SIMD fit in LDS, 7.9 KiB each; the
+// six-wave register budget, 80 VGPRs and a small spill, measured 3.5% faster
+// than five waves' 96 VGPRs); FP64: 384.
 constexpr int kGWaves = 1;
+constexpr int kGroupStack32 = 512;
+constexpr int kGroupStack64 = 384;
 
 template <bool F32, int STACK, bool OFF>
-__global__ void __launch_bounds__(kGWaves * PT_WAVE) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kSmallStack ? 6 : 1, 8)))
+__global__ void __launch_bounds__(kGWaves * PT_WAVE) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kGroupStack32 ? 6 : 1, 8)))
 grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
                        const int32_t* __restrict__ gnb, const uint32_t* __restrict__ ngroups,
                        const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
@@ -1883,14 +1893,18 @@ grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
   if (gi < g_hi) {
     const int G = gnb[gi];
     const int64_t b0 = gfirst[gi];
-    if (G == 1)
-      gravity_bucket<F32, STACK>(W, b0, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
-                                 ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
-                                 acc != 0);
-    else
-      gravity_group<F32, STACK>(W, b0, G, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
-                                ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
-                                acc != 0);
+    // a group of one bucket, or a group whose residual list overflowed: one
+    // walk per bucket
+    const bool done =
+        G > 1 && gravity_group<F32, STACK>(W, b0, G, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf,
+                                           ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
+                                           c_close, overflow, acc != 0);
+    if (!done) {
+      for (int k = 0; k < G; k++)
+        gravity_bucket<F32, STACK>(W, b0 + k, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy,
+                                   oz, ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
+                                   acc != 0);
+    }
   }
   if (lane == 0) {
     if (overflow) atomicExch(err, 1);
@@ -2121,10 +2135,10 @@ int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
   else PT_GROUPED_LAUNCH(F32, STACK, false)
   if (precision == 0) {
     if (big_stack) PT_GROUPED_LAUNCH_OFF(false, kBigStack);
-    else PT_GROUPED_LAUNCH_OFF(false, kSmallStack);
+    else PT_GROUPED_LAUNCH_OFF(false, kGroupStack64);
   } else {
     if (big_stack) PT_GROUPED_LAUNCH_OFF(true, kBigStack);
-    else PT_GROUPED_LAUNCH_OFF(true, kSmallStack);
+    else PT_GROUPED_LAUNCH_OFF(true, kGroupStack32);
   }
 #undef PT_GROUPED_LAUNCH_OFF
 #undef PT_GROUPED_LAUNCH

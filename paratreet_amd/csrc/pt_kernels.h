@@ -187,9 +187,11 @@ int pt_launch_ewald_hexa(int64_t n, const double* x, const double* y, const doub
 
 int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, double* xs,
                      double* ys, double* zs, double* ms, double* es, hipStream_t s);
+// Sorted slot i -> original slot perm[i] for four arrays; stage: 4 n doubles
+// (32-byte aligned).
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,
                      const double* a2, const double* a3, double* o0, double* o1, double* o2,
-                     double* o3, hipStream_t s);
+                     double* o3, double* stage, hipStream_t s);
 // Dual-tree traversal (pt_dual.hip): node -> bucket map, the node ids of two
 // keys, one breadth-first step over node pairs, and per-bucket lists from
 // records sorted by (bucket << 32 | source).

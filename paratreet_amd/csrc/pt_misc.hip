@@ -22,18 +22,33 @@ gather_kernel(const uint32_t* __restrict__ perm, int64_t n, const ParticleRec* _
   es[i] = r.e;
 }
 
+// Sorted-order results to the caller's order in two passes: one 32-byte
+// record per particle scattered to its original slot (a whole sector per
+// write instead of four 8-byte pieces of four lines), then the records split
+// into the four arrays with coalesced reads and writes.
+struct __align__(32) Res4 {
+  double a0, a1, a2, a3;
+};
+
 __global__ void __launch_bounds__(256)
 unsort_kernel(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ a0,
               const double* __restrict__ a1, const double* __restrict__ a2,
-              const double* __restrict__ a3, double* __restrict__ o0, double* __restrict__ o1,
-              double* __restrict__ o2, double* __restrict__ o3) {
+              const double* __restrict__ a3, Res4* __restrict__ stage) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   if (i >= n) return;
-  const uint32_t j = perm[i];
-  o0[j] = a0[i];
-  o1[j] = a1[i];
-  o2[j] = a2[i];
-  o3[j] = a3[i];
+  stage[perm[i]] = Res4{a0[i], a1[i], a2[i], a3[i]};
+}
+
+__global__ void __launch_bounds__(256)
+split4_kernel(const Res4* __restrict__ stage, int64_t n, double* __restrict__ o0,
+              double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3) {
+  const int64_t j = blockIdx.x * 256ll + threadIdx.x;
+  if (j >= n) return;
+  const Res4 r = stage[j];
+  o0[j] = r.a0;
+  o1[j] = r.a1;
+  o2[j] = r.a2;
+  o3[j] = r.a3;
 }
 
 // Particle::kick and Particle::perturb (src/Particle.C:7-21): the kick adds
@@ -512,9 +527,12 @@ int pt_launch_gather(const uint32_t* perm, int64_t n, const ParticleRec* rec, do
 
 int pt_launch_unsort(const uint32_t* perm, int64_t n, const double* a0, const double* a1,
                      const double* a2, const double* a3, double* o0, double* o1, double* o2,
-                     double* o3, hipStream_t s) {
+                     double* o3, double* stage, hipStream_t s) {
   if (n <= 0) return 0;
-  unsort_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, a0, a1, a2, a3, o0, o1, o2, o3);
+  Res4* st = reinterpret_cast<Res4*>(stage);
+  unsort_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, a0, a1, a2, a3, st);
+  if (hipGetLastError() != hipSuccess) return -1;
+  split4_kernel<<<grid_for(n), 256, 0, s>>>(st, n, o0, o1, o2, o3);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -187,15 +187,20 @@ def test_fused_path_matches_oracle(pair, grouping):
     check_fp32c(a32, acc, name + " fused")
 
 
-def test_walk_stack_overflow_rerun():
-    """A deep walk (30k particles in 8 halos, bucket 4, theta 0.1: a CPU
-    replay of the walk order reaches 226 pending entries) overflows the small
-    LDS walk stack; the evaluation reruns with the big stack and still
-    matches the oracle exactly (counters) and to 1e-12 (FP64)."""
+@pytest.mark.parametrize("grouping", [True, False], ids=["grouped", "per_bucket"])
+def test_walk_stack_overflow_rerun(grouping):
+    """A deep walk (30k particles in 8 halos, bucket 4, theta 0.1).  One walk
+    per bucket: a CPU replay of the walk order reaches 226 pending entries,
+    which overflows the 192-entry LDS stack, so the evaluation reruns with the
+    big stack.  Grouped: the same replay gives residual lists of up to 533
+    nodes, so 25 groups (FP32C, 512-entry stack) and 92 (FP64, 384) leave the
+    shared walk for one walk per member.  Both match the oracle exactly
+    (counters) and to 1e-12 (FP64)."""
     pos, _, m = ics.halo_mixture(30000, n_halos=8, seed=3)
     e = np.full(len(pos), 0.05)
     P = O.Pipeline(pos, m, e, bucket=4, theta=0.1)
     d = _gpu(pos, m, e, 4, 0.1)
+    d.set_fused_grouping(grouping)
     cnt = d.compute_forces(theta=0.1, precision=0)
     acc, pot = d.accelerations()
     oacc, opot, ocnt = P.gravity_unsorted()
