@@ -1605,15 +1605,172 @@ __device__ __forceinline__ void finish_member(W_t& W, const Group& g, const Lane
   __builtin_amdgcn_wave_barrier();
 }
 
-// Walk stack entries the residual list leaves to the member walks.  A group
-// whose residual list would take more is evaluated by per-bucket walks.
+// Walk stack entries the residual lists leave to the member walks.  A shared
+// walk whose residual list would take more gives up (see gravity_group).
 constexpr int kMemberReserve = 96;
+// Particles of a sub-run: after the group's shared walk, runs of consecutive
+// members of at most this many particles walk once more together below the
+// group's residual nodes.
+constexpr int kSubTargets = 16;
+
+// One shared-walk visit: each active lane's node is classified for every
+// member of the run of buckets mb (mG of them, union box ub) and lands in
+// the pending stack (opened by all), the closed-node buffer (closed by all
+// and far for every target), the leaf buffer (a leaf), or the residual list
+// growing down from res_top (anything else).  used_top: entries above
+// res_top held by enclosing residual lists.  Returns false, writing nothing,
+// when the entries would not fit with kMemberReserve to spare.
+template <bool F32, int STACK>
+__device__ __forceinline__ bool shared_visit(
+    WaveLds<F32, STACK>& W, int32_t node, bool active, const TreeNode* __restrict__ nodes,
+    const Bucket* __restrict__ mb, int mG, const Bucket& ub, const BucketF& ubf, float emax,
+    double gf, double ox, double oy, double oz, int& sp, int& npc, int& npp, int& nres,
+    int res_top, int& sh_open, int& sh_close, int& lane_leaf_parts) {
+  const TreeNode nd = nodes[node];
+  const bool leaf = active && nd.type == PT_NODE_LEAF;
+  const bool internal = active && nd.type == PT_NODE_INTERNAL;
+  const double cx = nd.cx + ox, cy = nd.cy + oy, cz = nd.cz + oz;
+  const double dsq = pt_box_point_dsq(ub, cx, cy, cz);
+  bool op = false, cl = false;
+  if (internal) {
+    if (nd.count <= PT_MIN_PARTICLE_NODE) {
+      op = true;
+    } else {
+      const double rsq = nd.radius * nd.radius * gf;
+      if (dsq > rsq) {
+        cl = true;
+      } else if (pt_box_point_far_dsq(ub, cx, cy, cz) <= rsq) {
+        op = true;
+      } else {
+        bool any_open = false, any_closed = false;
+        for (int k = 0; k < mG; k++) {
+          const bool o = pt_box_point_dsq(mb[k], cx, cy, cz) <= rsq;
+          any_open |= o;
+          any_closed |= !o;
+        }
+        op = !any_closed;
+        cl = !any_open;
+      }
+    }
+  }
+  const bool shared_close = cl && node_far_dsq(dsq, nd.node_soft, ubf, emax);
+  const bool res = internal && !op && !shared_close;
+  const unsigned long long leafm = __ballot(leaf), closem = __ballot(shared_close),
+                           openm = __ballot(op), resm = __ballot(res);
+  const int n_open = __popcll(openm), n_res = __popcll(resm);
+  const int held = STACK - res_top + nres + n_res;  // residual entries after this visit
+  if (sp + n_open + held > STACK || held > STACK - kMemberReserve) return false;
+  __builtin_amdgcn_wave_barrier();
+  if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
+  if (res) W.stack[res_top - 1 - (nres + pt_rank_in(resm))] = node;
+  if (shared_close) W.pcbuf[npc + pt_rank_in(closem)] = node | kFarNode;
+  if (leaf) {
+    const int k = npp + pt_rank_in(leafm);
+    W.ppbeg[k] = nd.begin;
+    W.ppcnt[k] = leaf_far_dsq(dsq, nd.leaf_reach, nd.leaf_soft, ubf, emax) ? (nd.count | kFarLeaf)
+                                                                            : nd.count;
+  }
+  lane_leaf_parts += leaf ? nd.count : 0;
+  sp += n_open;
+  nres += n_res;
+  npc += __popcll(closem);
+  npp += __popcll(leafm);
+  sh_open += n_open;
+  sh_close += __popcll(closem);
+  __builtin_amdgcn_wave_barrier();
+  return true;
+}
+
+// The shared walk of the run of buckets bk[mb0, mb0 + mG) (mG >= 2, their
+// particles [first, first + ntot), ntot <= 64), from the root (nseed < 0)
+// or from the nseed residual nodes below seed_top (every member visits
+// them); residual nodes go below res_top into nres.  On success the shared
+// sums are written (base < 0: finish_group, stored or added per acc) or
+// added by the lanes base + t that wrote the group's sums (base >= 0), and
+// the counters take every member's shared calls.  Returns false with nothing
+// written or counted when the lists overflow.
+template <bool F32, int STACK>
+__device__ __forceinline__ bool shared_walk(
+    WaveLds<F32, STACK>& W, const Bucket* __restrict__ bk, int64_t mb0, int mG, int seed_top,
+    int nseed, int res_top, int& nres, int base, const TreeNode* __restrict__ nodes,
+    const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ pm, const double* __restrict__ pe, double gf, double ox,
+    double oy, double oz, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double* __restrict__ pot, unsigned long long& c_node,
+    unsigned long long& c_part, unsigned long long& c_open, unsigned long long& c_close,
+    bool acc) {
+  using W_t = WaveLds<F32, STACK>;
+  const int lane = threadIdx.x & (PT_WAVE - 1);
+  const Bucket* mb = bk + mb0;
+  Bucket ub = mb[0];
+  for (int k = 1; k < mG; k++) {
+    const Bucket& m = mb[k];
+    ub.lox = fmin(ub.lox, m.lox);
+    ub.loy = fmin(ub.loy, m.loy);
+    ub.loz = fmin(ub.loz, m.loz);
+    ub.hix = fmax(ub.hix, m.hix);
+    ub.hiy = fmax(ub.hiy, m.hiy);
+    ub.hiz = fmax(ub.hiz, m.hiz);
+    ub.soft = fmax(ub.soft, m.soft);
+  }
+  const int64_t first = ub.begin;
+  const int ntot = (int)((int64_t)mb[mG - 1].begin + mb[mG - 1].count - first);
+  const double gcx = (ub.lox + ub.hix) * 0.5, gcy = (ub.loy + ub.hiy) * 0.5,
+               gcz = (ub.loz + ub.hiz) * 0.5;
+  const BucketF ubf = bucket_frame(ub, gcx, gcy, gcz);
+  const Group g = make_group_range(first, ntot);
+  Lanes<F32> L;
+  load_targets<F32>(L, g, px, py, pz, pe, gcx, gcy, gcz);
+  int sp = 0, npc = 0, npp = 0;
+  int sh_open = 0, sh_close = 0, lane_leaf_parts = 0;
+  nres = 0;
+  // seeds 64 at a time (the root alone from the top), each batch walked to
+  // the end
+  const int nbatch = nseed < 0 ? 1 : nseed;
+  for (int r0 = 0; r0 < nbatch; r0 += PT_WAVE) {
+    const bool active = nseed < 0 ? lane == 0 : r0 + lane < nseed;
+    const int32_t node = nseed < 0 ? 0 : W.stack[seed_top - 1 - (active ? r0 + lane : r0)];
+    if (!shared_visit<F32, STACK>(W, node, active, nodes, mb, mG, ub, ubf, L.emax, gf, ox, oy, oz,
+                                  sp, npc, npp, nres, res_top, sh_open, sh_close,
+                                  lane_leaf_parts))
+      return false;
+    drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+                         gcx, gcy, gcz);
+    while (sp > 0) {
+      const int k = sp < 8 ? sp : 8;
+      sp -= k;
+      const int slot = lane >> 3;
+      const bool act = slot < k;
+      const int32_t child = W.stack[sp + (act ? slot : 0)] + (lane & 7);
+      if (!shared_visit<F32, STACK>(W, child, act, nodes, mb, mG, ub, ubf, L.emax, gf, ox, oy,
+                                    oz, sp, npc, npp, nres, res_top, sh_open, sh_close,
+                                    lane_leaf_parts))
+        return false;
+      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+                           gcx, gcy, gcz);
+    }
+  }
+  drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz, gcx,
+                       gcy, gcz);
+  // every member made each shared call
+  c_open += (unsigned long long)sh_open * mG;
+  c_close += (unsigned long long)sh_close * mG;
+  c_node += (unsigned long long)sh_close * ntot;
+  c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * ntot;
+  if (base < 0) finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
+  else finish_member<F32, W_t>(W, g, L, base, ax, ay, az, pot);
+  return true;
+}
 
 // One group of G >= 2 buckets starting at bucket b0 (every member holds at
-// most 63 particles, all of them at most 64).  Returns false, having written
-// nothing and counted nothing, when the shared walk's pending and residual
-// entries do not fit the stack with kMemberReserve entries to spare; the
-// caller then walks each member on its own.
+// most 63 particles, all of them at most 64): the group's shared walk, then
+// per sub-run of at most kSubTargets particles a shared walk below the
+// group's residual nodes, then each member below what is left residual
+// (below the group's residual nodes when its sub-run is a single bucket or
+// its shared walk gave up).  Returns false, having written nothing and
+// counted nothing, when the group's shared walk gives up; the caller then
+// walks each member on its own.
 template <bool F32, int STACK>
 __device__ __forceinline__ bool gravity_group(
     WaveLds<F32, STACK>& W, int64_t b0, int G, const Bucket* __restrict__ bk,
@@ -1626,158 +1783,79 @@ __device__ __forceinline__ bool gravity_group(
     unsigned long long& c_open, unsigned long long& c_close, bool& overflow, bool acc) {
   using W_t = WaveLds<F32, STACK>;
   const int lane = threadIdx.x & (PT_WAVE - 1);
-  // union box, largest member softening, target range
-  Bucket ub = bk[b0];
-  for (int k = 1; k < G; k++) {
-    const Bucket& m = bk[b0 + k];
-    ub.lox = fmin(ub.lox, m.lox);
-    ub.loy = fmin(ub.loy, m.loy);
-    ub.loz = fmin(ub.loz, m.loz);
-    ub.hix = fmax(ub.hix, m.hix);
-    ub.hiy = fmax(ub.hiy, m.hiy);
-    ub.hiz = fmax(ub.hiz, m.hiz);
-    ub.soft = fmax(ub.soft, m.soft);
-  }
-  const int64_t first = ub.begin;
-  const int ntot = (int)((int64_t)bk[b0 + G - 1].begin + bk[b0 + G - 1].count - first);
-  const double gcx = (ub.lox + ub.hix) * 0.5, gcy = (ub.loy + ub.hiy) * 0.5,
-               gcz = (ub.loz + ub.hiz) * 0.5;
-  const BucketF ubf = bucket_frame(ub, gcx, gcy, gcz);
+  const int64_t first = bk[b0].begin;
   int nres = 0;
-  {
-    // ---- shared walk
-    const Group g = make_group_range(first, ntot);
-    Lanes<F32> L;
-    load_targets<F32>(L, g, px, py, pz, pe, gcx, gcy, gcz);
-    int sp = 0, npc = 0, npp = 0;
-    int sh_open = 0, sh_close = 0, lane_leaf_parts = 0;
-    for (bool root = true; root || sp > 0; root = false) {
-      int32_t node;
-      bool active;
-      if (root) {
-        node = 0;
-        active = lane == 0;
-      } else {
-        const int k = sp < 8 ? sp : 8;
-        sp -= k;
-        const int slot = lane >> 3;
-        active = slot < k;
-        node = W.stack[sp + (active ? slot : 0)] + (lane & 7);
-      }
-      const TreeNode nd = nodes[node];
-      const bool leaf = active && nd.type == PT_NODE_LEAF;
-      const bool internal = active && nd.type == PT_NODE_INTERNAL;
-      const double cx = nd.cx + ox, cy = nd.cy + oy, cz = nd.cz + oz;
-      const double dsq = pt_box_point_dsq(ub, cx, cy, cz);
-      bool op = false, cl = false;
-      if (internal) {
-        if (nd.count <= PT_MIN_PARTICLE_NODE) {
-          op = true;
-        } else {
-          const double rsq = nd.radius * nd.radius * gf;
-          if (dsq > rsq) {
-            cl = true;
-          } else if (pt_box_point_far_dsq(ub, cx, cy, cz) <= rsq) {
-            op = true;
-          } else {
-            bool any_open = false, any_closed = false;
-            for (int k = 0; k < G; k++) {
-              const bool o = pt_box_point_dsq(bk[b0 + k], cx, cy, cz) <= rsq;
-              any_open |= o;
-              any_closed |= !o;
-            }
-            op = !any_closed;
-            cl = !any_open;
-          }
-        }
-      }
-      const bool shared_close = cl && node_far_dsq(dsq, nd.node_soft, ubf, L.emax);
-      const bool res = internal && !op && !shared_close;
-      const unsigned long long leafm = __ballot(leaf), closem = __ballot(shared_close),
-                               openm = __ballot(op), resm = __ballot(res);
-      const int n_open = __popcll(openm), n_res = __popcll(resm);
-      if (sp + n_open + nres + n_res > STACK || nres + n_res > STACK - kMemberReserve)
-        return false;
-      __builtin_amdgcn_wave_barrier();
-      if (op) W.stack[sp + pt_rank_in(openm)] = nd.first_child;
-      if (res) W.stack[STACK - 1 - (nres + pt_rank_in(resm))] = node;
-      if (shared_close) W.pcbuf[npc + pt_rank_in(closem)] = node | kFarNode;
-      if (leaf) {
-        const int k = npp + pt_rank_in(leafm);
-        W.ppbeg[k] = nd.begin;
-        W.ppcnt[k] = leaf_far_dsq(dsq, nd.leaf_reach, nd.leaf_soft, ubf, L.emax)
-                         ? (nd.count | kFarLeaf)
-                         : nd.count;
-      }
-      lane_leaf_parts += leaf ? nd.count : 0;
-      sp += n_open;
-      nres += n_res;
-      npc += __popcll(closem);
-      npp += __popcll(leafm);
-      sh_open += n_open;
-      sh_close += __popcll(closem);
-      __builtin_amdgcn_wave_barrier();
-      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
-                           gcx, gcy, gcz);
-    }
-    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
-                         gcx, gcy, gcz);
-    // every member made each shared call
-    c_open += (unsigned long long)sh_open * G;
-    c_close += (unsigned long long)sh_close * G;
-    c_node += (unsigned long long)sh_close * ntot;
-    c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * ntot;
-    finish_group<F32, W_t>(W, g, L, ax, ay, az, pot, acc);
-  }
+  if (!shared_walk<F32, STACK>(W, bk, b0, G, STACK, -1, STACK, nres, -1, nodes, gn, gn32, px, py,
+                               pz, pm, pe, gf, ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
+                               c_close, acc))
+    return false;
   if (nres == 0) return true;
-  // ---- each member from the residual nodes
-  for (int k = 0; k < G; k++) {
-    const Bucket b = bk[b0 + k];
-    const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
-                 ccz = (b.loz + b.hiz) * 0.5;
-    const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
-    const Group g = make_group(b, 0);
-    Lanes<F32> L;
-    load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
-    const int limit = STACK - nres;  // >= kMemberReserve
-    int sp = 0, npc = 0, npp = 0;
-    int w_open = 0, w_close = 0, lane_leaf_parts = 0;
-    // 64 residual nodes at a time, each batch walked to the end, so the
-    // member's stack holds at most one batch plus what its walk pushes
-    for (int r0 = 0; r0 < nres; r0 += PT_WAVE) {
-      const bool active = r0 + lane < nres;
-      const int32_t node = W.stack[STACK - 1 - (active ? r0 + lane : r0)];
-      const TreeNode nd = nodes[node];
-      const double dsq = pt_box_point_dsq(b, nd.cx + ox, nd.cy + oy, nd.cz + oz);
-      const bool op = active && pt_gv_open_dsq(nd, gf, dsq);
-      const bool cl = active && !op;
-      const unsigned long long openm = __ballot(op), closem = __ballot(cl);
-      const int n_open = __popcll(openm);
-      __builtin_amdgcn_wave_barrier();
-      if (op) W.stack[pt_rank_in(openm)] = nd.first_child;
-      if (cl)
-        W.pcbuf[npc + pt_rank_in(closem)] =
-            node_far_dsq(dsq, nd.node_soft, bf, L.emax) ? (node | kFarNode) : node;
-      sp = n_open;
-      npc += __popcll(closem);
-      w_open += n_open;
-      w_close += __popcll(closem);
-      __builtin_amdgcn_wave_barrier();
-      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
-                           ccx, ccy, ccz);
-      walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz,
-                             pm, pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close,
-                             lane_leaf_parts, overflow);
-      if (overflow) return true;
+  for (int k0 = 0, k1 = 0; k0 < G; k0 = k1) {
+    // the sub-run [k0, k1)
+    int tot = bk[b0 + k0].count;
+    for (k1 = k0 + 1; k1 < G && tot + bk[b0 + k1].count <= kSubTargets; k1++)
+      tot += bk[b0 + k1].count;
+    int seed_top = STACK, nseed = nres;
+    if (k1 - k0 >= 2) {
+      int nres2 = 0;
+      if (shared_walk<F32, STACK>(W, bk, b0 + k0, k1 - k0, STACK, nres, STACK - nres, nres2,
+                                  (int)(bk[b0 + k0].begin - first), nodes, gn, gn32, px, py, pz,
+                                  pm, pe, gf, ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
+                                  c_close, acc)) {
+        seed_top = STACK - nres;
+        nseed = nres2;
+      }
     }
-    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
-                         ccy, ccz);
-    const int64_t nt = b.count;
-    c_open += (unsigned)w_open;
-    c_close += (unsigned)w_close;
-    c_node += (unsigned long long)w_close * nt;
-    c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
-    finish_member<F32, W_t>(W, g, L, (int)(b.begin - first), ax, ay, az, pot);
+    if (nseed == 0) continue;
+    // ---- each member of the sub-run from the residual nodes it was left
+    for (int k = k0; k < k1; k++) {
+      const Bucket b = bk[b0 + k];
+      const double ccx = (b.lox + b.hix) * 0.5, ccy = (b.loy + b.hiy) * 0.5,
+                   ccz = (b.loz + b.hiz) * 0.5;
+      const BucketF bf = bucket_frame(b, ccx, ccy, ccz);
+      const Group g = make_group(b, 0);
+      Lanes<F32> L;
+      load_targets<F32>(L, g, px, py, pz, pe, ccx, ccy, ccz);
+      const int limit = seed_top - nseed;  // >= kMemberReserve
+      int sp = 0, npc = 0, npp = 0;
+      int w_open = 0, w_close = 0, lane_leaf_parts = 0;
+      // 64 residual nodes at a time, each batch walked to the end, so the
+      // member's stack holds at most one batch plus what its walk pushes
+      for (int r0 = 0; r0 < nseed; r0 += PT_WAVE) {
+        const bool active = r0 + lane < nseed;
+        const int32_t node = W.stack[seed_top - 1 - (active ? r0 + lane : r0)];
+        const TreeNode nd = nodes[node];
+        const double dsq = pt_box_point_dsq(b, nd.cx + ox, nd.cy + oy, nd.cz + oz);
+        const bool op = active && pt_gv_open_dsq(nd, gf, dsq);
+        const bool cl = active && !op;
+        const unsigned long long openm = __ballot(op), closem = __ballot(cl);
+        const int n_open = __popcll(openm);
+        __builtin_amdgcn_wave_barrier();
+        if (op) W.stack[pt_rank_in(openm)] = nd.first_child;
+        if (cl)
+          W.pcbuf[npc + pt_rank_in(closem)] =
+              node_far_dsq(dsq, nd.node_soft, bf, L.emax) ? (node | kFarNode) : node;
+        sp = n_open;
+        npc += __popcll(closem);
+        w_open += n_open;
+        w_close += __popcll(closem);
+        __builtin_amdgcn_wave_barrier();
+        drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
+                             ccx, ccy, ccz);
+        walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz,
+                               pm, pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close,
+                               lane_leaf_parts, overflow);
+        if (overflow) return true;
+      }
+      drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
+                           ccx, ccy, ccz);
+      const int64_t nt = b.count;
+      c_open += (unsigned)w_open;
+      c_close += (unsigned)w_close;
+      c_node += (unsigned long long)w_close * nt;
+      c_part += (unsigned long long)pt_wave_sum_int(lane_leaf_parts) * nt;
+      finish_member<F32, W_t>(W, g, L, (int)(b.begin - first), ax, ay, az, pot);
+    }
   }
   return true;
 }
@@ -1858,15 +1936,16 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
 // for an upper bound of the group count and the surplus waves find no group.
 // Waves per work-group of the grouped kernel, and its walk stack (pending
 // entries and, from the top, the residual list) per precision.  FP32C: 512
-// entries (five one-wave work-groups per SIMD fit in LDS, 7.9 KiB each; the
-// six-wave register budget, 80 VGPRs and a small spill, measured 3.5% faster
-// than five waves' 96 VGPRs); FP64: 384.
+// entries (7.9 KiB of LDS per one-wave work-group, five per SIMD would fit);
+// the kernel's ~105 VGPRs keep four waves per SIMD without spilling, which
+// measured 3.5% faster than a five-wave register budget (95 VGPRs and a
+// spill).  FP64: 384.
 constexpr int kGWaves = 1;
 constexpr int kGroupStack32 = 512;
 constexpr int kGroupStack64 = 384;
 
 template <bool F32, int STACK, bool OFF>
-__global__ void __launch_bounds__(kGWaves * PT_WAVE) __attribute__((amdgpu_waves_per_eu(F32 && STACK == kGroupStack32 ? 6 : 1, 8)))
+__global__ void __launch_bounds__(kGWaves * PT_WAVE)
 grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
                        const int32_t* __restrict__ gnb, const uint32_t* __restrict__ ngroups,
                        const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
