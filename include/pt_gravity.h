@@ -297,13 +297,16 @@ pt_status pt_get_accelerations(pt_ctx* ctx, double* acc, double* pot);
 pt_status pt_get_state(pt_ctx* ctx, double* pos, double* vel);       /* n x 3 each */
 pt_status pt_get_counters(pt_ctx* ctx, pt_counters* out);
 /* How pt_compute_forces walks the quadrupole build (no reference counterpart;
- * both give every bucket exactly the open()/node()/leaf() calls of
- * TransposedDownTraverser, src/Traverser.h:154-217).  Enabled (the default):
- * runs of consecutive buckets of at most 64 particles walk once with their
- * union box and share the sources every member closes or reaches as leaves,
- * each member then walking on its own below the nodes the members decide
- * differently.  Disabled: one walk per bucket. */
-pt_status pt_set_fused_grouping(pt_ctx* ctx, int enabled);
+ * every mode gives every bucket exactly the open()/node()/leaf() calls of
+ * TransposedDownTraverser, src/Traverser.h:154-217).  Grouped: runs of
+ * consecutive buckets of at most `cap` particles walk once with their union
+ * box and share the sources every member closes or reaches as leaves, then
+ * sub-runs of at most 16 particles the same way, each bucket walking on its
+ * own only below the nodes its run decided differently.  mode 0: one walk
+ * per bucket; 1 (the default): automatic (the cap from the particle count
+ * and the device's compute units, one walk per bucket for small sets);
+ * 2..64: grouped with that cap. */
+pt_status pt_set_fused_grouping(pt_ctx* ctx, int mode);
 pt_status pt_set_timing(pt_ctx* ctx, int enabled);
 pt_status pt_get_timings(pt_ctx* ctx, float ms[PT_T_COUNT]);
 

@@ -93,8 +93,10 @@ struct pt_ctx {
   DBuf<int32_t> ewcount;
   DBuf<unsigned long long> cpart;  // per-block counter partials
   DBuf<unsigned long long> wqueue;  // per-XCD work counters of the fused kernel
-  // bucket groups of the grouped fused kernel (pt_set_fused_grouping)
-  bool fused_grouping = true;
+  // bucket groups of the grouped fused kernel (pt_set_fused_grouping): 0 one
+  // walk per bucket, 1 automatic, 2..64 the groups' particle cap
+  int32_t fused_grouping = 1;
+  int32_t n_cus = 0;  // compute units of the device (0: not read yet)
   DBuf<int32_t> fg_tfirst, fg_tnb, fg_first, fg_nb;
   DBuf<uint32_t> fg_wcount, fg_woff, fg_ng;
   bool lists_valid = false;
@@ -956,6 +958,30 @@ pt_status pt_start_dual(pt_ctx* c, double theta, const double offset[3], uint64_
 
 // Fused walk + interactions (no lists), with a large-stack rerun if the
 // small walk stack (192 entries) overflowed for some bucket.
+// The particle cap of the grouped walk's bucket groups, 0 for one walk per
+// bucket.  Automatic: the largest of 64, 32 and 16 that still gives every
+// resident wave slot (four per SIMD) about six groups, 16 for smaller sets
+// down to one group per slot, and one walk per bucket below that (larger
+// groups share more of the walk, more groups keep the device busy to the
+// end).  Measured on 100k, 256k and 1M Plummer spheres and the 16M halo
+// mixture, this picks the fastest of those caps in each case.
+static int group_cap(pt_ctx* c) {
+  if (c->fused_grouping == 0) return 0;
+  if (c->fused_grouping >= 2) return c->fused_grouping;
+  if (c->n_cus == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    c->n_cus = cus;
+  }
+  const int64_t slots = (int64_t)c->n_cus * 16;
+  for (int cap = 64; cap >= 32; cap /= 2)
+    if (c->n >= 6 * cap * slots) return cap;
+  return c->n >= 16 * slots ? 16 : 0;
+}
+
 static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
                                pt_precision precision, pt_counters* counters) {
   pt_status st = check_ctx(c);
@@ -967,10 +993,12 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (precision != PT_FP64 && (st = check_fp32c_range(c))) return st;
   const int64_t nb = c->nb;
   const bool hexa = c->cfg.hexadecapole != 0;
-  const bool grouped = !hexa && c->fused_grouping;
+  const int cap = hexa ? 0 : group_cap(c);
+  const bool grouped = cap > 0;
   const int64_t nblk = std::max<int64_t>(
       hexa ? pt_hexa_gravity_blocks(nb)
-           : (grouped ? pt_grouped_gravity_blocks(nb, c->n) : pt_gravity_blocks(nb, (int)precision)),
+           : (grouped ? pt_grouped_gravity_blocks(nb, c->n, cap)
+                      : pt_gravity_blocks(nb, (int)precision)),
       1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
   PT_TRY_HIP(c, ensure(c->wqueue, 8));
@@ -986,7 +1014,7 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
     PT_TRY_HIP(c, ensure(c->fg_ng, 1));
     if ((st = ensure_scratch(c, pt_scan_scratch_bytes(nw)))) return st;
     Timer t(c, PT_T_TRAVERSE);
-    PT_TRY_LAUNCH(c, pt_launch_fgroups(c->bk.p, nb, c->fg_tfirst.p, c->fg_tnb.p, c->fg_wcount.p,
+    PT_TRY_LAUNCH(c, pt_launch_fgroups(c->bk.p, nb, cap, c->fg_tfirst.p, c->fg_tnb.p, c->fg_wcount.p,
                                        c->fg_woff.p, c->scratch.p, c->fg_ng.p, c->fg_first.p,
                                        c->fg_nb.p, c->stream));
   }
@@ -1028,7 +1056,7 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
         else if (grouped)
           PT_TRY_LAUNCH(c, pt_launch_grouped_gravity(
                                (int)precision, attempt == 1, c->bk.p, c->fg_first.p, c->fg_nb.p,
-                               c->fg_ng.p, nb, c->n, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
+                               c->fg_ng.p, nb, c->n, cap, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
                                c->ys.p, c->zs.p, c->ms.p, c->es.p, gf, off, c->axs.p, c->ays.p,
                                c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc, c->stream));
         else
@@ -1715,10 +1743,11 @@ pt_status pt_get_counters(pt_ctx* c, pt_counters* out) {
   return PT_OK;
 }
 
-pt_status pt_set_fused_grouping(pt_ctx* c, int enabled) {
+pt_status pt_set_fused_grouping(pt_ctx* c, int mode) {
   pt_status st = check_ctx(c);
   if (st) return st;
-  c->fused_grouping = enabled != 0;
+  if (mode < 0 || mode > 64) return fail(c, PT_ERR_ARG, "pt_set_fused_grouping: mode 0..64");
+  c->fused_grouping = mode;
   return PT_OK;
 }
 

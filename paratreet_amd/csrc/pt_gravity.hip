@@ -1796,7 +1796,7 @@ __device__ __forceinline__ bool gravity_group(
     for (k1 = k0 + 1; k1 < G && tot + bk[b0 + k1].count <= kSubTargets; k1++)
       tot += bk[b0 + k1].count;
     int seed_top = STACK, nseed = nres;
-    if (k1 - k0 >= 2) {
+    if (k1 - k0 >= 2 && k1 - k0 < G) {
       int nres2 = 0;
       if (shared_walk<F32, STACK>(W, bk, b0 + k0, k1 - k0, STACK, nres, STACK - nres, nres2,
                                   (int)(bk[b0 + k0].begin - first), nodes, gn, gn32, px, py, pz,
@@ -2002,15 +2002,16 @@ grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
 
 // Bucket groups of the grouped kernel, formed greedily inside windows of
 // kFgWindow consecutive buckets: a group closes when the next bucket would
-// take it past 64 particles, so a bucket of more than 64 is a group of its
-// own.  One wave per window: the lanes load the window's counts, the greedy
+// take it past cap (<= 64) particles, so a bucket of more than cap is a
+// group of its own.  One wave per window: the lanes load the window's counts, the greedy
 // pass runs on wave-uniform values.  Window w writes its groups to slots
 // [w kFgWindow, w kFgWindow + wcount[w]) of tfirst / tnb.
 constexpr int kFgWindow = PT_WAVE;
 
 __global__ void __launch_bounds__(256)
-fgroup_window_kernel(const Bucket* __restrict__ bk, int64_t nb, int32_t* __restrict__ tfirst,
-                     int32_t* __restrict__ tnb, uint32_t* __restrict__ wcount) {
+fgroup_window_kernel(const Bucket* __restrict__ bk, int64_t nb, int cap,
+                     int32_t* __restrict__ tfirst, int32_t* __restrict__ tnb,
+                     uint32_t* __restrict__ wcount) {
   const int64_t w = (int64_t)blockIdx.x * 4 + threadIdx.x / PT_WAVE;
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t nw = (nb + kFgWindow - 1) / kFgWindow;
@@ -2021,7 +2022,7 @@ fgroup_window_kernel(const Bucket* __restrict__ bk, int64_t nb, int32_t* __restr
   int slot = 0, start = 0, tot = 0;
   for (int k = 0; k < n; k++) {
     const int c = __builtin_amdgcn_readlane(cnt, k);
-    if (k > start && tot + c > PT_WAVE) {
+    if (k > start && tot + c > cap) {
       if (lane == 0) {
         tfirst[b0 + slot] = (int32_t)(b0 + start);
         tnb[b0 + slot] = k - start;
@@ -2170,23 +2171,25 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
 
 int64_t pt_fgroup_windows(int64_t nb) { return (nb + kFgWindow - 1) / kFgWindow; }
 
-// Two consecutive groups of a window hold more than 64 particles together,
-// so a window of T particles has at most 2 T / 65 + 1 groups.
-int64_t pt_fgroup_max_groups(int64_t nb, int64_t n) {
-  return pt_fgroup_windows(nb) + 2 * n / 65 + 1;
+// Two consecutive groups of a window hold more than cap particles together,
+// so a window of T particles has at most 2 T / (cap + 1) + 1 groups.
+int64_t pt_fgroup_max_groups(int64_t nb, int64_t n, int cap) {
+  return pt_fgroup_windows(nb) + 2 * n / (cap + 1) + 1;
 }
 
-int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n) {
-  const int64_t per_xcd = (pt_fgroup_max_groups(nb, n) + 7) / 8 + 1;
+int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n, int cap) {
+  const int64_t per_xcd = (pt_fgroup_max_groups(nb, n, cap) + 7) / 8 + 1;
   return 8 * ((per_xcd + kGWaves - 1) / kGWaves);
 }
 
-int pt_launch_fgroups(const Bucket* bk, int64_t nb, int32_t* tfirst, int32_t* tnb,
+int pt_launch_fgroups(const Bucket* bk, int64_t nb, int cap, int32_t* tfirst, int32_t* tnb,
                       uint32_t* wcount, uint32_t* woff, void* scratch, uint32_t* ngroups,
                       int32_t* gfirst, int32_t* gnb, hipStream_t s) {
   if (nb <= 0) return 0;
+  if (cap < 2 || cap > PT_WAVE) return -1;
   const int64_t nw = pt_fgroup_windows(nb);
-  fgroup_window_kernel<<<(unsigned)((nw + 3) / 4), 256, 0, s>>>(bk, nb, tfirst, tnb, wcount);
+  fgroup_window_kernel<<<(unsigned)((nw + 3) / 4), 256, 0, s>>>(bk, nb, cap, tfirst, tnb,
+                                                                 wcount);
   if (hipGetLastError() != hipSuccess) return -1;
   if (pt_exclusive_scan_u32(wcount, woff, nw, scratch, ngroups, s) != 0) return -1;
   fgroup_compact_kernel<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(nb, tfirst, tnb, wcount,
@@ -2196,14 +2199,15 @@ int pt_launch_fgroups(const Bucket* bk, int64_t nb, int32_t* tfirst, int32_t* tn
 
 int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
                               const int32_t* gfirst, const int32_t* gnb, const uint32_t* ngroups,
-                              int64_t nb, int64_t n, const TreeNode* nodes, const GravNode* gn,
+                              int64_t nb, int64_t n, int cap, const TreeNode* nodes,
+                              const GravNode* gn,
                               const GravNode32* gn32, const double* x, const double* y,
                               const double* z, const double* m, const double* e, double gf,
                               const double off[3], double* ax, double* ay, double* az,
                               double* pot, unsigned long long* partials, int32_t* err,
                               bool accumulate, hipStream_t s) {
   if (nb <= 0) return 0;
-  const unsigned grid = (unsigned)pt_grouped_gravity_blocks(nb, n);
+  const unsigned grid = (unsigned)pt_grouped_gravity_blocks(nb, n, cap);
   const bool has_off = off[0] != 0.0 || off[1] != 0.0 || off[2] != 0.0;
 #define PT_GROUPED_LAUNCH(F32, STACK, OFF)                                                     \
   grouped_gravity_kernel<F32, STACK, OFF><<<grid, kGWaves * PT_WAVE, 0, s>>>(                           \

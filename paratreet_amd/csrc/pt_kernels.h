@@ -145,23 +145,24 @@ int pt_launch_gravity(int precision, bool big_stack, const Bucket* bk, int64_t n
                       unsigned long long* queue, bool accumulate, hipStream_t s);
 
 // Bucket groups of the grouped fused kernel: runs of consecutive buckets of
-// at most 64 particles in all (a larger bucket alone), formed inside windows
-// of 64 buckets and compacted in order into gfirst / gnb, their number in
-// *ngroups (device).  tfirst / tnb / gfirst / gnb hold nb entries,
+// at most cap (2..64) particles in all (a larger bucket alone), formed inside
+// windows of 64 buckets and compacted in order into gfirst / gnb, their
+// number in *ngroups (device).  tfirst / tnb / gfirst / gnb hold nb entries,
 // wcount / woff pt_fgroup_windows(nb); scratch: pt_scan_scratch_bytes of the
 // window count.
 int64_t pt_fgroup_windows(int64_t nb);
-int64_t pt_fgroup_max_groups(int64_t nb, int64_t n);
-int pt_launch_fgroups(const Bucket* bk, int64_t nb, int32_t* tfirst, int32_t* tnb,
+int64_t pt_fgroup_max_groups(int64_t nb, int64_t n, int cap);
+int pt_launch_fgroups(const Bucket* bk, int64_t nb, int cap, int32_t* tfirst, int32_t* tnb,
                       uint32_t* wcount, uint32_t* woff, void* scratch, uint32_t* ngroups,
                       int32_t* gfirst, int32_t* gnb, hipStream_t s);
 // The fused walk + interactions with the shared walk per bucket group; the
 // same calls, counters and partials layout as pt_launch_gravity (partials:
-// pt_grouped_gravity_blocks(nb, n) work-groups).
-int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n);
+// pt_grouped_gravity_blocks(nb, n, cap) work-groups).
+int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n, int cap);
 int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
                               const int32_t* gfirst, const int32_t* gnb, const uint32_t* ngroups,
-                              int64_t nb, int64_t n, const TreeNode* nodes, const GravNode* gn,
+                              int64_t nb, int64_t n, int cap, const TreeNode* nodes,
+                              const GravNode* gn,
                               const GravNode32* gn32, const double* x, const double* y,
                               const double* z, const double* m, const double* e, double gf,
                               const double off[3], double* ax, double* ay, double* az,

@@ -436,10 +436,11 @@ class Driver:
         self._check(self.L.pt_get_state(self.ctx, _ptr(pos), _ptr(vel)))
         return pos, vel
 
-    def set_fused_grouping(self, on=True):
-        """Grouped (default) or per-bucket walk of compute_forces; the same
-        calls and counters either way."""
-        self._check(self.L.pt_set_fused_grouping(self.ctx, 1 if on else 0))
+    def set_fused_grouping(self, mode=True):
+        """The walk of compute_forces: True / 1 automatic (the default),
+        False / 0 one walk per bucket, 2..64 bucket groups of at most that
+        many particles; the same calls and counters in every mode."""
+        self._check(self.L.pt_set_fused_grouping(self.ctx, int(mode)))
 
     def set_timing(self, on=True):
         self._check(self.L.pt_set_timing(self.ctx, 1 if on else 0))

@@ -113,11 +113,13 @@ def test_dual_pairs_match_oracle(case):
             check_fp32c(a32[nz], acc[nz], label)
 
 
-def test_dual_tree_config_adds_to_down_traversal(case):
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
+def test_dual_tree_config_adds_to_down_traversal(case, grouping):
     from paratreet_amd.gravity import Configuration, Driver
     name, P, _, (pos, m, e, bucket) = case
     d = Driver(Configuration(max_particles_per_leaf=bucket, dual_tree=True))
     d.load(pos, m, e)
+    d.set_fused_grouping(grouping)
     cnt = d.compute_forces(theta=0.7, precision=0)
     acc, pot = d.accelerations()
     cnt32 = d.compute_forces(theta=0.7, precision=1)

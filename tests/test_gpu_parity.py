@@ -161,12 +161,12 @@ def test_forces_fp32c_precision(pair):
     check_fp32c(a32, a64, name + " (same lists)")
 
 
-@pytest.mark.parametrize("grouping", [True, False], ids=["grouped", "per_bucket"])
+@pytest.mark.parametrize("grouping", [64, 24, 0], ids=["grouped64", "grouped24", "per_bucket"])
 def test_fused_path_matches_oracle(pair, grouping):
     """pt_compute_forces (fused walk + interactions, no lists) against the
-    oracle, with the grouped walk (bucket runs sharing a walk) and with one
-    walk per bucket: counters exact, FP64 within 1e-12, FP32C within its
-    bounds."""
+    oracle, with the grouped walk (bucket runs of at most 64 or 24 particles
+    sharing a walk, then sub-runs of at most 16) and with one walk per
+    bucket: counters exact, FP64 within 1e-12, FP32C within its bounds."""
     name, P, d, _, _ = pair
     d.set_fused_grouping(grouping)
     try:
@@ -175,7 +175,7 @@ def test_fused_path_matches_oracle(pair, grouping):
         cnt32 = d.compute_forces(theta=P.theta, precision=1)
         a32, _ = d.accelerations()
     finally:
-        d.set_fused_grouping(True)
+        d.set_fused_grouping(1)
     oacc, opot, ocnt = P.gravity_unsorted()
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt), name
     e = rel_err(acc, oacc)
@@ -187,7 +187,7 @@ def test_fused_path_matches_oracle(pair, grouping):
     check_fp32c(a32, acc, name + " fused")
 
 
-@pytest.mark.parametrize("grouping", [True, False], ids=["grouped", "per_bucket"])
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
 def test_walk_stack_overflow_rerun(grouping):
     """A deep walk (30k particles in 8 halos, bucket 4, theta 0.1).  One walk
     per bucket: a CPU replay of the walk order reaches 226 pending entries,

@@ -22,8 +22,12 @@ def _driver(bucket, period, n_rep, hexa=False):
                                 nReplicas=n_rep, hexadecapole=hexa))
 
 
-@pytest.mark.parametrize("hexa", [False, True])
-def test_periodic_replicas_match_oracle(hexa):
+@pytest.mark.parametrize("hexa,grouping", [(False, 64), (False, 0), (True, 1)],
+                         ids=["quad-grouped64", "quad-per_bucket", "hexa"])
+def test_periodic_replicas_match_oracle(hexa, grouping):
+    """The replica sum (each replica's walk adding to the last) and the Ewald
+    correction against the oracle; the quadrupole build with the grouped walk
+    and with one walk per bucket (the HEXADECAPOLE build walks per bucket)."""
     rng = np.random.default_rng(5)
     n = 4000
     pos = rng.uniform(0.0, 2.0, (n, 3))
@@ -35,6 +39,7 @@ def test_periodic_replicas_match_oracle(hexa):
     oacc, opot, ocnt = P.gravity_replicas(1, period, ewald=True)
     d = _driver(8, period, 1, hexa)
     d.load(pos, m, e)
+    d.set_fused_grouping(grouping)
     cnt = d.compute_forces(theta=0.7, precision=0)
     acc, pot = d.accelerations()
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)

@@ -89,18 +89,24 @@ def test_lists_counters_and_fp64(case):
     check_fp32c(a32, acc, name + " list")
 
 
-def test_fused_fp64_and_fp32c(case):
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
+def test_fused_fp64_and_fp32c(case, grouping):
     name, P, d, theta = case
     oacc, opot, ocnt = P.gravity_unsorted()
-    cnt = d.compute_forces(theta=theta, precision=0)
-    acc, pot = d.accelerations()
+    d.set_fused_grouping(grouping)
+    try:
+        cnt = d.compute_forces(theta=theta, precision=0)
+        acc, pot = d.accelerations()
+        cnt32 = d.compute_forces(theta=theta, precision=1)
+        a32, _ = d.accelerations()
+    finally:
+        d.set_fused_grouping(1)
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt), name
     e = rel_err(acc, oacc)
     ep = np.abs(pot - opot) / np.abs(opot)
-    print("%s fused fp64 vs oracle: max rel acc %.3g pot %.3g" % (name, e.max(), ep.max()))
+    print("%s fused fp64 vs oracle (grouping %d): max rel acc %.3g pot %.3g" % (
+        name, grouping, e.max(), ep.max()))
     assert e.max() <= FP64_TOL and ep.max() <= FP64_TOL, name
-    cnt32 = d.compute_forces(theta=theta, precision=1)
-    a32, _ = d.accelerations()
     assert tuple(int(v) for v in cnt32) == tuple(int(v) for v in cnt), name
     check_fp32c(a32, acc, name + " fused")
 
