@@ -18,6 +18,7 @@
 namespace {
 
 constexpr int kMaxBoxParts = 1024;
+constexpr size_t kPinBytes = 512;  // pinned host words of the per-evaluation read-backs
 
 enum Stage {
   S_NONE = 0,
@@ -133,6 +134,10 @@ struct pt_ctx {
   DBuf<unsigned long long> dual_state;
   // scratch
   DBuf<char> scratch;
+  // the per-evaluation read-backs (tree sizes, counters): pinned host words
+  // and an event the host polls
+  void* h_pin = nullptr;
+  hipEvent_t sync_ev = nullptr;
   // timing
   bool timing = false;
   hipEvent_t ev[PT_T_COUNT][2] = {};
@@ -236,6 +241,24 @@ struct Timer {
   }
 };
 
+// Copies `bytes` (at most kPinBytes) from the device after everything queued
+// on the stream, through the pinned words, the host polling an event (a
+// pageable destination would make the runtime stage the copy and block).
+hipError_t read_back(pt_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c->h_pin || !c->sync_ev || bytes > kPinBytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+  }
+  hipError_t e = hipMemcpyAsync(c->h_pin, src, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->sync_ev, c->stream);
+  if (e != hipSuccess) return e;
+  while ((e = hipEventQuery(c->sync_ev)) == hipErrorNotReady) {
+  }
+  if (e != hipSuccess) return e;
+  memcpy(dst, c->h_pin, bytes);
+  return hipSuccess;
+}
+
 pt_status check_ctx(pt_ctx* c) {
   if (!c) return PT_ERR_ARG;
   hipError_t e = hipSetDevice(c->device);
@@ -323,6 +346,9 @@ pt_status pt_ctx_create(const pt_config* cfg, int device, void* stream, pt_ctx**
     (void)hipEventCreate(&c->ev[i][0]);
     (void)hipEventCreate(&c->ev[i][1]);
   }
+  if (hipHostMalloc(&c->h_pin, kPinBytes, hipHostMallocDefault) != hipSuccess) c->h_pin = nullptr;
+  if (hipEventCreateWithFlags(&c->sync_ev, hipEventDisableTiming) != hipSuccess)
+    c->sync_ev = nullptr;
   if (ensure(c->box, 6) != hipSuccess || ensure(c->bbox, 6) != hipSuccess ||
       ensure(c->box_parts, 6 * kMaxBoxParts) != hipSuccess || ensure(c->counters, 5) != hipSuccess ||
       ensure(c->derr, 1) != hipSuccess || ensure(c->u32_total, 4) != hipSuccess ||
@@ -372,6 +398,8 @@ void pt_ctx_destroy(pt_ctx* c) {
     if (c->ev[i][0]) (void)hipEventDestroy(c->ev[i][0]);
     if (c->ev[i][1]) (void)hipEventDestroy(c->ev[i][1]);
   }
+  if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -570,9 +598,7 @@ pt_status pt_build_tree(pt_ctx* c) {
                                         c->ms.p, c->es.p, c->stream));
     }
     int64_t stat[3 + PT_MAX_DEPTH + 2] = {0};
-    PT_TRY_HIP(c, hipMemcpyAsync(stat, c->tstat.p, sizeof(stat), hipMemcpyDeviceToHost,
-                                 c->stream));
-    PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    PT_TRY_HIP(c, read_back(c, stat, c->tstat.p, sizeof(stat)));
     if (c->sort_bits < 64) {
       // valid when every split compared only sorted bits: a split creating
       // depth d + 1 reads the top 3 (d + 1) + 1 bits
@@ -724,8 +750,7 @@ static pt_status finish_counters(pt_ctx* c, int64_t nblocks, int32_t* herr, bool
                                       c->stream));
   }
   unsigned long long h[5];
-  PT_TRY_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  PT_TRY_HIP(c, read_back(c, h, c->counters.p, sizeof(h)));
   c->h_counters.n_node_ints = h[0];
   c->h_counters.n_part_ints = h[1];
   c->h_counters.n_opens = h[2];
@@ -761,8 +786,8 @@ pt_status pt_traverse(pt_ctx* c, double theta, const double offset[3], pt_counte
     PT_TRY_LAUNCH(c, pt_exclusive_scan_i32_to_i64(c->pc_cnt.p, c->pc_off.p, nb, c->scratch.p,
                                                   c->pc_off.p + nb, c->stream));
     int64_t tot[2] = {0, 0};
-    PT_TRY_HIP(c, hipMemcpyAsync(&tot[0], c->pp_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
-    PT_TRY_HIP(c, hipMemcpyAsync(&tot[1], c->pc_off.p + nb, 8, hipMemcpyDeviceToHost, c->stream));
+    PT_TRY_HIP(c, read_back(c, &tot[0], c->pp_off.p + nb, 8));
+    PT_TRY_HIP(c, read_back(c, &tot[1], c->pc_off.p + nb, 8));
     int32_t herr = 0;
     if ((st = finish_counters(c, std::max<int64_t>(nblk, 1), &herr))) return st;
     if (herr) return fail(c, PT_ERR_CAPACITY, "traversal stack overflow (PT_WALK_STACK)");
@@ -1336,10 +1361,9 @@ pt_status group_lists(pt_ctx* c, double theta) {
                                                 c->gpc_off.p + ns, c->stream));
   int64_t tot[2] = {0, 0};
   int32_t herr = 0;
-  PT_TRY_HIP(c, hipMemcpyAsync(&tot[0], c->gpp_off.p + ns, 8, hipMemcpyDeviceToHost, c->stream));
-  PT_TRY_HIP(c, hipMemcpyAsync(&tot[1], c->gpc_off.p + ns, 8, hipMemcpyDeviceToHost, c->stream));
-  PT_TRY_HIP(c, hipMemcpyAsync(&herr, c->derr.p, 4, hipMemcpyDeviceToHost, c->stream));
-  PT_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  PT_TRY_HIP(c, read_back(c, &tot[0], c->gpp_off.p + ns, 8));
+  PT_TRY_HIP(c, read_back(c, &tot[1], c->gpc_off.p + ns, 8));
+  PT_TRY_HIP(c, read_back(c, &herr, c->derr.p, 4));
   if (herr) return fail(c, PT_ERR_CAPACITY, "group walk stack overflow");
   c->n_gpp = tot[0];
   c->n_gpc = tot[1];
