@@ -1018,12 +1018,11 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
   if (precision != PT_FP64 && (st = check_fp32c_range(c))) return st;
   const int64_t nb = c->nb;
   const bool hexa = c->cfg.hexadecapole != 0;
-  const int cap = hexa ? 0 : group_cap(c);
+  const int cap = group_cap(c);
   const bool grouped = cap > 0;
   const int64_t nblk = std::max<int64_t>(
-      hexa ? pt_hexa_gravity_blocks(nb)
-           : (grouped ? pt_grouped_gravity_blocks(nb, c->n, cap)
-                      : pt_gravity_blocks(nb, (int)precision)),
+      grouped ? pt_grouped_gravity_blocks(nb, c->n, cap)
+              : (hexa ? pt_hexa_gravity_blocks(nb) : pt_gravity_blocks(nb, (int)precision)),
       1);
   PT_TRY_HIP(c, ensure(c->cpart, nblk * 4));
   PT_TRY_HIP(c, ensure(c->wqueue, 8));
@@ -1071,19 +1070,21 @@ static pt_status gravity_fused(pt_ctx* c, double theta, const double offset[3],
       PT_TRY_LAUNCH(c, pt_launch_fill_u64(c->cpart.p, nblk * 4, 0ull, c->stream));
       {
         Timer t(c, PT_T_FORCE);
-        if (hexa)
+        if (grouped)
+          PT_TRY_LAUNCH(c, pt_launch_grouped_gravity(
+                               (int)precision, attempt == 1, c->bk.p, c->fg_first.p, c->fg_nb.p,
+                               c->fg_ng.p, nb, c->n, cap, c->tnode.p, c->gn.p, c->gn32.p,
+                               hexa ? c->ho.p : nullptr, hexa ? c->hn.p : nullptr, c->xs.p,
+                               c->ys.p, c->zs.p, c->ms.p, c->es.p, gf, c->gm, off, c->axs.p,
+                               c->ays.p, c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc,
+                               c->stream));
+        else if (hexa)
           PT_TRY_LAUNCH(c, pt_launch_hexa_gravity((int)precision, attempt == 1, c->bk.p, nb,
                                                   c->tnode.p, c->ho.p,
                                                   c->hn.p, c->xs.p, c->ys.p, c->zs.p, c->ms.p,
                                                   c->es.p, gf, c->gm, off, c->axs.p, c->ays.p,
                                                   c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc,
                                                   c->stream));
-        else if (grouped)
-          PT_TRY_LAUNCH(c, pt_launch_grouped_gravity(
-                               (int)precision, attempt == 1, c->bk.p, c->fg_first.p, c->fg_nb.p,
-                               c->fg_ng.p, nb, c->n, cap, c->tnode.p, c->gn.p, c->gn32.p, c->xs.p,
-                               c->ys.p, c->zs.p, c->ms.p, c->es.p, gf, off, c->axs.p, c->ays.p,
-                               c->azs.p, c->pots.p, c->cpart.p, c->derr.p, acc, c->stream));
         else
           PT_TRY_LAUNCH(c, pt_launch_gravity((int)precision, attempt == 1, c->bk.p, nb,
                                              c->tnode.p, c->gn.p, c->gn32.p, c->xs.p, c->ys.p,

@@ -1063,9 +1063,10 @@ __device__ __forceinline__ void hexa_pair(const SrcH& s, Lanes<false>& L, double
   }
 }
 
-// Stages up to 64 closed nodes (node id in the calling lane), kHexaStage at a
-// time, and evaluates them for the group's targets (slot s of a lane takes
-// sources s, s + S, ... of each part; the S partial sums are combined in
+// Stages up to 64 closed nodes (node id in the calling lane, kFarNode set
+// when no openSoftening test can hold for the group), kHexaStage at a time,
+// and evaluates them for the group's targets (slot s of a lane takes sources
+// s, s + S, ... of each part; the S partial sums are combined in
 // finish_group).
 template <typename W_t>
 __device__ __forceinline__ void do_nodes_hexa(W_t& W, int32_t id, int nsrc, const Group& g,
@@ -1075,13 +1076,14 @@ __device__ __forceinline__ void do_nodes_hexa(W_t& W, int32_t id, int nsrc, cons
   SrcH* st = reinterpret_cast<SrcH*>(W.stage);
   for (int h = 0; h < nsrc; h += kHexaStage) {
     if (lane >= h && lane < h + kHexaStage && lane < nsrc) {
-      const HexNode& s = hn[id];
+      const bool far = (id & kFarNode) != 0;
+      const HexNode& s = hn[id & ~kFarNode];
       SrcH r;
       r.cx = s.cx;
       r.cy = s.cy;
       r.cz = s.cz;
       r.mass = s.mass;
-      r.mono = pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, b) ? 1.0 : 0.0;
+      r.mono = !far && pt_open_softening(s.cx + ox, s.cy + oy, s.cz + oz, s.soft, b) ? 1.0 : 0.0;
 #pragma unroll
       for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = s.h[k];
       st[lane - h] = r;
@@ -1132,14 +1134,15 @@ __device__ __forceinline__ void do_nodes_hexa32(W_t& W, int32_t id, int nsrc, co
   const int lane = __lane_id();
   SrcH32* st = reinterpret_cast<SrcH32*>(W.stage);
   if (lane < nsrc) {
-    const HexNode& s = hn[id];
+    const bool far = (id & kFarNode) != 0;
+    const HexNode& s = hn[id & ~kFarNode];
     const double cx = s.cx + ox, cy = s.cy + oy, cz = s.cz + oz;
     SrcH32 r;
     r.cx = (float)(cx - ccx);
     r.cy = (float)(cy - ccy);
     r.cz = (float)(cz - ccz);
     r.mass = (float)s.mass;
-    r.mono = pt_open_softening(cx, cy, cz, s.soft, b) ? 1.f : 0.f;
+    r.mono = !far && pt_open_softening(cx, cy, cz, s.soft, b) ? 1.f : 0.f;
 #pragma unroll
     for (int k = 0; k < PT_HEXA_MOMS; k++) r.h[k] = (float)s.h[k];
     st[lane] = r;
@@ -1352,14 +1355,45 @@ hexa_gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* _
   }
 }
 
+// The LDS working set of a fused wave: the quadrupole build's, or the
+// HEXADECAPOLE build's (larger node records).
+template <bool F32, int STACK, bool HEXA>
+struct FusedLdsOf {
+  using type = WaveLds<F32, STACK>;
+};
+template <bool F32, int STACK>
+struct FusedLdsOf<F32, STACK, true> {
+  using type = HexaFusedLds<STACK>;
+};
+
+// Stages and evaluates up to one chunk of closed nodes (node id, with the
+// kFarNode flag, in the calling lane) in the build's form.
+template <bool F32, bool HEXA, typename W_t>
+__device__ __forceinline__ void stage_nodes(W_t& W, int32_t id, int nsrc, const Group& g,
+                                            Lanes<F32>& L, const GravNode* __restrict__ gn,
+                                            const GravNode32* __restrict__ gn32,
+                                            const HexNode* __restrict__ hn, const Bucket& b,
+                                            const BucketF& bf, double ox, double oy, double oz,
+                                            double ccx, double ccy, double ccz) {
+  if constexpr (HEXA) {
+    if constexpr (F32)
+      do_nodes_hexa32<W_t>(W, id, nsrc, g, L, hn, b, ox, oy, oz, ccx, ccy, ccz);
+    else
+      do_nodes_hexa<W_t>(W, id, nsrc, g, L, hn, b, ox, oy, oz);
+  } else {
+    do_nodes<F32, W_t>(W, id, nsrc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
+  }
+}
+
 // Evaluates every full chunk of the pending closed nodes (cw at a time) and
 // leaf sources (64 at a time).  A walk step starts with npc < cw and
 // npp < 64 and adds at most 64 to each, so the buffers (128 entries) never
 // overflow and each rest (< 64) fits one move.
-template <bool F32, typename W_t>
+template <bool F32, bool HEXA, typename W_t>
 __device__ __forceinline__ void drain_full(W_t& W, int& npc, int& npp, const Group& g,
                                            Lanes<F32>& L, const GravNode* __restrict__ gn,
-                                           const GravNode32* __restrict__ gn32, const Bucket& b,
+                                           const GravNode32* __restrict__ gn32,
+                                           const HexNode* __restrict__ hn, const Bucket& b,
                                            const BucketF& bf, const double* __restrict__ px,
                                            const double* __restrict__ py,
                                            const double* __restrict__ pz,
@@ -1367,10 +1401,10 @@ __device__ __forceinline__ void drain_full(W_t& W, int& npc, int& npp, const Gro
                                            const double* __restrict__ pe, double ox, double oy,
                                            double oz, double ccx, double ccy, double ccz) {
   const int lane = __lane_id();
-  const int cw = F32 ? PT_WAVE : g.cw;
+  const int cw = (F32 || HEXA) ? PT_WAVE : g.cw;
   while (npc >= cw) {
-    do_nodes<F32, W_t>(W, lane < cw ? W.pcbuf[lane] : 0, cw, g, L, gn, gn32, b, bf, ox, oy, oz,
-                       ccx, ccy, ccz);
+    stage_nodes<F32, HEXA, W_t>(W, lane < cw ? W.pcbuf[lane] : 0, cw, g, L, gn, gn32, hn, b, bf,
+                                ox, oy, oz, ccx, ccy, ccz);
     const int rest = npc - cw;
     const int32_t v = lane < rest ? W.pcbuf[cw + lane] : 0;
     __builtin_amdgcn_wave_barrier();
@@ -1395,10 +1429,11 @@ __device__ __forceinline__ void drain_full(W_t& W, int& npc, int& npp, const Gro
 }
 
 // Evaluates what is left in the buffers.
-template <bool F32, typename W_t>
+template <bool F32, bool HEXA, typename W_t>
 __device__ __forceinline__ void drain_rest(W_t& W, int npc, int npp, const Group& g,
                                            Lanes<F32>& L, const GravNode* __restrict__ gn,
-                                           const GravNode32* __restrict__ gn32, const Bucket& b,
+                                           const GravNode32* __restrict__ gn32,
+                                           const HexNode* __restrict__ hn, const Bucket& b,
                                            const BucketF& bf, const double* __restrict__ px,
                                            const double* __restrict__ py,
                                            const double* __restrict__ pz,
@@ -1408,7 +1443,7 @@ __device__ __forceinline__ void drain_rest(W_t& W, int npc, int npp, const Group
   const int lane = __lane_id();
   if (npc > 0) {
     const int32_t id = lane < npc ? W.pcbuf[lane] : 0;
-    do_nodes<F32, W_t>(W, id, npc, g, L, gn, gn32, b, bf, ox, oy, oz, ccx, ccy, ccz);
+    stage_nodes<F32, HEXA, W_t>(W, id, npc, g, L, gn, gn32, hn, b, bf, ox, oy, oz, ccx, ccy, ccz);
   }
   if (npp > 0) {
     const int32_t beg = lane < npp ? W.ppbeg[lane] : 0;
@@ -1421,22 +1456,22 @@ __device__ __forceinline__ void drain_rest(W_t& W, int npc, int npp, const Group
 // entries from `limit` up are kept), its chunks evaluated as they fill.
 // w_open / w_close: this walk's open() outcomes; lane_leaf_parts: the
 // source-leaf particles this lane visited.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool HEXA>
 __device__ __forceinline__ void walk_below(
-    WaveLds<F32, STACK>& W, int& sp, int& npc, int& npp, int limit, const Bucket& b,
+    typename FusedLdsOf<F32, STACK, HEXA>::type& W, int& sp, int& npc, int& npp, int limit, const Bucket& b,
     const BucketF& bf, const Group& g, Lanes<F32>& L, const TreeNode* __restrict__ nodes,
     const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+    const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn, double gm,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ pm, const double* __restrict__ pe, double gf, double ox,
     double oy, double oz, double ccx, double ccy, double ccz, int& w_open, int& w_close,
     int& lane_leaf_parts, bool& overflow) {
-  using W_t = WaveLds<F32, STACK>;
+  using W_t = typename FusedLdsOf<F32, STACK, HEXA>::type;
   WalkCounts wc;
   while (sp > 0) {
     unsigned long long leafm, closem;
-    const WalkLane r = pt_walk_step<STACK, false, false>(nodes, b, gf, ox, oy, oz, W.stack, sp,
-                                                         overflow, wc, leafm, closem, nullptr, 0.0,
-                                                         limit);
+    const WalkLane r = pt_walk_step<STACK, HEXA, false>(nodes, b, gf, ox, oy, oz, W.stack, sp,
+                                                        overflow, wc, leafm, closem, ho, gm, limit);
     w_open += wc.n_open;
     w_close += wc.n_close;
     lane_leaf_parts += r.is_leaf ? r.count : 0;
@@ -1453,24 +1488,25 @@ __device__ __forceinline__ void walk_below(
     npc += __popcll(closem);
     npp += __popcll(leafm);
     __builtin_amdgcn_wave_barrier();
-    drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
+    drain_full<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
                          ccy, ccz);
   }
 }
 
 // One target bucket of the fused kernel: the walk for the bucket box with
 // the bucket's chunks evaluated as they fill, counters added to the wave's.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool HEXA>
 __device__ __forceinline__ void gravity_bucket(
-    WaveLds<F32, STACK>& W, int64_t bi, const Bucket* __restrict__ bk,
+    typename FusedLdsOf<F32, STACK, HEXA>::type& W, int64_t bi, const Bucket* __restrict__ bk,
     const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
-    const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+    const GravNode32* __restrict__ gn32,
+    const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn, double gm, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pm,
     const double* __restrict__ pe, double gf, double ox, double oy, double oz,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     double* __restrict__ pot, unsigned long long& c_node, unsigned long long& c_part,
     unsigned long long& c_open, unsigned long long& c_close, bool& overflow, bool acc) {
-  using W_t = WaveLds<F32, STACK>;
+  using W_t = typename FusedLdsOf<F32, STACK, HEXA>::type;
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const Bucket b = bk[bi];
   const int64_t nt = b.count;
@@ -1488,7 +1524,7 @@ __device__ __forceinline__ void gravity_bucket(
     int lane_leaf_parts = 0;
     {
       WalkCounts wc;
-      const WalkLane r = pt_walk_root(nodes, b, gf, ox, oy, oz, W.stack, sp, wc);
+      const WalkLane r = pt_walk_root<HEXA>(nodes, b, gf, ox, oy, oz, W.stack, sp, wc, ho, gm);
       if (r.is_closed) {
         if (lane == 0) W.pcbuf[0] = 0;
         npc = 1;
@@ -1504,10 +1540,10 @@ __device__ __forceinline__ void gravity_bucket(
       lane_leaf_parts = lane == 0 ? wc.leaf_parts : 0;
     }
     __builtin_amdgcn_wave_barrier();
-    walk_below<F32, STACK>(W, sp, npc, npp, STACK, b, bf, g, L, nodes, gn, gn32, px, py, pz, pm,
+    walk_below<F32, STACK, HEXA>(W, sp, npc, npp, STACK, b, bf, g, L, nodes, gn, gn32, ho, hn, gm, px, py, pz, pm,
                            pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close, lane_leaf_parts,
                            overflow);
-    drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
+    drain_rest<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, b, bf, px, py, pz, pm, pe, ox, oy, oz, ccx,
                          ccy, ccz);
     if (count) {
       c_open += (unsigned)w_open;
@@ -1620,11 +1656,12 @@ constexpr int kSubTargets = 16;
 // growing down from res_top (anything else).  used_top: entries above
 // res_top held by enclosing residual lists.  Returns false, writing nothing,
 // when the entries would not fit with kMemberReserve to spare.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool HEXA>
 __device__ __forceinline__ bool shared_visit(
-    WaveLds<F32, STACK>& W, int32_t node, bool active, const TreeNode* __restrict__ nodes,
+    typename FusedLdsOf<F32, STACK, HEXA>::type& W, int32_t node, bool active, const TreeNode* __restrict__ nodes,
     const Bucket* __restrict__ mb, int mG, const Bucket& ub, const BucketF& ubf, float emax,
-    double gf, double ox, double oy, double oz, int& sp, int& npc, int& npp, int& nres,
+    double gf, const HexOpen* __restrict__ ho, double gm, double ox, double oy, double oz,
+    int& sp, int& npc, int& npp, int& nres,
     int res_top, int& sh_open, int& sh_close, int& lane_leaf_parts) {
   const TreeNode nd = nodes[node];
   const bool leaf = active && nd.type == PT_NODE_LEAF;
@@ -1637,14 +1674,24 @@ __device__ __forceinline__ bool shared_visit(
       op = true;
     } else {
       const double rsq = nd.radius * nd.radius * gf;
-      if (dsq > rsq) {
+      // HEXADECAPOLE: a member also opens the node when the softening spheres
+      // meet and the monopole sphere (radius * gm) reaches its box, so only a
+      // node beyond both spheres for U is closed for all
+      double msq = 0.0;
+      if constexpr (HEXA) {
+        const double mono_rad = nd.radius * gm;
+        msq = mono_rad * mono_rad;
+      }
+      if (dsq > rsq && (!HEXA || dsq > msq)) {
         cl = true;
       } else if (pt_box_point_far_dsq(ub, cx, cy, cz) <= rsq) {
         op = true;
       } else {
         bool any_open = false, any_closed = false;
         for (int k = 0; k < mG; k++) {
-          const bool o = pt_box_point_dsq(mb[k], cx, cy, cz) <= rsq;
+          bool o;
+          if constexpr (HEXA) o = pt_gv_open_hexa(nd, ho[node], mb[k], gf, gm, ox, oy, oz);
+          else o = pt_box_point_dsq(mb[k], cx, cy, cz) <= rsq;
           any_open |= o;
           any_closed |= !o;
         }
@@ -1689,18 +1736,19 @@ __device__ __forceinline__ bool shared_visit(
 // added by the lanes base + t that wrote the group's sums (base >= 0), and
 // the counters take every member's shared calls.  Returns false with nothing
 // written or counted when the lists overflow.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool HEXA>
 __device__ __forceinline__ bool shared_walk(
-    WaveLds<F32, STACK>& W, const Bucket* __restrict__ bk, int64_t mb0, int mG, int seed_top,
+    typename FusedLdsOf<F32, STACK, HEXA>::type& W, const Bucket* __restrict__ bk, int64_t mb0, int mG, int seed_top,
     int nseed, int res_top, int& nres, int base, const TreeNode* __restrict__ nodes,
     const GravNode* __restrict__ gn, const GravNode32* __restrict__ gn32,
+    const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn, double gm,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ pm, const double* __restrict__ pe, double gf, double ox,
     double oy, double oz, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, double* __restrict__ pot, unsigned long long& c_node,
     unsigned long long& c_part, unsigned long long& c_open, unsigned long long& c_close,
     bool acc) {
-  using W_t = WaveLds<F32, STACK>;
+  using W_t = typename FusedLdsOf<F32, STACK, HEXA>::type;
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const Bucket* mb = bk + mb0;
   Bucket ub = mb[0];
@@ -1731,11 +1779,11 @@ __device__ __forceinline__ bool shared_walk(
   for (int r0 = 0; r0 < nbatch; r0 += PT_WAVE) {
     const bool active = nseed < 0 ? lane == 0 : r0 + lane < nseed;
     const int32_t node = nseed < 0 ? 0 : W.stack[seed_top - 1 - (active ? r0 + lane : r0)];
-    if (!shared_visit<F32, STACK>(W, node, active, nodes, mb, mG, ub, ubf, L.emax, gf, ox, oy, oz,
+    if (!shared_visit<F32, STACK, HEXA>(W, node, active, nodes, mb, mG, ub, ubf, L.emax, gf, ho, gm, ox, oy, oz,
                                   sp, npc, npp, nres, res_top, sh_open, sh_close,
                                   lane_leaf_parts))
       return false;
-    drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+    drain_full<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
                          gcx, gcy, gcz);
     while (sp > 0) {
       const int k = sp < 8 ? sp : 8;
@@ -1743,15 +1791,15 @@ __device__ __forceinline__ bool shared_walk(
       const int slot = lane >> 3;
       const bool act = slot < k;
       const int32_t child = W.stack[sp + (act ? slot : 0)] + (lane & 7);
-      if (!shared_visit<F32, STACK>(W, child, act, nodes, mb, mG, ub, ubf, L.emax, gf, ox, oy,
+      if (!shared_visit<F32, STACK, HEXA>(W, child, act, nodes, mb, mG, ub, ubf, L.emax, gf, ho, gm, ox, oy,
                                     oz, sp, npc, npp, nres, res_top, sh_open, sh_close,
                                     lane_leaf_parts))
         return false;
-      drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
+      drain_full<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, ub, ubf, px, py, pz, pm, pe, ox, oy, oz,
                            gcx, gcy, gcz);
     }
   }
-  drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, ub, ubf, px, py, pz, pm, pe, ox, oy, oz, gcx,
+  drain_rest<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, ub, ubf, px, py, pz, pm, pe, ox, oy, oz, gcx,
                        gcy, gcz);
   // every member made each shared call
   c_open += (unsigned long long)sh_open * mG;
@@ -1771,21 +1819,22 @@ __device__ __forceinline__ bool shared_walk(
 // its shared walk gave up).  Returns false, having written nothing and
 // counted nothing, when the group's shared walk gives up; the caller then
 // walks each member on its own.
-template <bool F32, int STACK>
+template <bool F32, int STACK, bool HEXA>
 __device__ __forceinline__ bool gravity_group(
-    WaveLds<F32, STACK>& W, int64_t b0, int G, const Bucket* __restrict__ bk,
+    typename FusedLdsOf<F32, STACK, HEXA>::type& W, int64_t b0, int G, const Bucket* __restrict__ bk,
     const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
-    const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+    const GravNode32* __restrict__ gn32,
+    const HexOpen* __restrict__ ho, const HexNode* __restrict__ hn, double gm, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pm,
     const double* __restrict__ pe, double gf, double ox, double oy, double oz,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     double* __restrict__ pot, unsigned long long& c_node, unsigned long long& c_part,
     unsigned long long& c_open, unsigned long long& c_close, bool& overflow, bool acc) {
-  using W_t = WaveLds<F32, STACK>;
+  using W_t = typename FusedLdsOf<F32, STACK, HEXA>::type;
   const int lane = threadIdx.x & (PT_WAVE - 1);
   const int64_t first = bk[b0].begin;
   int nres = 0;
-  if (!shared_walk<F32, STACK>(W, bk, b0, G, STACK, -1, STACK, nres, -1, nodes, gn, gn32, px, py,
+  if (!shared_walk<F32, STACK, HEXA>(W, bk, b0, G, STACK, -1, STACK, nres, -1, nodes, gn, gn32, ho, hn, gm, px, py,
                                pz, pm, pe, gf, ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
                                c_close, acc))
     return false;
@@ -1798,8 +1847,8 @@ __device__ __forceinline__ bool gravity_group(
     int seed_top = STACK, nseed = nres;
     if (k1 - k0 >= 2 && k1 - k0 < G) {
       int nres2 = 0;
-      if (shared_walk<F32, STACK>(W, bk, b0 + k0, k1 - k0, STACK, nres, STACK - nres, nres2,
-                                  (int)(bk[b0 + k0].begin - first), nodes, gn, gn32, px, py, pz,
+      if (shared_walk<F32, STACK, HEXA>(W, bk, b0 + k0, k1 - k0, STACK, nres, STACK - nres, nres2,
+                                  (int)(bk[b0 + k0].begin - first), nodes, gn, gn32, ho, hn, gm, px, py, pz,
                                   pm, pe, gf, ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
                                   c_close, acc)) {
         seed_top = STACK - nres;
@@ -1826,7 +1875,9 @@ __device__ __forceinline__ bool gravity_group(
         const int32_t node = W.stack[seed_top - 1 - (active ? r0 + lane : r0)];
         const TreeNode nd = nodes[node];
         const double dsq = pt_box_point_dsq(b, nd.cx + ox, nd.cy + oy, nd.cz + oz);
-        const bool op = active && pt_gv_open_dsq(nd, gf, dsq);
+        bool op;
+        if constexpr (HEXA) op = active && pt_gv_open_hexa(nd, ho[node], b, gf, gm, ox, oy, oz);
+        else op = active && pt_gv_open_dsq(nd, gf, dsq);
         const bool cl = active && !op;
         const unsigned long long openm = __ballot(op), closem = __ballot(cl);
         const int n_open = __popcll(openm);
@@ -1840,14 +1891,14 @@ __device__ __forceinline__ bool gravity_group(
         w_open += n_open;
         w_close += __popcll(closem);
         __builtin_amdgcn_wave_barrier();
-        drain_full<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
+        drain_full<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, b, bf, px, py, pz, pm, pe, ox, oy, oz,
                              ccx, ccy, ccz);
-        walk_below<F32, STACK>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, px, py, pz,
+        walk_below<F32, STACK, HEXA>(W, sp, npc, npp, limit, b, bf, g, L, nodes, gn, gn32, ho, hn, gm, px, py, pz,
                                pm, pe, gf, ox, oy, oz, ccx, ccy, ccz, w_open, w_close,
                                lane_leaf_parts, overflow);
         if (overflow) return true;
       }
-      drain_rest<F32, W_t>(W, npc, npp, g, L, gn, gn32, b, bf, px, py, pz, pm, pe, ox, oy, oz,
+      drain_rest<F32, HEXA, W_t>(W, npc, npp, g, L, gn, gn32, hn, b, bf, px, py, pz, pm, pe, ox, oy, oz,
                            ccx, ccy, ccz);
       const int64_t nt = b.count;
       c_open += (unsigned)w_open;
@@ -1895,7 +1946,8 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
   if constexpr (F32) {
     const int64_t bi = pt_xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
     if (bi < nb)
-      gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+      gravity_bucket<F32, STACK, false>(W, bi, bk, nodes, gn, gn32, nullptr, nullptr, 0.0, px, py,
+                                        pz, pm, pe, gf, ox, oy, oz,
                                  ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
                                  acc != 0);
   } else {
@@ -1907,7 +1959,8 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
       // the next index is fetched now and used after this bucket
       unsigned long long next = 0;
       if (lane == 0) next = atomicAdd(&queue[xcd], 1ull);
-      gravity_bucket<F32, STACK>(W, bi, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy, oz,
+      gravity_bucket<F32, STACK, false>(W, bi, bk, nodes, gn, gn32, nullptr, nullptr, 0.0, px, py,
+                                        pz, pm, pe, gf, ox, oy, oz,
                                  ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
                                  acc != 0);
       bi = b_lo + (int64_t)__shfl(next, 0, PT_WAVE);
@@ -1943,20 +1996,22 @@ gravity_kernel(const Bucket* __restrict__ bk, int64_t nb, const TreeNode* __rest
 constexpr int kGWaves = 1;
 constexpr int kGroupStack32 = 512;
 constexpr int kGroupStack64 = 384;
+constexpr int kGroupStackHexa = 384;  // the HEXADECAPOLE build, both forms
 
-template <bool F32, int STACK, bool OFF>
+template <bool F32, int STACK, bool OFF, bool HEXA>
 __global__ void __launch_bounds__(kGWaves * PT_WAVE)
 grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ gfirst,
                        const int32_t* __restrict__ gnb, const uint32_t* __restrict__ ngroups,
                        const TreeNode* __restrict__ nodes, const GravNode* __restrict__ gn,
-                       const GravNode32* __restrict__ gn32, const double* __restrict__ px,
+                       const GravNode32* __restrict__ gn32, const HexOpen* __restrict__ ho,
+                       const HexNode* __restrict__ hn, const double* __restrict__ px,
                        const double* __restrict__ py, const double* __restrict__ pz,
                        const double* __restrict__ pm, const double* __restrict__ pe, double gf,
-                       double ox_in, double oy_in, double oz_in, double* __restrict__ ax,
+                       double gm, double ox_in, double oy_in, double oz_in, double* __restrict__ ax,
                        double* __restrict__ ay, double* __restrict__ az, double* __restrict__ pot,
                        unsigned long long* __restrict__ partials, int32_t* __restrict__ err,
                        int acc) {
-  using W_t = WaveLds<F32, STACK>;
+  using W_t = typename FusedLdsOf<F32, STACK, HEXA>::type;
   __shared__ W_t lds[kGWaves];
   __shared__ unsigned long long s_cnt[kGWaves][4];
   const double ox = OFF ? ox_in : -0.0, oy = OFF ? oy_in : -0.0, oz = OFF ? oz_in : -0.0;
@@ -1975,14 +2030,15 @@ grouped_gravity_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
     // a group of one bucket, or a group whose residual list overflowed: one
     // walk per bucket
     const bool done =
-        G > 1 && gravity_group<F32, STACK>(W, b0, G, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf,
-                                           ox, oy, oz, ax, ay, az, pot, c_node, c_part, c_open,
-                                           c_close, overflow, acc != 0);
+        G > 1 && gravity_group<F32, STACK, HEXA>(W, b0, G, bk, nodes, gn, gn32, ho, hn, gm, px,
+                                                 py, pz, pm, pe, gf, ox, oy, oz, ax, ay, az, pot,
+                                                 c_node, c_part, c_open, c_close, overflow,
+                                                 acc != 0);
     if (!done) {
       for (int k = 0; k < G; k++)
-        gravity_bucket<F32, STACK>(W, b0 + k, bk, nodes, gn, gn32, px, py, pz, pm, pe, gf, ox, oy,
-                                   oz, ax, ay, az, pot, c_node, c_part, c_open, c_close, overflow,
-                                   acc != 0);
+        gravity_bucket<F32, STACK, HEXA>(W, b0 + k, bk, nodes, gn, gn32, ho, hn, gm, px, py, pz,
+                                         pm, pe, gf, ox, oy, oz, ax, ay, az, pot, c_node, c_part,
+                                         c_open, c_close, overflow, acc != 0);
     }
   }
   if (lane == 0) {
@@ -2200,28 +2256,37 @@ int pt_launch_fgroups(const Bucket* bk, int64_t nb, int cap, int32_t* tfirst, in
 int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
                               const int32_t* gfirst, const int32_t* gnb, const uint32_t* ngroups,
                               int64_t nb, int64_t n, int cap, const TreeNode* nodes,
-                              const GravNode* gn,
-                              const GravNode32* gn32, const double* x, const double* y,
+                              const GravNode* gn, const GravNode32* gn32, const HexOpen* ho,
+                              const HexNode* hn, const double* x, const double* y,
                               const double* z, const double* m, const double* e, double gf,
-                              const double off[3], double* ax, double* ay, double* az,
+                              double gm, const double off[3], double* ax, double* ay, double* az,
                               double* pot, unsigned long long* partials, int32_t* err,
                               bool accumulate, hipStream_t s) {
   if (nb <= 0) return 0;
   const unsigned grid = (unsigned)pt_grouped_gravity_blocks(nb, n, cap);
   const bool has_off = off[0] != 0.0 || off[1] != 0.0 || off[2] != 0.0;
-#define PT_GROUPED_LAUNCH(F32, STACK, OFF)                                                     \
-  grouped_gravity_kernel<F32, STACK, OFF><<<grid, kGWaves * PT_WAVE, 0, s>>>(                           \
-      bk, gfirst, gnb, ngroups, nodes, gn, gn32, x, y, z, m, e, gf, off[0], off[1], off[2], ax, \
-      ay, az, pot, partials, err, accumulate ? 1 : 0)
-#define PT_GROUPED_LAUNCH_OFF(F32, STACK)               \
-  if (has_off) PT_GROUPED_LAUNCH(F32, STACK, true);     \
-  else PT_GROUPED_LAUNCH(F32, STACK, false)
-  if (precision == 0) {
-    if (big_stack) PT_GROUPED_LAUNCH_OFF(false, kBigStack);
-    else PT_GROUPED_LAUNCH_OFF(false, kGroupStack64);
+  const bool hexa = ho != nullptr;
+#define PT_GROUPED_LAUNCH(F32, STACK, OFF, HEXA)                                               \
+  grouped_gravity_kernel<F32, STACK, OFF, HEXA><<<grid, kGWaves * PT_WAVE, 0, s>>>(            \
+      bk, gfirst, gnb, ngroups, nodes, gn, gn32, ho, hn, x, y, z, m, e, gf, gm, off[0], off[1], \
+      off[2], ax, ay, az, pot, partials, err, accumulate ? 1 : 0)
+#define PT_GROUPED_LAUNCH_OFF(F32, STACK, HEXA)               \
+  if (has_off) PT_GROUPED_LAUNCH(F32, STACK, true, HEXA);     \
+  else PT_GROUPED_LAUNCH(F32, STACK, false, HEXA)
+  if (hexa) {
+    if (precision == 0) {
+      if (big_stack) PT_GROUPED_LAUNCH_OFF(false, kBigStack, true);
+      else PT_GROUPED_LAUNCH_OFF(false, kGroupStackHexa, true);
+    } else {
+      if (big_stack) PT_GROUPED_LAUNCH_OFF(true, kBigStack, true);
+      else PT_GROUPED_LAUNCH_OFF(true, kGroupStackHexa, true);
+    }
+  } else if (precision == 0) {
+    if (big_stack) PT_GROUPED_LAUNCH_OFF(false, kBigStack, false);
+    else PT_GROUPED_LAUNCH_OFF(false, kGroupStack64, false);
   } else {
-    if (big_stack) PT_GROUPED_LAUNCH_OFF(true, kBigStack);
-    else PT_GROUPED_LAUNCH_OFF(true, kGroupStack32);
+    if (big_stack) PT_GROUPED_LAUNCH_OFF(true, kBigStack, false);
+    else PT_GROUPED_LAUNCH_OFF(true, kGroupStack32, false);
   }
 #undef PT_GROUPED_LAUNCH_OFF
 #undef PT_GROUPED_LAUNCH

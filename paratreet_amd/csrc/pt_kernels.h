@@ -157,15 +157,17 @@ int pt_launch_fgroups(const Bucket* bk, int64_t nb, int cap, int32_t* tfirst, in
                       int32_t* gfirst, int32_t* gnb, hipStream_t s);
 // The fused walk + interactions with the shared walk per bucket group; the
 // same calls, counters and partials layout as pt_launch_gravity (partials:
-// pt_grouped_gravity_blocks(nb, n, cap) work-groups).
+// pt_grouped_gravity_blocks(nb, n, cap) work-groups).  ho / hn non-null: the
+// HEXADECAPOLE build (open() with gm, node() from hn), else the quadrupole
+// build (gn / gn32).
 int64_t pt_grouped_gravity_blocks(int64_t nb, int64_t n, int cap);
 int pt_launch_grouped_gravity(int precision, bool big_stack, const Bucket* bk,
                               const int32_t* gfirst, const int32_t* gnb, const uint32_t* ngroups,
                               int64_t nb, int64_t n, int cap, const TreeNode* nodes,
-                              const GravNode* gn,
-                              const GravNode32* gn32, const double* x, const double* y,
+                              const GravNode* gn, const GravNode32* gn32, const HexOpen* ho,
+                              const HexNode* hn, const double* x, const double* y,
                               const double* z, const double* m, const double* e, double gf,
-                              const double off[3], double* ax, double* ay, double* az,
+                              double gm, const double off[3], double* ax, double* ay, double* az,
                               double* pot, unsigned long long* partials, int32_t* err,
                               bool accumulate, hipStream_t s);
 

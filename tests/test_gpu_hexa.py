@@ -100,12 +100,18 @@ def test_hexa_forces_fp64_within_1e12(pair):
     assert e.max() <= FP64_TOL and ep.max() <= FP64_TOL, name
 
 
-def test_hexa_compute_forces_matches_oracle(pair):
-    """pt_compute_forces on a hexadecapole context (lists + hexadecapole
-    node()): counters exact, FP64 within 1e-12."""
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
+def test_hexa_compute_forces_matches_oracle(pair, grouping):
+    """pt_compute_forces on a hexadecapole context (fused walk with the
+    hexadecapole open() and node()), with bucket runs sharing a walk and with
+    one walk per bucket: counters exact, FP64 within 1e-12."""
     name, P, d, _ = pair
-    cnt = d.compute_forces(theta=P.theta, precision=0)
-    acc, pot = d.accelerations()
+    d.set_fused_grouping(grouping)
+    try:
+        cnt = d.compute_forces(theta=P.theta, precision=0)
+        acc, pot = d.accelerations()
+    finally:
+        d.set_fused_grouping(1)
     oacc, opot, ocnt = P.gravity_unsorted()
     assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt), name
     assert rel_err(acc, oacc).max() <= FP64_TOL, name
@@ -123,15 +129,20 @@ def test_hexa_lambs_meets_reference_accuracy():
     assert rms < 8e-4 and r.max() < 0.03
 
 
-def test_hexa_fp32c_against_fp64(pair):
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
+def test_hexa_fp32c_against_fp64(pair, grouping):
     """The compensated-FP32 form of the hexadecapole build (fused path): the
     same interactions (counters exact) within 1e-5 RMS of the FP64 GPU
     forces, whose agreement with the oracle is pinned above."""
     name, P, d, _ = pair
-    c64 = d.compute_forces(theta=P.theta, precision=0)
-    a64, p64 = d.accelerations()
-    c32 = d.compute_forces(theta=P.theta, precision=1)
-    a32, p32 = d.accelerations()
+    d.set_fused_grouping(grouping)
+    try:
+        c64 = d.compute_forces(theta=P.theta, precision=0)
+        a64, p64 = d.accelerations()
+        c32 = d.compute_forces(theta=P.theta, precision=1)
+        a32, p32 = d.accelerations()
+    finally:
+        d.set_fused_grouping(1)
     assert tuple(int(v) for v in c32) == tuple(int(v) for v in c64), name
     check_fp32c(a32, a64, name + " hexadecapole")
     rp = np.abs(p32 - p64) / np.abs(p64)
@@ -156,14 +167,17 @@ def test_hexa_rejects_fp32c_lists_and_adjoint():
     d.close()
 
 
-def test_hexa_walk_stack_overflow_rerun():
-    """The fused hexadecapole evaluation reruns with the big walk stack when
-    the small one overflows (deep halos, bucket 4, theta 0.1) and still
-    matches the oracle."""
+@pytest.mark.parametrize("grouping", [64, 0], ids=["grouped64", "per_bucket"])
+def test_hexa_walk_stack_overflow_rerun(grouping):
+    """Deep halos, bucket 4, theta 0.1: one walk per bucket overflows the
+    small walk stack and the evaluation reruns with the big one; the grouped
+    walk's residual lists overflow and their groups walk per bucket.  Both
+    match the oracle."""
     pos, _, m = ics.halo_mixture(30000, n_halos=8, seed=3)
     e = np.full(len(pos), 0.05)
     P = O.Pipeline(pos, m, e, bucket=4, theta=0.1, hexadecapole=True)
     d = _gpu(pos, m, e, 4)
+    d.set_fused_grouping(grouping)
     cnt = d.compute_forces(theta=0.1, precision=0)
     acc, _ = d.accelerations()
     d.close()

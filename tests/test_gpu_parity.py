@@ -213,6 +213,33 @@ def test_walk_stack_overflow_rerun(grouping):
     check_fp32c(a32, acc, "deep walk")
 
 
+@pytest.mark.parametrize("grouping", [64, 24], ids=["grouped64", "grouped24"])
+def test_grouped_walk_with_large_buckets(grouping):
+    """Buckets of up to 100 particles (bucket size 100, a clumpy halo
+    mixture): runs mix groups of several small buckets with buckets larger
+    than the cap, which form groups of one, and with buckets of more than 64
+    particles, which take the per-bucket walk's several target passes inside
+    the grouped kernel.  Counters exact, FP64 within 1e-12, FP32C within its
+    bounds."""
+    pos, _, m = ics.halo_mixture(20000, n_halos=8, seed=21)
+    e = np.full(len(pos), 0.02)
+    P = O.Pipeline(pos, m, e, bucket=100, theta=0.7)
+    counts = P.nodes["count"][P.bucket_nodes]
+    assert counts.max() > 64 and (counts <= 8).any()
+    d = _gpu(pos, m, e, 100, 0.7)
+    d.set_fused_grouping(grouping)
+    cnt = d.compute_forces(theta=0.7, precision=0)
+    acc, _ = d.accelerations()
+    cnt32 = d.compute_forces(theta=0.7, precision=1)
+    a32, _ = d.accelerations()
+    d.close()
+    oacc, _, ocnt = P.gravity_unsorted()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
+    assert rel_err(acc, oacc).max() <= FP64_TOL
+    assert tuple(cnt32) == tuple(cnt)
+    check_fp32c(a32, acc, "large buckets")
+
+
 def test_lambs_tree_error_vs_reference_direct():
     """Tree force error on the reference's accuracy fixture against its
     direct.acc: same order as the oracle's (quadrupole, theta 0.7)."""

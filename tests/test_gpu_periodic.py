@@ -22,12 +22,13 @@ def _driver(bucket, period, n_rep, hexa=False):
                                 nReplicas=n_rep, hexadecapole=hexa))
 
 
-@pytest.mark.parametrize("hexa,grouping", [(False, 64), (False, 0), (True, 1)],
-                         ids=["quad-grouped64", "quad-per_bucket", "hexa"])
+@pytest.mark.parametrize("hexa,grouping", [(False, 64), (False, 0), (True, 64), (True, 0)],
+                         ids=["quad-grouped64", "quad-per_bucket", "hexa-grouped64",
+                              "hexa-per_bucket"])
 def test_periodic_replicas_match_oracle(hexa, grouping):
     """The replica sum (each replica's walk adding to the last) and the Ewald
-    correction against the oracle; the quadrupole build with the grouped walk
-    and with one walk per bucket (the HEXADECAPOLE build walks per bucket)."""
+    correction against the oracle, in both builds, with the grouped walk and
+    with one walk per bucket."""
     rng = np.random.default_rng(5)
     n = 4000
     pos = rng.uniform(0.0, 2.0, (n, 3))
