@@ -253,6 +253,19 @@ __device__ __forceinline__ void pack_node(const PtTree& t, int64_t i, const CDat
   }
 }
 
+// One particle of the CentroidData leaf constructor: moments, box, and the
+// largest softening.
+__device__ __forceinline__ void leaf_add(CData& d, double& smax, double px, double py, double pz,
+                                         double pm, double pe) {
+  smax = pe > smax ? pe : smax;
+  mom_add_particle(d, px, py, pz, pm, pe);
+  const double p[3] = {px, py, pz};
+  for (int k = 0; k < 3; k++) {
+    if (p[k] < d.lo[k]) d.lo[k] = p[k];
+    if (p[k] > d.hi[k]) d.hi[k] = p[k];
+  }
+}
+
 // One thread per node: a leaf runs the sequential particle recurrence of the
 // CentroidData leaf constructor in sorted particle order; an empty leaf stores
 // the default CentroidData; internal nodes are left to the upward pass.  The
@@ -272,16 +285,26 @@ leaf_kernel(const PtTree t, int64_t nn, const double* __restrict__ x,
   if (ty == PT_NODE_LEAF) {
     const int64_t b = t.begin[id];
     const int64_t n = t.count[id];
-    for (int64_t i = b; i < b + n; i++) {
-      const double px = x[i], py = y[i], pz = z[i];
-      smax = e[i] > smax ? e[i] : smax;
-      mom_add_particle(d, px, py, pz, m[i], e[i]);
-      const double p[3] = {px, py, pz};
-      for (int k = 0; k < 3; k++) {
-        if (p[k] < d.lo[k]) d.lo[k] = p[k];
-        if (p[k] > d.hi[k]) d.hi[k] = p[k];
+    // the first kLeafPre particles' inputs are loaded together, then added
+    // in order (the same sequence as one at a time, without a memory round
+    // trip per particle)
+    constexpr int kLeafPre = 16;
+    double qx[kLeafPre], qy[kLeafPre], qz[kLeafPre], qm[kLeafPre], qe[kLeafPre];
+#pragma unroll
+    for (int k = 0; k < kLeafPre; k++) {
+      if (k < n) {
+        qx[k] = x[b + k];
+        qy[k] = y[b + k];
+        qz[k] = z[b + k];
+        qm[k] = m[b + k];
+        qe[k] = e[b + k];
       }
     }
+#pragma unroll
+    for (int k = 0; k < kLeafPre; k++) {
+      if (k < n) leaf_add(d, smax, qx[k], qy[k], qz[k], qm[k], qe[k]);
+    }
+    for (int64_t i = b + kLeafPre; i < b + n; i++) leaf_add(d, smax, x[i], y[i], z[i], m[i], e[i]);
     d.count = n;
     if (d.count > 1) {
       d.radius = 0.5 * len3(d.hi[0] - d.lo[0], d.hi[1] - d.lo[1], d.hi[2] - d.lo[2]);
@@ -314,9 +337,10 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
                                             double (*__restrict__ slab)[kUpFields],
                                             const PtPackOut& o) {
   const int32_t fc = t.first_child[p];
+  // every child's fields, type and order keys are loaded in one phase (an
+  // empty child's stored defaults and a leaf's order keys are not used)
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    if (t.type[fc + c] == PT_NODE_EMPTY) continue;
     const int64_t ch = fc + c;
     slab[c][0] = nd[PT_CD_SOFT * nn + ch];
     slab[c][1] = nd[PT_CD_MASS * nn + ch];
@@ -332,16 +356,16 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
   int ctype[8];
   int32_t h[8];
   uint64_t b[8];
+  uint64_t ckey[8];
   int n_leaf = 0, n_empty = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     ctype[c] = t.type[fc + c];
-    h[c] = -1;
-    b[c] = 0;
-    if (ctype[c] == PT_NODE_INTERNAL) {
-      h[c] = ord_h[fc + c];
-      b[c] = ord_b[fc + c];
-    }
+    ckey[c] = t.key[fc + c];
+    const int32_t hc = ord_h[fc + c];
+    const uint64_t bc = ord_b[fc + c];
+    h[c] = ctype[c] == PT_NODE_INTERNAL ? hc : -1;
+    b[c] = ctype[c] == PT_NODE_INTERNAL ? bc : 0;
     n_leaf += ctype[c] == PT_NODE_LEAF;
     n_empty += ctype[c] == PT_NODE_EMPTY;
   }
@@ -379,7 +403,7 @@ __device__ __forceinline__ void upward_node(const PtTree& t, int32_t p, double* 
     uint64_t bl = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      uint64_t r = left_align(t.key[fc + c]) >> 1;
+      uint64_t r = left_align(ckey[c]) >> 1;
       if (ctype[c] == PT_NODE_EMPTY) r |= (1ull << 63);
       bl = r > bl ? r : bl;
     }
