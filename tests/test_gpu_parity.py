@@ -291,6 +291,31 @@ def test_small_and_degenerate(n, bucket):
     d.close()
 
 
+@pytest.mark.parametrize("n,bucket", [(1, 12), (12, 12), (13, 12), (100, 1), (257, 4)])
+def test_small_and_degenerate_fused_grouped(n, bucket):
+    """The same small and degenerate sets through pt_compute_forces with the
+    grouped walk (runs of at most 64 particles): counters exact, FP64 within
+    1e-12, zero force where the oracle's is zero."""
+    rng = np.random.default_rng(n * 31 + bucket)
+    pos = rng.standard_normal((n, 3))
+    if n >= 100:
+        pos[: n // 3] = pos[0]  # coincident particles: identical keys, rsq == 0 pairs
+    m = np.full(n, 1.0 / n)
+    e = np.full(n, 0.01)
+    P = O.Pipeline(pos, m, e, bucket=bucket, theta=0.7)
+    d = _gpu(pos, m, e, bucket, 0.7)
+    d.set_fused_grouping(64)
+    cnt = d.compute_forces(theta=0.7, precision=0)
+    acc, _ = d.accelerations()
+    d.close()
+    oacc, _, ocnt = P.gravity_unsorted()
+    assert tuple(int(v) for v in cnt) == tuple(int(v) for v in ocnt)
+    nz = np.linalg.norm(oacc, axis=1) > 0
+    assert np.all(acc[~nz] == 0.0)
+    if nz.any():
+        assert rel_err(acc[nz], oacc[nz]).max() <= FP64_TOL
+
+
 def test_leapfrog_matches_oracle():
     pos, vel, m = ics.plummer(3000, seed=5)
     e = np.full(3000, 0.01)
