@@ -41,7 +41,7 @@ CONFIGS = {
 # (KiB per dispatch; the counters are read with factor 1.0, calibrated against
 # a device copy of known size).
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plummer1m_kernels.md")
-TRAFFIC_KERNEL = "grouped_gravity_kernel<true, 512, false>"
+TRAFFIC_KERNEL = "grouped_gravity_kernel<true, 512, false, false>"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 FLOPS_PP = 20.0
