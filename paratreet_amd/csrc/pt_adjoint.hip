@@ -318,8 +318,6 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
   __shared__ TgtA s_t[kAdjWaves][kANT];
   __shared__ Bucket s_bk[kAdjWaves][PT_GROUP_BUCKETS];
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
-  __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
-  __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
   __shared__ double s_far[kAdjWaves][kPassFar];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
@@ -427,29 +425,29 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
       const int incl_f = pt_wave_incl_scan(cf);
       const int near_total = __builtin_amdgcn_readlane(incl_n, PT_WAVE - 1);
       const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
-      // entries in slot order (near ones first), so the starts ascend
-      const unsigned long long nearm = __ballot(isnear);
-      const int pos = isnear ? pt_rank_in(nearm) : __popcll(nearm) + pt_rank_in(~nearm);
-      __builtin_amdgcn_wave_barrier();
-      s_beg[wave][pos] = beg;
-      s_exc[wave][pos] = isnear ? incl_n - cn : near_total + incl_f - cf;
-      s_msk[wave][pos] = (int32_t)msk;
-      __builtin_amdgcn_wave_barrier();
+      // each entry's first slot, near entries first
+      const int exc = isnear ? incl_n - cn : near_total + incl_f - cf;
+      int carry = 0;
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
         const bool wfar = s0 >= near_total;
+        // a slot's entry: the lane of the last entry start at or below it in
+        // this window (marks by start slot, a ballot of the marks), or the
+        // previous window's last entry
+        int32_t* mark = s_beg[wave];
+        mark[lane] = -1;
+        __builtin_amdgcn_wave_barrier();
+        if (cnt > 0 && exc >= s0 && exc < s0 + PT_WAVE) mark[exc - s0] = lane;
+        __builtin_amdgcn_wave_barrier();
+        const unsigned long long starts = __ballot(mark[lane] >= 0);
+        const unsigned long long upto = starts & (~0ull >> (PT_WAVE - 1 - lane));
+        const int leaf = upto ? mark[PT_WAVE - 1 - __clzll(upto)] : carry;
+        carry = __builtin_amdgcn_readlane(leaf, PT_WAVE - 1);
+        const int lbeg = __shfl(beg, leaf, PT_WAVE);
+        const int lexc = __shfl(exc, leaf, PT_WAVE);
+        const unsigned jmask = (unsigned)__shfl((int)msk, leaf, PT_WAVE);
         const int slot = s0 + lane;
         if (slot < total) {
-          // the last entry whose exclusive start is <= slot holds the slot
-          // (entries with zero count share the start of the next entry and
-          // come before it, so the last such entry is the non-empty one)
-          int lo = 0, hi = PT_WAVE - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_exc[wave][mid] <= slot) lo = mid;
-            else hi = mid - 1;
-          }
-          const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
-          const unsigned jmask = (unsigned)s_msk[wave][lo];
+          const int64_t j = (int64_t)lbeg + (slot - lexc);
           const double jx = px[j] + ox, jy = py[j] + oy, jz = pz[j] + oz;
           const double jm = pm[j], je = pe[j];
           double sgx = 0.0, sgy = 0.0, sgz = 0.0, sgm = 0.0;
@@ -634,8 +632,6 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
   __shared__ TgtA32 s_t[kAdjWaves][kANT];
   __shared__ Bucket s_bk[kAdjWaves][PT_GROUP_BUCKETS];
   __shared__ int32_t s_beg[kAdjWaves][PT_WAVE];
-  __shared__ int32_t s_exc[kAdjWaves][PT_WAVE];
-  __shared__ int32_t s_msk[kAdjWaves][PT_WAVE];
   __shared__ double s_far[kAdjWaves][kPassFar];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PT_WAVE);
   const int lane = threadIdx.x & (PT_WAVE - 1);
@@ -819,27 +815,29 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
       const int incl_f = pt_wave_incl_scan(cf);
       const int near_total = __builtin_amdgcn_readlane(incl_n, PT_WAVE - 1);
       const int total = near_total + __builtin_amdgcn_readlane(incl_f, PT_WAVE - 1);
-      // entries in slot order (near ones first), so the starts ascend
-      const unsigned long long nearm = __ballot(isnear);
-      const int pos = isnear ? pt_rank_in(nearm) : __popcll(nearm) + pt_rank_in(~nearm);
-      __builtin_amdgcn_wave_barrier();
-      s_beg[wave][pos] = beg;
-      s_exc[wave][pos] = isnear ? incl_n - cn : near_total + incl_f - cf;
-      s_msk[wave][pos] = (int32_t)msk;
-      __builtin_amdgcn_wave_barrier();
+      // each entry's first slot, near entries first
+      const int exc = isnear ? incl_n - cn : near_total + incl_f - cf;
+      int carry = 0;
       for (int s0 = 0; s0 < total; s0 += PT_WAVE) {
         const bool wfar = s0 >= near_total;
+        // a slot's entry: the lane of the last entry start at or below it in
+        // this window (marks by start slot, a ballot of the marks), or the
+        // previous window's last entry
+        int32_t* mark = s_beg[wave];
+        mark[lane] = -1;
+        __builtin_amdgcn_wave_barrier();
+        if (cnt > 0 && exc >= s0 && exc < s0 + PT_WAVE) mark[exc - s0] = lane;
+        __builtin_amdgcn_wave_barrier();
+        const unsigned long long starts = __ballot(mark[lane] >= 0);
+        const unsigned long long upto = starts & (~0ull >> (PT_WAVE - 1 - lane));
+        const int leaf = upto ? mark[PT_WAVE - 1 - __clzll(upto)] : carry;
+        carry = __builtin_amdgcn_readlane(leaf, PT_WAVE - 1);
+        const int lbeg = __shfl(beg, leaf, PT_WAVE);
+        const int lexc = __shfl(exc, leaf, PT_WAVE);
+        const unsigned jmask = (unsigned)__shfl((int)msk, leaf, PT_WAVE);
         const int slot = s0 + lane;
         if (slot < total) {
-          // the last entry whose exclusive start is <= slot holds the slot
-          int lo = 0, hi = PT_WAVE - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_exc[wave][mid] <= slot) lo = mid;
-            else hi = mid - 1;
-          }
-          const int64_t j = (int64_t)s_beg[wave][lo] + (slot - s_exc[wave][lo]);
-          const unsigned jmask = (unsigned)s_msk[wave][lo];
+          const int64_t j = (int64_t)lbeg + (slot - lexc);
           const float jx = (float)((px[j] + ox) - ccx), jy = (float)((py[j] + oy) - ccy),
                       jz = (float)((pz[j] + oz) - ccz);
           const float jm = (float)pm[j], je = (float)pe[j];
