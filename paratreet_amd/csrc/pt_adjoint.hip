@@ -208,51 +208,54 @@ __device__ __forceinline__ void node_entry_pairs(const GravNode& s, unsigned mas
         }
       } else {
         const double rx = (tt.x - s.cx) - ox, ry = (tt.y - s.cy) - oy, rz = (tt.z - s.cz) - oz;
-        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double r2 = fma(rx, rx, fma(ry, ry, rz * rz));
         const double ir = pt_rsqrt64(r2);
         const double r = r2 * ir;
         const SplineQ q = splineq_grad(r, ir, r2, s.soft + tt.e);
-        const double qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
-        const double qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
-        const double qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
-        const double qir = 0.5 * (qx * rx + qy * ry + qz * rz);
-        const double qir3 = q.b * s.mass + q.d * qir - q.c * s.tr;
-        const double rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
-        const double qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
-        const double Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
-        const double Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
-        const double Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
-        const double k1 = (s.mass * q.br + qir * q.dr_ - s.tr * q.cr) * ir;
+        const double qx = fma(s.qxx, rx, fma(s.qxy, ry, s.qxz * rz));
+        const double qy = fma(s.qxy, rx, fma(s.qyy, ry, s.qyz * rz));
+        const double qz = fma(s.qxz, rx, fma(s.qyz, ry, s.qzz * rz));
+        const double qir = 0.5 * fma(qx, rx, fma(qy, ry, qz * rz));
+        const double qir3 = fma(q.b, s.mass, fma(q.d, qir, -q.c * s.tr));
+        const double rw = fma(rx, tt.wx, fma(ry, tt.wy, rz * tt.wz));
+        const double qw = fma(qx, tt.wx, fma(qy, tt.wy, qz * tt.wz));
+        const double Qwx = fma(s.qxx, tt.wx, fma(s.qxy, tt.wy, s.qxz * tt.wz));
+        const double Qwy = fma(s.qxy, tt.wx, fma(s.qyy, tt.wy, s.qyz * tt.wz));
+        const double Qwz = fma(s.qxz, tt.wx, fma(s.qyz, tt.wy, s.qzz * tt.wz));
+        const double k1 = fma(s.mass, q.br, fma(qir, q.dr_, -s.tr * q.cr)) * ir;
         const double k2 = q.cr * ir;
-        const double Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
-                          (k2 * qw) * rx;
-        const double Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
-                          (k2 * qw) * ry;
-        const double Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
-                          (k2 * qw) * rz;
+        // J = -qir3 w + (k2 (q.w) - k1 (r.w)) r - d (r.w) q + c Q w
+        const double s1 = fma(k2, qw, -k1 * rw);
+        const double drw = q.d * rw;
+        const double Jx = fma(-qir3, tt.wx, fma(s1, rx, fma(-drw, qx, q.c * Qwx)));
+        const double Jy = fma(-qir3, tt.wy, fma(s1, ry, fma(-drw, qy, q.c * Qwy)));
+        const double Jz = fma(-qir3, tt.wz, fma(s1, rz, fma(-drw, qz, q.c * Qwz)));
         tg[t][0] += Jx;
         tg[t][1] += Jy;
         tg[t][2] += Jz;
         lam[0] -= Jx;
         lam[1] -= Jy;
         lam[2] -= Jz;
-        lam[3] -= q.b * rw;
-        const double hd = -0.5 * q.d * rw, hc = 0.5 * q.c * rw;
-        lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
-        lam[5] += 2.0 * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
-        lam[6] += 2.0 * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
-        lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
-        lam[8] += 2.0 * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
-        lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
-        lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * s.tr) * rw + q.ct * qw;
+        lam[3] = fma(-q.b, rw, lam[3]);
+        // d(a.w)/dQ per stored component: (-d (r.w) / 2) r r^T + (c (r.w) / 2) I
+        // + c sym(w r^T), written as r_k p_l + r_l p_k (+ the trace part) with
+        // p = -d (r.w) r / 2 + c w
+        const double hd = -0.5 * drw, hc = 0.5 * q.c * rw;
+        const double qpx = fma(hd, rx, q.c * tt.wx), qpy = fma(hd, ry, q.c * tt.wy),
+                     qpz = fma(hd, rz, q.c * tt.wz);
+        lam[4] += fma(rx, qpx, hc);
+        lam[5] += fma(ry, qpx, rx * qpy);
+        lam[6] += fma(rz, qpx, rx * qpz);
+        lam[7] += fma(ry, qpy, hc);
+        lam[8] += fma(rz, qpy, ry * qpz);
+        lam[9] += fma(rz, qpz, hc);
+        lam[10] += fma(-fma(q.bt, s.mass, fma(q.dt_, qir, -q.ct * s.tr)), rw, q.ct * qw);
       }
     }
   }
 }
 
-// One source particle against the pass's targets (FP64).  FAR: every pair is
-// beyond its softening (and so not the target itself).
-template <bool FAR>
+// One source particle against the pass's targets (FP64), the softened form.
 __device__ __forceinline__ void particle_pairs(double jx, double jy, double jz, double jm,
                                                double je, unsigned jmask,
                                                const TgtA* __restrict__ st, int ng,
@@ -264,15 +267,10 @@ __device__ __forceinline__ void particle_pairs(double jx, double jy, double jz, 
       const TgtA tt = st[t];
       const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
-      if (FAR || r2 != 0) {
+      if (r2 != 0) {
         const double ir = pt_rsqrt64(r2);
         double bb, db;
-        if (FAR) {
-          bb = ir * ir * ir;
-          db = -3.0 * bb * ir;
-        } else {
-          spline_b_db(r2 * ir, ir, je + tt.e, bb, db);
-        }
+        spline_b_db(r2 * ir, ir, je + tt.e, bb, db);
         const double dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
         const double f = db * ir * dw;
         const double Kx = jm * (bb * tt.wx + f * dx);
@@ -288,6 +286,47 @@ __device__ __forceinline__ void particle_pairs(double jx, double jy, double jz, 
       }
     }
   }
+}
+
+// One source particle of a far window (every pair beyond its softening, so
+// not the target itself) against all kANT targets of the pass (FP64), without
+// branches: a pair whose target bucket is not in the mask, and the padding
+// targets past the pass's count (bucket index 31, the position of a real
+// target, so r > 0), is evaluated with a zero cotangent w, which zeroes each
+// of its contributions (all are linear in w); consecutive targets' pairs then
+// interleave.  The source's position gradient is summed before its mass
+// factor.
+__device__ __forceinline__ void particle_pairs_far(double jx, double jy, double jz, double jm,
+                                                   unsigned jmask, const TgtA* __restrict__ st,
+                                                   double (&tg)[kANT][3], double& sgx,
+                                                   double& sgy, double& sgz, double& sgm) {
+  double kx = 0.0, ky = 0.0, kz = 0.0;
+#pragma unroll
+  for (int t = 0; t < kANT; t++) {
+    const TgtA tt = st[t];
+    const bool act = (jmask >> tt.bt) & 1u;
+    const double wx = act ? tt.wx : 0.0, wy = act ? tt.wy : 0.0, wz = act ? tt.wz : 0.0;
+    const double dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double ir = pt_rsqrt64(r2);
+    const double a2 = ir * ir;
+    const double bb = ir * a2;
+    const double dw = fma(dx, wx, fma(dy, wy, dz * wz));
+    const double f = -3.0 * (bb * a2) * dw;  // b'(r) (d.w) / r
+    const double Kx = fma(bb, wx, f * dx);
+    const double Ky = fma(bb, wy, f * dy);
+    const double Kz = fma(bb, wz, f * dz);
+    tg[t][0] = fma(-jm, Kx, tg[t][0]);
+    tg[t][1] = fma(-jm, Ky, tg[t][1]);
+    tg[t][2] = fma(-jm, Kz, tg[t][2]);
+    kx += Kx;
+    ky += Ky;
+    kz += Kz;
+    sgm = fma(bb, dw, sgm);
+  }
+  sgx += jm * kx;
+  sgy += jm * ky;
+  sgz += jm * kz;
 }
 
 // One wave per bucket group (consecutive buckets, at most PT_GROUP_TARGETS
@@ -358,6 +397,11 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
       for (int k = 1; k < nbk; k++)
         if (i >= s_bk[wave][k].begin) bt = k;
       s_t[wave][lane] = TgtA{px[i], py[i], pz[i], pe[i], wx[i], wy[i], wz[i], bt, 0};
+    } else if (lane < kANT) {
+      // padding for the branch-free far-window form: a real target's
+      // position, no bucket, zero cotangent
+      const int64_t i = first + g0;
+      s_t[wave][lane] = TgtA{px[i], py[i], pz[i], 0.0, 0.0, 0.0, 0.0, 31, 0};
     }
     __builtin_amdgcn_wave_barrier();
     double tg[kANT][3];
@@ -452,10 +496,9 @@ adjoint_group_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict__ 
           const double jm = pm[j], je = pe[j];
           double sgx = 0.0, sgy = 0.0, sgz = 0.0, sgm = 0.0;
           if (wfar)
-            particle_pairs<true>(jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz, sgm);
+            particle_pairs_far(jx, jy, jz, jm, jmask, s_t[wave], tg, sgx, sgy, sgz, sgm);
           else
-            particle_pairs<false>(jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz,
-                                  sgm);
+            particle_pairs(jx, jy, jz, jm, je, jmask, s_t[wave], ng, tg, sgx, sgy, sgz, sgm);
           atomicAdd(&gx[j], sgx);
           atomicAdd(&gy[j], sgy);
           atomicAdd(&gz[j], sgz);
