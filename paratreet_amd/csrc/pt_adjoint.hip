@@ -627,7 +627,7 @@ __device__ __forceinline__ void particle_pairs32(bool far, float jx, float jy, f
     if (t < ng && ((jmask >> st[t].bt) & 1u)) {
       const TgtA32 tt = st[t];
       const float dx = jx - tt.x, dy = jy - tt.y, dz = jz - tt.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       if (far || r2 != 0.0f) {
         const float ir = __builtin_amdgcn_rsqf(r2);
         float bb, db;
@@ -637,11 +637,11 @@ __device__ __forceinline__ void particle_pairs32(bool far, float jx, float jy, f
         } else {
           spline_b_db32(r2 * ir, ir, je + tt.e, bb, db);
         }
-        const float dw = dx * tt.wx + dy * tt.wy + dz * tt.wz;
+        const float dw = fmaf(dx, tt.wx, fmaf(dy, tt.wy, dz * tt.wz));
         const float f = db * ir * dw;
-        const float Kx = jm * (bb * tt.wx + f * dx);
-        const float Ky = jm * (bb * tt.wy + f * dy);
-        const float Kz = jm * (bb * tt.wz + f * dz);
+        const float Kx = jm * fmaf(bb, tt.wx, f * dx);
+        const float Ky = jm * fmaf(bb, tt.wy, f * dy);
+        const float Kz = jm * fmaf(bb, tt.wz, f * dz);
         tg[t][0] -= (double)Kx;
         tg[t][1] -= (double)Ky;
         tg[t][2] -= (double)Kz;
@@ -789,43 +789,46 @@ adjoint_group32_kernel(const Bucket* __restrict__ bk, const int32_t* __restrict_
               }
             } else {
               const float rx = tt.x - cx, ry = tt.y - cy, rz = tt.z - cz;
-              const float r2 = rx * rx + ry * ry + rz * rz;
+              const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
               const float ir = __builtin_amdgcn_rsqf(r2);
               const float r = r2 * ir;
               const SplineQ32 q = splineq_grad32(r, ir, r2, s.soft + tt.e);
-              const float qx = s.qxx * rx + s.qxy * ry + s.qxz * rz;
-              const float qy = s.qxy * rx + s.qyy * ry + s.qyz * rz;
-              const float qz = s.qxz * rx + s.qyz * ry + s.qzz * rz;
-              const float qir = 0.5f * (qx * rx + qy * ry + qz * rz);
-              const float qir3 = q.b * s.mass + q.d * qir - q.c * tr;
-              const float rw = rx * tt.wx + ry * tt.wy + rz * tt.wz;
-              const float qw = qx * tt.wx + qy * tt.wy + qz * tt.wz;
-              const float Qwx = s.qxx * tt.wx + s.qxy * tt.wy + s.qxz * tt.wz;
-              const float Qwy = s.qxy * tt.wx + s.qyy * tt.wy + s.qyz * tt.wz;
-              const float Qwz = s.qxz * tt.wx + s.qyz * tt.wy + s.qzz * tt.wz;
-              const float k1 = (s.mass * q.br + qir * q.dr_ - tr * q.cr) * ir;
+              const float qx = fmaf(s.qxx, rx, fmaf(s.qxy, ry, s.qxz * rz));
+              const float qy = fmaf(s.qxy, rx, fmaf(s.qyy, ry, s.qyz * rz));
+              const float qz = fmaf(s.qxz, rx, fmaf(s.qyz, ry, s.qzz * rz));
+              const float qir = 0.5f * fmaf(qx, rx, fmaf(qy, ry, qz * rz));
+              const float qir3 = fmaf(q.b, s.mass, fmaf(q.d, qir, -q.c * tr));
+              const float rw = fmaf(rx, tt.wx, fmaf(ry, tt.wy, rz * tt.wz));
+              const float qw = fmaf(qx, tt.wx, fmaf(qy, tt.wy, qz * tt.wz));
+              const float Qwx = fmaf(s.qxx, tt.wx, fmaf(s.qxy, tt.wy, s.qxz * tt.wz));
+              const float Qwy = fmaf(s.qxy, tt.wx, fmaf(s.qyy, tt.wy, s.qyz * tt.wz));
+              const float Qwz = fmaf(s.qxz, tt.wx, fmaf(s.qyz, tt.wy, s.qzz * tt.wz));
+              const float k1 = fmaf(s.mass, q.br, fmaf(qir, q.dr_, -tr * q.cr)) * ir;
               const float k2 = q.cr * ir;
-              const float Jx = -qir3 * tt.wx - (k1 * rw) * rx - (q.d * rw) * qx + q.c * Qwx +
-                               (k2 * qw) * rx;
-              const float Jy = -qir3 * tt.wy - (k1 * rw) * ry - (q.d * rw) * qy + q.c * Qwy +
-                               (k2 * qw) * ry;
-              const float Jz = -qir3 * tt.wz - (k1 * rw) * rz - (q.d * rw) * qz + q.c * Qwz +
-                               (k2 * qw) * rz;
+              // J = -qir3 w + (k2 (q.w) - k1 (r.w)) r - d (r.w) q + c Q w
+              const float s1 = fmaf(k2, qw, -k1 * rw);
+              const float drw = q.d * rw;
+              const float Jx = fmaf(-qir3, tt.wx, fmaf(s1, rx, fmaf(-drw, qx, q.c * Qwx)));
+              const float Jy = fmaf(-qir3, tt.wy, fmaf(s1, ry, fmaf(-drw, qy, q.c * Qwy)));
+              const float Jz = fmaf(-qir3, tt.wz, fmaf(s1, rz, fmaf(-drw, qz, q.c * Qwz)));
               tg[t][0] += (double)Jx;
               tg[t][1] += (double)Jy;
               tg[t][2] += (double)Jz;
               lam[0] -= Jx;
               lam[1] -= Jy;
               lam[2] -= Jz;
-              lam[3] -= q.b * rw;
-              const float hd = -0.5f * q.d * rw, hc = 0.5f * q.c * rw;
-              lam[4] += hd * rx * rx + hc + q.c * tt.wx * rx;
-              lam[5] += 2.0f * hd * rx * ry + q.c * (tt.wx * ry + tt.wy * rx);
-              lam[6] += 2.0f * hd * rx * rz + q.c * (tt.wx * rz + tt.wz * rx);
-              lam[7] += hd * ry * ry + hc + q.c * tt.wy * ry;
-              lam[8] += 2.0f * hd * ry * rz + q.c * (tt.wy * rz + tt.wz * ry);
-              lam[9] += hd * rz * rz + hc + q.c * tt.wz * rz;
-              lam[10] += -(q.bt * s.mass + q.dt_ * qir - q.ct * tr) * rw + q.ct * qw;
+              lam[3] = fmaf(-q.b, rw, lam[3]);
+              // d(a.w)/dQ as r_k p_l + r_l p_k (+ the trace part), p = -d (r.w) r / 2 + c w
+              const float hd = -0.5f * drw, hc = 0.5f * q.c * rw;
+              const float qpx = fmaf(hd, rx, q.c * tt.wx), qpy = fmaf(hd, ry, q.c * tt.wy),
+                          qpz = fmaf(hd, rz, q.c * tt.wz);
+              lam[4] += fmaf(rx, qpx, hc);
+              lam[5] += fmaf(ry, qpx, rx * qpy);
+              lam[6] += fmaf(rz, qpx, rx * qpz);
+              lam[7] += fmaf(ry, qpy, hc);
+              lam[8] += fmaf(rz, qpy, ry * qpz);
+              lam[9] += fmaf(rz, qpz, hc);
+              lam[10] += fmaf(-fmaf(q.bt, s.mass, fmaf(q.dt_, qir, -q.ct * tr)), rw, q.ct * qw);
             }
           }
         }
