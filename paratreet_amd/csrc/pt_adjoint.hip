@@ -81,7 +81,7 @@ struct SplineQ {
 };
 
 __device__ __forceinline__ SplineQ splineq_grad(double r, double dir, double r2, double twoh) {
-  SplineQ s;
+  double b, c, d, br, cr, dr_, bt, ct, dt_;
   if (r2 < twoh * twoh) {
     const double dih = 2.0 / twoh;
     const double u = dih * r;
@@ -89,43 +89,45 @@ __device__ __forceinline__ SplineQ splineq_grad(double r, double dir, double r2,
                  dih6 = dih5 * dih;
     double bD, cD, dD;  // derivatives with respect to dih at fixed r
     if (u < 1.0) {
-      s.b = dih3 * 4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u;
-      s.c = dih5 * 12.0 / 5.0 - 3.0 / 2.0 * u;
-      s.d = 3.0 / 2.0 * dih6 * dir;
-      s.br = dih * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
-      s.cr = -3.0 / 2.0 * dih;
-      s.dr_ = -3.0 / 2.0 * dih6 * dir * dir;
+      b = dih3 * 4.0 / 3.0 - 6.0 / 5.0 * u * u + 1.0 / 2.0 * u * u * u;
+      c = dih5 * 12.0 / 5.0 - 3.0 / 2.0 * u;
+      d = 3.0 / 2.0 * dih6 * dir;
+      br = dih * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
+      cr = -3.0 / 2.0 * dih;
+      dr_ = -3.0 / 2.0 * dih6 * dir * dir;
       bD = 4.0 * dih2 + r * (-12.0 / 5.0 * u + 3.0 / 2.0 * u * u);
       cD = 12.0 * dih4 - 3.0 / 2.0 * r;
       dD = 9.0 * dih5 * dir;
     } else {
       const double dir2 = dir * dir, dir3 = dir2 * dir;
-      s.b = -1.0 / 15.0 * dir3 + dih3 * 8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u -
+      b = -1.0 / 15.0 * dir3 + dih3 * 8.0 / 3.0 - 3.0 * u + 6.0 / 5.0 * u * u -
             1.0 / 6.0 * u * u * u;
-      s.c = -1.0 / 5.0 * dir3 * dir2 + 3.0 * dih4 * dir - dih5 * 12.0 / 5.0 + 1.0 / 2.0 * u;
-      s.d = -dir3 * dir3 * dir + 3.0 * dih4 * dir3 - 1.0 / 2.0 * dih6 * dir;
-      s.br = 1.0 / 5.0 * dir2 * dir2 + dih * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
-      s.cr = dir3 * dir3 - 3.0 * dih4 * dir2 + 1.0 / 2.0 * dih;
-      s.dr_ = 7.0 * dir3 * dir3 * dir2 - 9.0 * dih4 * dir2 * dir2 + 1.0 / 2.0 * dih6 * dir2;
+      c = -1.0 / 5.0 * dir3 * dir2 + 3.0 * dih4 * dir - dih5 * 12.0 / 5.0 + 1.0 / 2.0 * u;
+      d = -dir3 * dir3 * dir + 3.0 * dih4 * dir3 - 1.0 / 2.0 * dih6 * dir;
+      br = 1.0 / 5.0 * dir2 * dir2 + dih * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
+      cr = dir3 * dir3 - 3.0 * dih4 * dir2 + 1.0 / 2.0 * dih;
+      dr_ = 7.0 * dir3 * dir3 * dir2 - 9.0 * dih4 * dir2 * dir2 + 1.0 / 2.0 * dih6 * dir2;
       bD = 8.0 * dih2 + r * (-3.0 + 12.0 / 5.0 * u - 1.0 / 2.0 * u * u);
       cD = 12.0 * dih3 * dir - 12.0 * dih4 + 1.0 / 2.0 * r;
       dD = 12.0 * dih3 * dir3 - 3.0 * dih5 * dir;
     }
     const double dd_dt = -0.5 * dih2;  // d(dih)/d(twoh)
-    s.bt = bD * dd_dt;
-    s.ct = cD * dd_dt;
-    s.dt_ = dD * dd_dt;
+    bt = bD * dd_dt;
+    ct = cD * dd_dt;
+    dt_ = dD * dd_dt;
   } else {
     const double a = dir;
-    s.b = a * a * a;
-    s.c = 3.0 * s.b * a * a;
-    s.d = 5.0 * s.c * a * a;
-    s.br = -3.0 * s.b * a;
-    s.cr = -5.0 * s.c * a;
-    s.dr_ = -7.0 * s.d * a;
-    s.bt = s.ct = s.dt_ = 0.0;
+    b = a * a * a;
+    c = 3.0 * b * a * a;
+    d = 5.0 * c * a * a;
+    br = -3.0 * b * a;
+    cr = -5.0 * c * a;
+    dr_ = -7.0 * d * a;
+    bt = ct = dt_ = 0.0;
   }
-  return s;
+  // built once from scalars: field stores in the branches can be merged into
+  // one store through a pointer select, which keeps the struct in scratch memory
+  return SplineQ{b, c, d, br, cr, dr_, bt, ct, dt_};
 }
 
 struct TgtA {
@@ -561,7 +563,7 @@ struct SplineQ32 {
 
 // splineq_grad in FP32; ir = 1 / r.
 __device__ __forceinline__ SplineQ32 splineq_grad32(float r, float ir, float r2, float twoh) {
-  SplineQ32 s;
+  float b, c, d, br, cr, dr_, bt, ct, dt_;
   const float dir = ir;
   if (r2 < twoh * twoh) {
     const float dih = 2.0f * rcp32(twoh);
@@ -570,43 +572,45 @@ __device__ __forceinline__ SplineQ32 splineq_grad32(float r, float ir, float r2,
                 dih6 = dih5 * dih;
     float bD, cD, dD;
     if (u < 1.0f) {
-      s.b = dih3 * 4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u;
-      s.c = dih5 * 12.0f / 5.0f - 3.0f / 2.0f * u;
-      s.d = 3.0f / 2.0f * dih6 * dir;
-      s.br = dih * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
-      s.cr = -3.0f / 2.0f * dih;
-      s.dr_ = -3.0f / 2.0f * dih6 * dir * dir;
+      b = dih3 * 4.0f / 3.0f - 6.0f / 5.0f * u * u + 1.0f / 2.0f * u * u * u;
+      c = dih5 * 12.0f / 5.0f - 3.0f / 2.0f * u;
+      d = 3.0f / 2.0f * dih6 * dir;
+      br = dih * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
+      cr = -3.0f / 2.0f * dih;
+      dr_ = -3.0f / 2.0f * dih6 * dir * dir;
       bD = 4.0f * dih2 + r * (-12.0f / 5.0f * u + 3.0f / 2.0f * u * u);
       cD = 12.0f * dih4 - 3.0f / 2.0f * r;
       dD = 9.0f * dih5 * dir;
     } else {
       const float dir2 = dir * dir, dir3 = dir2 * dir;
-      s.b = -1.0f / 15.0f * dir3 + dih3 * 8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u -
+      b = -1.0f / 15.0f * dir3 + dih3 * 8.0f / 3.0f - 3.0f * u + 6.0f / 5.0f * u * u -
             1.0f / 6.0f * u * u * u;
-      s.c = -1.0f / 5.0f * dir3 * dir2 + 3.0f * dih4 * dir - dih5 * 12.0f / 5.0f + 1.0f / 2.0f * u;
-      s.d = -dir3 * dir3 * dir + 3.0f * dih4 * dir3 - 1.0f / 2.0f * dih6 * dir;
-      s.br = 1.0f / 5.0f * dir2 * dir2 + dih * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
-      s.cr = dir3 * dir3 - 3.0f * dih4 * dir2 + 1.0f / 2.0f * dih;
-      s.dr_ = 7.0f * dir3 * dir3 * dir2 - 9.0f * dih4 * dir2 * dir2 + 1.0f / 2.0f * dih6 * dir2;
+      c = -1.0f / 5.0f * dir3 * dir2 + 3.0f * dih4 * dir - dih5 * 12.0f / 5.0f + 1.0f / 2.0f * u;
+      d = -dir3 * dir3 * dir + 3.0f * dih4 * dir3 - 1.0f / 2.0f * dih6 * dir;
+      br = 1.0f / 5.0f * dir2 * dir2 + dih * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
+      cr = dir3 * dir3 - 3.0f * dih4 * dir2 + 1.0f / 2.0f * dih;
+      dr_ = 7.0f * dir3 * dir3 * dir2 - 9.0f * dih4 * dir2 * dir2 + 1.0f / 2.0f * dih6 * dir2;
       bD = 8.0f * dih2 + r * (-3.0f + 12.0f / 5.0f * u - 1.0f / 2.0f * u * u);
       cD = 12.0f * dih3 * dir - 12.0f * dih4 + 1.0f / 2.0f * r;
       dD = 12.0f * dih3 * dir3 - 3.0f * dih5 * dir;
     }
     const float dd_dt = -0.5f * dih2;
-    s.bt = bD * dd_dt;
-    s.ct = cD * dd_dt;
-    s.dt_ = dD * dd_dt;
+    bt = bD * dd_dt;
+    ct = cD * dd_dt;
+    dt_ = dD * dd_dt;
   } else {
     const float a = dir;
-    s.b = a * a * a;
-    s.c = 3.0f * s.b * a * a;
-    s.d = 5.0f * s.c * a * a;
-    s.br = -3.0f * s.b * a;
-    s.cr = -5.0f * s.c * a;
-    s.dr_ = -7.0f * s.d * a;
-    s.bt = s.ct = s.dt_ = 0.0f;
+    b = a * a * a;
+    c = 3.0f * b * a * a;
+    d = 5.0f * c * a * a;
+    br = -3.0f * b * a;
+    cr = -5.0f * c * a;
+    dr_ = -7.0f * d * a;
+    bt = ct = dt_ = 0.0f;
   }
-  return s;
+  // built once from scalars: field stores in the branches can be merged into
+  // one store through a pointer select, which keeps the struct in scratch memory
+  return SplineQ32{b, c, d, br, cr, dr_, bt, ct, dt_};
 }
 
 struct TgtA32 {
